@@ -1,0 +1,235 @@
+"""ctypes loaders for the checker libraries.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this package.
+
+* :class:`Oracle` wraps ``oracle/liboracle.so`` -- the scalar C restatement of
+  ``p7_MSVFilter`` (impl_sse/msvfilter.c:78-213 with impl_sse/ssvfilter.c:1364-1413).
+* :class:`Reference` wraps ``oracle/_ref/libmsvref.so`` -- the HMMER 3.1b1
+  reference compiled from its own sources plus ``oracle/ref_harness.c``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libmsvref.so")
+
+eslOK, eslEINVAL, eslERANGE, eslENORESULT = 0, 11, 16, 19
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class _OrcProfile(C.Structure):
+    _fields_ = [("M", C.c_int), ("Kp", C.c_int), ("rbv", C.c_void_p),
+                ("tbm_b", C.c_int), ("tec_b", C.c_int), ("base_b", C.c_int),
+                ("bias_b", C.c_int), ("scale_b", C.c_float)]
+
+
+class Oracle:
+    """The CPU restatement.  Profiles are dicts with keys rbv (Kp x M uint8),
+    tbm, tec, base, bias, scale (see cudahmmsearch_amd.ByteProfile)."""
+
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built; run make -C oracle")
+        lib = C.CDLL(path)
+        lib.orc_tjb.restype = C.c_int
+        lib.orc_tjb.argtypes = [C.c_int]
+        for f in ("orc_ssv", "orc_msv_bytes", "orc_msv"):
+            getattr(lib, f).restype = C.c_int
+            getattr(lib, f).argtypes = [C.POINTER(_OrcProfile), C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        lib.orc_score.restype = C.c_float
+        lib.orc_score.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float]
+        lib.orc_nullsc.restype = C.c_float
+        lib.orc_nullsc.argtypes = [C.c_int]
+        lib.orc_pass.restype = C.c_int
+        lib.orc_pass.argtypes = [C.c_float, C.c_int, C.c_float, C.c_float, C.c_double]
+        lib.orc_msv_batch.restype = C.c_int
+        lib.orc_msv_batch.argtypes = [C.POINTER(_OrcProfile), C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        self.lib = lib
+
+    @staticmethod
+    def _prof(p):
+        rbv = np.ascontiguousarray(p["rbv"], dtype=np.uint8)
+        s = _OrcProfile(int(rbv.shape[1]), int(rbv.shape[0]), rbv.ctypes.data,
+                        int(p["tbm"]), int(p["tec"]), int(p["base"]), int(p["bias"]), float(p["scale"]))
+        return s, rbv
+
+    def tjb(self, L):
+        return self.lib.orc_tjb(int(L))
+
+    def _one(self, fn, p, x):
+        s, keep = self._prof(p)
+        x = np.ascontiguousarray(x, dtype=np.uint8)
+        usc = C.c_float()
+        st = fn(C.byref(s), _p(x), int(x.size), C.byref(usc))
+        return st, usc.value
+
+    def msv(self, p, x):
+        return self._one(self.lib.orc_msv, p, x)
+
+    def ssv(self, p, x):
+        return self._one(self.lib.orc_ssv, p, x)
+
+    def msv_bytes(self, p, x):
+        return self._one(self.lib.orc_msv_bytes, p, x)
+
+    def msv_batch(self, p, res, off, lens, nthreads=8):
+        s, keep = self._prof(p)
+        res = np.ascontiguousarray(res, dtype=np.uint8)
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        lens = np.ascontiguousarray(lens, dtype=np.int32)
+        n = lens.size
+        usc = np.zeros(n, np.float32)
+        st = np.zeros(n, np.int32)
+        self.lib.orc_msv_batch(C.byref(s), _p(res), _p(off), _p(lens), n, _p(usc), _p(st), nthreads)
+        return st, usc
+
+    def score(self, xJ, tjb, base, scale):
+        return self.lib.orc_score(int(xJ), int(tjb), int(base), float(scale))
+
+    def nullsc(self, L):
+        return self.lib.orc_nullsc(int(L))
+
+    def passes(self, usc, L, mu, lam, F1=0.02):
+        return self.lib.orc_pass(float(usc), int(L), float(mu), float(lam), float(F1))
+
+
+class Reference:
+    """The compiled HMMER reference (only where oracle/_ref was built)."""
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built; run make -C oracle ref")
+        lib = C.CDLL(path)
+        lib.ref_profile_sample.restype = C.c_void_p
+        lib.ref_profile_sample.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int]
+        lib.ref_profile_from_probs.restype = C.c_void_p
+        lib.ref_profile_from_probs.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        lib.ref_profile_free.argtypes = [C.c_void_p]
+        lib.ref_profile_M.argtypes = [C.c_void_p]
+        lib.ref_profile_rbv.argtypes = [C.c_void_p, C.c_void_p]
+        lib.ref_profile_scalars.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ref_profile_hmm.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ref_profile_msc.argtypes = [C.c_void_p, C.c_void_p]
+        lib.ref_tjb.argtypes = [C.c_void_p, C.c_int]
+        for f in ("ref_msv", "ref_ssv"):
+            getattr(lib, f).argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        lib.ref_nullsc.restype = C.c_float
+        lib.ref_nullsc.argtypes = [C.c_void_p, C.c_int]
+        lib.ref_pass.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_double]
+        lib.ref_msv_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                      C.c_void_p, C.c_void_p, C.c_int]
+        lib.ref_iid.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_void_p]
+        lib.ref_uniform.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_void_p]
+        lib.ref_core_emit.restype = C.c_int64
+        lib.ref_core_emit.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]
+        self.lib = lib
+
+    def sample(self, seed, M, config_L=100, calibrate=True):
+        h = self.lib.ref_profile_sample(seed, M, config_L, 1 if calibrate else 0)
+        if not h:
+            raise RuntimeError("reference profile sampling failed")
+        return RefProfile(self, h)
+
+    def from_probs(self, mat, t, config_L=100, calibrate=False):
+        M = mat.shape[0] - 1
+        mat = np.ascontiguousarray(mat, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        h = self.lib.ref_profile_from_probs(M, _p(mat), _p(t), config_L, 1 if calibrate else 0)
+        if not h:
+            raise RuntimeError("reference profile construction failed")
+        return RefProfile(self, h)
+
+    def iid(self, seed, n, L):
+        out = np.zeros(n * L, np.uint8)
+        self.lib.ref_iid(seed, n, L, _p(out))
+        return out.reshape(n, L)
+
+    def uniform(self, seed, fast, n):
+        out = np.zeros(n, np.float64)
+        self.lib.ref_uniform(seed, fast, n, _p(out))
+        return out
+
+
+class RefProfile:
+    def __init__(self, ref, h):
+        self.ref, self.h, self.lib = ref, h, ref.lib
+        self.M = self.lib.ref_profile_M(h)
+
+    def __del__(self):
+        try:
+            self.lib.ref_profile_free(self.h)
+        except Exception:
+            pass
+
+    def byte_profile(self):
+        rbv = np.zeros(29 * self.M, np.uint8)
+        self.lib.ref_profile_rbv(self.h, _p(rbv))
+        si = np.zeros(5, np.int32)
+        sf = np.zeros(3, np.float32)
+        self.lib.ref_profile_scalars(self.h, _p(si), _p(sf))
+        return {"rbv": rbv.reshape(29, self.M), "tbm": int(si[0]), "tec": int(si[1]),
+                "base": int(si[3]), "bias": int(si[4]), "scale": float(sf[0]),
+                "mu": float(sf[1]), "lambda": float(sf[2])}
+
+    def hmm(self):
+        mat = np.zeros((self.M + 1) * 20, np.float32)
+        t = np.zeros((self.M + 1) * 7, np.float32)
+        self.lib.ref_profile_hmm(self.h, _p(mat), _p(t))
+        return mat.reshape(self.M + 1, 20), t.reshape(self.M + 1, 7)
+
+    def msc(self):
+        out = np.zeros(29 * self.M, np.float32)
+        self.lib.ref_profile_msc(self.h, _p(out))
+        return out.reshape(29, self.M)
+
+    def tjb(self, L):
+        return self.lib.ref_tjb(self.h, L)
+
+    @staticmethod
+    def _dsq(x):
+        return np.concatenate([[255], np.asarray(x, np.uint8), [255]]).astype(np.uint8)
+
+    def msv(self, x):
+        d = self._dsq(x)
+        usc = C.c_float()
+        st = self.lib.ref_msv(self.h, _p(d), len(x), C.byref(usc))
+        return st, usc.value
+
+    def ssv(self, x):
+        d = self._dsq(x)
+        usc = C.c_float()
+        st = self.lib.ref_ssv(self.h, _p(d), len(x), C.byref(usc))
+        return st, usc.value
+
+    def nullsc(self, L):
+        return self.lib.ref_nullsc(self.h, L)
+
+    def passes(self, L, usc, F1=0.02):
+        return self.lib.ref_pass(self.h, L, usc, F1)
+
+    def msv_batch(self, res, off, lens, nthreads=8):
+        res = np.ascontiguousarray(res, np.uint8)
+        off = np.ascontiguousarray(off, np.int64)
+        lens = np.ascontiguousarray(lens, np.int32)
+        n = lens.size
+        usc = np.zeros(n, np.float32)
+        st = np.zeros(n, np.int32)
+        self.lib.ref_msv_batch(self.h, _p(res), _p(off), _p(lens), n, _p(usc), _p(st), nthreads)
+        return st, usc
+
+    def core_emit(self, seed, n, cap=None):
+        cap = cap or n * (self.M * 40 + 2000)
+        out = np.zeros(cap, np.uint8)
+        lens = np.zeros(n, np.int32)
+        tot = self.lib.ref_core_emit(self.h, seed, n, _p(out), cap, _p(lens))
+        if tot < 0:
+            raise RuntimeError("core_emit buffer too small")
+        return out[:tot], lens

@@ -1,0 +1,419 @@
+"""cudahmmsearch_amd -- HMMER3's MSV filter (p7_MSVFilter) on AMD MI355X (gfx950).
+
+The product is ``libmsvgpu.so`` (hand-written HIP kernels plus a C++ host
+runtime behind the C ABI in ``include/msv_gpu.h`` and ``include/msv_prep.h``).
+This module binds that ABI with ctypes and mirrors the reference's Python-less
+C interface one to one:
+
+=========================  ===================================================
+this module                reference (forestcheng/cudahmmsearch, code/)
+=========================  ===================================================
+``sample_profile``         p7_hmm_Sample + p7_ProfileConfig + p7_oprofile_Convert
+                           (src/p7_hmm.c:877, src/modelconfig.c:49,
+                           src/impl_sse/p7_oprofile.c:840), lambda as p7_Lambda
+``MSVContext.calibrate``   the MSV part of p7_Calibrate (src/evalues.c:60-141)
+``MSVContext.msv_filter``  p7_MSVFilter (src/impl_sse/msvfilter.c:78)
+``MSVContext.search``      cudaHmmsearch's block loop (src/GPUFunction.cu:332-492)
+                           + the F1 rule of p7_Pipeline (src/p7_pipeline.c:667-677)
+``core_emit`` / ``iid``    p7_CoreEmit (src/emit.c:71), esl_rsq_xfIID
+=========================  ===================================================
+
+No CPU fallback exists: the GPU entry points raise if the library or a GPU is
+missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "MSVError", "ByteProfile", "MSVContext", "Database", "load_library", "sample_profile",
+    "profile_from_hmm", "core_emit", "iid", "tjb", "nullsc", "gen_database", "calibration_targets",
+    "eslOK", "eslERANGE", "MODE_PLAIN", "MODE_BATCHED",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmsvgpu.so")
+
+eslOK, eslEMEM, eslEINVAL, eslERANGE, eslENORESULT = 0, 5, 11, 16, 19
+MODE_PLAIN, MODE_BATCHED, FLAG_EAGER = 0, 1, 0x10
+KP = 29
+
+
+class MSVError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"status {code}: {msg}")
+        self.code = code
+
+
+class _Prof(C.Structure):
+    _fields_ = [("M", C.c_int32), ("Kp", C.c_int32), ("rbv", C.c_void_p),
+                ("tbm_b", C.c_uint8), ("tec_b", C.c_uint8), ("base_b", C.c_uint8), ("bias_b", C.c_uint8),
+                ("scale_b", C.c_float), ("mmu", C.c_float), ("mlambda", C.c_float)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("ms_fast", C.c_double), ("ms_slow", C.c_double), ("ms_finalize", C.c_double),
+                ("ms_total", C.c_double), ("cells", C.c_int64), ("n_slow", C.c_int64),
+                ("n_fast_launches", C.c_int32)]
+
+
+class _Rng(C.Structure):
+    _fields_ = [("fast", C.c_int), ("mt", C.c_uint32 * 624), ("mti", C.c_int), ("x", C.c_uint32),
+                ("seed", C.c_uint32)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libmsvgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MSVError(-1, f"{path} is not built (run __graft_entry__.build() or make -C cudahmmsearch_amd/csrc)")
+    lib = C.CDLL(path)
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    sig = {
+        "msvg_init": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "msvg_free": (None, [vp]),
+        "msvg_last_error": (C.c_char_p, [vp]),
+        "msvg_profile_add": (C.c_int, [vp, C.POINTER(_Prof), C.POINTER(i32)]),
+        "msvg_profile_remove": (C.c_int, [vp, i32]),
+        "msvg_profile_set_evparams": (C.c_int, [vp, i32, C.c_float, C.c_float]),
+        "msvg_calibrate_mu": (C.c_int, [vp, i32, C.c_double, C.POINTER(C.c_float)]),
+        "msvg_db_add": (C.c_int, [vp, vp, vp, vp, i64, C.POINTER(i32)]),
+        "msvg_db_remove": (C.c_int, [vp, i32]),
+        "msvg_db_size": (i64, [vp, i32]),
+        "msvg_search": (C.c_int, [vp, vp, i32, i32, C.c_double, C.c_int, vp, vp, vp]),
+        "msvg_search_device": (C.c_int, [vp, vp, i32, i32, C.c_double, C.c_int]),
+        "msvg_results": (C.c_int, [vp, vp, vp, vp]),
+        "msvg_last_stats": (C.c_int, [vp, C.POINTER(_Stats)]),
+        "msvg_MSVFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
+        "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
+        "msvh_rng_init": (None, [C.POINTER(_Rng), C.c_uint32, C.c_int]),
+        "msvh_random": (C.c_double, [C.POINTER(_Rng)]),
+        "msvh_hmm_sample": (C.c_int, [C.POINTER(_Rng), C.c_int, vp, vp, vp]),
+        "msvh_profile_convert": (C.c_int, [C.c_int, vp, C.c_int, vp, vp, C.POINTER(C.c_float), vp]),
+        "msvh_lambda": (C.c_double, [C.c_int, vp]),
+        "msvh_core_emit": (C.c_int, [C.POINTER(_Rng), C.c_int, vp, vp, vp, vp, C.c_int]),
+        "msvh_iid": (None, [C.POINTER(_Rng), C.c_int, vp]),
+        "msvh_tjb": (C.c_int, [C.c_int]),
+        "msvh_nullsc": (C.c_float, [C.c_int]),
+        "msvh_fit_mu": (C.c_double, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_double]),
+        "msvh_calibration_targets": (None, [C.c_int, C.c_int, vp]),
+        "msvh_gen_lengths": (i64, [C.c_uint32, i64, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
+                                   C.c_double, C.c_double, C.c_int, vp]),
+        "msvh_gen_residues": (None, [C.c_uint32, i64, vp, vp, vp, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# Host-side profile preparation (no GPU needed)
+# ---------------------------------------------------------------------------
+class Rng:
+    """Easel's RNG: Mersenne twister (esl_randomness_Create) or, with
+    fast=True, the LCG of esl_randomness_CreateFast."""
+
+    def __init__(self, seed, fast=False):
+        self._r = _Rng()
+        load_library().msvh_rng_init(C.byref(self._r), seed, 1 if fast else 0)
+
+    def random(self):
+        return load_library().msvh_random(C.byref(self._r))
+
+
+class ByteProfile:
+    """The MSV fields of a P7_OPROFILE (impl_sse.h:79-86), de-striped.
+
+    ``rbv`` is (29, M) uint8; scalars as in the reference; ``mu``/``lam`` are
+    the calibrated MSV Gumbel parameters (``None`` until calibrated)."""
+
+    def __init__(self, rbv, tbm, tec, base, bias, scale, mu=None, lam=None, hmm=None, tjb_config=None):
+        self.rbv = np.ascontiguousarray(rbv, dtype=np.uint8)
+        self.M = self.rbv.shape[1]
+        self.tbm, self.tec, self.base, self.bias = int(tbm), int(tec), int(base), int(bias)
+        self.scale = float(scale)
+        self.mu, self.lam = mu, lam
+        self.hmm = hmm
+        self.tjb_config = tjb_config
+
+    def as_dict(self):
+        return {"rbv": self.rbv, "tbm": self.tbm, "tec": self.tec, "base": self.base, "bias": self.bias,
+                "scale": self.scale, "mu": self.mu, "lambda": self.lam}
+
+    def _struct(self):
+        return _Prof(self.M, self.rbv.shape[0], self.rbv.ctypes.data, self.tbm, self.tec, self.base, self.bias,
+                     self.scale, float(self.mu if self.mu is not None else -99999.0),
+                     float(self.lam if self.lam is not None else -99999.0))
+
+
+def profile_from_hmm(mat, ins=None, t=None, config_L=100):
+    """p7_ProfileConfig match scores + p7_oprofile_Convert's MSV part for an
+    HMM's match emissions ``mat`` ((M+1) x 20, row 0 unused); lambda as p7_Lambda."""
+    lib = load_library()
+    mat = np.ascontiguousarray(mat, dtype=np.float32)
+    M = mat.shape[0] - 1
+    rbv = np.zeros((KP, M), np.uint8)
+    sc = np.zeros(5, np.int32)
+    scale = C.c_float()
+    rc = lib.msvh_profile_convert(M, _p(mat), config_L, _p(rbv), _p(sc), C.byref(scale), None)
+    if rc != 0:
+        raise MSVError(rc, "profile conversion failed")
+    lam = float(np.float32(lib.msvh_lambda(M, _p(mat))))
+    return ByteProfile(rbv, sc[0], sc[1], sc[3], sc[4], scale.value, lam=lam,
+                       hmm=(mat, ins, t), tjb_config=int(sc[2]))
+
+
+def match_scores(mat, config_L=100):
+    """Float match scores p7P_MSC (29 x M) as p7_ProfileConfig sets them."""
+    lib = load_library()
+    mat = np.ascontiguousarray(mat, dtype=np.float32)
+    M = mat.shape[0] - 1
+    rbv = np.zeros((KP, M), np.uint8)
+    msc = np.zeros((KP, M), np.float32)
+    sc = np.zeros(5, np.int32)
+    scale = C.c_float()
+    lib.msvh_profile_convert(M, _p(mat), config_L, _p(rbv), _p(sc), C.byref(scale), _p(msc))
+    return msc
+
+
+def sample_hmm(seed, M):
+    """p7_hmm_Sample with esl_randomness_Create(seed): (mat, ins, t)."""
+    lib = load_library()
+    r = Rng(seed)
+    mat = np.zeros((M + 1, 20), np.float32)
+    ins = np.zeros((M + 1, 20), np.float32)
+    t = np.zeros((M + 1, 7), np.float32)
+    rc = lib.msvh_hmm_sample(C.byref(r._r), M, _p(mat), _p(ins), _p(t))
+    if rc != 0:
+        raise MSVError(rc, "hmm sampling failed")
+    return mat, ins, t
+
+
+def sample_profile(seed, M, config_L=100):
+    """A sampled amino HMM turned into a byte profile (uncalibrated mu)."""
+    mat, ins, t = sample_hmm(seed, M)
+    return profile_from_hmm(mat, ins, t, config_L)
+
+
+def core_emit(hmm, seed, n, rng=None):
+    """n p7_CoreEmit samples (list of uint8 arrays) from esl_randomness_Create(seed)."""
+    lib = load_library()
+    mat, ins, t = (np.ascontiguousarray(a, dtype=np.float32) for a in hmm)
+    M = mat.shape[0] - 1
+    r = rng or Rng(seed)
+    out = []
+    cap = 64 * M + 4096
+    buf = np.zeros(cap, np.uint8)
+    saved = _Rng()
+    for _ in range(n):
+        C.memmove(C.byref(saved), C.byref(r._r), C.sizeof(_Rng))
+        while True:
+            L = lib.msvh_core_emit(C.byref(r._r), M, _p(mat), _p(ins), _p(t), _p(buf), cap)
+            if L >= 0:
+                break
+            if L != -1:
+                raise MSVError(L, "core emission reached an impossible state")
+            # buffer too small: restore the generator and retry with room to spare
+            C.memmove(C.byref(r._r), C.byref(saved), C.sizeof(_Rng))
+            cap *= 4
+            buf = np.zeros(cap, np.uint8)
+        out.append(buf[:L].copy())
+    return out
+
+
+def iid(seed, n, L):
+    """n i.i.d. background targets of length L from esl_randomness_Create(seed)."""
+    lib = load_library()
+    r = Rng(seed)
+    out = np.zeros((n, L), np.uint8)
+    for i in range(n):
+        lib.msvh_iid(C.byref(r._r), L, _p(out[i]))
+    return out
+
+
+def tjb(L):
+    return load_library().msvh_tjb(int(L))
+
+
+def nullsc(L):
+    return load_library().msvh_nullsc(int(L))
+
+
+def calibration_targets(N=200, L=200):
+    out = np.zeros(N * L, np.uint8)
+    load_library().msvh_calibration_targets(N, L, _p(out))
+    return out.reshape(N, L)
+
+
+class Database:
+    """Concatenated digital targets: ``res`` (uint8), ``off`` (int64), ``lens`` (int32)."""
+
+    def __init__(self, res, off, lens):
+        self.res = np.ascontiguousarray(res, dtype=np.uint8)
+        self.off = np.ascontiguousarray(off, dtype=np.int64)
+        self.lens = np.ascontiguousarray(lens, dtype=np.int32)
+
+    @property
+    def n(self):
+        return int(self.lens.size)
+
+    @classmethod
+    def from_list(cls, seqs):
+        lens = np.array([len(s) for s in seqs], np.int32)
+        off = np.zeros(len(seqs), np.int64)
+        if len(seqs):
+            off[1:] = np.cumsum(lens[:-1])
+        res = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if len(seqs) else np.zeros(0, np.uint8)
+        return cls(res, off, lens)
+
+    def seq(self, i):
+        return self.res[self.off[i]:self.off[i] + self.lens[i]]
+
+    @property
+    def residues(self):
+        return int(self.lens.sum(dtype=np.int64))
+
+
+def gen_database(seed, n, median, sigma, lo, hi, tail_frac=0.0, tail_xm=1000.0, tail_alpha=1.5, tail_max=35000,
+                 nthreads=8):
+    """Synthetic i.i.d. background database (see msvh_gen_lengths in msv_prep.h)."""
+    lib = load_library()
+    lens = np.zeros(n, np.int32)
+    total = lib.msvh_gen_lengths(seed, n, median, sigma, lo, hi, tail_frac, tail_xm, tail_alpha, tail_max,
+                                 _p(lens))
+    off = np.zeros(n, np.int64)
+    if n:
+        off[1:] = np.cumsum(lens[:-1], dtype=np.int64)
+    res = np.zeros(max(int(total), 1), np.uint8)
+    lib.msvh_gen_residues(seed, n, _p(lens), _p(off), _p(res), nthreads)
+    return Database(res[:total], off, lens)
+
+
+def plant(db, seqs, seed, frac):
+    """Replace a fraction of targets (chosen by numpy seed) with ``seqs``;
+    returns a new Database and the replaced indices."""
+    rs = np.random.default_rng(seed)
+    k = min(len(seqs), int(round(frac * db.n)))
+    idx = np.sort(rs.choice(db.n, size=k, replace=False))
+    seqs_all = None
+    lens = db.lens.copy()
+    lens[idx] = [len(s) for s in seqs[:k]]
+    off = np.zeros(db.n, np.int64)
+    off[1:] = np.cumsum(lens[:-1], dtype=np.int64)
+    res = np.empty(int(lens.sum(dtype=np.int64)), np.uint8)
+    mask = np.ones(db.n, bool)
+    mask[idx] = False
+    # copy unchanged targets in runs
+    keep = np.flatnonzero(mask)
+    for i in keep:
+        res[off[i]:off[i] + lens[i]] = db.res[db.off[i]:db.off[i] + db.lens[i]]
+    for j, i in enumerate(idx):
+        res[off[i]:off[i] + lens[i]] = seqs[j]
+    del seqs_all
+    return Database(res, off, lens), idx
+
+
+# ---------------------------------------------------------------------------
+# GPU context
+# ---------------------------------------------------------------------------
+class MSVContext:
+    """One GPU, one context (not shared between host threads)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self._c = C.c_void_p()
+        rc = self.lib.msvg_init(device, C.byref(self._c))
+        if rc != 0:
+            raise MSVError(rc, f"msvg_init failed on device {device} (is a GPU visible?)")
+        self._keep = {}
+
+    def close(self):
+        if self._c:
+            self.lib.msvg_free(self._c)
+            self._c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise MSVError(rc, self.lib.msvg_last_error(self._c).decode())
+
+    def add_profile(self, prof):
+        pid = C.c_int32()
+        s = prof._struct()
+        self._chk(self.lib.msvg_profile_add(self._c, C.byref(s), C.byref(pid)))
+        return pid.value
+
+    def remove_profile(self, pid):
+        self._chk(self.lib.msvg_profile_remove(self._c, pid))
+
+    def set_evparams(self, pid, mu, lam):
+        self._chk(self.lib.msvg_profile_set_evparams(self._c, pid, float(mu), float(lam)))
+
+    def calibrate(self, pid, lam):
+        """GPU p7_MSVMu; returns mu (float32) and stores (mu, lam) on the profile."""
+        mu = C.c_float()
+        self._chk(self.lib.msvg_calibrate_mu(self._c, pid, float(lam), C.byref(mu)))
+        self.set_evparams(pid, mu.value, lam)
+        return mu.value
+
+    def add_database(self, db):
+        did = C.c_int32()
+        self._chk(self.lib.msvg_db_add(self._c, _p(db.res) if db.res.size else None, _p(db.off), _p(db.lens),
+                                       db.n, C.byref(did)))
+        return did.value
+
+    def remove_database(self, did):
+        self._chk(self.lib.msvg_db_remove(self._c, did))
+
+    def search(self, pids, did, F1=0.02, mode=MODE_BATCHED):
+        """Scores of every profile against every target: (usc, status, pass),
+        each shaped [len(pids), n] in the caller's target order."""
+        ids = np.ascontiguousarray(pids, dtype=np.int32)
+        n = self.lib.msvg_db_size(self._c, did)
+        usc = np.zeros((ids.size, n), np.float32)
+        st = np.zeros((ids.size, n), np.uint8)
+        ps = np.zeros((ids.size, n), np.uint8)
+        self._chk(self.lib.msvg_search(self._c, _p(ids), ids.size, did, F1, mode, _p(usc), _p(st), _p(ps)))
+        return usc, st, ps
+
+    def search_device(self, pids, did, F1=0.02, mode=MODE_BATCHED):
+        ids = np.ascontiguousarray(pids, dtype=np.int32)
+        self._chk(self.lib.msvg_search_device(self._c, _p(ids), ids.size, did, F1, mode))
+
+    def stats(self):
+        s = _Stats()
+        self._chk(self.lib.msvg_last_stats(self._c, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+    def msv_filter(self, pid, x):
+        """p7_MSVFilter on one target (codes, no sentinels): (status, usc)."""
+        dsq = np.concatenate([[255], np.asarray(x, np.uint8), [255]]).astype(np.uint8)
+        sc = C.c_float()
+        rc = self.lib.msvg_MSVFilter(self._c, pid, _p(dsq), len(x), C.byref(sc))
+        if rc not in (eslOK, eslERANGE):
+            self._chk(rc)
+        return rc, sc.value
+
+    def set_tile(self, m_lo, m_hi, lanes, waves=0):
+        self._chk(self.lib.msvg_set_tile(self._c, m_lo, m_hi, lanes, waves))
+
+    def clear_tiles(self):
+        self._chk(self.lib.msvg_set_tile(self._c, 0, 0, 0, 0))

@@ -1,0 +1,109 @@
+// msv_device.h -- data layouts shared by the HIP kernels and the host runtime.
+//
+// Residues in HBM: one byte per residue, each target padded to a multiple of
+// 16 bytes with the null row index, targets stored in descending-length order.
+// Residue bytes are remapped at upload time to dense row indices 0..R-1 (the
+// codes that occur in the database); row R is the null row (emission -inf).
+//
+// Fast-stage emission table (one per profile x database x lane group size G):
+//   uint4 T[(R + 1) * QB * 64], QB = ceil(Q / 4)
+//   entry (x, jb, lane) holds 4 dwords = registers j = 4*jb .. 4*jb+3;
+//   each dword packs two f16 emissions for the two model slots of that lane:
+//     slot z = 2*(lane % G) (low half) and 2*(lane % G) + 1 (high half),
+//     model position k = j + z * Q + 1 (striped, as the reference stripes over
+//     16 bytes, impl_sse/p7_oprofile.c:625-630, here over 2G slots).
+//   value = (bias_b - rbv[x][k]) * 2^-11, or -1.0 for padding k > M and for the
+//   null row.  The f16 domain stores W - B0 scaled by 2^-11, so the begin score
+//   is 0.0 and the clamp bit of v_pk_add_f16 is the floor at the begin score.
+//
+// Slow-stage cost table (per profile x database): same striping with G = 64,
+// Qs = ceil(M / 128), uint32 C[(R + 1) * Qs * 64], dword = (cost_lo, cost_hi)
+// as two u16 holding rbv bytes (255 for padding and for the null row).
+#pragma once
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+#define MSVG_NULLCOST 255
+
+// Per-pair result word written by the fast and slow stages:
+//   bits 0..7 xJ, bits 8..9 code.
+enum { RES_OK = 0, RES_ERANGE = 1, RES_SLOW = 2 };
+
+struct SlowItem {          // one pair that needs the exact byte MSV
+  int32_t prof;            // index into the search's profile array
+  int32_t seq;             // sorted target index
+};
+
+struct ProfDev {           // per-profile constants seen by the kernels
+  const uint4 *ftab;       // fast table (global), may be null if not built
+  const uint32_t *stab;    // slow table (global)
+  int32_t M, Q, QB, Qs;
+  int32_t tbm, tec, base, bias;
+  int32_t slow_all;        // 1: route every pair to the exact byte MSV
+  uint16_t *res;           // result words for this profile, indexed by sorted target
+};
+
+struct DbDev {
+  const uint8_t *res;      // remapped, padded residues
+  const int64_t *off;      // sorted target -> byte offset
+  const int32_t *len;      // sorted target -> length
+  const uint8_t *tjb;      // sorted target -> tjb_b(L)
+  int64_t n;
+  int32_t R;               // residue rows in use; R is the null row
+};
+
+// Kernel argument blocks (passed by value).
+struct FastArgs {
+  ProfDev prof;
+  DbDev db;
+  const int32_t *chunk_start;  // dynamic mode: task ranges [chunk_start[c], chunk_start[c+1])
+  int32_t nchunks;
+  int32_t *counter;            // dynamic mode: next chunk (zeroed before the launch)
+  const int32_t *order;        // static mode: task t -> sorted target index
+  int32_t ntasks;              // static mode: number of tasks
+  int32_t mode;                // 0 dynamic chunks of sorted targets, 1 static round-robin
+  int32_t prof_idx;
+  SlowItem *slow;
+  int32_t *slow_count;
+  int32_t slow_cap;
+};
+
+struct SlowArgs {
+  const ProfDev *profs;
+  DbDev db;
+  const SlowItem *items;
+  const int32_t *count;
+  int32_t cap;
+};
+
+struct FinArgs {
+  const ProfDev *profs;
+  DbDev db;
+  const int64_t *sorted_pos;   // caller target t -> sorted index
+  const int32_t *lidx;         // sorted index -> distinct-length index
+  const float *usc_tab;        // [np][511]: usc for xJ - tjb + 255
+  const uint16_t *thr;         // [np][nL]: smallest passing xJ (256 = none)
+  int32_t nL;
+  float *usc;                  // [np][n], may be null
+  uint8_t *status;             // [np][n], may be null
+  uint8_t *pass;               // [np][n], may be null
+};
+
+// Column-segmented fast pass for long models: one launch per model segment
+// of 128*Q positions; the value leaving the segment's last position in row i
+// is stored in carry[off(s) + i] (f16 bits) and enters the next segment's
+// first position in row i+1.  partial[s] keeps the running maximum (units).
+struct SegArgs {
+  ProfDev prof;            // Q/QB are the segment's
+  DbDev db;
+  const uint4 *tab;        // segment table, fast-table layout with G = 64
+  uint16_t *carry;         // per target, indexed like the residues
+  uint16_t *partial;       // per sorted target
+  const int64_t *carry_base;
+  int32_t first, last;
+  int32_t ntasks;
+  int32_t prof_idx;
+  SlowItem *slow;
+  int32_t *slow_count;
+  int32_t slow_cap;
+};

@@ -1,0 +1,330 @@
+// msv_host.cpp -- host-side profile preparation and workload generation
+// (declared in include/msv_prep.h).  Every routine follows the float/double
+// evaluation order of the HMMER/Easel function it mirrors, because the byte
+// profile, tjb_b and the pass rule are defined by those roundings.  Build
+// with -ffp-contract=off and without fast-math.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/msv_prep.h"
+#include "msv_host.h"
+
+namespace {
+
+constexpr double kLog2 = 0.69314718055994529;   // eslCONST_LOG2, easel/easel.h:258
+
+uint32_t jenkins_mix3(uint32_t a, uint32_t b, uint32_t c) {   // esl_random.c:383-398
+  a -= b; a -= c; a ^= (c >> 13);
+  b -= c; b -= a; b ^= (a << 8);
+  c -= a; c -= b; c ^= (b >> 13);
+  a -= b; a -= c; a ^= (c >> 12);
+  b -= c; b -= a; b ^= (a << 16);
+  c -= a; c -= b; c ^= (b >> 5);
+  a -= b; a -= c; a ^= (c >> 3);
+  b -= c; b -= a; b ^= (a << 10);
+  c -= a; c -= b; c ^= (b >> 15);
+  return c;
+}
+
+void mt_fill(msvh_rng *r) {                      // esl_random.c:332-354 (MT19937 refill)
+  static const uint32_t mag01[2] = {0x0u, 0x9908b0dfu};
+  uint32_t y;
+  int z;
+  for (z = 0; z < 227; z++) {
+    y = (r->mt[z] & 0x80000000u) | (r->mt[z + 1] & 0x7fffffffu);
+    r->mt[z] = r->mt[z + 397] ^ (y >> 1) ^ mag01[y & 1u];
+  }
+  for (; z < 623; z++) {
+    y = (r->mt[z] & 0x80000000u) | (r->mt[z + 1] & 0x7fffffffu);
+    r->mt[z] = r->mt[z - 227] ^ (y >> 1) ^ mag01[y & 1u];
+  }
+  y = (r->mt[623] & 0x80000000u) | (r->mt[0] & 0x7fffffffu);
+  r->mt[623] = r->mt[396] ^ (y >> 1) ^ mag01[y & 1u];
+  r->mti = 0;
+}
+
+uint32_t rng_u32(msvh_rng *r) {
+  if (r->fast) {                                 // knuth(), esl_random.c:278-284
+    r->x = r->x * 69069u + 1u;
+    return r->x;
+  }
+  if (r->mti >= 624) mt_fill(r);                 // mersenne_twister(), :297-308
+  uint32_t x = r->mt[r->mti++];
+  x ^= (x >> 11);
+  x ^= (x << 7) & 0x9d2c5680u;
+  x ^= (x << 15) & 0xefc60000u;
+  x ^= (x >> 18);
+  return x;
+}
+
+double uniform_positive(msvh_rng *r) {           // esl_rnd_UniformPositive, :449-454
+  double x;
+  do { x = msvh_random(r); } while (x == 0.0);
+  return x;
+}
+
+// esl_dirichlet_FSampleUniform (esl_dirichlet.c:964-971): Gamma(1) deviates
+// (esl_rnd_Gamma -> gamma_integer with a = 1, esl_random.c:607-617, :657-671)
+// normalized by esl_vec_FNorm (esl_vectorops.c:842-850).
+void dirichlet_uniform(msvh_rng *r, int K, float *p) {
+  for (int x = 0; x < K; x++) {
+    double U = 1.;
+    U *= uniform_positive(r);
+    p[x] = (float) (-log(U));
+  }
+  float sum = 0.;
+  for (int x = 0; x < K; x++) sum += p[x];
+  if (sum != 0.0) for (int x = 0; x < K; x++) p[x] /= sum;
+  else            for (int x = 0; x < K; x++) p[x] = 1. / (float) K;
+}
+
+const float kAmino[20] = {                       // p7_AminoFrequencies, hmmer.c:163-186
+  0.0787945f, 0.0151600f, 0.0535222f, 0.0668298f, 0.0397062f, 0.0695071f, 0.0229198f,
+  0.0590092f, 0.0594422f, 0.0963728f, 0.0237718f, 0.0414386f, 0.0482904f, 0.0395639f,
+  0.0540978f, 0.0683364f, 0.0540687f, 0.0673417f, 0.0114135f, 0.0304133f};
+
+// unbiased_byteify (impl_sse/p7_oprofile.c:512-519).
+uint8_t unbiased_byteify(float scale_b, float sc) {
+  sc = -1.0f * roundf(scale_b * sc);
+  return (sc > 255.) ? 255 : (uint8_t) sc;
+}
+
+// biased_byteify (impl_sse/p7_oprofile.c:495-503).  The reference's
+// (uint8_t) cast of a negative float is carried out on x86-64 as a
+// conversion to a 32-bit integer whose low byte is kept; the same is written
+// out here so the result does not depend on this compiler's choice.
+uint8_t biased_byteify(float scale_b, int bias_b, float sc) {
+  sc = -1.0f * roundf(scale_b * sc);
+  if (sc > 255 - bias_b) return 255;
+  const uint8_t low = (uint8_t) (uint32_t) (int32_t) sc;
+  return (uint8_t) (low + bias_b);
+}
+
+}  // namespace
+
+extern "C" {
+
+void msvh_rng_init(msvh_rng *r, uint32_t seed, int fast) {   // esl_randomness_Init, :197-214
+  std::memset(r, 0, sizeof(*r));
+  r->fast = fast ? 1 : 0;
+  if (seed == 0) seed = 42;
+  r->seed = seed;
+  if (!r->fast) {
+    r->mt[0] = seed;                                           // mersenne_seed_table, :315-325
+    for (int z = 1; z < 624; z++) r->mt[z] = 69069u * r->mt[z - 1];
+    mt_fill(r);
+  } else {
+    r->x = jenkins_mix3(seed, 87654321u, 12345678u);
+    if (r->x == 0) r->x = 42;
+  }
+}
+
+double msvh_random(msvh_rng *r) { return ((double) rng_u32(r) / 4294967296.0); }   // esl_random, :271-276
+
+int msvh_fchoose(msvh_rng *r, const float *p, int N) {     // esl_rnd_FChoose, :718-738
+  double norm = 0.0, sum = 0.0;
+  const double roll = msvh_random(r);
+  for (int i = 0; i < N; i++) norm += p[i];
+  for (int i = 0; i < N; i++) {
+    sum += p[i];
+    if (roll < (sum / norm)) return i;
+  }
+  return N - 1;   // unreachable for a normalized p (the reference aborts here)
+}
+
+void msvh_background(float *f) { std::memcpy(f, kAmino, sizeof(kAmino)); }
+
+int msvh_hmm_sample(msvh_rng *r, int M, float *mat, float *ins, float *t) {   // p7_hmm_Sample
+  if (M < 1) return 11;
+  std::memset(mat, 0, sizeof(float) * (M + 1) * 20);
+  std::memset(ins, 0, sizeof(float) * (M + 1) * 20);
+  std::memset(t, 0, sizeof(float) * (M + 1) * 7);
+  t[5] = 1.0;                           // D_0 does not exist: t[0][DM] = 1 (p7_hmm.c:163)
+  for (int k = 0; k <= M; k++) {
+    if (k > 0) dirichlet_uniform(r, 20, mat + k * 20);
+    dirichlet_uniform(r, 20, ins + k * 20);
+    dirichlet_uniform(r, 3, t + k * 7);
+    dirichlet_uniform(r, 2, t + k * 7 + 3);
+    if (k > 0) dirichlet_uniform(r, 2, t + k * 7 + 5);
+  }
+  dirichlet_uniform(r, 2, t + M * 7);   // node M: TMM, TMI only
+  t[M * 7 + 2] = 0.;                    // MD
+  t[M * 7 + 5] = 1.0;                   // DM
+  t[M * 7 + 6] = 0.0;                   // DD
+  return 0;
+}
+
+int msvh_profile_convert(int M, const float *mat, int config_L, uint8_t *rbv, int *scalars, float *scale,
+                         float *msc_out) {
+  if (M < 1) return 11;
+  const float *f = kAmino;
+  // Match emission scores, modelconfig.c:138-150 with esl_abc_FExpectScVec
+  // (esl_alphabet.c:1399-1413, :1507-1513) for B, J, Z, O, U, X.
+  static const int degen[6][20] = {
+      /* B = N,D */ {0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0},
+      /* J = I,L */ {0, 0, 0, 0, 0, 0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+      /* Z = Q,E */ {0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0},
+      /* O = K   */ {0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+      /* U = C   */ {0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+      /* X = all */ {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}};
+  std::vector<float> msc((size_t) MSVH_KP * M);
+  float sc[MSVH_KP];
+  for (int k = 1; k <= M; k++) {
+    for (int x = 0; x < 20; x++) sc[x] = log(mat[k * 20 + x] / f[x]);
+    sc[20] = -INFINITY;
+    sc[27] = -INFINITY;
+    sc[28] = -INFINITY;
+    for (int d = 0; d < 6; d++) {
+      float result = 0., denom = 0.;
+      for (int i = 0; i < 20; i++)
+        if (degen[d][i]) { result += sc[i] * f[i]; denom += f[i]; }
+      result /= denom;
+      sc[21 + d] = result;
+    }
+    for (int x = 0; x < MSVH_KP; x++) msc[(size_t) x * M + (k - 1)] = sc[x];
+  }
+  // mf_conversion, p7_oprofile.c:602-640.  The maximum runs over the real
+  // residues' match and insert scores; inserts are 0 for k < M.
+  float max = 0.0;
+  for (int x = 0; x < 20; x++)
+    for (int k = 1; k <= M; k++) max = std::max(max, msc[(size_t) x * M + (k - 1)]);
+  const float scale_b = 3.0 / kLog2;
+  const int base_b = 190;
+  const int bias_b = unbiased_byteify(scale_b, -1.0 * max);
+  for (int x = 0; x < MSVH_KP; x++)
+    for (int k = 1; k <= M; k++)
+      rbv[(size_t) x * M + (k - 1)] = biased_byteify(scale_b, bias_b, msc[(size_t) x * M + (k - 1)]);
+  scalars[0] = unbiased_byteify(scale_b, logf(2.0f / ((float) M * (float) (M + 1))));   // tbm_b
+  scalars[1] = unbiased_byteify(scale_b, logf(0.5f));                                    // tec_b
+  scalars[2] = unbiased_byteify(scale_b, logf(3.0f / (float) (config_L + 3)));           // tjb_b
+  scalars[3] = base_b;
+  scalars[4] = bias_b;
+  *scale = scale_b;
+  if (msc_out) std::memcpy(msc_out, msc.data(), sizeof(float) * msc.size());
+  return 0;
+}
+
+double msvh_lambda(int M, const float *mat) {
+  // p7_Lambda (evalues.c:186-192) with p7_MeanMatchRelativeEntropy
+  // (modelstats.c:81-98) and esl_vec_FRelEntropy (esl_vectorops.c:950-962).
+  double KL = 0.;
+  for (int k = 1; k <= M; k++) {
+    const float *p = mat + k * 20;
+    float kl = 0.;
+    bool inf = false;
+    for (int i = 0; i < 20; i++)
+      if (p[i] > 0.) {
+        if (kAmino[i] == 0.) { inf = true; break; }
+        kl += p[i] * log(p[i] / kAmino[i]);
+      }
+    const float rel = inf ? INFINITY : (float) (1.44269504 * kl);
+    KL += rel;
+  }
+  KL /= (double) M;
+  return kLog2 + 1.44 / ((double) M * KL);
+}
+
+int msvh_core_emit(msvh_rng *r, int M, const float *mat, const float *ins, const float *t, uint8_t *out, int cap) {
+  // p7_CoreEmit, emit.c:71-148.  States B=0, M=1, I=2, D=3, E=4.
+  enum { B, MS, IS, DS, ES };
+  int k = 0, n = 0, st = B;
+  while (st != ES) {
+    switch (st) {
+      case B:
+      case MS: {
+        const int c = msvh_fchoose(r, t + k * 7, 3);
+        st = (c == 0) ? MS : (c == 1) ? IS : DS;
+        break;
+      }
+      case IS: st = (msvh_fchoose(r, t + k * 7 + 3, 2) == 0) ? MS : IS; break;
+      case DS: st = (msvh_fchoose(r, t + k * 7 + 5, 2) == 0) ? MS : DS; break;
+    }
+    if (st == MS || st == DS) k++;
+    if (k == M + 1) {
+      if (st == MS) { st = ES; k = 0; }
+      else return -2;   // the reference raises eslECORRUPT here
+    }
+    int x = -1;
+    if (st == MS) x = msvh_fchoose(r, mat + k * 20, 20);
+    else if (st == IS) x = msvh_fchoose(r, ins + k * 20, 20);
+    if (x >= 0) {
+      if (n >= cap) return -1;
+      out[n++] = (uint8_t) x;
+    }
+  }
+  return n;
+}
+
+void msvh_iid(msvh_rng *r, int L, uint8_t *out) {          // esl_rsq_xfIID
+  for (int i = 0; i < L; i++) out[i] = (uint8_t) msvh_fchoose(r, kAmino, 20);
+}
+
+int msvh_tjb(int L) { return msvh_tjb_impl(L); }
+float msvh_nullsc(int L) { return msvh_nullsc_impl(L); }
+
+double msvh_fit_mu(const float *usc, const int32_t *status, int N, int L, int base_b, float scale_b, double lambda) {
+  const float maxsc = (255 - base_b) / scale_b;             // evalues.c:248
+  const float nullsc = msvh_nullsc_impl(L);
+  double esum = 0.;
+  for (int i = 0; i < N; i++) {
+    const float sc = (status[i] == 16) ? maxsc : usc[i];
+    const double xv = (sc - nullsc) / kLog2;                // evalues.c:270
+    esum += exp(-lambda * xv);                              // esl_gumbel.c:498-500
+  }
+  return -log(esum / N) / lambda;
+}
+
+void msvh_calibration_targets(int N, int L, uint8_t *out) {
+  msvh_rng r;
+  msvh_rng_init(&r, 42, 1);
+  for (int i = 0; i < N; i++) msvh_iid(&r, L, out + (size_t) i * L);
+}
+
+int64_t msvh_gen_lengths(uint32_t seed, int64_t n, double median, double sigma, int lo, int hi, double tail_frac,
+                         double tail_xm, double tail_alpha, int tail_max, int32_t *lens) {
+  msvh_rng r;
+  msvh_rng_init(&r, seed, 0);
+  int64_t total = 0;
+  const double two_pi = 6.283185307179586;
+  for (int64_t i = 0; i < n; i++) {
+    int L;
+    if (tail_frac > 0.0 && msvh_random(&r) < tail_frac) {
+      const double u = uniform_positive(&r);
+      double x = tail_xm * pow(u, -1.0 / tail_alpha);
+      if (x > tail_max) x = tail_max;
+      L = (int) x;
+    } else {
+      const double u1 = uniform_positive(&r), u2 = msvh_random(&r);
+      const double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+      double x = median * exp(sigma * z);
+      L = (int) std::lround(x);
+      if (L < lo) L = lo;
+      if (L > hi) L = hi;
+    }
+    lens[i] = L;
+    total += L;
+  }
+  return total;
+}
+
+void msvh_gen_residues(uint32_t seed, int64_t n, const int32_t *lens, const int64_t *off, uint8_t *res, int nthreads) {
+  const int64_t nblocks = (n + 4095) / 4096;
+  if (nthreads < 1) nthreads = 1;
+  auto work = [&](int tid) {
+    for (int64_t b = tid; b < nblocks; b += nthreads) {
+      msvh_rng r;
+      msvh_rng_init(&r, (uint32_t) (seed * 1000003u + (uint32_t) b + 1u), 0);
+      const int64_t i1 = std::min<int64_t>(n, (b + 1) * 4096);
+      for (int64_t i = b * 4096; i < i1; i++) msvh_iid(&r, lens[i], res + off[i]);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nthreads; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
+}
+
+}  // extern "C"

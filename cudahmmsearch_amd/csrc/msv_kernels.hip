@@ -1,0 +1,594 @@
+// msv_kernels.hip -- gfx950 kernels for HMMER3's MSV filter.
+//
+// Three stages per (profile, target) pair, together bit-exact to the
+// reference composite p7_MSVFilter (impl_sse/msvfilter.c:78-213):
+//
+//  1. msv_fast    J-free max-plus recurrence over all cells with the begin
+//                 score B0 as a floor -- the quantity the reference's SSV path
+//                 computes (impl_sse/ssvfilter.c:1364-1413) -- in packed f16
+//                 (2 cells per VGPR, clamp bit = floor), followed by SSV's
+//                 decision rules.  Pairs whose J state could fire are queued.
+//  2. msv_slow    the byte-saturating MSV row loop with E/J/B updates and the
+//                 255-overflow test, literally as impl_sse/msvfilter.c:112-210,
+//                 in packed u16 lanes, for the queued pairs.
+//  3. msv_finalize  usc and the F1 pass bit from host-built tables.
+//
+// Wavefront layout: a lane group of G lanes (16, 32 or 64) owns one target;
+// its 2G f16 slots hold model positions striped as k = j + z*Q + 1 (slot z,
+// register j).  The diagonal dependency of the last register moves one slot
+// per row through a DPP lane shift (wave_shr:1 / row_shr:1) plus v_alignbit.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "msv_device.h"
+
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+namespace {
+
+__device__ __forceinline__ h2v as_h2(uint32_t v) { return __builtin_bit_cast(h2v, v); }
+__device__ __forceinline__ uint32_t h2_u32(h2v v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ u16x2 as_u2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t u2_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// One DP cell pair: clamp(W' + e) to [0, 1] -> v_pk_add_f16 ... clamp.
+__device__ __forceinline__ uint32_t cell(uint32_t w, uint32_t e) {
+  const h2v zero = {(_Float16)0.0f, (_Float16)0.0f};
+  const h2v one = {(_Float16)1.0f, (_Float16)1.0f};
+  h2v s = as_h2(w) + as_h2(e);
+  s = __builtin_elementwise_min(__builtin_elementwise_max(s, zero), one);
+  return h2_u32(s);
+}
+// Running maximum over cells -> v_pk_maximum3_f16.
+__device__ __forceinline__ uint32_t max3h(uint32_t a, uint32_t b, uint32_t c) {
+  return h2_u32(__builtin_elementwise_maximum(__builtin_elementwise_maximum(as_h2(a), as_h2(b)), as_h2(c)));
+}
+__device__ __forceinline__ uint32_t max2h(uint32_t a, uint32_t b) {
+  return h2_u32(__builtin_elementwise_maximum(as_h2(a), as_h2(b)));
+}
+
+// Value entering slot 0 of register 0 from slot 2G-1 of the last register,
+// per lane group: slot s <- slot s-1, group-first slot <- 0 (the floor).
+template <int G>
+__device__ __forceinline__ uint32_t shift_slot(uint32_t last, int lane) {
+  uint32_t prev;
+  if constexpr (G == 64) {
+    prev = __builtin_amdgcn_update_dpp(0u, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
+  } else if constexpr (G == 16) {
+    prev = __builtin_amdgcn_update_dpp(0u, last, 0x111 /* row_shr:1 */, 0xf, 0xf, true);
+  } else {
+    prev = __builtin_amdgcn_update_dpp(0u, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
+    prev = (lane == 32) ? 0u : prev;
+  }
+  return __builtin_amdgcn_alignbit(last, prev, 16);
+}
+
+__device__ __forceinline__ uint32_t byte_of(const uint4 &c, int rr) {
+  const uint32_t w = (rr < 4) ? c.x : (rr < 8) ? c.y : (rr < 12) ? c.z : c.w;
+  return (w >> (8 * (rr & 3))) & 0xffu;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Stage 1: fast J-free pass.
+// ---------------------------------------------------------------------------
+
+
+// The decision the reference reaches from the J-free maximum, returning the
+// result word.  Rp is max(W - B0) over all cells, in units (0..2048).
+//   SSV regime (tjb+tbm+tec+bias < 127), ssvfilter.c:1370-1410:
+//     overflow -> eslERANGE; J possible (xE - tec > base) -> byte MSV;
+//     else xJ = max(B0, cells) - tec.
+//   Guard regime: the byte MSV itself; while J never fires its B is the
+//   constant B0 of msvfilter.c:124, so its cells are the same J-free cells.
+__device__ __forceinline__ uint32_t decide(int Rp, int tjb, const ProfDev &pd) {
+  const int tjbm = (tjb + pd.tbm) & 255;             // msvfilter.c:120
+  const int B0 = max(0, pd.base - tjbm);             // msvfilter.c:124
+  const int R = Rp + B0;
+  const bool guard = (tjb + pd.tbm + pd.tec + pd.bias) >= 127;
+  if (pd.slow_all) return RES_SLOW << 8;
+  if (R >= 255 - pd.bias) return RES_ERANGE << 8;
+  if (R - pd.tec > pd.base) return RES_SLOW << 8;
+  if (guard) {
+    if (Rp == 0) return RES_SLOW << 8;                // exact sub-B0 maximum unknown
+    return (uint32_t) max(0, R - pd.tec);
+  }
+  return (uint32_t) (R - pd.tec);
+}
+
+template <int G, int Q>
+__device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, int64_t s, int lane) {
+  constexpr int QB = (Q + 3) / 4;
+  const bool valid = s < a.db.n;
+  const int L = valid ? a.db.len[s] : 0;
+  const int Lpad = (L + 15) & ~15;
+  const uint8_t *p = a.db.res + (valid ? a.db.off[s] : 0);
+
+  int Lmax = Lpad;
+  if constexpr (G < 64) {
+#pragma unroll
+    for (int o = 32; o >= G; o >>= 1) Lmax = max(Lmax, __shfl_xor(Lmax, o, 64));
+  }
+  Lmax = __builtin_amdgcn_readfirstlane(Lmax);
+
+  const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
+  const uint4 null4 = make_uint4(nb, nb, nb, nb);
+  const uint4 *tl = lds + lane;
+
+  uint32_t r[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) r[j] = 0u;
+  uint32_t acc = 0u;
+
+  for (int i0 = 0; i0 < Lmax; i0 += 16) {
+    uint4 c = *reinterpret_cast<const uint4 *>(p + (i0 < Lpad ? i0 : 0));
+    if (i0 >= Lpad) c = null4;
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+      const uint32_t x = byte_of(c, rr);
+      const uint4 *tp = tl + x * (QB * 64);
+      uint32_t e[QB * 4];
+#pragma unroll
+      for (int jb = 0; jb < QB; ++jb) {
+        const uint4 v = tp[jb * 64];
+        e[4 * jb + 0] = v.x; e[4 * jb + 1] = v.y; e[4 * jb + 2] = v.z; e[4 * jb + 3] = v.w;
+      }
+      const uint32_t sh = shift_slot<G>(r[Q - 1], lane);
+#pragma unroll
+      for (int j = Q - 1; j >= 1; --j) r[j] = cell(r[j - 1], e[j]);
+      r[0] = cell(sh, e[0]);
+      int j = 0;
+#pragma unroll
+      for (; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);
+      if constexpr ((Q & 1) != 0) acc = max2h(acc, r[Q - 1]);
+    }
+  }
+
+  // Group maximum of both f16 halves, then the reference's decision.
+  const h2v ah = as_h2(acc);
+  float m = fmaxf((float) ah.x, (float) ah.y);
+#pragma unroll
+  for (int o = G / 2; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if (valid && (lane % G) == 0) {
+    const int Rp = (int) (m * 2048.0f + 0.5f);
+    const uint32_t w = decide(Rp, a.db.tjb[s], a.prof);
+    a.prof.res[s] = (uint16_t) w;
+    if ((w >> 8) == RES_SLOW) {
+      const int k = atomicAdd(a.slow_count, 1);
+      if (k < a.slow_cap) a.slow[k] = SlowItem{a.prof_idx, (int32_t) s};
+    }
+  }
+}
+
+template <int G, int Q>
+__global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
+  extern __shared__ uint4 lds[];
+  constexpr int QB = (Q + 3) / 4;
+  constexpr int NG = 64 / G;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+
+  const int ntab = (a.db.R + 1) * QB * 64;
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = a.prof.ftab[i];
+  __syncthreads();
+
+  const int g = lane / G;
+  if (a.mode == 0) {
+    for (;;) {
+      int c = 0;
+      if (lane == 0) c = atomicAdd(a.counter, 1);
+      c = __builtin_amdgcn_readfirstlane(c);
+      if (c >= a.nchunks) break;
+      const int t1 = a.chunk_start[c + 1];
+      for (int t = a.chunk_start[c]; t < t1; ++t) fast_task<G, Q>(a, lds, (int64_t) t * NG + g, lane);
+    }
+  } else {
+    for (int t = blockIdx.x * nwaves + wave; t < a.ntasks; t += gridDim.x * nwaves)
+      fast_task<G, Q>(a, lds, a.order[t], lane);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 1b: the same J-free pass over one column segment of a long model.
+// One target per wave (G = 64), static round-robin over sorted targets.
+// ---------------------------------------------------------------------------
+template <int Q>
+__global__ void __launch_bounds__(1024) msv_seg(SegArgs a) {
+  extern __shared__ uint4 lds[];
+  constexpr int QB = (Q + 3) / 4;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int ntab = (a.db.R + 1) * QB * 64;
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = a.tab[i];
+  __syncthreads();
+  const uint4 *tl = lds + lane;
+
+  for (int64_t s = (int64_t) blockIdx.x * nwaves + wave; s < a.db.n; s += (int64_t) gridDim.x * nwaves) {
+    const int L = a.db.len[s];
+    const int Lpad = (L + 15) & ~15;
+    const int64_t base = a.db.off[s];
+    const uint8_t *p = a.db.res + base;
+    uint16_t *cy = a.carry + base;
+    uint32_t r[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) r[j] = 0u;
+    uint32_t acc = 0u;
+    uint32_t cin_last = 0u;          // carry entering row i0 (value of row i0-1)
+
+    for (int i0 = 0; i0 < Lpad; i0 += 16) {
+      const uint4 c = *reinterpret_cast<const uint4 *>(p + i0);
+      uint32_t cin = 0u;
+      if (!a.first && lane < 16) cin = cy[i0 + lane];
+      uint32_t cout = 0u;
+#pragma unroll
+      for (int rr = 0; rr < 16; ++rr) {
+        const uint32_t x = byte_of(c, rr);
+        const uint4 *tp = tl + x * (QB * 64);
+        uint32_t e[QB * 4];
+#pragma unroll
+        for (int jb = 0; jb < QB; ++jb) {
+          const uint4 v = tp[jb * 64];
+          e[4 * jb + 0] = v.x; e[4 * jb + 1] = v.y; e[4 * jb + 2] = v.z; e[4 * jb + 3] = v.w;
+        }
+        const uint32_t last = r[Q - 1];
+        uint32_t prev = __builtin_amdgcn_update_dpp(0u, last, 0x138, 0xf, 0xf, true);
+        if (!a.first) {
+          const uint32_t ci = (rr == 0) ? cin_last : (uint32_t) __builtin_amdgcn_readlane((int) cin, rr - 1);
+          prev = (lane == 0) ? (ci << 16) : prev;
+        }
+        const uint32_t sh = __builtin_amdgcn_alignbit(last, prev, 16);
+#pragma unroll
+        for (int j = Q - 1; j >= 1; --j) r[j] = cell(r[j - 1], e[j]);
+        r[0] = cell(sh, e[0]);
+        int j = 0;
+#pragma unroll
+        for (; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);
+        if constexpr ((Q & 1) != 0) acc = max2h(acc, r[Q - 1]);
+        if (!a.last) {
+          const uint32_t co = ((uint32_t) __builtin_amdgcn_readlane((int) r[Q - 1], 63)) >> 16;
+          cout = (lane == rr) ? co : cout;
+        }
+      }
+      if (!a.first) cin_last = (uint32_t) __builtin_amdgcn_readlane((int) cin, 15);
+      if (!a.last && lane < 16) cy[i0 + lane] = (uint16_t) cout;
+    }
+
+    const h2v ah = as_h2(acc);
+    float m = fmaxf((float) ah.x, (float) ah.y);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (lane == 0) {
+      int Rp = (int) (m * 2048.0f + 0.5f);
+      if (!a.first) Rp = max(Rp, (int) a.partial[s]);
+      if (!a.last) {
+        a.partial[s] = (uint16_t) Rp;
+      } else {
+        const uint32_t w = decide(Rp, a.db.tjb[s], a.prof);
+        a.prof.res[s] = (uint16_t) w;
+        if ((w >> 8) == RES_SLOW) {
+          const int k = atomicAdd(a.slow_count, 1);
+          if (k < a.slow_cap) a.slow[k] = SlowItem{a.prof_idx, (int32_t) s};
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 2: exact byte MSV (msvfilter.c:112-210) for queued pairs.
+// One wave per pair; the DP row lives in LDS, one dword column per lane.
+// ---------------------------------------------------------------------------
+
+
+#define SLOW_WAVES 4
+#define SLOW_QMAX 128
+
+__global__ void __launch_bounds__(64 * SLOW_WAVES) msv_slow(SlowArgs a) {
+  __shared__ uint32_t dp[SLOW_WAVES][SLOW_QMAX][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int n = min(*a.count, a.cap);
+  uint32_t *mydp = &dp[wave][0][0];
+
+  for (int it = blockIdx.x * SLOW_WAVES + wave; it < n; it += gridDim.x * SLOW_WAVES) {
+    const SlowItem item = a.items[it];
+    const ProfDev pd = a.profs[item.prof];
+    const int64_t s = item.seq;
+    const int L = a.db.len[s];
+    const uint8_t *p = a.db.res + a.db.off[s];
+    const int Qs = pd.Qs;
+    const int tjb = a.db.tjb[s];
+    const int tjbm = (tjb + pd.tbm) & 255;                       // :120
+    const u16x2 biasv = {(unsigned short) pd.bias, (unsigned short) pd.bias};
+    const u16x2 c255 = {255, 255};
+
+    for (int j = 0; j < Qs; ++j) mydp[j * 64 + lane] = 0u;       // :114
+    int xJ = 0;                                                  // :115
+    int xB = max(0, pd.base - tjbm);                             // :124
+    int code = RES_OK;
+
+    for (int i0 = 0; i0 < L; i0 += 16) {
+      const uint4 c = *reinterpret_cast<const uint4 *>(p + i0);
+      const int nr = min(16, L - i0);
+      for (int rr = 0; rr < nr; ++rr) {
+        const uint32_t x = byte_of(c, rr);
+        const uint32_t *cp = pd.stab + (size_t) x * Qs * 64 + lane;
+        const u16x2 xBv = {(unsigned short) xB, (unsigned short) xB};
+        const uint32_t last = mydp[(Qs - 1) * 64 + lane];
+        const uint32_t prev = __builtin_amdgcn_update_dpp(0u, last, 0x138, 0xf, 0xf, true);
+        uint32_t mpv = __builtin_amdgcn_alignbit(last, prev, 16);  // :146, zero shifted in
+        u16x2 xE = {0, 0};
+        for (int j = 0; j < Qs; ++j) {
+          u16x2 sv = __builtin_elementwise_max(as_u2(mpv), xBv);             // :150
+          sv = __builtin_elementwise_min(sv + biasv, c255);                    // :151
+          sv = __builtin_elementwise_sub_sat(sv, as_u2(cp[j * 64]));           // :152
+          xE = __builtin_elementwise_max(xE, sv);                              // :153
+          mpv = mydp[j * 64 + lane];
+          mydp[j * 64 + lane] = u2_u32(sv);
+        }
+        int m = max((int) xE.x, (int) xE.y);
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+        m = __builtin_amdgcn_readfirstlane(m);
+        if (m + pd.bias >= 255) { code = RES_ERANGE; break; }                 // :160-185
+        const int e = max(0, m - pd.tec);                                      // :187
+        xJ = max(xJ, e);                                                       // :188
+        xB = max(0, max(pd.base, xJ) - tjbm);                                  // :190-191
+      }
+      if (code == RES_ERANGE) break;
+    }
+    if (lane == 0) pd.res[s] = (uint16_t) (code == RES_ERANGE ? (RES_ERANGE << 8) : xJ);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 3: usc, status and the F1 pass bit, in the caller's target order.
+// ---------------------------------------------------------------------------
+
+
+__global__ void __launch_bounds__(256) msv_finalize(FinArgs a) {
+  const int p = blockIdx.y;
+  const ProfDev pd = a.profs[p];
+  const int64_t n = a.db.n;
+  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
+    const int64_t pos = a.sorted_pos[t];
+    const uint32_t w = pd.res[pos];
+    const uint32_t code = w >> 8;
+    float usc;
+    uint8_t st, ps;
+    if (code == RES_ERANGE) {
+      usc = __builtin_inff(); st = 16; ps = 1;               // eslERANGE, P = 0
+    } else if (code == RES_OK) {
+      const int xJ = (int) (w & 255u);
+      usc = a.usc_tab[p * 511 + xJ - (int) a.db.tjb[pos] + 255];
+      st = 0;
+      ps = (xJ >= (int) a.thr[(int64_t) p * a.nL + a.lidx[pos]]) ? 1 : 0;
+    } else {
+      usc = __builtin_bit_cast(float, 0x7fc00000u); st = 255; ps = 0;  // unresolved: reported as an error
+    }
+    const int64_t o = (int64_t) p * n + t;
+    if (a.usc) a.usc[o] = usc;
+    if (a.status) a.status[o] = st;
+    if (a.pass) a.pass[o] = ps;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Table builders and small utilities.
+// ---------------------------------------------------------------------------
+// Fast table: uint4 T[(R+1) * QB * 64]; see msv_device.h.
+__global__ void build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R,
+                                 int G, int Q, int bias, uint32_t *out) {
+  const int QB = (Q + 3) / 4;
+  const int64_t total = (int64_t) (R + 1) * QB * 64 * 4;
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t) gridDim.x * blockDim.x) {
+    const int e = (int) (i & 3);
+    const int lane = (int) ((i >> 2) & 63);
+    const int64_t rest = i >> 8;
+    const int jb = (int) (rest % QB);
+    const int x = (int) (rest / QB);
+    const int j = 4 * jb + e;
+    const int t = lane % G;
+    uint32_t word = 0;
+    for (int h = 0; h < 2; ++h) {
+      const int z = 2 * t + h;
+      const int k = j + z * Q + 1;
+      _Float16 v = (_Float16) -1.0f;
+      if (x < R && j < Q && k <= M) {
+        const int cost = rbv[(int64_t) rowcode[x] * M + (k - 1)];
+        v = (_Float16) ((float) (bias - cost) * (1.0f / 2048.0f));
+      }
+      word |= (uint32_t) __builtin_bit_cast(unsigned short, v) << (16 * h);
+    }
+    out[i] = word;
+  }
+}
+
+// Segment table: fast-table layout with G = 64 for positions k0+1 .. k0+128*Q.
+__global__ void build_seg_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int Q, int k0, int bias,
+                                uint32_t *out) {
+  const int QB = (Q + 3) / 4;
+  const int64_t total = (int64_t) (R + 1) * QB * 64 * 4;
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t) gridDim.x * blockDim.x) {
+    const int e = (int) (i & 3);
+    const int lane = (int) ((i >> 2) & 63);
+    const int64_t rest = i >> 8;
+    const int jb = (int) (rest % QB);
+    const int x = (int) (rest / QB);
+    const int j = 4 * jb + e;
+    uint32_t word = 0;
+    for (int h = 0; h < 2; ++h) {
+      const int z = 2 * lane + h;
+      const int k = k0 + j + z * Q + 1;
+      _Float16 v = (_Float16) -1.0f;
+      if (x < R && j < Q && k <= M) {
+        const int cost = rbv[(int64_t) rowcode[x] * M + (k - 1)];
+        v = (_Float16) ((float) (bias - cost) * (1.0f / 2048.0f));
+      }
+      word |= (uint32_t) __builtin_bit_cast(unsigned short, v) << (16 * h);
+    }
+    out[i] = word;
+  }
+}
+
+// Slow table: uint32 C[(R+1) * Qs * 64]; dword = (cost of slot 2*lane, of slot 2*lane+1).
+__global__ void build_slow_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int Qs, uint32_t *out) {
+  const int64_t total = (int64_t) (R + 1) * Qs * 64;
+  for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t) gridDim.x * blockDim.x) {
+    const int lane = (int) (i & 63);
+    const int64_t rest = i >> 6;
+    const int j = (int) (rest % Qs);
+    const int x = (int) (rest / Qs);
+    uint32_t word = 0;
+    for (int h = 0; h < 2; ++h) {
+      const int z = 2 * lane + h;
+      const int k = j + z * Qs + 1;
+      uint32_t cost = MSVG_NULLCOST;
+      if (x < R && k <= M) cost = rbv[(int64_t) rowcode[x] * M + (k - 1)];
+      word |= cost << (16 * h);
+    }
+    out[i] = word;
+  }
+}
+
+// Zero n int32 words (used inside captured graphs in place of a memset node).
+__global__ void fill_zero_i32(int32_t *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch dispatch (called from msv_runtime.cpp).
+// ---------------------------------------------------------------------------
+typedef void (*fast_fn)(FastArgs);
+
+template <int G, int Q>
+static hipError_t launch_fast_t(const FastArgs &a, int grid, int block, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((msv_fast<G, Q>), dim3(grid), dim3(block), lds, st, a);
+  return hipGetLastError();
+}
+
+template <int G, int... Qs>
+static hipError_t dispatch_q(int q, const FastArgs &a, int grid, int block, size_t lds, hipStream_t st,
+                             std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipErrorInvalidValue;
+  (void) ((q == Qs ? (err = launch_fast_t<G, Qs>(a, grid, block, lds, st), true) : false) || ...);
+  return err;
+}
+
+// Q values compiled per lane-group size: every Q from 1 to 32.
+#define MSVG_QSEQ std::integer_sequence<int, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, \
+  17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32>
+
+#define MSVG_SEGQ std::integer_sequence<int, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, \
+  17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28>
+
+template <int G, int Q>
+static hipError_t set_lds_t() {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&msv_fast<G, Q>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+template <int G, int... Qs>
+static hipError_t set_lds_all(std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipSuccess;
+  ((err = (err == hipSuccess ? set_lds_t<G, Qs>() : err)), ...);
+  return err;
+}
+
+template <int Q>
+static hipError_t launch_seg_t(const SegArgs &a, int grid, int block, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((msv_seg<Q>), dim3(grid), dim3(block), lds, st, a);
+  return hipGetLastError();
+}
+template <int... Qs>
+static hipError_t dispatch_seg(int q, const SegArgs &a, int grid, int block, size_t lds, hipStream_t st,
+                               std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipErrorInvalidValue;
+  (void) ((q == Qs ? (err = launch_seg_t<Qs>(a, grid, block, lds, st), true) : false) || ...);
+  return err;
+}
+template <int Q>
+static hipError_t set_lds_seg_t() {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&msv_seg<Q>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+template <int... Qs>
+static hipError_t set_lds_seg(std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipSuccess;
+  ((err = (err == hipSuccess ? set_lds_seg_t<Qs>() : err)), ...);
+  return err;
+}
+
+extern "C" hipError_t msvk_launch_seg(int Q, const SegArgs *a, int grid, int block, size_t lds, hipStream_t st) {
+  if (Q < 1 || Q > 28) return hipErrorInvalidValue;
+  return dispatch_seg(Q, *a, grid, block, lds, st, MSVG_SEGQ{});
+}
+
+extern "C" hipError_t msvk_build_seg_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int Q, int k0,
+                                           int bias, uint32_t *out, hipStream_t st) {
+  const int QB = (Q + 3) / 4;
+  const int64_t total = (int64_t) (R + 1) * QB * 256;
+  int grid = (int) ((total + 255) / 256);
+  if (grid > 65535) grid = 65535;
+  hipLaunchKernelGGL(build_seg_table, dim3(grid), dim3(256), 0, st, rbv, M, rowcode, R, Q, k0, bias, out);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t msvk_prepare() {
+  hipError_t e;
+  if ((e = set_lds_seg(MSVG_SEGQ{})) != hipSuccess) return e;
+  if ((e = set_lds_all<16>(MSVG_QSEQ{})) != hipSuccess) return e;
+  if ((e = set_lds_all<32>(MSVG_QSEQ{})) != hipSuccess) return e;
+  return set_lds_all<64>(MSVG_QSEQ{});
+}
+
+extern "C" hipError_t msvk_launch_fast(int G, int Q, const FastArgs *a, int grid, int block, size_t lds,
+                                       hipStream_t st) {
+  if (Q < 1 || Q > 32) return hipErrorInvalidValue;
+  switch (G) {
+    case 16: return dispatch_q<16>(Q, *a, grid, block, lds, st, MSVG_QSEQ{});
+    case 32: return dispatch_q<32>(Q, *a, grid, block, lds, st, MSVG_QSEQ{});
+    case 64: return dispatch_q<64>(Q, *a, grid, block, lds, st, MSVG_QSEQ{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
+extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(msv_slow, dim3(grid), dim3(64 * SLOW_WAVES), 0, st, *a);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st) {
+  int gx = (int) ((n + 255) / 256);
+  if (gx > 4096) gx = 4096;
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(msv_finalize, dim3(gx, np), dim3(256), 0, st, *a);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
+                                            int Q, int bias, uint32_t *out, hipStream_t st) {
+  const int QB = (Q + 3) / 4;
+  const int64_t total = (int64_t) (R + 1) * QB * 256;
+  int grid = (int) ((total + 255) / 256);
+  if (grid > 65535) grid = 65535;
+  hipLaunchKernelGGL(build_fast_table, dim3(grid), dim3(256), 0, st, rbv, M, Kp, rowcode, R, G, Q, bias, out);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t msvk_build_slow_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int Qs,
+                                            uint32_t *out, hipStream_t st) {
+  const int64_t total = (int64_t) (R + 1) * Qs * 64;
+  int grid = (int) ((total + 255) / 256);
+  if (grid > 65535) grid = 65535;
+  hipLaunchKernelGGL(build_slow_table, dim3(grid), dim3(256), 0, st, rbv, M, rowcode, R, Qs, out);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t msvk_fill_zero(int32_t *p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(fill_zero_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}

@@ -1,0 +1,845 @@
+// msv_runtime.cpp -- host runtime behind include/msv_gpu.h.
+//
+// Owns device memory, the resident length-sorted target database, per-profile
+// device tables, the launch schedule (plain or batched/persistent), hipGraph
+// capture of batched searches, and the host tables that turn the device's
+// integer results into usc and the F1 pass bit with the reference's own float
+// and double expressions:
+//   usc     impl_sse/msvfilter.c:208-210
+//   nullsc  p7_bg.c:191-198 (p7_bg_SetLength) and :359-363 (p7_bg_NullOne)
+//   pass    p7_pipeline.c:667-677 with esl_gumbel_surv, easel/esl_gumbel.c:129-137
+//   tjb_b   impl_sse/p7_oprofile.c:921-925 via unbiased_byteify :512-519
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/msv_gpu.h"
+#include "msv_device.h"
+#include "msv_host.h"
+#include "../../include/msv_prep.h"
+
+extern "C" hipError_t msvk_prepare();
+extern "C" hipError_t msvk_launch_fast(int G, int Q, const FastArgs *a, int grid, int block, size_t lds, hipStream_t st);
+extern "C" hipError_t msvk_launch_seg(int Q, const SegArgs *a, int grid, int block, size_t lds, hipStream_t st);
+extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st);
+extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st);
+extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
+                                            int Q, int bias, uint32_t *out, hipStream_t st);
+extern "C" hipError_t msvk_build_seg_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int Q,
+                                           int k0, int bias, uint32_t *out, hipStream_t st);
+extern "C" hipError_t msvk_build_slow_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int Qs,
+                                            uint32_t *out, hipStream_t st);
+extern "C" hipError_t msvk_fill_zero(int32_t *p, int n, hipStream_t st);
+
+namespace {
+
+constexpr int kLdsBytes = 160 * 1024;
+constexpr int kQMax = 32;           // registers per lane in the fast kernels
+constexpr int kSegQMax = 28;        // registers per lane in one model segment (at most 3584 positions)
+
+// Segment width: the most registers whose table fits in LDS for R rows.
+int seg_q(int R) {
+  const int qb = kLdsBytes / ((R + 1) * 1024);
+  return std::max(1, std::min(kSegQMax, 4 * qb));
+}
+
+struct DevMem {
+  void *p = nullptr;
+  size_t n = 0;
+  DevMem() = default;
+  DevMem(const DevMem &) = delete;
+  DevMem &operator=(const DevMem &) = delete;
+  ~DevMem() { if (p) (void) hipFree(p); }
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n && p) return hipSuccess;
+    if (p) { (void) hipFree(p); p = nullptr; n = 0; }
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e == hipSuccess) n = bytes;
+    return e;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Profile {
+  int M = 0, Kp = 0;
+  std::vector<uint8_t> rbv;        // Kp * M
+  int tbm = 0, tec = 0, base = 0, bias = 0;
+  float scale = 0.f, mu = 0.f, lambda = 0.f;
+  DevMem d_rbv;
+};
+
+struct Db {
+  int64_t n = 0, total_res = 0, packed = 0;
+  int R = 0;
+  std::vector<uint8_t> rowcode;         // row -> residue code
+  int maxcode = -1;
+  std::vector<int32_t> len_sorted;
+  std::vector<int64_t> sorted_of;       // caller index -> sorted index
+  std::vector<int32_t> Ls;              // distinct lengths
+  std::vector<int32_t> lidx_sorted;
+  std::vector<uint8_t> tjb_sorted;
+  DevMem d_res, d_off, d_len, d_tjb, d_lidx, d_sorted_pos, d_order, d_rowcode;
+  std::map<int, std::pair<std::unique_ptr<DevMem>, int>> chunks;   // G -> (chunk_start, nchunks)
+};
+
+struct Tables {                          // per (profile, db)
+  std::map<int, std::unique_ptr<DevMem>> fast;   // G -> table
+  std::vector<std::unique_ptr<DevMem>> seg;      // segment tables (M > fast limit)
+  std::unique_ptr<DevMem> slow;
+};
+
+struct Tile { int lo, hi, lanes, waves; };
+
+struct Bufs {                             // device buffers one search writes
+  DevMem d_res, d_prof, d_slow, d_counters, d_usc, d_status, d_pass, d_thr, d_usctab, d_seg;
+  int64_t slow_cap = 0;
+};
+
+struct Plan {                             // a captured batched search with its own buffers
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  Bufs bufs;
+  ~Plan() {
+    if (exec) (void) hipGraphExecDestroy(exec);
+    if (graph) (void) hipGraphDestroy(graph);
+  }
+};
+
+}  // namespace
+
+struct msvg_ctx {
+  int device = 0;
+  int ncu = 256;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int next_id = 1;
+  std::map<int, std::unique_ptr<Profile>> profiles;
+  std::map<int, std::unique_ptr<Db>> dbs;
+  std::map<std::pair<int, int>, std::unique_ptr<Tables>> tables;
+  std::map<std::tuple<int, int, double>, std::pair<std::vector<uint16_t>, std::vector<float>>> finals;
+  std::vector<Tile> tiles;
+  Bufs eager;                      // buffers of plain and eager searches
+  Bufs *last = nullptr;            // buffers holding the latest results
+  int64_t slow_cap = 0;            // slow-queue capacity for the next search
+  int last_np = 0, last_db = -1;
+  msvg_stats stats{};
+  hipEvent_t ev[8] = {};
+  std::map<std::string, std::unique_ptr<Plan>> plans;
+  // single-target drop-in
+  int scratch_db = -1;
+};
+
+namespace {
+
+int fail(msvg_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hipfail(msvg_ctx *c, hipError_t e, const char *what) {
+  return fail(c, MSVG_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, expr)                                   \
+  do {                                                      \
+    hipError_t _e = (expr);                                 \
+    if (_e != hipSuccess) return hipfail((ctx), _e, #expr); \
+  } while (0)
+
+int qfor(int M, int slots) { return (M + slots - 1) / slots; }
+
+// Lane-group size and waves per workgroup for a profile of length M.
+void choose_tile(const msvg_ctx *c, int M, int R, int &G, int &waves) {
+  for (const Tile &t : c->tiles)
+    if (M >= t.lo && M <= t.hi) { G = t.lanes; waves = t.waves; return; }
+  // Built-in cost model: VALU instructions per wave-row divided by the model
+  // cells the wave advances, over the sizes whose tables fit in LDS.
+  double best = 1e30;
+  G = 64;
+  for (int g : {16, 32, 64}) {
+    const int Q = qfor(M, 2 * g);
+    if (Q > kQMax) continue;
+    const size_t lds = (size_t) (R + 1) * ((Q + 3) / 4) * 1024;
+    if (lds > (size_t) kLdsBytes) continue;
+    const double v = Q + (Q + 1) / 2 + 4 + (g == 32 ? 1 : 0);
+    const double cost = v / ((64 / g) * (double) M);
+    if (cost < best - 1e-12) { best = cost; G = g; }
+  }
+  waves = 0;
+}
+
+size_t fast_lds(int R, int Q) { return (size_t) (R + 1) * ((Q + 3) / 4) * 1024; }
+
+// Workgroup size and grid for a persistent launch whose table uses lds bytes.
+void persistent_shape(const msvg_ctx *c, size_t lds, int waves_hint, int &grid, int &block) {
+  int per_cu = (int) std::max<size_t>(1, std::min<size_t>(8, kLdsBytes / std::max<size_t>(lds, 1)));
+  int waves = waves_hint > 0 ? waves_hint : std::min(16, std::max(4, (16 + per_cu - 1) / per_cu));
+  block = 64 * waves;
+  grid = c->ncu * per_cu;
+}
+
+float usc_of(int d, int base, float scale) {
+  // msvfilter.c:208-210 with xJ - tjb_b = d
+  float sc = ((float) d - (float) base);
+  sc /= scale;
+  sc -= 3.0;
+  return sc;
+}
+
+// The F1 pass decision of p7_pipeline.c:667-677 for a finite usc.
+bool passes(float usc, int L, float mu, float lambda, double F1) {
+  const float nullsc = msvh_nullsc(L);
+  const float seq_score = (usc - nullsc) / 0.69314718055994529;
+  const double y = (double) lambda * ((double) seq_score - (double) mu);
+  const double ey = -std::exp(-y);
+  const double P = (std::fabs(ey) < 5e-9) ? -ey : 1 - std::exp(ey);
+  return !(P > F1);
+}
+
+const std::pair<std::vector<uint16_t>, std::vector<float>> &final_tables(msvg_ctx *c, int pid, int dbid, double F1) {
+  auto key = std::make_tuple(pid, dbid, F1);
+  auto it = c->finals.find(key);
+  if (it != c->finals.end()) return it->second;
+  const Profile &p = *c->profiles[pid];
+  const Db &d = *c->dbs[dbid];
+  std::vector<float> usc(511);
+  for (int k = 0; k < 511; ++k) usc[k] = usc_of(k - 255, p.base, p.scale);
+  std::vector<uint16_t> thr(d.Ls.size());
+  int guess = 128;
+  for (size_t li = 0; li < d.Ls.size(); ++li) {
+    const int L = d.Ls[li];
+    const int tjb = msvh_tjb(L);
+    auto ok = [&](int xJ) { return passes(usc[xJ - tjb + 255], L, p.mu, p.lambda, F1); };
+    int t;
+    if (!ok(255)) t = 256;
+    else if (ok(0)) t = 0;
+    else {
+      // smallest passing xJ: the score is monotone in xJ.  Start from the
+      // previous length's threshold and widen, then bisect.
+      int lo = std::max(0, guess - 2), hi = std::min(255, guess + 2);
+      while (lo > 0 && ok(lo)) lo = std::max(0, lo - 8);
+      while (!ok(hi)) hi = std::min(255, hi + 8);
+      if (ok(lo)) t = lo;  // lo == 0 passing handled above; keep for safety
+      else {
+        while (hi - lo > 1) { int m = (lo + hi) / 2; if (ok(m)) hi = m; else lo = m; }
+        t = hi;
+      }
+    }
+    thr[li] = (uint16_t) t;
+    if (t <= 255) guess = t;
+  }
+  return c->finals.emplace(key, std::make_pair(std::move(thr), std::move(usc))).first->second;
+}
+
+int build_chunks(msvg_ctx *c, Db &d, int G) {
+  auto it = d.chunks.find(G);
+  if (it != d.chunks.end()) return MSVG_OK;
+  const int NG = 64 / G;
+  const int64_t ntasks = (d.n + NG - 1) / NG;
+  // Chunks of roughly equal residue count; about 16 per wave of a full chip.
+  const int64_t waves = (int64_t) c->ncu * 16;
+  const int64_t target = std::max<int64_t>(4096, d.total_res / (waves * 16));
+  std::vector<int32_t> cs;
+  cs.push_back(0);
+  int64_t acc = 0;
+  for (int64_t t = 0; t < ntasks; ++t) {
+    acc += d.len_sorted[t * NG];     // the longest in the task sets its rows
+    if (acc >= target) { cs.push_back((int32_t) (t + 1)); acc = 0; }
+  }
+  if (cs.back() != ntasks) cs.push_back((int32_t) ntasks);
+  auto mem = std::make_unique<DevMem>();
+  HIPCHK(c, mem->ensure(cs.size() * sizeof(int32_t)));
+  HIPCHK(c, hipMemcpy(mem->p, cs.data(), cs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  d.chunks.emplace(G, std::make_pair(std::move(mem), (int) cs.size() - 1));
+  return MSVG_OK;
+}
+
+Tables &tables_for(msvg_ctx *c, int pid, int dbid) {
+  auto &t = c->tables[{pid, dbid}];
+  if (!t) t = std::make_unique<Tables>();
+  return *t;
+}
+
+int ensure_fast_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **out) {
+  Profile &p = *c->profiles[pid];
+  Db &d = *c->dbs[dbid];
+  Tables &t = tables_for(c, pid, dbid);
+  auto it = t.fast.find(G);
+  if (it == t.fast.end()) {
+    const int Q = qfor(p.M, 2 * G);
+    auto mem = std::make_unique<DevMem>();
+    HIPCHK(c, mem->ensure(fast_lds(d.R, Q)));
+    HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, d.d_rowcode.as<uint8_t>(), d.R, G, Q, p.bias,
+                                    mem->as<uint32_t>(), c->stream));
+    it = t.fast.emplace(G, std::move(mem)).first;
+  }
+  *out = it->second->as<uint32_t>();
+  return MSVG_OK;
+}
+
+int ensure_seg_tables(msvg_ctx *c, int pid, int dbid) {
+  Profile &p = *c->profiles[pid];
+  Db &d = *c->dbs[dbid];
+  Tables &t = tables_for(c, pid, dbid);
+  if (!t.seg.empty()) return MSVG_OK;
+  const int span = 128 * seg_q(d.R);
+  for (int k0 = 0; k0 < p.M; k0 += span) {
+    const int Mseg = std::min(span, p.M - k0);
+    const int Q = qfor(Mseg, 128);
+    auto mem = std::make_unique<DevMem>();
+    HIPCHK(c, mem->ensure(fast_lds(d.R, Q)));
+    HIPCHK(c, msvk_build_seg_table(p.d_rbv.as<uint8_t>(), p.M, d.d_rowcode.as<uint8_t>(), d.R, Q, k0, p.bias,
+                                   mem->as<uint32_t>(), c->stream));
+    t.seg.push_back(std::move(mem));
+  }
+  return MSVG_OK;
+}
+
+int ensure_slow_table(msvg_ctx *c, int pid, int dbid, const uint32_t **out) {
+  Profile &p = *c->profiles[pid];
+  Db &d = *c->dbs[dbid];
+  Tables &t = tables_for(c, pid, dbid);
+  if (!t.slow) {
+    const int Qs = qfor(p.M, 128);
+    auto mem = std::make_unique<DevMem>();
+    HIPCHK(c, mem->ensure((size_t) (d.R + 1) * Qs * 64 * 4));
+    HIPCHK(c, msvk_build_slow_table(p.d_rbv.as<uint8_t>(), p.M, d.d_rowcode.as<uint8_t>(), d.R, Qs,
+                                    mem->as<uint32_t>(), c->stream));
+    t.slow = std::move(mem);
+  }
+  *out = t.slow->as<uint32_t>();
+  return MSVG_OK;
+}
+
+int check_ids(msvg_ctx *c, const int32_t *ids, int np, int dbid) {
+  if (!c) return MSVG_EINVAL;
+  if (np < 1 || !ids) return fail(c, MSVG_EINVAL, "no profiles given");
+  auto dit = c->dbs.find(dbid);
+  if (dit == c->dbs.end()) return fail(c, MSVG_EINVAL, "unknown database id");
+  for (int i = 0; i < np; ++i) {
+    auto pit = c->profiles.find(ids[i]);
+    if (pit == c->profiles.end()) return fail(c, MSVG_EINVAL, "unknown profile id");
+    if (dit->second->maxcode >= pit->second->Kp)
+      return fail(c, MSVG_EINVAL, "database holds residue codes outside the profile's alphabet");
+  }
+  return MSVG_OK;
+}
+
+// Enqueue one complete search on c->stream.  When timed, per-stage events are
+// recorded (never inside a graph capture).
+// When capturing, nothing may be copied from host memory (the graph would
+// replay a copy of a buffer that no longer exists) and nothing may be
+// synchronized; the warm-up call made just before the capture leaves every
+// table and parameter block in b, and replays of the graph refill them
+// first (Plan upload in search_device).
+int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, double F1, int mode, bool timed,
+                   bool capturing, int64_t *cells_out) {
+  Db &d = *c->dbs[dbid];
+  const int64_t n = d.n;
+  int rc;
+  std::vector<ProfDev> pds(np);
+  std::vector<int> Gs(np), waves(np), segmented(np);
+  int64_t cells = 0;
+  for (int i = 0; i < np; ++i) {
+    Profile &p = *c->profiles[ids[i]];
+    ProfDev &pd = pds[i];
+    int G = 64, w = 0;
+    if (mode == MSVG_MODE_BATCHED) choose_tile(c, p.M, d.R, G, w);
+    const int Q = qfor(p.M, 2 * G);
+    segmented[i] = (Q > kQMax || fast_lds(d.R, Q) > (size_t) kLdsBytes) ? 1 : 0;
+    if (segmented[i]) G = 64;
+    Gs[i] = G; waves[i] = w;
+    pd.ftab = nullptr;
+    if (!segmented[i]) {
+      const uint32_t *ft;
+      if ((rc = ensure_fast_table(c, ids[i], dbid, G, &ft)) != MSVG_OK) return rc;
+      pd.ftab = reinterpret_cast<const uint4 *>(ft);
+    } else if ((rc = ensure_seg_tables(c, ids[i], dbid)) != MSVG_OK) {
+      return rc;
+    }
+    const uint32_t *st;
+    if ((rc = ensure_slow_table(c, ids[i], dbid, &st)) != MSVG_OK) return rc;
+    pd.stab = st;
+    pd.M = p.M; pd.Q = qfor(p.M, 2 * G); pd.QB = (pd.Q + 3) / 4; pd.Qs = qfor(p.M, 128);
+    pd.tbm = p.tbm; pd.tec = p.tec; pd.base = p.base; pd.bias = p.bias;
+    pd.slow_all = 0;
+    cells += (int64_t) p.M * d.total_res;
+  }
+  *cells_out = cells;
+
+  // Per-search buffers.
+  HIPCHK(c, b.d_res.ensure((size_t) np * n * sizeof(uint16_t)));
+  for (int i = 0; i < np; ++i) pds[i].res = b.d_res.as<uint16_t>() + (size_t) i * n;
+  HIPCHK(c, b.d_prof.ensure(np * sizeof(ProfDev)));
+  if (!capturing) HIPCHK(c, hipMemcpyAsync(b.d_prof.p, pds.data(), np * sizeof(ProfDev), hipMemcpyHostToDevice, c->stream));
+  if (c->slow_cap < std::max<int64_t>(4096, (int64_t) np * n / 32)) c->slow_cap = std::max<int64_t>(4096, (int64_t) np * n / 32);
+  if (!capturing) b.slow_cap = c->slow_cap;
+  HIPCHK(c, b.d_slow.ensure((size_t) b.slow_cap * sizeof(SlowItem)));
+  HIPCHK(c, b.d_counters.ensure((size_t) (np + 2) * sizeof(int32_t)));
+  HIPCHK(c, b.d_usc.ensure((size_t) np * n * sizeof(float)));
+  HIPCHK(c, b.d_status.ensure((size_t) np * n));
+  HIPCHK(c, b.d_pass.ensure((size_t) np * n));
+
+  // Host-built finalize tables.
+  std::vector<uint16_t> thr((size_t) np * d.Ls.size());
+  std::vector<float> utab((size_t) np * 511);
+  for (int i = 0; i < np; ++i) {
+    const auto &ft = final_tables(c, ids[i], dbid, F1);
+    std::copy(ft.first.begin(), ft.first.end(), thr.begin() + (size_t) i * d.Ls.size());
+    std::copy(ft.second.begin(), ft.second.end(), utab.begin() + (size_t) i * 511);
+  }
+  HIPCHK(c, b.d_thr.ensure(thr.size() * sizeof(uint16_t) + 16));
+  HIPCHK(c, b.d_usctab.ensure(utab.size() * sizeof(float)));
+  if (!capturing) {
+    HIPCHK(c, hipMemcpyAsync(b.d_thr.p, thr.data(), thr.size() * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.d_usctab.p, utab.data(), utab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+
+  DbDev dd;
+  dd.res = d.d_res.as<uint8_t>(); dd.off = d.d_off.as<int64_t>(); dd.len = d.d_len.as<int32_t>();
+  dd.tjb = d.d_tjb.as<uint8_t>(); dd.n = n; dd.R = d.R;
+  int32_t *counters = b.d_counters.as<int32_t>();
+  int32_t *slow_count = counters + np;
+
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  HIPCHK(c, msvk_fill_zero(counters, np + 2, c->stream));
+
+  for (int i = 0; i < np; ++i) {
+    const Profile &p = *c->profiles[ids[i]];
+    if (segmented[i]) {
+      // Long model: column segments of kSegQ registers, carrying the last
+      // column of each row from one segment to the next.
+      Tables &t = tables_for(c, ids[i], dbid);
+      HIPCHK(c, b.d_seg.ensure((size_t) d.packed * sizeof(uint16_t) + (size_t) d.n * sizeof(uint16_t) + 64));
+      const int span = 128 * seg_q(d.R);
+      int si = 0;
+      for (int k0 = 0; k0 < p.M; k0 += span, ++si) {
+        const int Mseg = std::min(span, p.M - k0);
+        const int Q = qfor(Mseg, 128);
+        SegArgs a;
+        a.prof = pds[i];
+        a.prof.Q = Q; a.prof.QB = (Q + 3) / 4;
+        a.db = dd;
+        a.tab = t.seg[si]->as<uint4>();
+        a.carry = b.d_seg.as<uint16_t>();
+        a.partial = reinterpret_cast<uint16_t *>(b.d_seg.as<uint8_t>() + (size_t) d.packed * sizeof(uint16_t));
+        a.first = (k0 == 0); a.last = (k0 + span >= p.M);
+        a.prof_idx = i; a.slow = b.d_slow.as<SlowItem>(); a.slow_count = slow_count; a.slow_cap = (int32_t) b.slow_cap;
+        a.carry_base = d.d_off.as<int64_t>();
+        const size_t lds = fast_lds(d.R, Q);
+        int grid, block;
+        persistent_shape(c, lds, 0, grid, block);
+        a.ntasks = (int32_t) n;
+        HIPCHK(c, msvk_launch_seg(Q, &a, grid, block, lds, c->stream));
+      }
+      continue;
+    }
+    FastArgs a;
+    a.prof = pds[i];
+    a.db = dd;
+    a.prof_idx = i;
+    a.slow = b.d_slow.as<SlowItem>();
+    a.slow_count = slow_count;
+    a.slow_cap = (int32_t) b.slow_cap;
+    a.counter = counters + i;
+    const int G = Gs[i];
+    const size_t lds = fast_lds(d.R, pds[i].Q);
+    int grid, block;
+    if (mode == MSVG_MODE_BATCHED) {
+      if ((rc = build_chunks(c, d, G)) != MSVG_OK) return rc;
+      auto &ch = d.chunks[G];
+      a.mode = 0;
+      a.chunk_start = ch.first->as<int32_t>();
+      a.nchunks = ch.second;
+      a.order = nullptr; a.ntasks = 0;
+      persistent_shape(c, lds, waves[i], grid, block);
+    } else {
+      a.mode = 1;
+      a.chunk_start = nullptr; a.nchunks = 0;
+      a.order = d.d_order.as<int32_t>();
+      a.ntasks = (int32_t) n;
+      block = 256;
+      grid = c->ncu * (int) std::max<size_t>(1, std::min<size_t>(8, kLdsBytes / lds));
+    }
+    HIPCHK(c, msvk_launch_fast(G, pds[i].Q, &a, grid, block, lds, c->stream));
+  }
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+
+  SlowArgs sa;
+  sa.profs = b.d_prof.as<ProfDev>();
+  sa.db = dd;
+  sa.items = b.d_slow.as<SlowItem>();
+  sa.count = slow_count;
+  sa.cap = (int32_t) b.slow_cap;
+  HIPCHK(c, msvk_launch_slow(&sa, c->ncu * 4, c->stream));
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+
+  FinArgs fa;
+  fa.profs = b.d_prof.as<ProfDev>();
+  fa.db = dd;
+  fa.sorted_pos = d.d_sorted_pos.as<int64_t>();
+  fa.lidx = d.d_lidx.as<int32_t>();
+  fa.usc_tab = b.d_usctab.as<float>();
+  fa.thr = b.d_thr.as<uint16_t>();
+  fa.nL = (int32_t) d.Ls.size();
+  fa.usc = b.d_usc.as<float>();
+  fa.status = b.d_status.as<uint8_t>();
+  fa.pass = b.d_pass.as<uint8_t>();
+  HIPCHK(c, msvk_launch_finalize(&fa, np, n, c->stream));
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  return MSVG_OK;
+}
+
+int after_search(msvg_ctx *c, Bufs &b, int np, int dbid, int64_t cells, bool timed) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int32_t cnt = 0;
+  HIPCHK(c, hipMemcpy(&cnt, b.d_counters.as<int32_t>() + np, sizeof(int32_t), hipMemcpyDeviceToHost));
+  c->last = &b;
+  c->stats.cells = cells;
+  c->stats.n_slow = cnt;
+  if (timed) {
+    float a = 0, b = 0, f = 0, t = 0;
+    HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HIPCHK(c, hipEventElapsedTime(&f, c->ev[2], c->ev[3]));
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+    c->stats.ms_fast = a; c->stats.ms_slow = b; c->stats.ms_finalize = f; c->stats.ms_total = t;
+  }
+  c->last_np = np;
+  c->last_db = dbid;
+  if (cnt > b.slow_cap) return MSVG_ENORESULT;     // caller re-runs with a larger queue
+  return MSVG_OK;
+}
+
+int search_device(msvg_ctx *c, const int32_t *ids, int np, int dbid, double F1, int mode_flags) {
+  int rc = check_ids(c, ids, np, dbid);
+  if (rc != MSVG_OK) return rc;
+  const bool eager = (mode_flags & MSVG_FLAG_EAGER) != 0;
+  const int mode = mode_flags & 0xf;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    int64_t cells = 0;
+    if (mode == MSVG_MODE_BATCHED && !eager) {
+      std::string key = std::to_string(dbid) + ":" + std::to_string(F1) + ":" + std::to_string(c->slow_cap);
+      for (int i = 0; i < np; ++i) key += "," + std::to_string(ids[i]);
+      auto &plan = c->plans[key];
+      if (!plan) {
+        // Warm every cache (tables, chunks, buffers) outside the capture, then
+        // capture the launch sequence once.
+        auto pl = std::make_unique<Plan>();
+        if ((rc = enqueue_search(c, pl->bufs, ids, np, dbid, F1, mode, false, false, &cells)) != MSVG_OK) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        rc = enqueue_search(c, pl->bufs, ids, np, dbid, F1, mode, false, true, &cells);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != MSVG_OK) { if (g) (void) hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");
+        pl->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0));
+        plan = std::move(pl);
+      }
+      for (int i = 0; i < np; ++i) cells += (int64_t) c->profiles[ids[i]]->M * c->dbs[dbid]->total_res;
+      HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+      HIPCHK(c, hipGraphLaunch(plan->exec, c->stream));
+      HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      float t = 0;
+      HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
+      rc = after_search(c, plan->bufs, np, dbid, cells, false);
+      c->stats.ms_total = t; c->stats.ms_fast = c->stats.ms_slow = c->stats.ms_finalize = 0;
+      c->stats.n_fast_launches = np;
+    } else {
+      if ((rc = enqueue_search(c, c->eager, ids, np, dbid, F1, mode, true, false, &cells)) != MSVG_OK) return rc;
+      rc = after_search(c, c->eager, np, dbid, cells, true);
+      c->stats.n_fast_launches = np;
+    }
+    if (rc != MSVG_ENORESULT) return rc;
+    c->slow_cap = (int64_t) c->stats.n_slow + 4096;     // queue overflowed: grow and redo
+    c->last = nullptr;
+    c->plans.clear();
+  }
+  return fail(c, MSVG_EMEM, "slow-path queue could not be sized");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int msvg_init(int device, msvg_ctx **ret) {
+  if (!ret) return MSVG_EINVAL;
+  *ret = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return MSVG_EDEVICE;
+  if (hipSetDevice(device) != hipSuccess) return MSVG_EDEVICE;
+  auto c = std::make_unique<msvg_ctx>();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return MSVG_EDEVICE;
+  c->ncu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MSVG_EDEVICE;
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return MSVG_EDEVICE;
+  if (msvk_prepare() != hipSuccess) return MSVG_EDEVICE;
+  *ret = c.release();
+  return MSVG_OK;
+}
+
+void msvg_free(msvg_ctx *c) {
+  if (!c) return;
+  (void) hipSetDevice(c->device);
+  (void) hipStreamSynchronize(c->stream);
+  c->plans.clear();
+  c->tables.clear();
+  c->dbs.clear();
+  c->profiles.clear();
+  for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
+  if (c->stream) (void) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *msvg_last_error(const msvg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int msvg_profile_add(msvg_ctx *c, const msvg_oprofile *om, int32_t *ret_id) {
+  if (!c || !om || !ret_id || !om->rbv) return fail(c, MSVG_EINVAL, "null argument");
+  if (om->M < 1 || om->M > 16384) return fail(c, MSVG_EINVAL, "M must be in 1..16384");
+  if (om->Kp < 1 || om->Kp > 255) return fail(c, MSVG_EINVAL, "Kp out of range");
+  if ((int) om->bias_b + (int) om->base_b >= 255)
+    return fail(c, MSVG_EINVAL, "bias_b + base_b >= 255 is outside the supported range");
+  (void) hipSetDevice(c->device);
+  auto p = std::make_unique<Profile>();
+  p->M = om->M; p->Kp = om->Kp;
+  p->rbv.assign(om->rbv, om->rbv + (size_t) om->M * om->Kp);
+  p->tbm = om->tbm_b; p->tec = om->tec_b; p->base = om->base_b; p->bias = om->bias_b;
+  p->scale = om->scale_b; p->mu = om->mmu; p->lambda = om->mlambda;
+  HIPCHK(c, p->d_rbv.ensure(p->rbv.size()));
+  HIPCHK(c, hipMemcpy(p->d_rbv.p, p->rbv.data(), p->rbv.size(), hipMemcpyHostToDevice));
+  const int id = c->next_id++;
+  c->profiles[id] = std::move(p);
+  *ret_id = id;
+  return MSVG_OK;
+}
+
+int msvg_profile_remove(msvg_ctx *c, int32_t id) {
+  if (!c || !c->profiles.count(id)) return fail(c, MSVG_EINVAL, "unknown profile id");
+  (void) hipStreamSynchronize(c->stream);
+  for (auto it = c->tables.begin(); it != c->tables.end();)
+    it = (it->first.first == id) ? c->tables.erase(it) : std::next(it);
+  for (auto it = c->finals.begin(); it != c->finals.end();)
+    it = (std::get<0>(it->first) == id) ? c->finals.erase(it) : std::next(it);
+  c->last = nullptr;
+  c->plans.clear();
+  c->profiles.erase(id);
+  return MSVG_OK;
+}
+
+int msvg_db_add(msvg_ctx *c, const uint8_t *res, const int64_t *off, const int32_t *len, int64_t n, int32_t *ret_id) {
+  if (!c || !ret_id || n < 0 || (n > 0 && (!res || !off || !len))) return fail(c, MSVG_EINVAL, "null argument");
+  if (n >= (int64_t) 1 << 31) return fail(c, MSVG_EINVAL, "too many targets for one database");
+  (void) hipSetDevice(c->device);
+  auto d = std::make_unique<Db>();
+  d->n = n;
+  // Codes present.
+  bool seen[256] = {false};
+  int64_t total = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (len[i] < 0) return fail(c, MSVG_EINVAL, "negative target length");
+    const uint8_t *s = res + off[i];
+    for (int32_t k = 0; k < len[i]; ++k) seen[s[k]] = true;
+    total += len[i];
+  }
+  uint8_t rowof[256];
+  for (int x = 0; x < 256; ++x)
+    if (seen[x]) { rowof[x] = (uint8_t) d->rowcode.size(); d->rowcode.push_back((uint8_t) x); d->maxcode = x; }
+  if (d->rowcode.empty()) d->rowcode.push_back(0);
+  d->R = (int) d->rowcode.size();
+  if (d->R > 254) return fail(c, MSVG_EINVAL, "too many distinct residue codes");
+  d->total_res = total;
+  // Length-sorted order (descending; ties by caller index for determinism).
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return len[a] > len[b]; });
+  d->sorted_of.resize(n);
+  d->len_sorted.resize(n);
+  d->tjb_sorted.resize(n);
+  std::vector<int64_t> off_sorted(n);
+  int64_t bytes = 0;
+  for (int64_t s = 0; s < n; ++s) {
+    const int64_t i = order[s];
+    d->sorted_of[i] = s;
+    d->len_sorted[s] = len[i];
+    d->tjb_sorted[s] = (uint8_t) msvh_tjb(len[i]);
+    off_sorted[s] = bytes;
+    bytes += ((int64_t) len[i] + 15) / 16 * 16;
+  }
+  d->packed = bytes + 64;
+  std::vector<uint8_t> packed((size_t) bytes + 64, (uint8_t) d->R);
+  for (int64_t s = 0; s < n; ++s) {
+    const int64_t i = order[s];
+    const uint8_t *src = res + off[i];
+    uint8_t *dst = packed.data() + off_sorted[s];
+    for (int32_t k = 0; k < len[i]; ++k) dst[k] = rowof[src[k]];
+  }
+  // Distinct lengths.
+  std::vector<int32_t> ls(d->len_sorted);
+  std::sort(ls.begin(), ls.end());
+  ls.erase(std::unique(ls.begin(), ls.end()), ls.end());
+  d->Ls = ls;
+  d->lidx_sorted.resize(n);
+  for (int64_t s = 0; s < n; ++s)
+    d->lidx_sorted[s] = (int32_t) (std::lower_bound(ls.begin(), ls.end(), d->len_sorted[s]) - ls.begin());
+  std::vector<int32_t> order32(n);
+  for (int64_t i = 0; i < n; ++i) order32[i] = (int32_t) d->sorted_of[i];
+
+  HIPCHK(c, d->d_res.ensure(packed.size()));
+  HIPCHK(c, hipMemcpy(d->d_res.p, packed.data(), packed.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, d->d_off.ensure(std::max<int64_t>(n, 1) * 8));
+  HIPCHK(c, hipMemcpy(d->d_off.p, off_sorted.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, d->d_len.ensure(std::max<int64_t>(n, 1) * 4));
+  HIPCHK(c, hipMemcpy(d->d_len.p, d->len_sorted.data(), n * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, d->d_tjb.ensure(std::max<int64_t>(n, 1)));
+  HIPCHK(c, hipMemcpy(d->d_tjb.p, d->tjb_sorted.data(), n, hipMemcpyHostToDevice));
+  HIPCHK(c, d->d_lidx.ensure(std::max<int64_t>(n, 1) * 4));
+  HIPCHK(c, hipMemcpy(d->d_lidx.p, d->lidx_sorted.data(), n * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, d->d_sorted_pos.ensure(std::max<int64_t>(n, 1) * 8));
+  HIPCHK(c, hipMemcpy(d->d_sorted_pos.p, d->sorted_of.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, d->d_order.ensure(std::max<int64_t>(n, 1) * 4));
+  HIPCHK(c, hipMemcpy(d->d_order.p, order32.data(), n * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, d->d_rowcode.ensure(d->rowcode.size()));
+  HIPCHK(c, hipMemcpy(d->d_rowcode.p, d->rowcode.data(), d->rowcode.size(), hipMemcpyHostToDevice));
+  const int id = c->next_id++;
+  c->dbs[id] = std::move(d);
+  *ret_id = id;
+  return MSVG_OK;
+}
+
+int msvg_db_remove(msvg_ctx *c, int32_t id) {
+  if (!c || !c->dbs.count(id)) return fail(c, MSVG_EINVAL, "unknown database id");
+  (void) hipStreamSynchronize(c->stream);
+  for (auto it = c->tables.begin(); it != c->tables.end();)
+    it = (it->first.second == id) ? c->tables.erase(it) : std::next(it);
+  for (auto it = c->finals.begin(); it != c->finals.end();)
+    it = (std::get<1>(it->first) == id) ? c->finals.erase(it) : std::next(it);
+  c->last = nullptr;
+  c->plans.clear();
+  c->dbs.erase(id);
+  if (c->last_db == id) c->last_db = -1;
+  return MSVG_OK;
+}
+
+int64_t msvg_db_size(msvg_ctx *c, int32_t id) {
+  if (!c || !c->dbs.count(id)) return -1;
+  return c->dbs[id]->n;
+}
+
+int msvg_search_device(msvg_ctx *c, const int32_t *ids, int32_t np, int32_t dbid, double F1, int mode) {
+  if (!c) return MSVG_EINVAL;
+  if ((mode & 0xf) != MSVG_MODE_PLAIN && (mode & 0xf) != MSVG_MODE_BATCHED) return fail(c, MSVG_EINVAL, "unknown mode");
+  (void) hipSetDevice(c->device);
+  return search_device(c, ids, np, dbid, F1, mode);
+}
+
+int msvg_results(msvg_ctx *c, float *usc, uint8_t *status, uint8_t *pass) {
+  if (!c || c->last_db < 0 || !c->last) return fail(c, MSVG_EINVAL, "no results");
+  Bufs &b = *c->last;
+  const size_t cnt = (size_t) c->last_np * c->dbs[c->last_db]->n;
+  if (usc) HIPCHK(c, hipMemcpy(usc, b.d_usc.p, cnt * sizeof(float), hipMemcpyDeviceToHost));
+  if (status) HIPCHK(c, hipMemcpy(status, b.d_status.p, cnt, hipMemcpyDeviceToHost));
+  if (pass) HIPCHK(c, hipMemcpy(pass, b.d_pass.p, cnt, hipMemcpyDeviceToHost));
+  if (status) {
+    for (size_t i = 0; i < cnt; ++i)
+      if (status[i] == 255) return fail(c, MSVG_EDEVICE, "a pair was left unresolved by the device stages");
+  }
+  return MSVG_OK;
+}
+
+int msvg_search(msvg_ctx *c, const int32_t *ids, int32_t np, int32_t dbid, double F1, int mode, float *usc,
+                uint8_t *status, uint8_t *pass) {
+  int rc = msvg_search_device(c, ids, np, dbid, F1, mode);
+  if (rc != MSVG_OK) return rc;
+  std::vector<uint8_t> st;
+  if (!status) { st.resize((size_t) np * c->dbs[dbid]->n); status = st.data(); }
+  return msvg_results(c, usc, status, pass);
+}
+
+int msvg_last_stats(msvg_ctx *c, msvg_stats *out) {
+  if (!c || !out) return MSVG_EINVAL;
+  *out = c->stats;
+  return MSVG_OK;
+}
+
+int msvg_set_tile(msvg_ctx *c, int32_t lo, int32_t hi, int32_t lanes, int32_t waves) {
+  if (!c) return MSVG_EINVAL;
+  if (lanes == 0) { c->tiles.clear(); c->last = nullptr; c->plans.clear(); return MSVG_OK; }
+  if (lanes != 16 && lanes != 32 && lanes != 64) return fail(c, MSVG_EINVAL, "lanes must be 16, 32 or 64");
+  if (waves < 0 || waves > 16) return fail(c, MSVG_EINVAL, "waves must be 0..16");
+  c->tiles.push_back(Tile{lo, hi, lanes, waves});
+  c->last = nullptr;
+  c->plans.clear();
+  return MSVG_OK;
+}
+
+int msvg_profile_set_evparams(msvg_ctx *c, int32_t id, float mmu, float mlambda) {
+  if (!c || !c->profiles.count(id)) return fail(c, MSVG_EINVAL, "unknown profile id");
+  c->profiles[id]->mu = mmu;
+  c->profiles[id]->lambda = mlambda;
+  for (auto it = c->finals.begin(); it != c->finals.end();)
+    it = (std::get<0>(it->first) == id) ? c->finals.erase(it) : std::next(it);
+  c->last = nullptr;
+  c->plans.clear();
+  return MSVG_OK;
+}
+
+int msvg_calibrate_mu(msvg_ctx *c, int32_t id, double lambda, float *ret_mu) {
+  if (!c || !ret_mu || !c->profiles.count(id)) return fail(c, MSVG_EINVAL, "unknown profile id");
+  const int N = 200, L = 200;                       // EmN, EmL: evalues.c:67-68
+  std::vector<uint8_t> res((size_t) N * L);
+  msvh_calibration_targets(N, L, res.data());
+  std::vector<int64_t> off(N);
+  std::vector<int32_t> len(N, L);
+  for (int i = 0; i < N; ++i) off[i] = (int64_t) i * L;
+  int32_t db;
+  int rc = msvg_db_add(c, res.data(), off.data(), len.data(), N, &db);
+  if (rc != MSVG_OK) return rc;
+  std::vector<float> usc(N);
+  std::vector<uint8_t> st(N);
+  rc = msvg_search(c, &id, 1, db, 0.02, MSVG_MODE_PLAIN, usc.data(), st.data(), nullptr);
+  msvg_db_remove(c, db);
+  if (rc != MSVG_OK) return rc;
+  std::vector<int32_t> st32(st.begin(), st.end());
+  const Profile &p = *c->profiles[id];
+  *ret_mu = (float) msvh_fit_mu(usc.data(), st32.data(), N, L, p.base, p.scale, lambda);
+  return MSVG_OK;
+}
+
+int msvg_MSVFilter(msvg_ctx *c, int32_t pid, const uint8_t *dsq, int L, float *ret_sc) {
+  if (!c || !dsq || !ret_sc || L < 0) return fail(c, MSVG_EINVAL, "null argument");
+  if (!c->profiles.count(pid)) return fail(c, MSVG_EINVAL, "unknown profile id");
+  // One-target database holding dsq[1..L]; reused by the next call.
+  if (c->scratch_db >= 0) msvg_db_remove(c, c->scratch_db);
+  c->scratch_db = -1;
+  const int64_t off = 0;
+  int32_t id;
+  int rc = msvg_db_add(c, dsq + 1, &off, &L, 1, &id);
+  if (rc != MSVG_OK) return rc;
+  c->scratch_db = id;
+  float usc;
+  uint8_t st;
+  rc = msvg_search(c, &pid, 1, id, 0.02, MSVG_MODE_PLAIN, &usc, &st, nullptr);
+  if (rc != MSVG_OK) return rc;
+  *ret_sc = usc;
+  return st == 16 ? MSVG_ERANGE : MSVG_OK;
+}
+
+}  // extern "C"

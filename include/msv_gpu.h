@@ -1,0 +1,127 @@
+/* msv_gpu.h -- C ABI of the MI355X MSV filter (HMMER3 p7_MSVFilter on gfx950).
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point
+ * returns an Easel-compatible status (easel/easel.h:95-114); none calls exit()
+ * or lets a C++ exception cross the boundary.  Host buffers are owned by the
+ * caller; device buffers by the library.  A context is bound to one GPU and
+ * must not be used from two host threads at once (use one context per thread,
+ * as HMMER gives each worker its own P7_OMX / P7_OPROFILE clone,
+ * hmmsearch.c:429-439, cudaHmmsearch.cu:508).
+ *
+ * Reference interfaces replaced (paths in forestcheng/cudahmmsearch, code/):
+ *   msvg_MSVFilter     int p7_MSVFilter(const ESL_DSQ *dsq, int L,
+ *                        const P7_OPROFILE *om, P7_OMX *ox, float *ret_sc)
+ *                        src/impl_sse/impl_sse.h:327, src/impl_sse/msvfilter.c:78
+ *   msvg_search        the GPU block loop of cudaHmmsearch: loadCUDA()
+ *                        src/GPUFunction.cu:332-492 (GPUFunction.h:72-76) feeding
+ *                        usc into p7_PipelineShort() (src/p7_pipeline.c:887-918)
+ *                        with the MSV pass rule of p7_Pipeline (p7_pipeline.c:667-677)
+ *   msvg_profile_add   allocCUDAFixed()/loadOM() (GPUFunction.cu:260-313): the
+ *                        MSV fields of P7_OPROFILE (impl_sse.h:79-86) to the GPU
+ *   msvg_db_add        loadDBQuad() + the per-block H2D copies (GPUFunction.cu:
+ *                        219-255, 420-457), here once, length-sorted and resident
+ *   msvg_init/free     getGPUInfo()/freeMem() (GPUFunction.cu:494-520)
+ */
+#ifndef MSV_GPU_H
+#define MSV_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: the values of Easel's (easel/easel.h:95-114). */
+#define MSVG_OK         0   /* eslOK      */
+#define MSVG_EMEM       5   /* eslEMEM    */
+#define MSVG_EINVAL    11   /* eslEINVAL  */
+#define MSVG_ERANGE    16   /* eslERANGE: score overflowed, usc = +inf */
+#define MSVG_ENORESULT 19   /* eslENORESULT */
+#define MSVG_EDEVICE  100   /* HIP runtime error (no Easel counterpart) */
+
+/* Search modes. */
+#define MSVG_MODE_PLAIN    0  /* one launch per profile, targets in caller order, static schedule */
+#define MSVG_MODE_BATCHED  1  /* length-sorted, dynamic persistent schedule, tuned tiles, hipGraph */
+#define MSVG_FLAG_EAGER  0x10 /* or'ed into BATCHED: launch without the graph, with per-stage timing */
+
+typedef struct msvg_ctx msvg_ctx;
+
+/* The MSV part of P7_OPROFILE (impl_sse.h:79-86) plus the two calibrated
+ * E-value parameters the pass rule reads (om->evparam[p7_MMU], [p7_MLAMBDA]).
+ * rbv is de-striped: rbv[x * M + (k-1)] is the biased cost of residue x at
+ * node k (impl_sse/p7_oprofile.c:625-630), x in 0..Kp-1. */
+typedef struct {
+  int32_t        M;
+  int32_t        Kp;
+  const uint8_t *rbv;
+  uint8_t        tbm_b, tec_b, base_b, bias_b;
+  float          scale_b;
+  float          mmu, mlambda;
+} msvg_oprofile;
+
+/* Per-search statistics (kernel times from HIP events on the search stream). */
+typedef struct {
+  double  ms_fast;         /* sum of fast-stage launch durations */
+  double  ms_slow;         /* exact byte-MSV stage */
+  double  ms_finalize;
+  double  ms_total;        /* first launch start to last launch end */
+  int64_t cells;           /* sum over pairs of M * L */
+  int64_t n_slow;          /* pairs that ran the exact byte MSV */
+  int32_t n_fast_launches;
+} msvg_stats;
+
+int         msvg_init(int device, msvg_ctx **ret_ctx);
+void        msvg_free(msvg_ctx *ctx);
+const char *msvg_last_error(const msvg_ctx *ctx);
+
+/* Profiles: copied to the device; the caller's buffers may be freed after.
+ * EINVAL if M < 1, Kp < 21, or bias_b + base_b >= 255 (a range no profile
+ * built by p7_oprofile_Convert reaches). */
+int msvg_profile_add(msvg_ctx *ctx, const msvg_oprofile *om, int32_t *ret_id);
+int msvg_profile_remove(msvg_ctx *ctx, int32_t id);
+
+/* Replace a profile's calibrated MSV E-value parameters. */
+int msvg_profile_set_evparams(msvg_ctx *ctx, int32_t id, float mmu, float mlambda);
+
+/* p7_MSVMu (src/evalues.c:240-286) on the GPU: scores the 200 i.i.d. length-200
+ * calibration targets of p7_Calibrate (RNG esl_randomness_CreateFast(42),
+ * evalues.c:81) with this profile and fits the Gumbel location for lambda. */
+int msvg_calibrate_mu(msvg_ctx *ctx, int32_t id, double lambda, float *ret_mu);
+
+/* Target database: residues are digital codes 0..Kp-1 without sentinels,
+ * target i at res[off[i] .. off[i]+len[i]).  Copied to the device once,
+ * remapped to the codes present, padded and length-sorted. */
+int msvg_db_add(msvg_ctx *ctx, const uint8_t *res, const int64_t *off, const int32_t *len,
+                int64_t n, int32_t *ret_id);
+int msvg_db_remove(msvg_ctx *ctx, int32_t id);
+int64_t msvg_db_size(msvg_ctx *ctx, int32_t id);
+
+/* Score every listed profile against every target.  Outputs are row-major
+ * [np][n] in the caller's target order; any may be NULL:
+ *   usc    MSV score in nats, bit-identical to p7_MSVFilter's *ret_sc
+ *   status 0 (eslOK) or 16 (eslERANGE, usc = +inf)
+ *   pass   1 if the target passes the F1 filter of p7_Pipeline (P <= F1). */
+int msvg_search(msvg_ctx *ctx, const int32_t *profile_ids, int32_t np, int32_t db_id,
+                double F1, int mode, float *usc, uint8_t *status, uint8_t *pass);
+
+/* The same, leaving results on the device (for timing with inputs resident);
+ * msvg_results copies the last device-resident results to the host. */
+int msvg_search_device(msvg_ctx *ctx, const int32_t *profile_ids, int32_t np, int32_t db_id,
+                       double F1, int mode);
+int msvg_results(msvg_ctx *ctx, float *usc, uint8_t *status, uint8_t *pass);
+int msvg_last_stats(msvg_ctx *ctx, msvg_stats *out);
+
+/* Drop-in for p7_MSVFilter: dsq is 1-based with sentinels at 0 and L+1
+ * (easel.h:308-309).  Returns MSVG_OK or MSVG_ERANGE; *ret_sc as the
+ * reference sets it. */
+int msvg_MSVFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, int L, float *ret_sc);
+
+/* Tile choice for the batched mode: lanes per target (16, 32, 64) and waves
+ * per workgroup for profiles of length M; 0 restores the built-in choice.
+ * The autotuner in the Python package persists its findings through this. */
+int msvg_set_tile(msvg_ctx *ctx, int32_t M_lo, int32_t M_hi, int32_t lanes, int32_t waves);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

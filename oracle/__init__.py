@@ -49,6 +49,10 @@ class Oracle:
         lib.orc_nullsc.argtypes = [C.c_int]
         lib.orc_pass.restype = C.c_int
         lib.orc_pass.argtypes = [C.c_float, C.c_int, C.c_float, C.c_float, C.c_double]
+        lib.orc_jfree_max.restype = C.c_int
+        lib.orc_jfree_max.argtypes = [C.POINTER(_OrcProfile), C.c_void_p, C.c_int]
+        lib.orc_decide.restype = C.c_int
+        lib.orc_decide.argtypes = [C.POINTER(_OrcProfile), C.c_int, C.c_int, C.POINTER(C.c_int)]
         lib.orc_msv_batch.restype = C.c_int
         lib.orc_msv_batch.argtypes = [C.POINTER(_OrcProfile), C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int, C.c_void_p, C.c_void_p, C.c_int]
@@ -91,6 +95,17 @@ class Oracle:
         self.lib.orc_msv_batch(C.byref(s), _p(res), _p(off), _p(lens), n, _p(usc), _p(st), nthreads)
         return st, usc
 
+    def jfree_max(self, p, x):
+        s, keep = self._prof(p)
+        x = np.ascontiguousarray(x, dtype=np.uint8)
+        return self.lib.orc_jfree_max(C.byref(s), _p(x), int(x.size))
+
+    def decide(self, p, Rp, L):
+        s, keep = self._prof(p)
+        xJ = C.c_int(-1)
+        code = self.lib.orc_decide(C.byref(s), int(Rp), int(L), C.byref(xJ))
+        return code, xJ.value
+
     def score(self, xJ, tjb, base, scale):
         return self.lib.orc_score(int(xJ), int(tjb), int(base), float(scale))
 
@@ -128,6 +143,10 @@ class Reference:
                                       C.c_void_p, C.c_void_p, C.c_int]
         lib.ref_iid.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_void_p]
         lib.ref_uniform.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_void_p]
+        lib.ref_profile_from_file.restype = C.c_void_p
+        lib.ref_profile_from_file.argtypes = [C.c_char_p, C.c_int]
+        lib.ref_profile_Kp.argtypes = [C.c_void_p]
+        lib.ref_digitize.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
         lib.ref_core_emit.restype = C.c_int64
         lib.ref_core_emit.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]
         self.lib = lib
@@ -145,6 +164,12 @@ class Reference:
         h = self.lib.ref_profile_from_probs(M, _p(mat), _p(t), config_L, 1 if calibrate else 0)
         if not h:
             raise RuntimeError("reference profile construction failed")
+        return RefProfile(self, h)
+
+    def from_file(self, path, which=0):
+        h = self.lib.ref_profile_from_file(path.encode(), which)
+        if not h:
+            raise RuntimeError(f"cannot read model {which} of {path}")
         return RefProfile(self, h)
 
     def iid(self, seed, n, L):
@@ -170,12 +195,13 @@ class RefProfile:
             pass
 
     def byte_profile(self):
-        rbv = np.zeros(29 * self.M, np.uint8)
+        kp = self.Kp
+        rbv = np.zeros(kp * self.M, np.uint8)
         self.lib.ref_profile_rbv(self.h, _p(rbv))
         si = np.zeros(5, np.int32)
         sf = np.zeros(3, np.float32)
         self.lib.ref_profile_scalars(self.h, _p(si), _p(sf))
-        return {"rbv": rbv.reshape(29, self.M), "tbm": int(si[0]), "tec": int(si[1]),
+        return {"rbv": rbv.reshape(kp, self.M), "tbm": int(si[0]), "tec": int(si[1]),
                 "base": int(si[3]), "bias": int(si[4]), "scale": float(sf[0]),
                 "mu": float(sf[1]), "lambda": float(sf[2])}
 
@@ -186,12 +212,23 @@ class RefProfile:
         return mat.reshape(self.M + 1, 20), t.reshape(self.M + 1, 7)
 
     def msc(self):
-        out = np.zeros(29 * self.M, np.float32)
+        out = np.zeros(self.Kp * self.M, np.float32)
         self.lib.ref_profile_msc(self.h, _p(out))
-        return out.reshape(29, self.M)
+        return out.reshape(self.Kp, self.M)
 
     def tjb(self, L):
         return self.lib.ref_tjb(self.h, L)
+
+    @property
+    def Kp(self):
+        return self.lib.ref_profile_Kp(self.h)
+
+    def digitize(self, text):
+        out = np.zeros(len(text) + 1, np.uint8)
+        n = self.lib.ref_digitize(self.h, text.encode(), _p(out))
+        if n < 0:
+            raise ValueError("text has characters outside the alphabet")
+        return out[:n]
 
     @staticmethod
     def _dsq(x):

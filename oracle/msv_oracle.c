@@ -225,3 +225,54 @@ int orc_msv_batch(const orc_profile *p, const uint8_t *res, const int64_t *off,
   pthread_mutex_destroy(&b.lock);
   return 0;
 }
+
+/* ---- checker for the GPU decomposition (not a reference restatement) ----
+ * The HIP path computes, per pair, Rp = max over all cells of
+ * W(i,k) - B0 with W(i,k) = max(W(i-1,k-1) + bias - rbv[x_i][k], B0) and
+ * B0 = max(0, base - ((tjb + tbm) mod 256)), then reaches the composite's
+ * result from Rp alone unless the J state may fire.  These two functions
+ * state that in plain integers so tests can confirm on the CPU that
+ * decide(jfree(x)) equals orc_msv(x) whenever it does not defer. */
+int orc_jfree_max(const orc_profile *p, const uint8_t *x, int L)
+{
+  int M = p->M, i, k;
+  int tjb = orc_tjb(L);
+  int B0 = p->base_b - ((tjb + p->tbm_b) & 255);
+  int *prev = malloc(sizeof(int) * (M + 1)), *cur = malloc(sizeof(int) * (M + 1)), *t;
+  int best = 0;
+  if (B0 < 0) B0 = 0;
+  for (k = 0; k <= M; k++) prev[k] = B0;
+  for (i = 0; i < L; i++) {
+    cur[0] = B0;
+    for (k = 1; k <= M; k++) {
+      int w = prev[k - 1] + p->bias_b - p->rbv[x[i] * M + k - 1];
+      if (w < B0) w = B0;
+      cur[k] = w;
+      if (w - B0 > best) best = w - B0;
+    }
+    t = prev; prev = cur; cur = t;
+  }
+  free(prev); free(cur);
+  return best;
+}
+
+/* Returns 0 (eslOK, *xJ set), 16 (eslERANGE) or 19 (defer to the byte MSV). */
+int orc_decide(const orc_profile *p, int Rp, int L, int *xJ)
+{
+  int tjb = orc_tjb(L);
+  int tjbm = (tjb + p->tbm_b) & 255;
+  int B0 = p->base_b - tjbm;
+  int R, guard;
+  if (B0 < 0) B0 = 0;
+  R = Rp + B0;
+  guard = (tjb + p->tbm_b + p->tec_b + p->bias_b) >= 127;
+  if (R >= 255 - p->bias_b) return ORC_eslERANGE;
+  if (R - p->tec_b > p->base_b) return ORC_eslENORESULT;
+  if (guard) {
+    if (Rp == 0) return ORC_eslENORESULT;
+    *xJ = (R - p->tec_b) > 0 ? R - p->tec_b : 0;
+    return ORC_eslOK;
+  }
+  *xJ = R - p->tec_b;
+  return ORC_eslOK;
+}

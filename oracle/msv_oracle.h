@@ -33,4 +33,8 @@ int   orc_pass(float usc, int L, float mu, float lambda, double F1);
 int   orc_msv_batch(const orc_profile *p, const uint8_t *res, const int64_t *off,
                     const int32_t *len, int n, float *usc, int32_t *status, int nthreads);
 
+/* Checker for the GPU path's decomposition (see msv_oracle.c). */
+int orc_jfree_max(const orc_profile *p, const uint8_t *x, int L);
+int orc_decide(const orc_profile *p, int Rp, int L, int *xJ);
+
 #endif

@@ -321,3 +321,37 @@ int64_t ref_core_emit(void *vh, uint32_t seed, int n, uint8_t *out, int64_t cap,
   esl_randomness_Destroy(r);
   return tot;
 }
+
+/* ---- real models from HMMER3 files (p7_hmmfile_OpenE / p7_hmmfile_Read,
+ * src/p7_hmmfile.c), configured as hmmsearch does (p7_ProfileConfig L=100,
+ * p7_LOCAL; cudaHmmsearch.cu:499-502).  The file's STATS lines provide the
+ * calibrated MSV mu and lambda. ---- */
+void *ref_profile_from_file(const char *path, int which)
+{
+  P7_HMMFILE *hfp = NULL;
+  REFPROF *h = calloc(1, sizeof(REFPROF));
+  int i;
+  if (p7_hmmfile_OpenE(path, NULL, &hfp, NULL) != eslOK) { free(h); return NULL; }
+  for (i = 0; i <= which; i++) {
+    if (h->hmm) { p7_hmm_Destroy(h->hmm); h->hmm = NULL; }
+    if (p7_hmmfile_Read(hfp, &h->abc, &h->hmm) != eslOK) { p7_hmmfile_Close(hfp); refprof_free(h); return NULL; }
+  }
+  p7_hmmfile_Close(hfp);
+  if (refprof_configure(h, 100, 0) != 0) { refprof_free(h); return NULL; }
+  return h;
+}
+
+int ref_profile_Kp(void *vh) { return ((REFPROF *) vh)->abc->Kp; }
+
+/* Digitize text with the profile's alphabet (esl_abc_Digitize); out gets
+ * codes for positions 1..n (no sentinels).  Returns n, or -1 on bad input. */
+int ref_digitize(void *vh, const char *text, uint8_t *out)
+{
+  REFPROF *h = vh;
+  int n = (int) strlen(text);
+  ESL_DSQ *dsq = malloc(n + 2);
+  if (esl_abc_Digitize(h->abc, text, dsq) != eslOK) { free(dsq); return -1; }
+  memcpy(out, dsq + 1, n);
+  free(dsq);
+  return n;
+}

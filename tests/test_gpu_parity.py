@@ -1,0 +1,219 @@
+"""GPU parity: the HIP path against the golden vectors of the compiled
+reference and against the oracle (oracle/msv_oracle.c) on seeded workloads.
+Bar: bit-exact usc (float bits), status and F1 pass bit for every pair."""
+import numpy as np
+import pytest
+
+import cudahmmsearch_amd as ca
+from conftest import golden_cases, golden_profile
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = ca.MSVContext(0)
+    yield c
+    c.close()
+
+
+def _bp(p):
+    return ca.ByteProfile(p["rbv"], p["tbm"], p["tec"], p["base"], p["bias"], p["scale"], p["mu"], p["lambda"])
+
+
+def _oracle_pass(O, ust, uusc, lens, mu, lam, F1=0.02):
+    return np.array([1 if ust[i] == 16 else O.passes(uusc[i], lens[i], mu, lam, F1) for i in range(len(lens))],
+                    np.uint8)
+
+
+@pytest.mark.parametrize("mode", [ca.MODE_PLAIN, ca.MODE_BATCHED])
+def test_golden_cases(ctx, golden, mode):
+    """Every golden case (sampled and shipped models, edge lengths, degenerate
+    residues, planted homologs, ERANGE and SSV fall-backs) bit-exact."""
+    z, man = golden
+    nprof = len(man["profiles"])
+    for pi in range(nprof):
+        p = golden_profile(z, man, pi)
+        rows = golden_cases(z, pi)
+        print(f"golden profile {p['name']} M={p['M']} cases={len(rows)} mode={mode}", flush=True)
+        pid = ctx.add_profile(_bp(p))
+        db = ca.Database.from_list([r[2] for r in rows])
+        did = ctx.add_database(db)
+        usc, st, ps = ctx.search([pid], did, mode=mode)
+        idx = np.array([r[0] for r in rows])
+        assert np.array_equal(st[0].astype(np.int32), z["case_status"][idx]), p["name"]
+        assert np.array_equal(usc[0].view(np.uint32), z["case_usc_bits"][idx]), p["name"]
+        assert np.array_equal(ps[0], z["case_pass"][idx]), p["name"]
+        ctx.remove_database(did)
+        ctx.remove_profile(pid)
+
+
+def test_fn3_vs_7less_drome_passes(ctx, golden):
+    """tutorial/fn3.out:110 -- fn3 against 7LESS_DROME: 'Passed MSV filter: 1'."""
+    z, man = golden
+    pi = [m["name"] for m in man["profiles"]].index("fn3")
+    c = man["named_cases"]["7LESS_DROME"]
+    p = golden_profile(z, man, pi)
+    pid = ctx.add_profile(_bp(p))
+    x = z["res"][z["case_off"][c]:z["case_off"][c] + z["case_len"][c]]
+    did = ctx.add_database(ca.Database.from_list([x]))
+    usc, st, ps = ctx.search([pid], did)
+    assert ps[0, 0] == 1
+    ctx.remove_database(did)
+    ctx.remove_profile(pid)
+
+
+def test_calibration_mu_matches_reference(ctx, golden):
+    """GPU p7_MSVMu reproduces p7_Calibrate's mu bit for bit."""
+    z, man = golden
+    for pi, meta in enumerate(man["profiles"]):
+        if meta["seed"] is None or meta["M"] > 2500:
+            continue
+        prof = ca.sample_profile(meta["seed"], meta["M"])
+        p = golden_profile(z, man, pi)
+        assert np.array_equal(prof.rbv, p["rbv"])
+        pid = ctx.add_profile(prof)
+        mu = ctx.calibrate(pid, prof.lam)
+        print(f"calibrate {meta['name']}: mu={mu} ref={p['mu']}", flush=True)
+        assert np.float32(mu).view(np.uint32) == meta["mu_bits"], meta["name"]
+        assert np.float32(prof.lam).view(np.uint32) == meta["lambda_bits"], meta["name"]
+        ctx.remove_profile(pid)
+
+
+def _workload(seed, M, n, median, sigma, lo, hi, planted=0.02, **kw):
+    prof = ca.sample_profile(seed, M)
+    db = ca.gen_database(seed + 1, n, median, sigma, lo, hi, **kw)
+    hom = ca.core_emit(prof.hmm, seed + 2, max(1, int(planted * n)))
+    db, _ = ca.plant(db, hom, seed + 3, planted)
+    return prof, db
+
+
+def _check_vs_oracle(ctx, prof, db, mode, pid=None, did=None, sample=None):
+    O = Oracle()
+    if pid is None:
+        pid = ctx.add_profile(prof)
+        prof.mu = ctx.calibrate(pid, prof.lam)
+    if did is None:
+        did = ctx.add_database(db)
+    usc, st, ps = ctx.search([pid], did, mode=mode)
+    idx = np.arange(db.n) if sample is None else np.sort(np.random.default_rng(0).choice(db.n, sample, replace=False))
+    sub = ca.Database.from_list([db.seq(i) for i in idx])
+    ost, ousc = O.msv_batch(prof.as_dict(), sub.res, sub.off, sub.lens, nthreads=16)
+    assert np.array_equal(st[0, idx].astype(np.int32), ost)
+    assert np.array_equal(usc[0, idx].view(np.uint32), ousc.view(np.uint32))
+    opass = _oracle_pass(O, ost, ousc, sub.lens, prof.mu, prof.lam)
+    assert np.array_equal(ps[0, idx], opass)
+    return pid, did, usc, st, ps
+
+
+@pytest.mark.parametrize("mode", [ca.MODE_PLAIN, ca.MODE_BATCHED])
+def test_plain_profile_vs_oracle(ctx, mode):
+    """M = 400 against 4000 log-normal targets with 2% planted homologs."""
+    prof, db = _workload(3, 400, 4000, 350, 0.7, 20, 10000)
+    print(f"M=400 n={db.n} residues={db.residues} mode={mode}", flush=True)
+    pid, did, *_ = _check_vs_oracle(ctx, prof, db, mode)
+    ctx.remove_database(did)
+    ctx.remove_profile(pid)
+
+
+@pytest.mark.parametrize("M", [1, 7, 33, 64, 65, 128, 129, 300, 513, 1024, 1025, 2049, 4096])
+def test_model_lengths_vs_oracle(ctx, M):
+    """Tile boundaries of every lane-group size (16/32/64 lanes, Q = 1..32)."""
+    prof, db = _workload(100 + M, M, 400, 300, 0.8, 1, 3000)
+    print(f"M={M} n={db.n}", flush=True)
+    pid, did, *_ = _check_vs_oracle(ctx, prof, db, ca.MODE_BATCHED)
+    usc2, st2, ps2 = ctx.search([pid], did, mode=ca.MODE_PLAIN)
+    usc, st, ps = ctx.search([pid], did, mode=ca.MODE_BATCHED)
+    assert np.array_equal(usc.view(np.uint32), usc2.view(np.uint32))
+    ctx.remove_database(did)
+    ctx.remove_profile(pid)
+
+
+@pytest.mark.parametrize("M", [4097, 6000, 10000])
+def test_long_profiles_vs_oracle(ctx, M):
+    """Column-segmented path (tables beyond LDS)."""
+    prof, db = _workload(200 + M, M, 60, 1200, 0.6, 100, 6000, planted=0.05)
+    print(f"long M={M} n={db.n}", flush=True)
+    pid, did, *_ = _check_vs_oracle(ctx, prof, db, ca.MODE_BATCHED)
+    ctx.remove_database(did)
+    ctx.remove_profile(pid)
+
+
+def test_batched_multi_profile(ctx):
+    """Eight profiles of uneven M against length-skewed targets: batched equals
+    plain equals the oracle on a sample."""
+    seeds = list(range(10, 18))
+    Ms = [50, 180, 420, 777, 1200, 1800, 2300, 2500]
+    profs = [ca.sample_profile(s, M) for s, M in zip(seeds, Ms)]
+    db = ca.gen_database(5, 3000, 300, 0.6, 20, 5000, tail_frac=0.02, tail_xm=1000, tail_alpha=1.5, tail_max=12000)
+    hom = ca.core_emit(profs[2].hmm, 9, 60)
+    db, _ = ca.plant(db, hom, 11, 0.02)
+    pids = []
+    for p in profs:
+        pid = ctx.add_profile(p)
+        p.mu = ctx.calibrate(pid, p.lam)
+        pids.append(pid)
+    did = ctx.add_database(db)
+    ub, sb, pb = ctx.search(pids, did, mode=ca.MODE_BATCHED)
+    up, sp, pp = ctx.search(pids, did, mode=ca.MODE_PLAIN)
+    assert np.array_equal(ub.view(np.uint32), up.view(np.uint32))
+    assert np.array_equal(sb, sp) and np.array_equal(pb, pp)
+    O = Oracle()
+    idx = np.sort(np.random.default_rng(1).choice(db.n, 150, replace=False))
+    sub = ca.Database.from_list([db.seq(i) for i in idx])
+    for j, p in enumerate(profs):
+        ost, ousc = O.msv_batch(p.as_dict(), sub.res, sub.off, sub.lens, nthreads=16)
+        assert np.array_equal(sb[j, idx].astype(np.int32), ost), p.M
+        assert np.array_equal(ub[j, idx].view(np.uint32), ousc.view(np.uint32)), p.M
+        assert np.array_equal(pb[j, idx], _oracle_pass(O, ost, ousc, sub.lens, p.mu, p.lam)), p.M
+    # determinism: a second batched run (graph replay) is identical
+    ub2, sb2, pb2 = ctx.search(pids, did, mode=ca.MODE_BATCHED)
+    assert np.array_equal(ub.view(np.uint32), ub2.view(np.uint32))
+    ctx.remove_database(did)
+    for pid in pids:
+        ctx.remove_profile(pid)
+
+
+def test_dropin_msvfilter(ctx, golden):
+    """msvg_MSVFilter, the p7_MSVFilter signature, on golden cases."""
+    z, man = golden
+    for pi in (0, 4, 13):
+        p = golden_profile(z, man, pi)
+        pid = ctx.add_profile(_bp(p))
+        for c, _, x in golden_cases(z, pi)[:12]:
+            st, usc = ctx.msv_filter(pid, x)
+            assert st == z["case_status"][c]
+            assert np.float32(usc).view(np.uint32) == z["case_usc_bits"][c]
+        ctx.remove_profile(pid)
+
+
+def test_empty_and_zero_length(ctx):
+    prof = ca.sample_profile(3, 400)
+    pid = ctx.add_profile(prof)
+    prof.mu = ctx.calibrate(pid, prof.lam)
+    did = ctx.add_database(ca.Database.from_list([]))
+    usc, st, ps = ctx.search([pid], did)
+    assert usc.shape == (1, 0)
+    ctx.remove_database(did)
+    db = ca.Database.from_list([np.zeros(0, np.uint8), np.array([3], np.uint8), np.zeros(0, np.uint8)])
+    _check_vs_oracle(ctx, prof, db, ca.MODE_BATCHED, pid=pid)
+    ctx.remove_profile(pid)
+
+
+def test_order_invariance(ctx):
+    """Permuting the database permutes the results (sorting is internal)."""
+    prof, db = _workload(21, 300, 1500, 300, 0.8, 10, 4000)
+    pid = ctx.add_profile(prof)
+    prof.mu = ctx.calibrate(pid, prof.lam)
+    did = ctx.add_database(db)
+    u1, s1, p1 = ctx.search([pid], did)
+    perm = np.random.default_rng(3).permutation(db.n)
+    db2 = ca.Database.from_list([db.seq(i) for i in perm])
+    did2 = ctx.add_database(db2)
+    u2, s2, p2 = ctx.search([pid], did2)
+    assert np.array_equal(u1[0, perm].view(np.uint32), u2[0].view(np.uint32))
+    assert np.array_equal(p1[0, perm], p2[0])
+    ctx.remove_database(did)
+    ctx.remove_database(did2)
+    ctx.remove_profile(pid)

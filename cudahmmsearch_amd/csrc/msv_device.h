@@ -26,7 +26,9 @@
 #define MSVG_NULLCOST 255
 
 // Per-pair result word written by the fast and slow stages:
-//   bits 0..7 xJ, bits 8..9 code.
+//   bits 0..7 xJ, bits 8..9 code.  A deferred pair (RES_SLOW) carries bit 0
+//   = 1 when only the J state is in doubt (the J-aware f16 stage resolves it)
+//   and 0 when the exact sub-B0 maximum is needed (byte stage).
 enum { RES_OK = 0, RES_ERANGE = 1, RES_SLOW = 2 };
 
 struct SlowItem {          // one pair that needs the exact byte MSV
@@ -66,6 +68,20 @@ struct FastArgs {
   SlowItem *slow;
   int32_t *slow_count;
   int32_t slow_cap;
+  SlowItem *jslow;             // this profile's J-aware queue (null: use the byte queue)
+  int32_t *jslow_count;
+  int32_t jslow_cap;
+};
+
+// J-aware stage: the MSV recurrence with per-row E/J/B updates in the f16
+// domain of the fast stage, one target per wave, for one profile's queue.
+struct JArgs {
+  ProfDev prof;
+  DbDev db;
+  const uint4 *tab;            // the profile's fast table for 64-lane groups
+  const SlowItem *items;
+  const int32_t *count;
+  int32_t cap;
 };
 
 struct SlowArgs {

@@ -19,6 +19,8 @@
 // per row through a DPP lane shift (wave_shr:1 / row_shr:1) plus v_alignbit.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
+#include <utility>
 #include "msv_device.h"
 
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
@@ -68,6 +70,42 @@ __device__ __forceinline__ uint32_t byte_of(const uint4 &c, int rr) {
   return (w >> (8 * (rr & 3))) & 0xffu;
 }
 
+// v_writelane_b32: lane `lane` of dst takes the wave-uniform value val.
+template <int lane>
+__device__ __forceinline__ uint32_t write_lane(uint32_t dst, uint32_t val) {
+  asm("v_writelane_b32 %0, %1, %2" : "+v"(dst) : "s"(val), "i"(lane));
+  return dst;
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [I0, N).
+template <int I0, int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (I0 < N) {
+    f(std::integral_constant<int, I0>{});
+    static_for<I0 + 1, N>(f);
+  }
+}
+
+// Ends a DP row: the row's cells are materialized here and nothing of the
+// next row is computed before (stops the compiler from interleaving unrolled
+// rows along diagonals, which keeps sixteen rows of cells live at once).
+template <int Q>
+__device__ __forceinline__ void row_fence(uint32_t (&r)[Q]) {
+#pragma unroll
+  for (int j = 0; j < Q; ++j) asm volatile("" : "+v"(r[j]));
+}
+
+// The QB uint4 table entries of residue row x for this lane.
+template <int QB>
+__device__ __forceinline__ void load_row(const uint4 *tl, uint32_t x, uint32_t (&e)[QB * 4]) {
+  const uint4 *tp = tl + x * (QB * 64);
+#pragma unroll
+  for (int jb = 0; jb < QB; ++jb) {
+    const uint4 v = tp[jb * 64];
+    e[4 * jb + 0] = v.x; e[4 * jb + 1] = v.y; e[4 * jb + 2] = v.z; e[4 * jb + 3] = v.w;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -89,12 +127,24 @@ __device__ __forceinline__ uint32_t decide(int Rp, int tjb, const ProfDev &pd) {
   const bool guard = (tjb + pd.tbm + pd.tec + pd.bias) >= 127;
   if (pd.slow_all) return RES_SLOW << 8;
   if (R >= 255 - pd.bias) return RES_ERANGE << 8;
-  if (R - pd.tec > pd.base) return RES_SLOW << 8;
+  if (R - pd.tec > pd.base) return (RES_SLOW << 8) | 1u;  // only J is in doubt
   if (guard) {
     if (Rp == 0) return RES_SLOW << 8;                // exact sub-B0 maximum unknown
     return (uint32_t) max(0, R - pd.tec);
   }
   return (uint32_t) (R - pd.tec);
+}
+
+// Queue a deferred pair: J-only doubts go to the profile's J-aware queue when
+// it has one, everything else to the byte-MSV queue.
+__device__ __forceinline__ void defer(uint32_t w, int prof_idx, int64_t s, SlowItem *slow, int32_t *slow_count,
+                                      int32_t slow_cap, SlowItem *jslow, int32_t *jslow_count, int32_t jslow_cap) {
+  if ((w & 1u) && jslow) {
+    const int k = atomicAdd(jslow_count, 1);
+    if (k < jslow_cap) { jslow[k] = SlowItem{prof_idx, (int32_t) s}; return; }
+  }
+  const int k = atomicAdd(slow_count, 1);
+  if (k < slow_cap) slow[k] = SlowItem{prof_idx, (int32_t) s};
 }
 
 template <int G, int Q>
@@ -121,9 +171,13 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
   for (int j = 0; j < Q; ++j) r[j] = 0u;
   uint32_t acc = 0u;
 
+  // Residues arrive 16 at a time; the next group is loaded while the current
+  // one is processed.
+  uint4 cnext = *reinterpret_cast<const uint4 *>(p);
   for (int i0 = 0; i0 < Lmax; i0 += 16) {
-    uint4 c = *reinterpret_cast<const uint4 *>(p + (i0 < Lpad ? i0 : 0));
-    if (i0 >= Lpad) c = null4;
+    uint4 c = (i0 < Lpad) ? cnext : null4;
+    const int i1 = i0 + 16;
+    cnext = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
       const uint32_t x = byte_of(c, rr);
@@ -142,6 +196,8 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
 #pragma unroll
       for (; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);
       if constexpr ((Q & 1) != 0) acc = max2h(acc, r[Q - 1]);
+      row_fence<Q>(r);
+      asm volatile("" : "+v"(acc));
     }
   }
 
@@ -154,10 +210,8 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
     const int Rp = (int) (m * 2048.0f + 0.5f);
     const uint32_t w = decide(Rp, a.db.tjb[s], a.prof);
     a.prof.res[s] = (uint16_t) w;
-    if ((w >> 8) == RES_SLOW) {
-      const int k = atomicAdd(a.slow_count, 1);
-      if (k < a.slow_cap) a.slow[k] = SlowItem{a.prof_idx, (int32_t) s};
-    }
+    if ((w >> 8) == RES_SLOW)
+      defer(w, a.prof_idx, s, a.slow, a.slow_count, a.slow_cap, a.jslow, a.jslow_count, a.jslow_cap);
   }
 }
 
@@ -218,27 +272,28 @@ __global__ void __launch_bounds__(1024) msv_seg(SegArgs a) {
     uint32_t acc = 0u;
     uint32_t cin_last = 0u;          // carry entering row i0 (value of row i0-1)
 
+    // Rows are pipelined by hand: the table entries of row i+1 are read while
+    // row i is computed, and no other reordering across rows is allowed
+    // (the compiler would otherwise hoist all sixteen rows' reads).
+    uint4 c = *reinterpret_cast<const uint4 *>(p);
+    uint32_t e[QB * 4];
+    load_row<QB>(tl, byte_of(c, 0), e);
     for (int i0 = 0; i0 < Lpad; i0 += 16) {
-      const uint4 c = *reinterpret_cast<const uint4 *>(p + i0);
+      const int i1 = i0 + 16;
+      const uint4 cn = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
       uint32_t cin = 0u;
       if (!a.first && lane < 16) cin = cy[i0 + lane];
       uint32_t cout = 0u;
-#pragma unroll
-      for (int rr = 0; rr < 16; ++rr) {
-        const uint32_t x = byte_of(c, rr);
-        const uint4 *tp = tl + x * (QB * 64);
-        uint32_t e[QB * 4];
-#pragma unroll
-        for (int jb = 0; jb < QB; ++jb) {
-          const uint4 v = tp[jb * 64];
-          e[4 * jb + 0] = v.x; e[4 * jb + 1] = v.y; e[4 * jb + 2] = v.z; e[4 * jb + 3] = v.w;
-        }
+      static_for<0, 16>([&](auto rrc) {
+        constexpr int rr = decltype(rrc)::value;
+        uint32_t en[QB * 4];
+        load_row<QB>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
         const uint32_t last = r[Q - 1];
         uint32_t prev = __builtin_amdgcn_update_dpp(0u, last, 0x138, 0xf, 0xf, true);
-        if (!a.first) {
-          const uint32_t ci = (rr == 0) ? cin_last : (uint32_t) __builtin_amdgcn_readlane((int) cin, rr - 1);
-          prev = (lane == 0) ? (ci << 16) : prev;
-        }
+        // carry from the previous segment (all zero in the first segment,
+        // which is what the lane shift supplies there anyway)
+        const uint32_t ci = (rr == 0) ? cin_last : (uint32_t) __builtin_amdgcn_readlane((int) cin, rr == 0 ? 0 : rr - 1);
+        prev = write_lane<0>(prev, ci << 16);
         const uint32_t sh = __builtin_amdgcn_alignbit(last, prev, 16);
 #pragma unroll
         for (int j = Q - 1; j >= 1; --j) r[j] = cell(r[j - 1], e[j]);
@@ -247,13 +302,16 @@ __global__ void __launch_bounds__(1024) msv_seg(SegArgs a) {
 #pragma unroll
         for (; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);
         if constexpr ((Q & 1) != 0) acc = max2h(acc, r[Q - 1]);
-        if (!a.last) {
-          const uint32_t co = ((uint32_t) __builtin_amdgcn_readlane((int) r[Q - 1], 63)) >> 16;
-          cout = (lane == rr) ? co : cout;
-        }
-      }
-      if (!a.first) cin_last = (uint32_t) __builtin_amdgcn_readlane((int) cin, 15);
+        const uint32_t co = ((uint32_t) __builtin_amdgcn_readlane((int) r[Q - 1], 63)) >> 16;
+        cout = write_lane<rr>(cout, co);
+#pragma unroll
+        for (int k = 0; k < QB * 4; ++k) e[k] = en[k];
+        row_fence<Q>(r);
+        asm volatile("" : "+v"(acc));
+      });
+      cin_last = (uint32_t) __builtin_amdgcn_readlane((int) cin, 15);
       if (!a.last && lane < 16) cy[i0 + lane] = (uint16_t) cout;
+      c = cn;
     }
 
     const h2v ah = as_h2(acc);
@@ -268,12 +326,97 @@ __global__ void __launch_bounds__(1024) msv_seg(SegArgs a) {
       } else {
         const uint32_t w = decide(Rp, a.db.tjb[s], a.prof);
         a.prof.res[s] = (uint16_t) w;
-        if ((w >> 8) == RES_SLOW) {
-          const int k = atomicAdd(a.slow_count, 1);
-          if (k < a.slow_cap) a.slow[k] = SlowItem{a.prof_idx, (int32_t) s};
+        if ((w >> 8) == RES_SLOW)
+          defer(w, a.prof_idx, s, a.slow, a.slow_count, a.slow_cap, nullptr, nullptr, 0);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 2a: J-aware pass for pairs whose J state may fire (msvfilter.c:
+// 137-203 with exact integers).  Same f16 cell domain as the fast stage
+// (W - B0 scaled by 2^-11, floor 0), plus per row: max with the current
+// begin score B_i - B0, the wave's row maximum, the overflow test and the
+// E -> J -> B update.  Cells below B0 are stored as B0; that changes neither
+// max(M, xB) (xB >= B0) nor any row maximum that can move J past base, and
+// the pairs queued here have final xJ > base, so the result is exact.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_max_f32(float v) {
+  // Row-wise DPP reduction, then across rows; lane 63 ends with the maximum.
+  v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, true)));
+  v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, true)));
+  v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, true)));
+  v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, true)));
+  v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x142, 0xa, 0xf, false)));
+  v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x143, 0xc, 0xf, false)));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
+template <int Q>
+__global__ void __launch_bounds__(256) msv_jrow(JArgs a) {
+  extern __shared__ uint4 lds[];
+  constexpr int QB = (Q + 3) / 4;
+  const int n = min(*a.count, a.cap);
+  if (n == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int ntab = (a.db.R + 1) * QB * 64;
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = a.tab[i];
+  __syncthreads();
+  const uint4 *tl = lds + lane;
+  const ProfDev &pd = a.prof;
+
+  for (int it = blockIdx.x * nwaves + wave; it < n; it += gridDim.x * nwaves) {
+    const int64_t s = a.items[it].seq;
+    const int L = a.db.len[s];
+    const uint8_t *p = a.db.res + a.db.off[s];
+    const int tjb = a.db.tjb[s];
+    const int tjbm = (tjb + pd.tbm) & 255;                       // msvfilter.c:120
+    const int B0 = max(0, pd.base - tjbm);                       // :124
+    int xJ = 0, code = RES_OK;
+    uint32_t bv = 0u;                                            // (xB - B0) / 2048 in both halves
+    uint32_t r[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) r[j] = 0u;
+
+    for (int i0 = 0; i0 < L && code == RES_OK; i0 += 16) {
+      const uint4 c = *reinterpret_cast<const uint4 *>(p + i0);
+      const int nr = min(16, L - i0);
+#pragma unroll
+      for (int rr = 0; rr < 16; ++rr) {
+        if (rr < nr && code == RES_OK) {
+          const uint32_t x = byte_of(c, rr);
+          const uint4 *tp = tl + x * (QB * 64);
+          uint32_t e[QB * 4];
+#pragma unroll
+          for (int jb = 0; jb < QB; ++jb) {
+            const uint4 v = tp[jb * 64];
+            e[4 * jb + 0] = v.x; e[4 * jb + 1] = v.y; e[4 * jb + 2] = v.z; e[4 * jb + 3] = v.w;
+          }
+          const uint32_t sh = shift_slot<64>(r[Q - 1], lane);
+#pragma unroll
+          for (int j = Q - 1; j >= 1; --j) r[j] = cell(max2h(r[j - 1], bv), e[j]);   // :150-152
+          r[0] = cell(max2h(sh, bv), e[0]);
+          uint32_t rm = 0u;
+          int j = 0;
+#pragma unroll
+          for (; j + 1 < Q; j += 2) rm = max3h(rm, r[j], r[j + 1]);                  // :153
+          if constexpr ((Q & 1) != 0) rm = max2h(rm, r[Q - 1]);
+          const h2v rh = as_h2(rm);
+          const float m = wave_max_f32(fmaxf((float) rh.x, (float) rh.y));
+          const int xE = (int) (m * 2048.0f + 0.5f) + B0;
+          if (xE + pd.bias >= 255) { code = RES_ERANGE; break; }                     // :160-185
+          xJ = max(xJ, max(0, xE - pd.tec));                                         // :187-188
+          const int xB = max(0, max(pd.base, xJ) - tjbm);                            // :190-191
+          const _Float16 bh = (_Float16) ((float) (xB - B0) * (1.0f / 2048.0f));
+          const uint32_t b16 = __builtin_bit_cast(unsigned short, bh);
+          bv = b16 | (b16 << 16);
         }
       }
     }
+    if (lane == 0) pd.res[s] = (uint16_t) (code == RES_ERANGE ? (RES_ERANGE << 8) : xJ);
   }
 }
 
@@ -537,9 +680,39 @@ extern "C" hipError_t msvk_build_seg_table(const uint8_t *rbv, int M, const uint
   return hipGetLastError();
 }
 
+template <int Q>
+static hipError_t launch_jrow_t(const JArgs &a, int grid, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((msv_jrow<Q>), dim3(grid), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
+template <int... Qs>
+static hipError_t dispatch_jrow(int q, const JArgs &a, int grid, size_t lds, hipStream_t st,
+                                std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipErrorInvalidValue;
+  (void) ((q == Qs ? (err = launch_jrow_t<Qs>(a, grid, lds, st), true) : false) || ...);
+  return err;
+}
+template <int Q>
+static hipError_t set_lds_jrow_t() {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&msv_jrow<Q>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+template <int... Qs>
+static hipError_t set_lds_jrow(std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipSuccess;
+  ((err = (err == hipSuccess ? set_lds_jrow_t<Qs>() : err)), ...);
+  return err;
+}
+
+extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st) {
+  if (Q < 1 || Q > 32) return hipErrorInvalidValue;
+  return dispatch_jrow(Q, *a, grid, lds, st, MSVG_QSEQ{});
+}
+
 extern "C" hipError_t msvk_prepare() {
   hipError_t e;
   if ((e = set_lds_seg(MSVG_SEGQ{})) != hipSuccess) return e;
+  if ((e = set_lds_jrow(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_all<16>(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_all<32>(MSVG_QSEQ{})) != hipSuccess) return e;
   return set_lds_all<64>(MSVG_QSEQ{});

@@ -30,6 +30,7 @@ extern "C" hipError_t msvk_prepare();
 extern "C" hipError_t msvk_launch_fast(int G, int Q, const FastArgs *a, int grid, int block, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_seg(int Q, const SegArgs *a, int grid, int block, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st);
+extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st);
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
                                             int Q, int bias, uint32_t *out, hipStream_t st);
@@ -99,8 +100,9 @@ struct Tables {                          // per (profile, db)
 struct Tile { int lo, hi, lanes, waves; };
 
 struct Bufs {                             // device buffers one search writes
-  DevMem d_res, d_prof, d_slow, d_counters, d_usc, d_status, d_pass, d_thr, d_usctab, d_seg;
+  DevMem d_res, d_prof, d_slow, d_counters, d_usc, d_status, d_pass, d_thr, d_usctab, d_seg, d_jslow;
   int64_t slow_cap = 0;
+  int64_t jcap = 0;                      // per-profile J-aware queue capacity
 };
 
 struct Plan {                             // a captured batched search with its own buffers
@@ -347,6 +349,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   int rc;
   std::vector<ProfDev> pds(np);
   std::vector<int> Gs(np), waves(np), segmented(np);
+  std::vector<const uint4 *> jtab(np);
   int64_t cells = 0;
   for (int i = 0; i < np; ++i) {
     Profile &p = *c->profiles[ids[i]];
@@ -367,6 +370,13 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     }
     const uint32_t *st;
     if ((rc = ensure_slow_table(c, ids[i], dbid, &st)) != MSVG_OK) return rc;
+    const int Q64 = qfor(p.M, 128);
+    jtab[i] = nullptr;
+    if (Q64 <= kQMax && fast_lds(d.R, Q64) <= (size_t) kLdsBytes) {
+      const uint32_t *jt;
+      if ((rc = ensure_fast_table(c, ids[i], dbid, 64, &jt)) != MSVG_OK) return rc;
+      jtab[i] = reinterpret_cast<const uint4 *>(jt);
+    }
     pd.stab = st;
     pd.M = p.M; pd.Q = qfor(p.M, 2 * G); pd.QB = (pd.Q + 3) / 4; pd.Qs = qfor(p.M, 128);
     pd.tbm = p.tbm; pd.tec = p.tec; pd.base = p.base; pd.bias = p.bias;
@@ -383,7 +393,9 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   if (c->slow_cap < std::max<int64_t>(4096, (int64_t) np * n / 32)) c->slow_cap = std::max<int64_t>(4096, (int64_t) np * n / 32);
   if (!capturing) b.slow_cap = c->slow_cap;
   HIPCHK(c, b.d_slow.ensure((size_t) b.slow_cap * sizeof(SlowItem)));
-  HIPCHK(c, b.d_counters.ensure((size_t) (np + 2) * sizeof(int32_t)));
+  HIPCHK(c, b.d_counters.ensure((size_t) (2 * np + 2) * sizeof(int32_t)));
+  if (!capturing) b.jcap = std::max<int64_t>(1024, n / 64);
+  HIPCHK(c, b.d_jslow.ensure((size_t) np * b.jcap * sizeof(SlowItem)));
   HIPCHK(c, b.d_usc.ensure((size_t) np * n * sizeof(float)));
   HIPCHK(c, b.d_status.ensure((size_t) np * n));
   HIPCHK(c, b.d_pass.ensure((size_t) np * n));
@@ -411,7 +423,8 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   int32_t *slow_count = counters + np;
 
   if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  HIPCHK(c, msvk_fill_zero(counters, np + 2, c->stream));
+  HIPCHK(c, msvk_fill_zero(counters, 2 * np + 2, c->stream));
+  int32_t *jcounts = counters + np + 1;
 
   for (int i = 0; i < np; ++i) {
     const Profile &p = *c->profiles[ids[i]];
@@ -451,6 +464,9 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     a.slow_count = slow_count;
     a.slow_cap = (int32_t) b.slow_cap;
     a.counter = counters + i;
+    a.jslow = jtab[i] ? b.d_jslow.as<SlowItem>() + (size_t) i * b.jcap : nullptr;
+    a.jslow_count = jcounts + i;
+    a.jslow_cap = (int32_t) b.jcap;
     const int G = Gs[i];
     const size_t lds = fast_lds(d.R, pds[i].Q);
     int grid, block;
@@ -471,6 +487,19 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       grid = c->ncu * (int) std::max<size_t>(1, std::min<size_t>(8, kLdsBytes / lds));
     }
     HIPCHK(c, msvk_launch_fast(G, pds[i].Q, &a, grid, block, lds, c->stream));
+    if (jtab[i]) {
+      JArgs ja;
+      ja.prof = pds[i];
+      ja.db = dd;
+      ja.tab = jtab[i];
+      ja.items = a.jslow;
+      ja.count = a.jslow_count;
+      ja.cap = a.jslow_cap;
+      const int Q64 = qfor(p.M, 128);
+      const size_t jl = fast_lds(d.R, Q64);
+      const int per_cu = (int) std::max<size_t>(1, std::min<size_t>(4, kLdsBytes / jl));
+      HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jl, c->stream));
+    }
   }
   if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
 
@@ -501,11 +530,15 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
 
 int after_search(msvg_ctx *c, Bufs &b, int np, int dbid, int64_t cells, bool timed) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  int32_t cnt = 0;
-  HIPCHK(c, hipMemcpy(&cnt, b.d_counters.as<int32_t>() + np, sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<int32_t> cn(2 * np + 2);
+  HIPCHK(c, hipMemcpy(cn.data(), b.d_counters.p, cn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int32_t cnt = cn[np];
+  int64_t jtot = 0;
+  for (int i = 0; i < np; ++i) jtot += std::min<int64_t>(cn[np + 1 + i], b.jcap);
   c->last = &b;
   c->stats.cells = cells;
   c->stats.n_slow = cnt;
+  c->stats.n_jrow = jtot;
   if (timed) {
     float a = 0, b = 0, f = 0, t = 0;
     HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));

@@ -67,6 +67,7 @@ typedef struct {
   double  ms_total;        /* first launch start to last launch end */
   int64_t cells;           /* sum over pairs of M * L */
   int64_t n_slow;          /* pairs that ran the exact byte MSV */
+  int64_t n_jrow;          /* pairs that ran the J-aware f16 stage */
   int32_t n_fast_launches;
 } msvg_stats;
 

@@ -63,7 +63,7 @@ struct FastArgs {
   int32_t *counter;            // dynamic mode: next chunk (zeroed before the launch)
   const int32_t *order;        // static mode: task t -> sorted target index
   int32_t ntasks;              // static mode: number of tasks
-  int32_t mode;                // 0 dynamic chunks of sorted targets, 1 static round-robin
+  int32_t mode;                // 0 dynamic chunks, 1 static round-robin, 2 one chunk per workgroup
   int32_t prof_idx;
   SlowItem *slow;
   int32_t *slow_count;

@@ -238,6 +238,15 @@ __global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
       const int t1 = a.chunk_start[c + 1];
       for (int t = a.chunk_start[c]; t < t1; ++t) fast_task<G, Q>(a, lds, (int64_t) t * NG + g, lane);
     }
+  } else if (a.mode == 2) {
+    // One chunk per workgroup, waves interleaved over its tasks: the launch
+    // is not persistent, so workgroups of concurrent searches (one per
+    // stream) share the CUs as they come and go.
+    const int c = blockIdx.x;
+    if (c < a.nchunks) {
+      const int t1 = a.chunk_start[c + 1];
+      for (int t = a.chunk_start[c] + wave; t < t1; t += nwaves) fast_task<G, Q>(a, lds, (int64_t) t * NG + g, lane);
+    }
   } else {
     for (int t = blockIdx.x * nwaves + wave; t < a.ntasks; t += gridDim.x * nwaves)
       fast_task<G, Q>(a, lds, a.order[t], lane);
@@ -597,6 +606,18 @@ __global__ void build_slow_table(const uint8_t *rbv, int M, const uint8_t *rowco
   }
 }
 
+// Passing targets of one profile, compacted: index (caller order) and usc.
+// The order of the output is unspecified; the host sorts it.
+__global__ void __launch_bounds__(256) msv_compact(const uint8_t *pass, const float *usc, int64_t n,
+                                                   int32_t *count, int32_t *idx, float *out_usc, int32_t cap) {
+  for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t) gridDim.x * blockDim.x) {
+    if (pass[t]) {
+      const int k = atomicAdd(count, 1);
+      if (k < cap) { idx[k] = (int32_t) t; out_usc[k] = usc[t]; }
+    }
+  }
+}
+
 // Zero n int32 words (used inside captured graphs in place of a memset node).
 __global__ void fill_zero_i32(int32_t *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -758,6 +779,15 @@ extern "C" hipError_t msvk_build_slow_table(const uint8_t *rbv, int M, const uin
   int grid = (int) ((total + 255) / 256);
   if (grid > 65535) grid = 65535;
   hipLaunchKernelGGL(build_slow_table, dim3(grid), dim3(256), 0, st, rbv, M, rowcode, R, Qs, out);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t msvk_compact(const uint8_t *pass, const float *usc, int64_t n, int32_t *count, int32_t *idx,
+                                   float *out_usc, int32_t cap, hipStream_t st) {
+  int grid = (int) ((n + 255) / 256);
+  if (grid > 8192) grid = 8192;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(msv_compact, dim3(grid), dim3(256), 0, st, pass, usc, n, count, idx, out_usc, cap);
   return hipGetLastError();
 }
 

@@ -12,6 +12,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -270,7 +276,7 @@ Tables &tables_for(msvg_ctx *c, int pid, int dbid) {
   return *t;
 }
 
-int ensure_fast_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **out) {
+int ensure_fast_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **out, hipStream_t st) {
   Profile &p = *c->profiles[pid];
   Db &d = *c->dbs[dbid];
   Tables &t = tables_for(c, pid, dbid);
@@ -280,14 +286,14 @@ int ensure_fast_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **ou
     auto mem = std::make_unique<DevMem>();
     HIPCHK(c, mem->ensure(fast_lds(d.R, Q)));
     HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, d.d_rowcode.as<uint8_t>(), d.R, G, Q, p.bias,
-                                    mem->as<uint32_t>(), c->stream));
+                                    mem->as<uint32_t>(), st));
     it = t.fast.emplace(G, std::move(mem)).first;
   }
   *out = it->second->as<uint32_t>();
   return MSVG_OK;
 }
 
-int ensure_seg_tables(msvg_ctx *c, int pid, int dbid) {
+int ensure_seg_tables(msvg_ctx *c, int pid, int dbid, hipStream_t st) {
   Profile &p = *c->profiles[pid];
   Db &d = *c->dbs[dbid];
   Tables &t = tables_for(c, pid, dbid);
@@ -299,13 +305,13 @@ int ensure_seg_tables(msvg_ctx *c, int pid, int dbid) {
     auto mem = std::make_unique<DevMem>();
     HIPCHK(c, mem->ensure(fast_lds(d.R, Q)));
     HIPCHK(c, msvk_build_seg_table(p.d_rbv.as<uint8_t>(), p.M, d.d_rowcode.as<uint8_t>(), d.R, Q, k0, p.bias,
-                                   mem->as<uint32_t>(), c->stream));
+                                   mem->as<uint32_t>(), st));
     t.seg.push_back(std::move(mem));
   }
   return MSVG_OK;
 }
 
-int ensure_slow_table(msvg_ctx *c, int pid, int dbid, const uint32_t **out) {
+int ensure_slow_table(msvg_ctx *c, int pid, int dbid, const uint32_t **out, hipStream_t st) {
   Profile &p = *c->profiles[pid];
   Db &d = *c->dbs[dbid];
   Tables &t = tables_for(c, pid, dbid);
@@ -314,7 +320,7 @@ int ensure_slow_table(msvg_ctx *c, int pid, int dbid, const uint32_t **out) {
     auto mem = std::make_unique<DevMem>();
     HIPCHK(c, mem->ensure((size_t) (d.R + 1) * Qs * 64 * 4));
     HIPCHK(c, msvk_build_slow_table(p.d_rbv.as<uint8_t>(), p.M, d.d_rowcode.as<uint8_t>(), d.R, Qs,
-                                    mem->as<uint32_t>(), c->stream));
+                                    mem->as<uint32_t>(), st));
     t.slow = std::move(mem);
   }
   *out = t.slow->as<uint32_t>();
@@ -342,8 +348,11 @@ int check_ids(msvg_ctx *c, const int32_t *ids, int np, int dbid) {
 // synchronized; the warm-up call made just before the capture leaves every
 // table and parameter block in b, and replays of the graph refill them
 // first (Plan upload in search_device).
+// sched: 0 persistent (batched mode), 2 one chunk per workgroup (serving:
+// concurrent searches on separate streams share the CUs).
 int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, double F1, int mode, bool timed,
-                   bool capturing, int64_t *cells_out) {
+                   bool capturing, int64_t *cells_out, hipStream_t st = nullptr, int sched = 0) {
+  if (!st) st = c->stream;
   Db &d = *c->dbs[dbid];
   const int64_t n = d.n;
   int rc;
@@ -363,21 +372,21 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     pd.ftab = nullptr;
     if (!segmented[i]) {
       const uint32_t *ft;
-      if ((rc = ensure_fast_table(c, ids[i], dbid, G, &ft)) != MSVG_OK) return rc;
+      if ((rc = ensure_fast_table(c, ids[i], dbid, G, &ft, st)) != MSVG_OK) return rc;
       pd.ftab = reinterpret_cast<const uint4 *>(ft);
-    } else if ((rc = ensure_seg_tables(c, ids[i], dbid)) != MSVG_OK) {
+    } else if ((rc = ensure_seg_tables(c, ids[i], dbid, st)) != MSVG_OK) {
       return rc;
     }
-    const uint32_t *st;
-    if ((rc = ensure_slow_table(c, ids[i], dbid, &st)) != MSVG_OK) return rc;
+    const uint32_t *stab;
+    if ((rc = ensure_slow_table(c, ids[i], dbid, &stab, st)) != MSVG_OK) return rc;
     const int Q64 = qfor(p.M, 128);
     jtab[i] = nullptr;
     if (Q64 <= kQMax && fast_lds(d.R, Q64) <= (size_t) kLdsBytes) {
       const uint32_t *jt;
-      if ((rc = ensure_fast_table(c, ids[i], dbid, 64, &jt)) != MSVG_OK) return rc;
+      if ((rc = ensure_fast_table(c, ids[i], dbid, 64, &jt, st)) != MSVG_OK) return rc;
       jtab[i] = reinterpret_cast<const uint4 *>(jt);
     }
-    pd.stab = st;
+    pd.stab = stab;
     pd.M = p.M; pd.Q = qfor(p.M, 2 * G); pd.QB = (pd.Q + 3) / 4; pd.Qs = qfor(p.M, 128);
     pd.tbm = p.tbm; pd.tec = p.tec; pd.base = p.base; pd.bias = p.bias;
     pd.slow_all = 0;
@@ -389,7 +398,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   HIPCHK(c, b.d_res.ensure((size_t) np * n * sizeof(uint16_t)));
   for (int i = 0; i < np; ++i) pds[i].res = b.d_res.as<uint16_t>() + (size_t) i * n;
   HIPCHK(c, b.d_prof.ensure(np * sizeof(ProfDev)));
-  if (!capturing) HIPCHK(c, hipMemcpyAsync(b.d_prof.p, pds.data(), np * sizeof(ProfDev), hipMemcpyHostToDevice, c->stream));
+  if (!capturing) HIPCHK(c, hipMemcpyAsync(b.d_prof.p, pds.data(), np * sizeof(ProfDev), hipMemcpyHostToDevice, st));
   if (c->slow_cap < std::max<int64_t>(4096, (int64_t) np * n / 32)) c->slow_cap = std::max<int64_t>(4096, (int64_t) np * n / 32);
   if (!capturing) b.slow_cap = c->slow_cap;
   HIPCHK(c, b.d_slow.ensure((size_t) b.slow_cap * sizeof(SlowItem)));
@@ -411,9 +420,9 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   HIPCHK(c, b.d_thr.ensure(thr.size() * sizeof(uint16_t) + 16));
   HIPCHK(c, b.d_usctab.ensure(utab.size() * sizeof(float)));
   if (!capturing) {
-    HIPCHK(c, hipMemcpyAsync(b.d_thr.p, thr.data(), thr.size() * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b.d_usctab.p, utab.data(), utab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.d_thr.p, thr.data(), thr.size() * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(b.d_usctab.p, utab.data(), utab.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipStreamSynchronize(st));
   }
 
   DbDev dd;
@@ -422,8 +431,8 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   int32_t *counters = b.d_counters.as<int32_t>();
   int32_t *slow_count = counters + np;
 
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  HIPCHK(c, msvk_fill_zero(counters, 2 * np + 2, c->stream));
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[0], st));
+  HIPCHK(c, msvk_fill_zero(counters, 2 * np + 2, st));
   int32_t *jcounts = counters + np + 1;
 
   for (int i = 0; i < np; ++i) {
@@ -452,7 +461,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
         int grid, block;
         persistent_shape(c, lds, 0, grid, block);
         a.ntasks = (int32_t) n;
-        HIPCHK(c, msvk_launch_seg(Q, &a, grid, block, lds, c->stream));
+        HIPCHK(c, msvk_launch_seg(Q, &a, grid, block, lds, st));
       }
       continue;
     }
@@ -473,11 +482,12 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     if (mode == MSVG_MODE_BATCHED) {
       if ((rc = build_chunks(c, d, G)) != MSVG_OK) return rc;
       auto &ch = d.chunks[G];
-      a.mode = 0;
+      a.mode = sched == 2 ? 2 : 0;
       a.chunk_start = ch.first->as<int32_t>();
       a.nchunks = ch.second;
       a.order = nullptr; a.ntasks = 0;
       persistent_shape(c, lds, waves[i], grid, block);
+      if (sched == 2) { grid = std::max(1, ch.second); block = 256; }
     } else {
       a.mode = 1;
       a.chunk_start = nullptr; a.nchunks = 0;
@@ -486,7 +496,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       block = 256;
       grid = c->ncu * (int) std::max<size_t>(1, std::min<size_t>(8, kLdsBytes / lds));
     }
-    HIPCHK(c, msvk_launch_fast(G, pds[i].Q, &a, grid, block, lds, c->stream));
+    HIPCHK(c, msvk_launch_fast(G, pds[i].Q, &a, grid, block, lds, st));
     if (jtab[i]) {
       JArgs ja;
       ja.prof = pds[i];
@@ -498,10 +508,10 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       const int Q64 = qfor(p.M, 128);
       const size_t jl = fast_lds(d.R, Q64);
       const int per_cu = (int) std::max<size_t>(1, std::min<size_t>(4, kLdsBytes / jl));
-      HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jl, c->stream));
+      HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jl, st));
     }
   }
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[1], st));
 
   SlowArgs sa;
   sa.profs = b.d_prof.as<ProfDev>();
@@ -509,8 +519,8 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   sa.items = b.d_slow.as<SlowItem>();
   sa.count = slow_count;
   sa.cap = (int32_t) b.slow_cap;
-  HIPCHK(c, msvk_launch_slow(&sa, c->ncu * 4, c->stream));
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  HIPCHK(c, msvk_launch_slow(&sa, c->ncu * 4, st));
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[2], st));
 
   FinArgs fa;
   fa.profs = b.d_prof.as<ProfDev>();
@@ -523,8 +533,8 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   fa.usc = b.d_usc.as<float>();
   fa.status = b.d_status.as<uint8_t>();
   fa.pass = b.d_pass.as<uint8_t>();
-  HIPCHK(c, msvk_launch_finalize(&fa, np, n, c->stream));
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  HIPCHK(c, msvk_launch_finalize(&fa, np, n, st));
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[3], st));
   return MSVG_OK;
 }
 
@@ -873,6 +883,336 @@ int msvg_MSVFilter(msvg_ctx *c, int32_t pid, const uint8_t *dsq, int L, float *r
   if (rc != MSVG_OK) return rc;
   *ret_sc = usc;
   return st == 16 ? MSVG_ERANGE : MSVG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching over one resident database.
+// ---------------------------------------------------------------------------
+extern "C" hipError_t msvk_compact(const uint8_t *pass, const float *usc, int64_t n, int32_t *count, int32_t *idx,
+                                   float *out_usc, int32_t cap, hipStream_t st);
+
+namespace {
+
+struct ServerQuery {
+  int64_t ticket = 0;
+  msvg_oprofile desc{};
+  std::vector<uint8_t> rbv;
+  double t_submit = 0, t_admit = -1, t_done = -1;
+  int state = 0;                  // 0 queued, 1 on the GPU, 2 done
+  int status = MSVG_OK;
+  int slot = -1;
+  int64_t cells = 0;
+  size_t lds = 0, dev = 0;
+  std::vector<int32_t> idx;
+  std::vector<float> usc;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  Bufs bufs;
+  DevMem d_cidx, d_cusc, d_ccount;
+  int32_t *h_count = nullptr;     // pinned
+  ServerQuery *q = nullptr;
+};
+
+}  // namespace
+
+struct msvg_server {
+  msvg_ctx *ctx = nullptr;
+  int db = -1;
+  double F1 = 0.02;
+  int64_t hbm_budget = 0;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<ServerQuery *> pending;
+  std::map<int64_t, std::unique_ptr<ServerQuery>> queries;
+  std::map<std::string, int> prof_cache;
+  std::thread th;
+  std::atomic<bool> stopping{false};
+  std::chrono::steady_clock::time_point t0;
+  int64_t next_ticket = 1;
+  msvg_server_info st{};
+  double busy_since = -1;
+  size_t lds_in_use = 0, dev_in_use = 0;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+std::string profile_key(const ServerQuery &q) {
+  std::string k(reinterpret_cast<const char *>(q.rbv.data()), q.rbv.size());
+  char buf[128];
+  std::snprintf(buf, sizeof buf, "|%d|%d|%d|%d|%d|%d|%a|%a|%a", q.desc.M, q.desc.Kp, q.desc.tbm_b, q.desc.tec_b,
+                q.desc.base_b, q.desc.bias_b, q.desc.scale_b, q.desc.mmu, q.desc.mlambda);
+  return k + buf;
+}
+
+// LDS a query's fast-stage workgroups hold while it runs.
+size_t query_lds(msvg_ctx *c, const Db &d, int M) {
+  int G = 64, w = 0;
+  choose_tile(c, M, d.R, G, w);
+  const int Q = qfor(M, 2 * G);
+  if (Q > kQMax || fast_lds(d.R, Q) > (size_t) kLdsBytes) return fast_lds(d.R, qfor(std::min(M, 128 * seg_q(d.R)), 128));
+  return fast_lds(d.R, Q);
+}
+
+int server_start_query(msvg_server *s, Slot &sl, ServerQuery &q) {
+  msvg_ctx *c = s->ctx;
+  const Db &d = *c->dbs[s->db];
+  int pid;
+  const std::string key = profile_key(q);
+  auto it = s->prof_cache.find(key);
+  if (it != s->prof_cache.end()) {
+    pid = it->second;
+  } else {
+    q.desc.rbv = q.rbv.data();
+    int32_t id;
+    int rc = msvg_profile_add(c, &q.desc, &id);
+    if (rc != MSVG_OK) return rc;
+    pid = id;
+    s->prof_cache[key] = pid;
+  }
+  int64_t cells = 0;
+  int rc = enqueue_search(c, sl.bufs, &pid, 1, s->db, s->F1, MSVG_MODE_BATCHED, false, false, &cells, sl.stream, 2);
+  if (rc != MSVG_OK) return rc;
+  q.cells = cells;
+  HIPCHK(c, sl.d_ccount.ensure(sizeof(int32_t)));
+  HIPCHK(c, sl.d_cidx.ensure((size_t) std::max<int64_t>(d.n, 1) * sizeof(int32_t)));
+  HIPCHK(c, sl.d_cusc.ensure((size_t) std::max<int64_t>(d.n, 1) * sizeof(float)));
+  HIPCHK(c, msvk_fill_zero(sl.d_ccount.as<int32_t>(), 1, sl.stream));
+  HIPCHK(c, msvk_compact(sl.bufs.d_pass.as<uint8_t>(), sl.bufs.d_usc.as<float>(), d.n, sl.d_ccount.as<int32_t>(),
+                         sl.d_cidx.as<int32_t>(), sl.d_cusc.as<float>(), (int32_t) std::max<int64_t>(d.n, 1),
+                         sl.stream));
+  HIPCHK(c, hipMemcpyAsync(sl.h_count, sl.d_ccount.p, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
+  HIPCHK(c, hipEventRecord(sl.done, sl.stream));
+  return MSVG_OK;
+}
+
+int server_finish_query(msvg_server *s, Slot &sl, ServerQuery &q) {
+  msvg_ctx *c = s->ctx;
+  // Deferred-pair queues must not have overflowed (they are sized n/32 and
+  // n/64 per profile); a search that overflowed reports an error.
+  int32_t cn[4];
+  HIPCHK(c, hipMemcpy(cn, sl.bufs.d_counters.p, 4 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (cn[1] > sl.bufs.slow_cap) return fail(c, MSVG_EMEM, "deferred-pair queue overflow");
+  const int32_t k = *sl.h_count;
+  q.idx.resize(k);
+  q.usc.resize(k);
+  if (k > 0) {
+    HIPCHK(c, hipMemcpy(q.idx.data(), sl.d_cidx.p, (size_t) k * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(q.usc.data(), sl.d_cusc.p, (size_t) k * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<int32_t> ord(k);
+    for (int32_t i = 0; i < k; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return q.idx[a] < q.idx[b]; });
+    std::vector<int32_t> i2(k);
+    std::vector<float> u2(k);
+    for (int32_t i = 0; i < k; ++i) { i2[i] = q.idx[ord[i]]; u2[i] = q.usc[ord[i]]; }
+    q.idx.swap(i2);
+    q.usc.swap(u2);
+  }
+  return MSVG_OK;
+}
+
+void server_loop(msvg_server *s) {
+  msvg_ctx *c = s->ctx;
+  (void) hipSetDevice(c->device);
+  const Db &d = *c->dbs[s->db];
+  for (;;) {
+    bool progressed = false;
+    // Completions.
+    for (auto &slp : s->slots) {
+      Slot &sl = *slp;
+      if (!sl.q) continue;
+      if (hipEventQuery(sl.done) != hipSuccess) continue;
+      ServerQuery *q = sl.q;
+      const int rc = server_finish_query(s, sl, *q);
+      std::lock_guard<std::mutex> lk(s->mu);
+      q->status = rc;
+      q->state = 2;
+      q->t_done = s->now();
+      s->lds_in_use -= q->lds;
+      s->dev_in_use -= q->dev;
+      sl.q = nullptr;
+      s->st.completed++;
+      s->st.running--;
+      if (s->st.running == 0 && s->busy_since >= 0) { s->st.busy_s += q->t_done - s->busy_since; s->busy_since = -1; }
+      s->cv.notify_all();
+      progressed = true;
+    }
+    // Admissions: a free slot, the HBM budget and the LDS budget (all running
+    // queries' workgroups must be able to share a CU) decide.
+    for (;;) {
+      ServerQuery *q = nullptr;
+      Slot *free_slot = nullptr;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stopping || s->pending.empty()) break;
+        for (auto &slp : s->slots)
+          if (!slp->q) { free_slot = slp.get(); break; }
+        if (!free_slot) break;
+        ServerQuery *cand = s->pending.front();
+        cand->lds = query_lds(c, d, cand->desc.M);
+        cand->dev = (size_t) d.n * 16 + (size_t) cand->desc.M * 29 * 8;
+        const bool lds_ok = s->st.running == 0 || s->lds_in_use + cand->lds <= (size_t) kLdsBytes;
+        const bool hbm_ok = s->hbm_budget <= 0 || s->st.running == 0 ||
+                            (int64_t) (s->dev_in_use + cand->dev) <= s->hbm_budget;
+        if (!lds_ok || !hbm_ok) break;
+        s->pending.pop_front();
+        q = cand;
+        q->state = 1;
+        q->slot = (int) (free_slot - s->slots[0].get());
+        q->t_admit = s->now();
+        s->lds_in_use += q->lds;
+        s->dev_in_use += q->dev;
+        s->st.admitted++;
+        s->st.pending--;
+        s->st.running++;
+        if (s->busy_since < 0) s->busy_since = q->t_admit;
+      }
+      free_slot->q = q;
+      const int rc = server_start_query(s, *free_slot, *q);
+      if (rc != MSVG_OK) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        q->status = rc;
+        q->state = 2;
+        q->t_done = s->now();
+        s->lds_in_use -= q->lds;
+        s->dev_in_use -= q->dev;
+        free_slot->q = nullptr;
+        s->st.completed++;
+        s->st.running--;
+        if (s->st.running == 0 && s->busy_since >= 0) { s->st.busy_s += q->t_done - s->busy_since; s->busy_since = -1; }
+        s->cv.notify_all();
+      }
+      progressed = true;
+    }
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (s->stopping && s->st.running == 0) break;
+      if (!progressed) s->cv.wait_for(lk, std::chrono::microseconds(s->st.running ? 50 : 2000));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int msvg_server_start(msvg_ctx *c, int32_t db_id, int32_t slots, double F1, int64_t hbm_budget, msvg_server **ret) {
+  if (!c || !ret) return MSVG_EINVAL;
+  if (!c->dbs.count(db_id)) return fail(c, MSVG_EINVAL, "unknown database id");
+  if (slots < 1 || slots > 16) return fail(c, MSVG_EINVAL, "slots must be 1..16");
+  (void) hipSetDevice(c->device);
+  auto s = std::make_unique<msvg_server>();
+  s->ctx = c;
+  s->db = db_id;
+  s->F1 = F1;
+  s->hbm_budget = hbm_budget;
+  for (int i = 0; i < slots; ++i) {
+    auto sl = std::make_unique<Slot>();
+    HIPCHK(c, hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
+    HIPCHK(c, hipHostMalloc((void **) &sl->h_count, sizeof(int32_t), hipHostMallocDefault));
+    s->slots.push_back(std::move(sl));
+  }
+  s->t0 = std::chrono::steady_clock::now();
+  msvg_server *raw = s.get();
+  s->th = std::thread(server_loop, raw);
+  *ret = s.release();
+  return MSVG_OK;
+}
+
+int msvg_server_submit(msvg_server *s, const msvg_oprofile *om, int64_t *ret_ticket) {
+  if (!s || !om || !om->rbv || !ret_ticket) return MSVG_EINVAL;
+  if (om->M < 1 || om->M > 16384 || om->Kp < 1 || om->Kp > 255 || (int) om->bias_b + (int) om->base_b >= 255)
+    return MSVG_EINVAL;
+  auto q = std::make_unique<ServerQuery>();
+  q->desc = *om;
+  q->rbv.assign(om->rbv, om->rbv + (size_t) om->M * om->Kp);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stopping) return MSVG_EINVAL;
+  q->ticket = s->next_ticket++;
+  q->t_submit = s->now();
+  *ret_ticket = q->ticket;
+  s->pending.push_back(q.get());
+  s->queries[q->ticket] = std::move(q);
+  s->st.submitted++;
+  s->st.pending++;
+  s->st.max_pending = std::max<int64_t>(s->st.max_pending, (int64_t) s->pending.size());
+  s->cv.notify_all();
+  return MSVG_OK;
+}
+
+int msvg_server_wait(msvg_server *s, int64_t ticket, double timeout_s, msvg_query_result *out, int32_t *idx,
+                     float *usc, int64_t cap) {
+  if (!s || !out) return MSVG_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->queries.find(ticket);
+  if (it == s->queries.end()) return MSVG_EINVAL;
+  ServerQuery *q = it->second.get();
+  auto ready = [&] { return q->state == 2; };
+  if (timeout_s < 0) s->cv.wait(lk, ready);
+  else if (!s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
+    out->ticket = ticket;
+    out->t_submit = q->t_submit;
+    out->t_admit = q->t_admit;
+    out->t_done = -1;
+    return MSVG_ENORESULT;
+  }
+  out->ticket = ticket;
+  out->status = q->status;
+  out->M = q->desc.M;
+  out->n_pass = (int64_t) q->idx.size();
+  out->cells = q->cells;
+  out->t_submit = q->t_submit;
+  out->t_admit = q->t_admit;
+  out->t_done = q->t_done;
+  if (idx && cap > 0) {
+    const int64_t k = std::min<int64_t>(cap, (int64_t) q->idx.size());
+    std::copy(q->idx.begin(), q->idx.begin() + k, idx);
+    if (usc) std::copy(q->usc.begin(), q->usc.begin() + k, usc);
+  }
+  s->queries.erase(it);
+  return MSVG_OK;
+}
+
+int msvg_server_stats(msvg_server *s, msvg_server_info *out) {
+  if (!s || !out) return MSVG_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->st;
+  out->uptime_s = s->now();
+  if (s->busy_since >= 0) out->busy_s += out->uptime_s - s->busy_since;
+  return MSVG_OK;
+}
+
+int msvg_server_stop(msvg_server *s) {
+  if (!s) return MSVG_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+    for (ServerQuery *q : s->pending) { q->state = 2; q->status = MSVG_ENORESULT; }
+    s->st.pending = 0;
+    s->pending.clear();
+    s->cv.notify_all();
+  }
+  if (s->th.joinable()) s->th.join();
+  msvg_ctx *c = s->ctx;
+  for (auto &sl : s->slots) {
+    (void) hipStreamSynchronize(sl->stream);
+    (void) hipEventDestroy(sl->done);
+    (void) hipStreamDestroy(sl->stream);
+    (void) hipHostFree(sl->h_count);
+  }
+  for (auto &kv : s->prof_cache) (void) msvg_profile_remove(c, kv.second);
+  delete s;
+  return MSVG_OK;
 }
 
 }  // extern "C"

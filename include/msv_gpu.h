@@ -122,6 +122,48 @@ int msvg_MSVFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, int L,
  * The autotuner in the Python package persists its findings through this. */
 int msvg_set_tile(msvg_ctx *ctx, int32_t M_lo, int32_t M_hi, int32_t lanes, int32_t waves);
 
+/* ---- Serving: continuous batching over one resident database ----
+ * The counterpart of the reference's hmmpgmd worker (src/hmmdwrkr.c:532-667),
+ * which keeps a RAM-resident database (cachedb.h:19-34) and runs each search
+ * command over it.  Here queries (profiles) arrive at any time; a server
+ * thread admits each as soon as a slot, the HBM budget and the LDS budget
+ * allow, and runs it on its own stream, so queries that arrive while others
+ * are on the GPU start at once and their workgroups share the CUs.  While a
+ * server runs on a context, make no other calls on that context. */
+typedef struct msvg_server msvg_server;
+
+typedef struct {
+  int64_t ticket;
+  int32_t status;          /* MSVG_OK, or the error the query ended with   */
+  int32_t M;
+  int64_t n_pass;          /* targets passing F1 (ERANGE included)          */
+  int64_t cells;           /* M * residues of the database                  */
+  double  t_submit, t_admit, t_done;   /* seconds since the server started */
+} msvg_query_result;
+
+typedef struct {
+  int64_t submitted, admitted, completed, pending, running;
+  int64_t max_pending;
+  double  busy_s;          /* time with at least one query on the GPU       */
+  double  uptime_s;
+} msvg_server_info;
+
+/* slots: queries on the GPU at once (1..16); hbm_budget: device bytes the
+ * server may hold for running queries (0: no limit); F1 as in msvg_search. */
+int msvg_server_start(msvg_ctx *ctx, int32_t db_id, int32_t slots, double F1, int64_t hbm_budget,
+                      msvg_server **ret);
+/* Queue a query; the profile is copied.  Thread-safe. */
+int msvg_server_submit(msvg_server *srv, const msvg_oprofile *om, int64_t *ret_ticket);
+/* Wait up to timeout_s (< 0: forever) for a query.  On MSVG_OK fills *out
+ * and, if idx is not NULL, up to cap passing targets (caller order,
+ * ascending) with their usc; the query's results are then released.
+ * Returns MSVG_ENORESULT if the query has not finished in time. */
+int msvg_server_wait(msvg_server *srv, int64_t ticket, double timeout_s, msvg_query_result *out, int32_t *idx,
+                     float *usc, int64_t cap);
+int msvg_server_stats(msvg_server *srv, msvg_server_info *out);
+/* Stop admitting, finish the queries on the GPU, drop the queued ones. */
+int msvg_server_stop(msvg_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,13 @@ generators in cudahmmsearch_amd):
   c4  8 sampled profiles, M in {6000, 8000, 10000, 12000} x 2 (seeds 80..87)
       vs 500,000 targets, log-normal median 1200, sigma 0.6, clipped
       100..35000, seed 7
+  c5  serving: resident database of 10,000,000 targets (log-normal median
+      350, sigma 0.7, clipped 20..10000, seed 8); queries drawn from 64
+      sampled profiles with M log-normal (median 250, sigma 0.6, clipped
+      30..4000, seeds 1000..1063), calibrated beforehand (as HMM files carry
+      their STATS lines); Poisson arrivals at --rate req/s for --duration s.
+      Also measures the service capacity (closed loop, every slot busy) and
+      p50/p99 at 70% of it, where the queue is stable.
 """
 import argparse
 import json
@@ -136,6 +143,93 @@ def copy_probe(nbytes=256 << 20):
     log(f"copy probe: {nbytes} bytes device to device")
 
 
+def serving_phase(ca, srv, profs, rate, duration, rs, drain):
+    """Open-loop Poisson arrivals; returns latencies (ms) of finished queries
+    and the counts of submitted / finished / unfinished ones."""
+    t0 = time.perf_counter()
+    tickets = []
+    nxt = 0.0
+    while True:
+        nxt += rs.exponential(1.0 / rate)
+        if nxt > duration:
+            break
+        now = time.perf_counter() - t0
+        if nxt > now:
+            time.sleep(nxt - now)
+        tickets.append(srv.submit(profs[int(rs.integers(len(profs)))]))
+    lat, cells, unfinished = [], 0, 0
+    for t in tickets:
+        r = srv.wait(t, timeout=(-1.0 if drain else 0.0), want_hits=False)
+        if r is None:
+            unfinished += 1
+            continue
+        info = r[0]
+        lat.append((info["t_done"] - info["t_submit"]) * 1e3)
+        cells += info["cells"]
+    return np.array(lat), len(tickets), unfinished, cells, time.perf_counter() - t0
+
+
+def serving(ca, args):
+    db = ca.gen_database(8, 10_000_000, 350, 0.7, 20, 10000, nthreads=16)
+    rs = np.random.default_rng(2005)
+    Ms = np.clip(np.round(np.exp(np.log(250.0) + 0.6 * rs.standard_normal(64))), 30, 4000).astype(int)
+    profs = [ca.sample_profile(1000 + i, int(M)) for i, M in enumerate(Ms)]
+    log(f"serving database: {db.n} targets, {db.residues} residues; {len(profs)} query profiles, "
+        f"median M {int(np.median(Ms))}")
+    ctx = ca.MSVContext(0)
+    for p in profs:
+        pid = ctx.add_profile(p)
+        p.mu = ctx.calibrate(pid, p.lam)
+        ctx.remove_profile(pid)
+    did = ctx.add_database(db)
+    meanM = float(np.mean(Ms))
+    # warm tables and thresholds for every query profile
+    srv = ca.Server(ctx, did, slots=args.slots)
+    for t in [srv.submit(p) for p in profs]:
+        srv.wait(t, want_hits=False)
+    # 1. offered load
+    lat, sub, unf, cells, wall = serving_phase(ca, srv, profs, args.rate, args.duration, rs, drain=False)
+    st = srv.stats()
+    srv.stop()
+    log(f"offered {args.rate}/s: submitted {sub}, finished {lat.size}, unfinished {unf}")
+    # 2. service capacity: every slot busy
+    srv = ca.Server(ctx, did, slots=args.slots)
+    t0 = time.perf_counter()
+    n_cap = 12 * args.slots
+    tk = [srv.submit(profs[int(rs.integers(len(profs)))]) for _ in range(n_cap)]
+    cap_cells = sum(srv.wait(t, want_hits=False)[0]["cells"] for t in tk)
+    cap_s = time.perf_counter() - t0
+    srv.stop()
+    cap_qps = n_cap / cap_s
+    log(f"capacity: {cap_qps:.1f} queries/s, {cap_cells / cap_s / 1e9:.0f} GCUPS")
+    # 3. stable load: 70% of capacity, drained
+    srv = ca.Server(ctx, did, slots=args.slots)
+    rate2 = 0.7 * cap_qps
+    lat2, sub2, _, cells2, wall2 = serving_phase(ca, srv, profs, rate2, max(10.0, args.duration / 3), rs, drain=True)
+    srv.stop()
+    ctx.close()
+    pct = lambda a, q: round(float(np.percentile(a, q)), 2) if a.size else None  # noqa: E731
+    out = {
+        "metric": f"MSV serving p99 latency at {args.rate:g} req/s on 1 MI355X",
+        "value": pct(lat, 99), "unit": "ms", "n_gpus": 1, "steps": sub, "warmup": len(profs),
+        "ms_per_step": pct(lat, 50), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"c5: {db.n} resident targets ({db.residues} residues), 64 query profiles "
+                               f"M log-normal median 250 (mean {meanM:.0f}), Poisson {args.rate:g} req/s "
+                               f"for {args.duration:g} s, {args.slots} slots",
+                   "cells_per_query_mean": round(meanM * db.residues)},
+        "offered": {"rate_per_s": args.rate, "submitted": sub, "finished": int(lat.size), "unfinished": unf,
+                    "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_queue": st["max_pending"],
+                    "gcups_while_serving": round(cells / max(st["busy_s"], 1e-9) / 1e9, 1),
+                    "note": "latencies of finished queries only; the offered load exceeds capacity, so the "
+                            "queue grows for the whole run and the unfinished queries would wait longer"},
+        "capacity": {"queries_per_s": round(cap_qps, 2), "gcups": round(cap_cells / cap_s / 1e9, 1)},
+        "stable": {"rate_per_s": round(rate2, 2), "submitted": sub2, "p50_ms": pct(lat2, 50),
+                   "p99_ms": pct(lat2, 99), "gcups": round(cells2 / wall2 / 1e9, 1)},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +239,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run a 256 MiB D2D copy first (counter scaling)")
     ap.add_argument("--batched-only", action="store_true", help="time only the batched mode (profiling runs)")
+    ap.add_argument("--rate", type=float, default=500.0, help="c5: offered queries per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
+    ap.add_argument("--slots", type=int, default=4, help="c5: queries on the GPU at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -152,6 +249,9 @@ def main():
         sys.exit(2)
 
     import cudahmmsearch_amd as ca
+    if args.workload == "c5":
+        serving(ca, args)
+        return
     if args.copy_probe:
         copy_probe()
     t0 = time.perf_counter()

@@ -58,6 +58,17 @@ class _Stats(C.Structure):
                 ("n_fast_launches", C.c_int32)]
 
 
+class _QRes(C.Structure):
+    _fields_ = [("ticket", C.c_int64), ("status", C.c_int32), ("M", C.c_int32), ("n_pass", C.c_int64),
+                ("cells", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class _SInfo(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
+                ("pending", C.c_int64), ("running", C.c_int64), ("max_pending", C.c_int64),
+                ("busy_s", C.c_double), ("uptime_s", C.c_double)]
+
+
 class _Rng(C.Structure):
     _fields_ = [("fast", C.c_int), ("mt", C.c_uint32 * 624), ("mti", C.c_int), ("x", C.c_uint32),
                 ("seed", C.c_uint32)]
@@ -92,6 +103,11 @@ def load_library(path=LIB_PATH):
         "msvg_last_stats": (C.c_int, [vp, C.POINTER(_Stats)]),
         "msvg_MSVFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
+        "msvg_server_start": (C.c_int, [vp, i32, i32, C.c_double, i64, C.POINTER(vp)]),
+        "msvg_server_submit": (C.c_int, [vp, C.POINTER(_Prof), C.POINTER(i64)]),
+        "msvg_server_wait": (C.c_int, [vp, i64, C.c_double, C.POINTER(_QRes), vp, vp, i64]),
+        "msvg_server_stats": (C.c_int, [vp, C.POINTER(_SInfo)]),
+        "msvg_server_stop": (C.c_int, [vp]),
         "msvh_rng_init": (None, [C.POINTER(_Rng), C.c_uint32, C.c_int]),
         "msvh_random": (C.c_double, [C.POINTER(_Rng)]),
         "msvh_hmm_sample": (C.c_int, [C.POINTER(_Rng), C.c_int, vp, vp, vp]),
@@ -332,13 +348,21 @@ def plant(db, seqs, seed, frac):
 class MSVContext:
     """One GPU, one context (not shared between host threads)."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, tiles="default"):
+        """tiles: "default" loads the persisted autotuner table if present,
+        None uses the built-in cost model, or a list of tile dicts."""
         self.lib = load_library()
         self._c = C.c_void_p()
         rc = self.lib.msvg_init(device, C.byref(self._c))
         if rc != 0:
             raise MSVError(rc, f"msvg_init failed on device {device} (is a GPU visible?)")
         self._keep = {}
+        if tiles == "default":
+            from . import autotune
+            tiles = autotune.load()
+        if tiles:
+            for t in tiles:
+                self.set_tile(t["lo"], t["hi"], t["lanes"], t["waves"])
 
     def close(self):
         if self._c:
@@ -417,3 +441,55 @@ class MSVContext:
 
     def clear_tiles(self):
         self._chk(self.lib.msvg_set_tile(self._c, 0, 0, 0, 0))
+
+
+class Server:
+    """Continuous-batching MSV server over one resident database (see
+    msvg_server_* in include/msv_gpu.h).  While it runs, make no other calls
+    on its context."""
+
+    def __init__(self, ctx, did, slots=4, F1=0.02, hbm_budget=0):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.n = self.lib.msvg_db_size(ctx._c, did)
+        self._s = C.c_void_p()
+        ctx._chk(self.lib.msvg_server_start(ctx._c, did, slots, F1, hbm_budget, C.byref(self._s)))
+
+    def submit(self, prof):
+        t = C.c_int64()
+        s = prof._struct()
+        rc = self.lib.msvg_server_submit(self._s, C.byref(s), C.byref(t))
+        if rc != 0:
+            raise MSVError(rc, "submit refused")
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0, want_hits=True):
+        """None if not finished within timeout; else (info dict, idx, usc)."""
+        r = _QRes()
+        cap = self.n if want_hits else 0
+        idx = np.zeros(max(cap, 1), np.int32)
+        usc = np.zeros(max(cap, 1), np.float32)
+        rc = self.lib.msvg_server_wait(self._s, ticket, timeout, C.byref(r), _p(idx) if cap else None,
+                                       _p(usc) if cap else None, cap)
+        if rc == eslENORESULT:
+            return None
+        if rc != 0:
+            raise MSVError(rc, "wait failed")
+        info = {k: getattr(r, k) for k, _ in _QRes._fields_}
+        k = int(r.n_pass) if cap else 0
+        return info, idx[:k].copy(), usc[:k].copy()
+
+    def stats(self):
+        i = _SInfo()
+        self.lib.msvg_server_stats(self._s, C.byref(i))
+        return {k: getattr(i, k) for k, _ in _SInfo._fields_}
+
+    def stop(self):
+        if self._s:
+            self.lib.msvg_server_stop(self._s)
+            self._s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

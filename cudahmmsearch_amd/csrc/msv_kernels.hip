@@ -13,7 +13,7 @@
 //                 in packed u16 lanes, for the queued pairs.
 //  3. msv_finalize  usc and the F1 pass bit from host-built tables.
 //
-// Wavefront layout: a lane group of G lanes (16, 32 or 64) owns one target;
+// Wavefront layout: a lane group of G lanes (8, 16, 32 or 64) owns one target;
 // its 2G f16 slots hold model positions striped as k = j + z*Q + 1 (slot z,
 // register j).  The diagonal dependency of the last register moves one slot
 // per row through a DPP lane shift (wave_shr:1 / row_shr:1) plus v_alignbit.
@@ -58,6 +58,9 @@ __device__ __forceinline__ uint32_t shift_slot(uint32_t last, int lane) {
     prev = __builtin_amdgcn_update_dpp(0u, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
   } else if constexpr (G == 16) {
     prev = __builtin_amdgcn_update_dpp(0u, last, 0x111 /* row_shr:1 */, 0xf, 0xf, true);
+  } else if constexpr (G == 8) {
+    prev = __builtin_amdgcn_update_dpp(0u, last, 0x111 /* row_shr:1 */, 0xf, 0xf, true);
+    prev = (lane & 7) ? prev : 0u;       // the second group of each DPP row starts at lane 8
   } else {
     prev = __builtin_amdgcn_update_dpp(0u, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
     prev = (lane == 32) ? 0u : prev;
@@ -732,6 +735,7 @@ extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t l
 
 extern "C" hipError_t msvk_prepare() {
   hipError_t e;
+  if ((e = set_lds_all<8>(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_seg(MSVG_SEGQ{})) != hipSuccess) return e;
   if ((e = set_lds_jrow(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_all<16>(MSVG_QSEQ{})) != hipSuccess) return e;
@@ -743,6 +747,7 @@ extern "C" hipError_t msvk_launch_fast(int G, int Q, const FastArgs *a, int grid
                                        hipStream_t st) {
   if (Q < 1 || Q > 32) return hipErrorInvalidValue;
   switch (G) {
+    case 8: return dispatch_q<8>(Q, *a, grid, block, lds, st, MSVG_QSEQ{});
     case 16: return dispatch_q<16>(Q, *a, grid, block, lds, st, MSVG_QSEQ{});
     case 32: return dispatch_q<32>(Q, *a, grid, block, lds, st, MSVG_QSEQ{});
     case 64: return dispatch_q<64>(Q, *a, grid, block, lds, st, MSVG_QSEQ{});

@@ -172,12 +172,12 @@ void choose_tile(const msvg_ctx *c, int M, int R, int &G, int &waves) {
   // cells the wave advances, over the sizes whose tables fit in LDS.
   double best = 1e30;
   G = 64;
-  for (int g : {16, 32, 64}) {
+  for (int g : {8, 16, 32, 64}) {
     const int Q = qfor(M, 2 * g);
     if (Q > kQMax) continue;
     const size_t lds = (size_t) (R + 1) * ((Q + 3) / 4) * 1024;
     if (lds > (size_t) kLdsBytes) continue;
-    const double v = Q + (Q + 1) / 2 + 4 + (g == 32 ? 1 : 0);
+    const double v = Q + (Q + 1) / 2 + 4 + ((g == 32 || g == 8) ? 1 : 0);
     const double cost = v / ((64 / g) * (double) M);
     if (cost < best - 1e-12) { best = cost; G = g; }
   }
@@ -825,7 +825,7 @@ int msvg_last_stats(msvg_ctx *c, msvg_stats *out) {
 int msvg_set_tile(msvg_ctx *c, int32_t lo, int32_t hi, int32_t lanes, int32_t waves) {
   if (!c) return MSVG_EINVAL;
   if (lanes == 0) { c->tiles.clear(); c->last = nullptr; c->plans.clear(); return MSVG_OK; }
-  if (lanes != 16 && lanes != 32 && lanes != 64) return fail(c, MSVG_EINVAL, "lanes must be 16, 32 or 64");
+  if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) return fail(c, MSVG_EINVAL, "lanes must be 8, 16, 32 or 64");
   if (waves < 0 || waves > 16) return fail(c, MSVG_EINVAL, "waves must be 0..16");
   c->tiles.push_back(Tile{lo, hi, lanes, waves});
   c->last = nullptr;

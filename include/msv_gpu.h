@@ -117,7 +117,7 @@ int msvg_last_stats(msvg_ctx *ctx, msvg_stats *out);
  * reference sets it. */
 int msvg_MSVFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, int L, float *ret_sc);
 
-/* Tile choice for the batched mode: lanes per target (16, 32, 64) and waves
+/* Tile choice for the batched mode: lanes per target (8, 16, 32, 64) and waves
  * per workgroup for profiles of length M; 0 restores the built-in choice.
  * The autotuner in the Python package persists its findings through this. */
 int msvg_set_tile(msvg_ctx *ctx, int32_t M_lo, int32_t M_hi, int32_t lanes, int32_t waves);

@@ -217,3 +217,29 @@ def test_order_invariance(ctx):
     ctx.remove_database(did)
     ctx.remove_database(did2)
     ctx.remove_profile(pid)
+
+
+@pytest.mark.parametrize("M", [100, 400, 700, 1500])
+def test_every_tile_same_result(ctx, M):
+    """Tiles (lanes per target, waves per workgroup) change speed, never results."""
+    prof, db = _workload(300 + M, M, 800, 300, 0.7, 20, 3000)
+    pid = ctx.add_profile(prof)
+    prof.mu = ctx.calibrate(pid, prof.lam)
+    did = ctx.add_database(db)
+    ref = None
+    for lanes in (8, 16, 32, 64):
+        if (M + 2 * lanes - 1) // (2 * lanes) > 32:
+            continue
+        for waves in (4, 16):
+            ctx.clear_tiles()
+            ctx.set_tile(1, 1 << 20, lanes, waves)
+            out = ctx.search([pid], did, mode=ca.MODE_BATCHED)
+            if ref is None:
+                ref = out
+            else:
+                assert np.array_equal(out[0].view(np.uint32), ref[0].view(np.uint32)), (lanes, waves)
+                assert np.array_equal(out[2], ref[2])
+    ctx.clear_tiles()
+    _check_vs_oracle(ctx, prof, db, ca.MODE_BATCHED, pid=pid, did=did, sample=200)
+    ctx.remove_database(did)
+    ctx.remove_profile(pid)

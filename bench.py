@@ -184,7 +184,7 @@ def serving(ca, args):
     did = ctx.add_database(db)
     meanM = float(np.mean(Ms))
     # warm tables and thresholds for every query profile
-    srv = ca.Server(ctx, did, slots=args.slots)
+    srv = ca.Server(ctx, did, slots=args.slots, shared=args.shared)
     for t in [srv.submit(p) for p in profs]:
         srv.wait(t, want_hits=False)
     # 1. offered load
@@ -193,7 +193,7 @@ def serving(ca, args):
     srv.stop()
     log(f"offered {args.rate}/s: submitted {sub}, finished {lat.size}, unfinished {unf}")
     # 2. service capacity: every slot busy
-    srv = ca.Server(ctx, did, slots=args.slots)
+    srv = ca.Server(ctx, did, slots=args.slots, shared=args.shared)
     t0 = time.perf_counter()
     n_cap = 12 * args.slots
     tk = [srv.submit(profs[int(rs.integers(len(profs)))]) for _ in range(n_cap)]
@@ -203,7 +203,7 @@ def serving(ca, args):
     cap_qps = n_cap / cap_s
     log(f"capacity: {cap_qps:.1f} queries/s, {cap_cells / cap_s / 1e9:.0f} GCUPS")
     # 3. stable load: 70% of capacity, drained
-    srv = ca.Server(ctx, did, slots=args.slots)
+    srv = ca.Server(ctx, did, slots=args.slots, shared=args.shared)
     rate2 = 0.7 * cap_qps
     lat2, sub2, _, cells2, wall2 = serving_phase(ca, srv, profs, rate2, max(10.0, args.duration / 3), rs, drain=True)
     srv.stop()
@@ -216,7 +216,8 @@ def serving(ca, args):
         "dtype": "u8", "data": "synthetic",
         "config": {"workload": f"c5: {db.n} resident targets ({db.residues} residues), 64 query profiles "
                                f"M log-normal median 250 (mean {meanM:.0f}), Poisson {args.rate:g} req/s "
-                               f"for {args.duration:g} s, {args.slots} slots",
+                               f"for {args.duration:g} s, {args.slots} slots, "
+                               f"{'shared' if args.shared else 'persistent'} launches",
                    "cells_per_query_mean": round(meanM * db.residues)},
         "offered": {"rate_per_s": args.rate, "submitted": sub, "finished": int(lat.size), "unfinished": unf,
                     "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_queue": st["max_pending"],
@@ -242,6 +243,7 @@ def main():
     ap.add_argument("--rate", type=float, default=500.0, help="c5: offered queries per second")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--slots", type=int, default=4, help="c5: queries on the GPU at once")
+    ap.add_argument("--shared", action="store_true", help="c5: one chunk per workgroup (queries share CUs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -294,8 +296,13 @@ def main():
     fast_ms_per_launch = se["ms_fast"] / max(1, se["n_fast_launches"])
     cells_per_launch = cells / len(pids)
     achieved = INSTR_PER_CELL * cells_per_launch / (fast_ms_per_launch / 1e3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh)["bytes_per_launch"]     # rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled
     roof = {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": round(PEAK_LANE_INSTR / 1e12, 3),
-            "unit": "T lane-instr/s", "frac": round(achieved / PEAK_LANE_INSTR, 4), "traffic": None,
+            "unit": "T lane-instr/s", "frac": round(achieved / PEAK_LANE_INSTR, 4), "traffic": traffic,
             "kernel": "msv_fast (+ msv_jrow for the pairs whose J state may fire)",
             "ms_per_launch": round(fast_ms_per_launch, 3),
             "cells_per_s": round(cells_per_launch / (fast_ms_per_launch / 1e3), 1),

@@ -103,7 +103,7 @@ def load_library(path=LIB_PATH):
         "msvg_last_stats": (C.c_int, [vp, C.POINTER(_Stats)]),
         "msvg_MSVFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
-        "msvg_server_start": (C.c_int, [vp, i32, i32, C.c_double, i64, C.POINTER(vp)]),
+        "msvg_server_start": (C.c_int, [vp, i32, i32, C.c_double, i64, i32, C.POINTER(vp)]),
         "msvg_server_submit": (C.c_int, [vp, C.POINTER(_Prof), C.POINTER(i64)]),
         "msvg_server_wait": (C.c_int, [vp, i64, C.c_double, C.POINTER(_QRes), vp, vp, i64]),
         "msvg_server_stats": (C.c_int, [vp, C.POINTER(_SInfo)]),
@@ -448,11 +448,15 @@ class Server:
     msvg_server_* in include/msv_gpu.h).  While it runs, make no other calls
     on its context."""
 
-    def __init__(self, ctx, did, slots=4, F1=0.02, hbm_budget=0):
+    def __init__(self, ctx, did, slots=4, F1=0.02, hbm_budget=0, shared=False):
+        """shared=False: persistent launches (queries start as the running one
+        drains); shared=True: one chunk per workgroup (running queries share
+        the CUs from the start)."""
         self.ctx, self.lib = ctx, ctx.lib
         self.n = self.lib.msvg_db_size(ctx._c, did)
         self._s = C.c_void_p()
-        ctx._chk(self.lib.msvg_server_start(ctx._c, did, slots, F1, hbm_budget, C.byref(self._s)))
+        ctx._chk(self.lib.msvg_server_start(ctx._c, did, slots, F1, hbm_budget, 1 if shared else 0,
+                                            C.byref(self._s)))
 
     def submit(self, prof):
         t = C.c_int64()

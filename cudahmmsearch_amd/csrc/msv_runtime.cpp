@@ -403,7 +403,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   if (!capturing) b.slow_cap = c->slow_cap;
   HIPCHK(c, b.d_slow.ensure((size_t) b.slow_cap * sizeof(SlowItem)));
   HIPCHK(c, b.d_counters.ensure((size_t) (2 * np + 2) * sizeof(int32_t)));
-  if (!capturing) b.jcap = std::max<int64_t>(1024, n / 64);
+  if (!capturing) b.jcap = std::max<int64_t>(1024, n / 16);
   HIPCHK(c, b.d_jslow.ensure((size_t) np * b.jcap * sizeof(SlowItem)));
   HIPCHK(c, b.d_usc.ensure((size_t) np * n * sizeof(float)));
   HIPCHK(c, b.d_status.ensure((size_t) np * n));
@@ -925,6 +925,7 @@ struct msvg_server {
   int db = -1;
   double F1 = 0.02;
   int64_t hbm_budget = 0;
+  int sched = 0;                  // 0 persistent launches, 2 one chunk per workgroup
   std::vector<std::unique_ptr<Slot>> slots;
   std::mutex mu;
   std::condition_variable cv;
@@ -980,7 +981,8 @@ int server_start_query(msvg_server *s, Slot &sl, ServerQuery &q) {
     s->prof_cache[key] = pid;
   }
   int64_t cells = 0;
-  int rc = enqueue_search(c, sl.bufs, &pid, 1, s->db, s->F1, MSVG_MODE_BATCHED, false, false, &cells, sl.stream, 2);
+  int rc = enqueue_search(c, sl.bufs, &pid, 1, s->db, s->F1, MSVG_MODE_BATCHED, false, false, &cells, sl.stream,
+                          s->sched);
   if (rc != MSVG_OK) return rc;
   q.cells = cells;
   HIPCHK(c, sl.d_ccount.ensure(sizeof(int32_t)));
@@ -1008,16 +1010,8 @@ int server_finish_query(msvg_server *s, Slot &sl, ServerQuery &q) {
   if (k > 0) {
     HIPCHK(c, hipMemcpy(q.idx.data(), sl.d_cidx.p, (size_t) k * sizeof(int32_t), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(q.usc.data(), sl.d_cusc.p, (size_t) k * sizeof(float), hipMemcpyDeviceToHost));
-    std::vector<int32_t> ord(k);
-    for (int32_t i = 0; i < k; ++i) ord[i] = i;
-    std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return q.idx[a] < q.idx[b]; });
-    std::vector<int32_t> i2(k);
-    std::vector<float> u2(k);
-    for (int32_t i = 0; i < k; ++i) { i2[i] = q.idx[ord[i]]; u2[i] = q.usc[ord[i]]; }
-    q.idx.swap(i2);
-    q.usc.swap(u2);
   }
-  return MSVG_OK;
+  return MSVG_OK;             // the waiting caller orders the hits (msvg_server_wait)
 }
 
 void server_loop(msvg_server *s) {
@@ -1060,7 +1054,9 @@ void server_loop(msvg_server *s) {
         ServerQuery *cand = s->pending.front();
         cand->lds = query_lds(c, d, cand->desc.M);
         cand->dev = (size_t) d.n * 16 + (size_t) cand->desc.M * 29 * 8;
-        const bool lds_ok = s->st.running == 0 || s->lds_in_use + cand->lds <= (size_t) kLdsBytes;
+        // Shared scheduling: the running queries' workgroups must fit a CU
+        // together.  Persistent launches queue behind each other anyway.
+        const bool lds_ok = s->sched != 2 || s->st.running == 0 || s->lds_in_use + cand->lds <= (size_t) kLdsBytes;
         const bool hbm_ok = s->hbm_budget <= 0 || s->st.running == 0 ||
                             (int64_t) (s->dev_in_use + cand->dev) <= s->hbm_budget;
         if (!lds_ok || !hbm_ok) break;
@@ -1105,8 +1101,10 @@ void server_loop(msvg_server *s) {
 
 extern "C" {
 
-int msvg_server_start(msvg_ctx *c, int32_t db_id, int32_t slots, double F1, int64_t hbm_budget, msvg_server **ret) {
+int msvg_server_start(msvg_ctx *c, int32_t db_id, int32_t slots, double F1, int64_t hbm_budget, int32_t flags,
+                      msvg_server **ret) {
   if (!c || !ret) return MSVG_EINVAL;
+  if (flags != MSVG_SERVE_PERSISTENT && flags != MSVG_SERVE_SHARED) return fail(c, MSVG_EINVAL, "unknown flags");
   if (!c->dbs.count(db_id)) return fail(c, MSVG_EINVAL, "unknown database id");
   if (slots < 1 || slots > 16) return fail(c, MSVG_EINVAL, "slots must be 1..16");
   (void) hipSetDevice(c->device);
@@ -1115,6 +1113,7 @@ int msvg_server_start(msvg_ctx *c, int32_t db_id, int32_t slots, double F1, int6
   s->db = db_id;
   s->F1 = F1;
   s->hbm_budget = hbm_budget;
+  s->sched = (flags == MSVG_SERVE_SHARED) ? 2 : 0;
   for (int i = 0; i < slots; ++i) {
     auto sl = std::make_unique<Slot>();
     HIPCHK(c, hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
@@ -1166,20 +1165,28 @@ int msvg_server_wait(msvg_server *s, int64_t ticket, double timeout_s, msvg_quer
     out->t_done = -1;
     return MSVG_ENORESULT;
   }
-  out->ticket = ticket;
-  out->status = q->status;
-  out->M = q->desc.M;
-  out->n_pass = (int64_t) q->idx.size();
-  out->cells = q->cells;
-  out->t_submit = q->t_submit;
-  out->t_admit = q->t_admit;
-  out->t_done = q->t_done;
-  if (idx && cap > 0) {
-    const int64_t k = std::min<int64_t>(cap, (int64_t) q->idx.size());
-    std::copy(q->idx.begin(), q->idx.begin() + k, idx);
-    if (usc) std::copy(q->usc.begin(), q->usc.begin() + k, usc);
-  }
+  std::unique_ptr<ServerQuery> own = std::move(it->second);
   s->queries.erase(it);
+  lk.unlock();
+  out->ticket = ticket;
+  out->status = own->status;
+  out->M = own->desc.M;
+  out->n_pass = (int64_t) own->idx.size();
+  out->cells = own->cells;
+  out->t_submit = own->t_submit;
+  out->t_admit = own->t_admit;
+  out->t_done = own->t_done;
+  if (idx && cap > 0) {
+    const int64_t k = (int64_t) own->idx.size();
+    std::vector<int32_t> ord(k);
+    for (int64_t i = 0; i < k; ++i) ord[i] = (int32_t) i;
+    std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return own->idx[a] < own->idx[b]; });
+    const int64_t m = std::min<int64_t>(cap, k);
+    for (int64_t i = 0; i < m; ++i) {
+      idx[i] = own->idx[ord[i]];
+      if (usc) usc[i] = own->usc[ord[i]];
+    }
+  }
   return MSVG_OK;
 }
 

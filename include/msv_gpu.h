@@ -148,10 +148,16 @@ typedef struct {
   double  uptime_s;
 } msvg_server_info;
 
-/* slots: queries on the GPU at once (1..16); hbm_budget: device bytes the
- * server may hold for running queries (0: no limit); F1 as in msvg_search. */
+/* slots: queries admitted to the GPU at once (1..16); hbm_budget: device
+ * bytes the server may hold for running queries (0: no limit); F1 as in
+ * msvg_search.  flags: MSVG_SERVE_PERSISTENT (each query's fast stage is one
+ * persistent launch; an admitted query's workgroups start as the running
+ * ones retire) or MSVG_SERVE_SHARED (one chunk per workgroup, so running
+ * queries' workgroups interleave on the CUs from the start). */
+#define MSVG_SERVE_PERSISTENT 0
+#define MSVG_SERVE_SHARED     1
 int msvg_server_start(msvg_ctx *ctx, int32_t db_id, int32_t slots, double F1, int64_t hbm_budget,
-                      msvg_server **ret);
+                      int32_t flags, msvg_server **ret);
 /* Queue a query; the profile is copied.  Thread-safe. */
 int msvg_server_submit(msvg_server *srv, const msvg_oprofile *om, int64_t *ret_ticket);
 /* Wait up to timeout_s (< 0: forever) for a query.  On MSVG_OK fills *out

@@ -8,7 +8,8 @@ import cudahmmsearch_amd as ca
 pytestmark = pytest.mark.gpu
 
 
-def test_alone_equals_mixed():
+@pytest.mark.parametrize("shared", [False, True])
+def test_alone_equals_mixed(shared):
     ctx = ca.MSVContext(0)
     db = ca.gen_database(31, 20000, 300, 0.7, 20, 6000, nthreads=8)
     profs = [ca.sample_profile(500 + i, M) for i, M in enumerate([60, 180, 333, 512, 900, 1500])]
@@ -22,7 +23,7 @@ def test_alone_equals_mixed():
         usc, st, ps = ctx.search([pid], did)
         direct.append((np.flatnonzero(ps[0]).astype(np.int32), usc[0][ps[0] == 1]))
         ctx.remove_profile(pid)
-    srv = ca.Server(ctx, did, slots=4)
+    srv = ca.Server(ctx, did, slots=4, shared=shared)
     alone = []
     for p in profs:
         t = srv.submit(p)

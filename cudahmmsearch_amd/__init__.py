@@ -103,6 +103,7 @@ def load_library(path=LIB_PATH):
         "msvg_last_stats": (C.c_int, [vp, C.POINTER(_Stats)]),
         "msvg_MSVFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
+        "msvg_test_alloc_budget": (None, [C.c_long]),
         "msvg_server_start": (C.c_int, [vp, i32, i32, C.c_double, i64, i32, C.POINTER(vp)]),
         "msvg_server_submit": (C.c_int, [vp, C.POINTER(_Prof), C.POINTER(i64)]),
         "msvg_server_wait": (C.c_int, [vp, i64, C.c_double, C.POINTER(_QRes), vp, vp, i64]),

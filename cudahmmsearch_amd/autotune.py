@@ -26,8 +26,11 @@ LANES = (8, 16, 32, 64)
 WAVES = (4, 8, 16)
 
 
-def feasible(M, lanes):
-    return (M + 2 * lanes - 1) // (2 * lanes) <= 32
+def feasible(M, lanes, rows=21):
+    """Q <= 32 registers and the table (rows x ceil(Q/4) KiB) within 160 KiB
+    of LDS for a database of 20 residue codes plus the null row."""
+    Q = (M + 2 * lanes - 1) // (2 * lanes)
+    return Q <= 32 and rows * ((Q + 3) // 4) * 1024 <= 160 * 1024
 
 
 def time_tile(ctx, pid, did, lanes, waves, reps=3):
@@ -56,6 +59,10 @@ def tune(ctx, db, edges=BIN_EDGES, seed=900, log=print):
                 continue
             for waves in WAVES:
                 cand[f"{lanes}x{waves}"] = time_tile(ctx, pid, did, lanes, waves)
+        if not cand:
+            log(f"M {lo}-{hi}: no tile fits LDS over the whole bin; left to the built-in choice")
+            ctx.remove_profile(pid)
+            continue
         key = min(cand, key=cand.get)
         lanes, waves = (int(v) for v in key.split("x"))
         table.append({"lo": lo, "hi": hi, "lanes": lanes, "waves": waves, "ms": round(cand[key], 4),

@@ -122,4 +122,7 @@ struct SegArgs {
   SlowItem *slow;
   int32_t *slow_count;
   int32_t slow_cap;
+  SlowItem *jslow;             // the profile's J-aware queue, or null
+  int32_t *jslow_count;
+  int32_t jslow_cap;
 };

@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(1024) msv_seg(SegArgs a) {
         const uint32_t w = decide(Rp, a.db.tjb[s], a.prof);
         a.prof.res[s] = (uint16_t) w;
         if ((w >> 8) == RES_SLOW)
-          defer(w, a.prof_idx, s, a.slow, a.slow_count, a.slow_cap, nullptr, nullptr, 0);
+          defer(w, a.prof_idx, s, a.slow, a.slow_count, a.slow_cap, a.jslow, a.jslow_count, a.jslow_cap);
       }
     }
   }

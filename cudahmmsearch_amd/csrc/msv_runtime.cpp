@@ -20,6 +20,7 @@
 #include <thread>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -58,6 +59,24 @@ int seg_q(int R) {
   return std::max(1, std::min(kSegQMax, 4 * qb));
 }
 
+// Failure injection for tests: MSVG_TEST_FAIL_ALLOC=k makes every device
+// allocation after the first k fail as if the device were out of memory
+// (checked on the host; no device call is made for the failing request).
+std::atomic<long> g_alloc_budget{-2};
+
+bool alloc_allowed() {
+  long b = g_alloc_budget.load();
+  if (b == -2) {
+    const char *env = std::getenv("MSVG_TEST_FAIL_ALLOC");
+    b = env ? std::atol(env) : -1;
+    long expect = -2;
+    g_alloc_budget.compare_exchange_strong(expect, b);
+    b = g_alloc_budget.load();
+  }
+  if (b < 0) return true;
+  return g_alloc_budget.fetch_sub(1) > 0;
+}
+
 struct DevMem {
   void *p = nullptr;
   size_t n = 0;
@@ -68,6 +87,7 @@ struct DevMem {
   hipError_t ensure(size_t bytes) {
     if (bytes <= n && p) return hipSuccess;
     if (p) { (void) hipFree(p); p = nullptr; n = 0; }
+    if (!alloc_allowed()) return hipErrorOutOfMemory;
     hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
     if (e == hipSuccess) n = bytes;
     return e;
@@ -166,8 +186,18 @@ int qfor(int M, int slots) { return (M + slots - 1) / slots; }
 
 // Lane-group size and waves per workgroup for a profile of length M.
 void choose_tile(const msvg_ctx *c, int M, int R, int &G, int &waves) {
+  // An installed tile applies only where its table fits this database's
+  // LDS footprint; otherwise the cost model below decides.
   for (const Tile &t : c->tiles)
-    if (M >= t.lo && M <= t.hi) { G = t.lanes; waves = t.waves; return; }
+    if (M >= t.lo && M <= t.hi) {
+      const int Q = qfor(M, 2 * t.lanes);
+      if (Q <= kQMax && (size_t) (R + 1) * ((Q + 3) / 4) * 1024 <= (size_t) kLdsBytes) {
+        G = t.lanes;
+        waves = t.waves;
+        return;
+      }
+      break;
+    }
   // Built-in cost model: VALU instructions per wave-row divided by the model
   // cells the wave advances, over the sizes whose tables fit in LDS.
   double best = 1e30;
@@ -456,12 +486,28 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
         a.partial = reinterpret_cast<uint16_t *>(b.d_seg.as<uint8_t>() + (size_t) d.packed * sizeof(uint16_t));
         a.first = (k0 == 0); a.last = (k0 + span >= p.M);
         a.prof_idx = i; a.slow = b.d_slow.as<SlowItem>(); a.slow_count = slow_count; a.slow_cap = (int32_t) b.slow_cap;
+        a.jslow = jtab[i] ? b.d_jslow.as<SlowItem>() + (size_t) i * b.jcap : nullptr;
+        a.jslow_count = jcounts + i;
+        a.jslow_cap = (int32_t) b.jcap;
         a.carry_base = d.d_off.as<int64_t>();
         const size_t lds = fast_lds(d.R, Q);
         int grid, block;
         persistent_shape(c, lds, 0, grid, block);
         a.ntasks = (int32_t) n;
         HIPCHK(c, msvk_launch_seg(Q, &a, grid, block, lds, st));
+      }
+      if (jtab[i]) {
+        JArgs ja;
+        ja.prof = pds[i];
+        ja.db = dd;
+        ja.tab = jtab[i];
+        ja.items = b.d_jslow.as<SlowItem>() + (size_t) i * b.jcap;
+        ja.count = jcounts + i;
+        ja.cap = (int32_t) b.jcap;
+        const int Q64 = qfor(p.M, 128);
+        const size_t jl = fast_lds(d.R, Q64);
+        const int per_cu = (int) std::max<size_t>(1, std::min<size_t>(4, kLdsBytes / jl));
+        HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jl, st));
       }
       continue;
     }
@@ -653,6 +699,8 @@ void msvg_free(msvg_ctx *c) {
 }
 
 const char *msvg_last_error(const msvg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void msvg_test_alloc_budget(long k) { g_alloc_budget.store(k); }
 
 int msvg_profile_add(msvg_ctx *c, const msvg_oprofile *om, int32_t *ret_id) {
   if (!c || !om || !ret_id || !om->rbv) return fail(c, MSVG_EINVAL, "null argument");

@@ -117,6 +117,11 @@ int msvg_last_stats(msvg_ctx *ctx, msvg_stats *out);
  * reference sets it. */
 int msvg_MSVFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, int L, float *ret_sc);
 
+/* Test hook: after k more device allocations, fail the rest as out of
+ * memory (k < 0: never).  The environment variable MSVG_TEST_FAIL_ALLOC sets
+ * the initial value.  No counterpart in the reference. */
+void msvg_test_alloc_budget(long k);
+
 /* Tile choice for the batched mode: lanes per target (8, 16, 32, 64) and waves
  * per workgroup for profiles of length M; 0 restores the built-in choice.
  * The autotuner in the Python package persists its findings through this. */

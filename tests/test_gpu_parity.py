@@ -243,3 +243,17 @@ def test_every_tile_same_result(ctx, M):
     _check_vs_oracle(ctx, prof, db, ca.MODE_BATCHED, pid=pid, did=did, sample=200)
     ctx.remove_database(did)
     ctx.remove_profile(pid)
+
+
+def test_deferred_pairs_take_the_j_stage(ctx):
+    """Planted homologs whose J state may fire are resolved by the J-aware
+    stage, not the byte stage, for every lane-group size the tile table may
+    choose (M = 1000 sits where a 16-lane table would not fit LDS)."""
+    for M in (300, 1000):
+        prof, db = _workload(700 + M, M, 3000, 300, 0.7, 20, 3000, planted=0.03)
+        pid, did, *_ = _check_vs_oracle(ctx, prof, db, ca.MODE_BATCHED, sample=300)
+        ctx.search_device([pid], did, mode=ca.MODE_BATCHED | ca.FLAG_EAGER)
+        st = ctx.stats()
+        assert st["n_jrow"] > 0 and st["n_slow"] == 0, (M, st)
+        ctx.remove_database(did)
+        ctx.remove_profile(pid)

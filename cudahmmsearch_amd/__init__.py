@@ -413,6 +413,8 @@ class MSVContext:
         each shaped [len(pids), n] in the caller's target order."""
         ids = np.ascontiguousarray(pids, dtype=np.int32)
         n = self.lib.msvg_db_size(self._c, did)
+        if n < 0:
+            raise MSVError(eslEINVAL, f"no database {did}")
         usc = np.zeros((ids.size, n), np.float32)
         st = np.zeros((ids.size, n), np.uint8)
         ps = np.zeros((ids.size, n), np.uint8)
@@ -455,6 +457,8 @@ class Server:
         the CUs from the start)."""
         self.ctx, self.lib = ctx, ctx.lib
         self.n = self.lib.msvg_db_size(ctx._c, did)
+        if self.n < 0:
+            raise MSVError(eslEINVAL, f"no database {did}")
         self._s = C.c_void_p()
         ctx._chk(self.lib.msvg_server_start(ctx._c, did, slots, F1, hbm_budget, 1 if shared else 0,
                                             C.byref(self._s)))

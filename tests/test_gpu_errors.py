@@ -42,8 +42,15 @@ def test_invalid_arguments():
     assert e.value.code == 11
     pid = ctx.add_profile(prof)
     with pytest.raises(ca.MSVError) as e:
-        ctx.search([pid + 1000], 12345)
+        ctx.search([pid], 12345)                    # unknown database
     assert e.value.code == 11
+    with pytest.raises(ca.MSVError) as e:
+        ctx.search_device([pid], 12345)             # the same, refused by the library itself
+    assert e.value.code == 11 and "database" in str(e.value)
+    good = ctx.add_database(ca.Database.from_list([np.array([0, 1, 2], np.uint8)]))
+    with pytest.raises(ca.MSVError) as e:
+        ctx.search([pid + 1000], good)              # unknown profile
+    assert e.value.code == 11 and "profile" in str(e.value)
     # residue codes beyond the profile's alphabet
     x = np.array([0, 1, 40], np.uint8)
     did = ctx.add_database(ca.Database.from_list([x]))

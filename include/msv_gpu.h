@@ -76,8 +76,8 @@ void        msvg_free(msvg_ctx *ctx);
 const char *msvg_last_error(const msvg_ctx *ctx);
 
 /* Profiles: copied to the device; the caller's buffers may be freed after.
- * EINVAL if M < 1, Kp < 21, or bias_b + base_b >= 255 (a range no profile
- * built by p7_oprofile_Convert reaches). */
+ * EINVAL if M < 1 or M > 16384, Kp outside 1..255, or bias_b + base_b >= 255
+ * (a range no profile built by p7_oprofile_Convert reaches). */
 int msvg_profile_add(msvg_ctx *ctx, const msvg_oprofile *om, int32_t *ret_id);
 int msvg_profile_remove(msvg_ctx *ctx, int32_t id);
 

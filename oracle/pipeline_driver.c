@@ -1,0 +1,143 @@
+/* pipeline_driver -- test infrastructure: a serial hmmsearch over the
+ * reference's own library, linked twice (oracle/Makefile `dropin`):
+ *
+ *   oracle/_ref/pipeline_ref      with the reference's SSE p7_MSVFilter
+ *   oracle/_ref/pipeline_mi355x   with cudahmmsearch_amd/hmmer/msvfilter_mi355x.c
+ *                                 (p7_MSVFilter on the GPU) in its place
+ *
+ * Identical output files from the two are the drop-in check: every
+ * p7_MSVFilter call made by p7_Pipeline goes through the replaced symbol, so
+ * the MSV pass counts, the hits that survive the later filters and every
+ * per-target MSV score below must agree bit for bit.
+ *
+ * The loop is hmmsearch's serial_loop (src/hmmsearch.c:1285-1313) with the
+ * per-query set-up of src/hmmsearch.c:474-510 and the report calls of
+ * src/hmmsearch.c:549-559; the stopwatch is left out so the output holds no
+ * timing.  Own code; only reference functions are called.
+ *
+ *   pipeline_driver <hmmfile> <seqfile> <outfile> [F1 [nobias]]
+ *
+ * After each query's report it lists, for every target, p7_MSVFilter's
+ * status and the float bits of its score.
+ */
+#include "p7_config.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "easel.h"
+#include "esl_alphabet.h"
+#include "esl_sq.h"
+#include "esl_sqio.h"
+#include "hmmer.h"
+
+static void die(const char *msg, const char *arg)
+{
+  fprintf(stderr, "pipeline_driver: %s %s\n", msg, arg ? arg : "");
+  exit(1);
+}
+
+int main(int argc, char **argv)
+{
+  char         errbuf[eslERRBUFSIZE];
+  P7_HMMFILE  *hfp = NULL;
+  ESL_SQFILE  *sqfp = NULL;
+  ESL_ALPHABET *abc = NULL;
+  P7_HMM      *hmm = NULL;
+  P7_BG       *bg = NULL;
+  FILE        *ofp;
+  double       F1 = -1.0;
+  int          nobias = 0, nquery = 0, hstatus, sstatus;
+
+  if (argc < 4) die("usage: pipeline_driver <hmmfile> <seqfile> <outfile> [F1 [nobias]]", NULL);
+  if (argc > 4) F1 = atof(argv[4]);
+  if (argc > 5) nobias = (strcmp(argv[5], "nobias") == 0);
+  if (p7_hmmfile_OpenE(argv[1], NULL, &hfp, errbuf) != eslOK) die("cannot open HMM file", argv[1]);
+  if ((ofp = fopen(argv[3], "w")) == NULL) die("cannot write", argv[3]);
+
+  while ((hstatus = p7_hmmfile_Read(hfp, &abc, &hmm)) == eslOK) {
+    P7_PROFILE  *gm = p7_profile_Create(hmm->M, abc);
+    P7_OPROFILE *om = p7_oprofile_Create(hmm->M, abc);
+    P7_OPROFILE *wom;
+    P7_PIPELINE *pli;
+    P7_TOPHITS  *th = p7_tophits_Create();
+    P7_OMX      *ox;
+    ESL_SQ      *sq;
+    int64_t      nseq = 0;
+    char        *lines = NULL;
+    size_t       nlines = 0, cap = 0;
+
+    if (nquery++ == 0) {
+      bg = p7_bg_Create(abc);
+      if (esl_sqfile_OpenDigital(abc, argv[2], eslSQFILE_UNKNOWN, NULL, &sqfp) != eslOK) die("cannot open", argv[2]);
+    } else if (esl_sqfile_Position(sqfp, 0) != eslOK) die("cannot rewind", argv[2]);
+
+    fprintf(ofp, "Query:       %s  [M=%d]\n", hmm->name, hmm->M);
+    p7_ProfileConfig(hmm, bg, gm, 100, p7_LOCAL);
+    p7_oprofile_Convert(gm, om);
+    wom = p7_oprofile_Clone(om);
+    pli = p7_pipeline_Create(NULL, om->M, 100, FALSE, p7_SEARCH_SEQS);
+    if (F1 > 0.0) pli->F1 = F1;
+    if (nobias) pli->do_biasfilter = FALSE;
+    p7_pli_NewModel(pli, wom, bg);
+    ox = p7_omx_Create(om->M, 0, 0);
+    sq = esl_sq_CreateDigital(abc);
+
+    while ((sstatus = esl_sqio_Read(sqfp, sq)) == eslOK) {
+      float usc = 0.0f;
+      int   st;
+      union { float f; uint32_t u; } bits;
+      char  line[256];
+      int   n;
+
+      p7_pli_NewSeq(pli, sq);
+      p7_bg_SetLength(bg, sq->n);
+      p7_oprofile_ReconfigLength(wom, sq->n);
+      p7_Pipeline(pli, wom, bg, sq, th);
+
+      st = p7_MSVFilter(sq->dsq, (int) sq->n, wom, ox, &usc);
+      bits.f = usc;
+      n = snprintf(line, sizeof line, "msv %s %" PRId64 " %d %08x\n", sq->name, (int64_t) sq->n, st, bits.u);
+      if (n < 0 || n >= (int) sizeof line) die("sequence name too long:", sq->name);
+      if (nlines + n + 1 > cap) {
+        cap = 2 * cap + 4096;
+        if ((lines = realloc(lines, cap)) == NULL) die("out of memory", NULL);
+      }
+      memcpy(lines + nlines, line, n);
+      nlines += n;
+      nseq++;
+      esl_sq_Reuse(sq);
+      p7_pipeline_Reuse(pli);
+    }
+    if (sstatus != eslEOF) die("sequence file parse failed:", argv[2]);
+
+    p7_tophits_SortBySortkey(th);
+    p7_tophits_Threshold(th, pli);
+    p7_tophits_Targets(ofp, th, pli, 120);  fprintf(ofp, "\n\n");
+    p7_tophits_Domains(ofp, th, pli, 120);  fprintf(ofp, "\n\n");
+    p7_tophits_TabularTargets(ofp, hmm->name, hmm->acc, th, pli, 1);
+    p7_pli_Statistics(ofp, pli, NULL);
+    fprintf(ofp, "//\n");
+    if (nlines) fwrite(lines, 1, nlines, ofp);
+    fprintf(ofp, "targets %" PRId64 "\n", nseq);
+
+    free(lines);
+    esl_sq_Destroy(sq);
+    p7_omx_Destroy(ox);
+    p7_pipeline_Destroy(pli);
+    p7_tophits_Destroy(th);
+    p7_oprofile_Destroy(wom);
+    p7_oprofile_Destroy(om);
+    p7_profile_Destroy(gm);
+    p7_hmm_Destroy(hmm);
+  }
+  if (hstatus != eslEOF) die("HMM file parse failed:", argv[1]);
+
+  fclose(ofp);
+  if (sqfp) esl_sqfile_Close(sqfp);
+  if (bg) p7_bg_Destroy(bg);
+  if (abc) esl_alphabet_Destroy(abc);
+  p7_hmmfile_Close(hfp);
+  return 0;
+}

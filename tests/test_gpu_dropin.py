@@ -1,0 +1,190 @@
+"""The drop-in check at the caller.
+
+p7_Pipeline (src/p7_pipeline.c:646-...) calls p7_MSVFilter for every target
+(p7_pipeline.c:671) and derives the MSV pass from its score.  The serial
+search of oracle/pipeline_driver.c runs that pipeline over the reference's
+own library twice: oracle/_ref/pipeline_ref with the reference's SSE
+p7_MSVFilter, oracle/_ref/pipeline_mi355x with
+cudahmmsearch_amd/hmmer/msvfilter_mi355x.c (libmsvgpu on the GPU) in its
+place, nothing else changed.  Their output files -- hit tables, domain
+alignments, tabular output, the pipeline's "Passed MSV filter" counts and a
+line per target with p7_MSVFilter's status and score bits -- must be
+byte-identical.  Inputs: the reference's tutorial models and sequences
+(tests/golden/hmmer/, copied by make_inputs.py) and a seeded synthetic
+protein database with planted fragments of real family members, zero-length
+and long targets.
+"""
+import gzip
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REFBIN = os.path.join(ROOT, "oracle", "_ref", "pipeline_ref")
+GPUBIN = os.path.join(ROOT, "oracle", "_ref", "pipeline_mi355x")
+INPUTS = os.path.join(ROOT, "tests", "golden", "hmmer")
+AMINO = "ACDEFGHIKLMNPQRSTVWY"
+# Robinson & Robinson background frequencies, as p7_AminoFrequencies (hmmer.c:163-186) lists them.
+BGF = np.array([0.0787945, 0.0151600, 0.0535222, 0.0668298, 0.0397062, 0.0695071, 0.0229198, 0.0590092,
+                0.0594422, 0.0963728, 0.0237718, 0.0414386, 0.0482904, 0.0395639, 0.0540978, 0.0683364,
+                0.0540687, 0.0673417, 0.0114135, 0.0304133])
+
+needs_bins = pytest.mark.skipif(not (os.path.exists(REFBIN) and os.path.exists(GPUBIN)),
+                                reason="pipeline binaries not built (make -C oracle ref dropin)")
+
+
+def _unpack(name, dst):
+    out = os.path.join(dst, name)
+    with gzip.open(os.path.join(INPUTS, name + ".gz"), "rb") as src, open(out, "wb") as fh:
+        shutil.copyfileobj(src, fh)
+    return out
+
+
+def _read_fasta(path):
+    seqs, cur = [], None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(">"):
+                cur = []
+                seqs.append(cur)
+            elif cur is not None:
+                cur.append(line.strip())
+    return ["".join(s) for s in seqs]
+
+
+def _embl_sequence(path):
+    out, on = [], False
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("SQ"):
+                on = True
+            elif line.startswith("//"):
+                break
+            elif on:
+                out.append("".join(ch for ch in line if ch.isalpha()).upper())
+    return "".join(out)
+
+
+def synthetic_fasta(path, n, seed, donors):
+    """n targets: i.i.d. background residues (log-normal lengths, median 300),
+    every 7th with a fragment of a real family member planted, a few
+    zero-length targets, a few of 5000-9000 residues, and a sprinkling of
+    the ambiguity codes X, B, Z."""
+    rs = np.random.default_rng(seed)
+    lens = np.clip(rs.lognormal(np.log(300), 0.6, n), 1, 4000).astype(int)
+    with open(path, "w") as fh:
+        for i in range(n):
+            L = int(lens[i])
+            if i % 97 == 5:
+                L = 0
+            elif i % 211 == 7:
+                L = int(rs.integers(5000, 9000))
+            s = "".join(np.array(list(AMINO))[rs.choice(20, L, p=BGF / BGF.sum())])
+            if L and i % 7 == 3:
+                d = donors[int(rs.integers(len(donors)))]
+                w = int(rs.integers(40, min(300, len(d)) + 1))
+                a = int(rs.integers(0, len(d) - w + 1))
+                at = int(rs.integers(0, L + 1))
+                s = s[:at] + d[a:a + w] + s[at:]
+            if s and i % 13 == 0:
+                s = list(s)
+                for _ in range(3):
+                    s[int(rs.integers(len(s)))] = "XBZ"[int(rs.integers(3))]
+                s = "".join(s)
+            fh.write(f">t{i} synthetic target {i}\n")
+            for k in range(0, len(s), 60):
+                fh.write(s[k:k + 60] + "\n")
+            if not s:
+                fh.write("\n")
+
+
+def _run_both(tmp, hmm, seqs, extra=()):
+    outs = []
+    for b in (REFBIN, GPUBIN):
+        out = os.path.join(tmp, os.path.basename(b) + ".out")
+        r = subprocess.run([b, hmm, seqs, out, *extra], capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, f"{os.path.basename(b)} failed: {r.stderr[-2000:]}"
+        with open(out) as fh:
+            outs.append(fh.read())
+    return outs
+
+
+def _check_same(ref, gpu):
+    assert "Passed MSV filter:" in ref
+    if ref != gpu:
+        rl, gl = ref.splitlines(), gpu.splitlines()
+        for i, (a, b) in enumerate(zip(rl, gl)):
+            assert a == b, f"first difference at line {i + 1}:\n ref: {a}\n gpu: {b}"
+        assert len(rl) == len(gl), "outputs differ in length"
+
+
+@pytest.mark.gpu
+@needs_bins
+def test_tutorial_fn3_vs_7less(tmp_path):
+    hmm = _unpack("fn3.hmm", tmp_path)
+    seq = _unpack("7LESS_DROME", tmp_path)
+    ref, gpu = _run_both(str(tmp_path), hmm, seq)
+    _check_same(ref, gpu)
+    # the shipped answer tutorial/fn3.out:110: the one target passes the MSV filter
+    assert any(l.startswith("Passed MSV filter:") and l.split()[3] == "1" for l in ref.splitlines())
+
+
+@pytest.mark.gpu
+@needs_bins
+def test_tutorial_globins(tmp_path):
+    ref, gpu = _run_both(str(tmp_path), _unpack("globins4.hmm", tmp_path), _unpack("globins45.fa", tmp_path))
+    _check_same(ref, gpu)
+
+
+@pytest.mark.gpu
+@needs_bins
+def test_tutorial_dna(tmp_path):
+    ref, gpu = _run_both(str(tmp_path), _unpack("MADE1.hmm", tmp_path), _unpack("dna_target.fa", tmp_path))
+    _check_same(ref, gpu)
+
+
+def _models(tmp, names):
+    """One HMM file holding the named models in order (a multi-query search
+    when there are several; the models in one file share a format)."""
+    path = os.path.join(tmp, "_".join(n.split(".")[0] for n in names) + ".hmm")
+    with open(path, "w") as fh:
+        for m in names:
+            with open(_unpack(m, tmp)) as src:
+                fh.write(src.read())
+    return path
+
+
+def _donors(tmp):
+    return _read_fasta(_unpack("globins45.fa", tmp)) + [_embl_sequence(_unpack("7LESS_DROME", tmp))]
+
+
+@pytest.mark.gpu
+@needs_bins
+@pytest.mark.parametrize("models,extra", [
+    (("fn3.hmm", "globins4.hmm"), ()),
+    (("fn3.hmm", "globins4.hmm"), ("0.3", "nobias")),
+    (("Pkinase.hmm",), ()),
+], ids=["fn3+globins4", "fn3+globins4-F1=0.3-nobias", "Pkinase-3e"])
+def test_synthetic_targets(tmp_path, models, extra):
+    tmp = str(tmp_path)
+    seqs = os.path.join(tmp, "targets.fa")
+    synthetic_fasta(seqs, 700, 2024, _donors(tmp))
+    ref, gpu = _run_both(tmp, _models(tmp, models), seqs, extra)
+    _check_same(ref, gpu)
+    assert ref.count("\nmsv ") == len(models) * 700
+    passed = [int(l.split()[3]) for l in ref.splitlines() if l.startswith("Passed MSV filter:")]
+    assert len(passed) == len(models) and all(p > 0 for p in passed)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd") or not os.path.exists(GPUBIN),
+                    reason="needs a machine without a GPU and the built drop-in")
+def test_dropin_fails_loudly_without_gpu(tmp_path):
+    """No silent CPU fallback: without a device the drop-in p7_MSVFilter
+    raises through Easel's exception handler and the search stops."""
+    r = subprocess.run([GPUBIN, _unpack("fn3.hmm", tmp_path), _unpack("HBB_HUMAN", tmp_path),
+                        os.path.join(str(tmp_path), "o.txt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "no MI355X device" in r.stderr

@@ -147,6 +147,8 @@ struct msvg_ctx {
   int device = 0;
   int ncu = 256;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;      // batched mode: J-aware launches beside the fast chain
+  hipEvent_t fork = nullptr, join = nullptr;
   std::string err;
   int next_id = 1;
   std::map<int, std::unique_ptr<Profile>> profiles;
@@ -380,9 +382,28 @@ int check_ids(msvg_ctx *c, const int32_t *ids, int np, int dbid) {
 // first (Plan upload in search_device).
 // sched: 0 persistent (batched mode), 2 one chunk per workgroup (serving:
 // concurrent searches on separate streams share the CUs).
+// side: when given (batched searches on c->stream), each profile's J-aware
+// launch goes to this stream as soon as the profile's fast launch is done, so
+// it runs beside the next profiles' fast launches instead of after them
+// (each J-aware launch lasts as long as its longest pair, one wave per pair);
+// the stream joins back before the byte stage.
 int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, double F1, int mode, bool timed,
-                   bool capturing, int64_t *cells_out, hipStream_t st = nullptr, int sched = 0) {
+                   bool capturing, int64_t *cells_out, hipStream_t st = nullptr, int sched = 0,
+                   hipStream_t side = nullptr) {
   if (!st) st = c->stream;
+  bool side_used = false;
+  // J-aware launch of profile i on the side stream (or in line on st).
+  auto launch_j = [&](const JArgs &ja, int Q64, size_t jl, int grid) -> int {
+    hipStream_t js = st;
+    if (side) {
+      HIPCHK(c, hipEventRecord(c->fork, st));
+      HIPCHK(c, hipStreamWaitEvent(side, c->fork, 0));
+      js = side;
+      side_used = true;
+    }
+    HIPCHK(c, msvk_launch_jrow(Q64, &ja, grid, jl, js));
+    return MSVG_OK;
+  };
   Db &d = *c->dbs[dbid];
   const int64_t n = d.n;
   int rc;
@@ -507,7 +528,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
         const int Q64 = qfor(p.M, 128);
         const size_t jl = fast_lds(d.R, Q64);
         const int per_cu = (int) std::max<size_t>(1, std::min<size_t>(4, kLdsBytes / jl));
-        HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jl, st));
+        if ((rc = launch_j(ja, Q64, jl, c->ncu * per_cu)) != MSVG_OK) return rc;
       }
       continue;
     }
@@ -554,8 +575,12 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       const int Q64 = qfor(p.M, 128);
       const size_t jl = fast_lds(d.R, Q64);
       const int per_cu = (int) std::max<size_t>(1, std::min<size_t>(4, kLdsBytes / jl));
-      HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jl, st));
+      if ((rc = launch_j(ja, Q64, jl, c->ncu * per_cu)) != MSVG_OK) return rc;
     }
+  }
+  if (side_used) {
+    HIPCHK(c, hipEventRecord(c->join, side));
+    HIPCHK(c, hipStreamWaitEvent(st, c->join, 0));
   }
   if (timed) HIPCHK(c, hipEventRecord(c->ev[1], st));
 
@@ -624,10 +649,11 @@ int search_device(msvg_ctx *c, const int32_t *ids, int np, int dbid, double F1, 
         // Warm every cache (tables, chunks, buffers) outside the capture, then
         // capture the launch sequence once.
         auto pl = std::make_unique<Plan>();
-        if ((rc = enqueue_search(c, pl->bufs, ids, np, dbid, F1, mode, false, false, &cells)) != MSVG_OK) return rc;
+        if ((rc = enqueue_search(c, pl->bufs, ids, np, dbid, F1, mode, false, false, &cells, nullptr, 0,
+                                 c->side)) != MSVG_OK) return rc;
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-        rc = enqueue_search(c, pl->bufs, ids, np, dbid, F1, mode, false, true, &cells);
+        rc = enqueue_search(c, pl->bufs, ids, np, dbid, F1, mode, false, true, &cells, nullptr, 0, c->side);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc != MSVG_OK) { if (g) (void) hipGraphDestroy(g); return rc; }
@@ -647,7 +673,8 @@ int search_device(msvg_ctx *c, const int32_t *ids, int np, int dbid, double F1, 
       c->stats.ms_total = t; c->stats.ms_fast = c->stats.ms_slow = c->stats.ms_finalize = 0;
       c->stats.n_fast_launches = np;
     } else {
-      if ((rc = enqueue_search(c, c->eager, ids, np, dbid, F1, mode, true, false, &cells)) != MSVG_OK) return rc;
+      if ((rc = enqueue_search(c, c->eager, ids, np, dbid, F1, mode, true, false, &cells, nullptr, 0,
+                               mode == MSVG_MODE_BATCHED ? c->side : nullptr)) != MSVG_OK) return rc;
       rc = after_search(c, c->eager, np, dbid, cells, true);
       c->stats.n_fast_launches = np;
     }
@@ -678,6 +705,9 @@ int msvg_init(int device, msvg_ctx **ret) {
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return MSVG_EDEVICE;
   c->ncu = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MSVG_EDEVICE;
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) return MSVG_EDEVICE;
+  if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess) return MSVG_EDEVICE;
+  if (hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) return MSVG_EDEVICE;
   for (auto &e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return MSVG_EDEVICE;
   if (msvk_prepare() != hipSuccess) return MSVG_EDEVICE;
@@ -694,6 +724,9 @@ void msvg_free(msvg_ctx *c) {
   c->dbs.clear();
   c->profiles.clear();
   for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
+  if (c->fork) (void) hipEventDestroy(c->fork);
+  if (c->join) (void) hipEventDestroy(c->join);
+  if (c->side) { (void) hipStreamSynchronize(c->side); (void) hipStreamDestroy(c->side); }
   if (c->stream) (void) hipStreamDestroy(c->stream);
   delete c;
 }

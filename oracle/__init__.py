@@ -3,8 +3,9 @@
 TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` import this package.
 
-* :class:`Oracle` wraps ``oracle/liboracle.so`` -- the scalar C restatement of
-  ``p7_MSVFilter`` (impl_sse/msvfilter.c:78-213 with impl_sse/ssvfilter.c:1364-1413).
+* :class:`Oracle` wraps ``oracle/liboracle.so`` -- the scalar C restatements of
+  ``p7_MSVFilter`` (impl_sse/msvfilter.c:78-213 with impl_sse/ssvfilter.c:1364-1413)
+  and ``p7_ViterbiFilter`` (impl_sse/vitfilter.c:84-250).
 * :class:`Reference` wraps ``oracle/_ref/libmsvref.so`` -- the HMMER 3.1b1
   reference compiled from its own sources plus ``oracle/ref_harness.c``.
 """
@@ -28,6 +29,23 @@ class _OrcProfile(C.Structure):
     _fields_ = [("M", C.c_int), ("Kp", C.c_int), ("rbv", C.c_void_p),
                 ("tbm_b", C.c_int), ("tec_b", C.c_int), ("base_b", C.c_int),
                 ("bias_b", C.c_int), ("scale_b", C.c_float)]
+
+
+class _OrcVProfile(C.Structure):
+    _fields_ = [("M", C.c_int), ("Kp", C.c_int), ("rwv", C.c_void_p), ("twv", C.c_void_p),
+                ("e_loop", C.c_int), ("e_move", C.c_int), ("base_w", C.c_int), ("ddbound_w", C.c_int),
+                ("scale_w", C.c_float), ("nj", C.c_float)]
+
+
+def vprofile_from_fields(f):
+    """Viterbi profile dict in model order (k = 1..M) from ref_vit_fields'
+    padded arrays: {rwv (Kp x M int16), twv (8 x M int16), e_loop, e_move,
+    base_w, ddbound_w, scale_w, nj, vmu, vlambda}."""
+    M = f["M"]
+    return {"M": M, "rwv": np.ascontiguousarray(f["rwv_pad"][:, :M]),
+            "twv": np.ascontiguousarray(f["twv_pad"][:, :M]), "e_loop": f["e_loop"], "e_move": f["e_move"],
+            "base_w": f["base_w"], "ddbound_w": f["ddbound_w"], "scale_w": f["scale_w"], "nj": f["nj"],
+            "vmu": f["vmu"], "vlambda": f["vlambda"]}
 
 
 class Oracle:
@@ -55,6 +73,13 @@ class Oracle:
         lib.orc_decide.argtypes = [C.POINTER(_OrcProfile), C.c_int, C.c_int, C.POINTER(C.c_int)]
         lib.orc_msv_batch.restype = C.c_int
         lib.orc_msv_batch.argtypes = [C.POINTER(_OrcProfile), C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        lib.orc_vit_move.restype = C.c_int
+        lib.orc_vit_move.argtypes = [C.POINTER(_OrcVProfile), C.c_int]
+        lib.orc_vit.restype = C.c_int
+        lib.orc_vit.argtypes = [C.POINTER(_OrcVProfile), C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        lib.orc_vit_batch.restype = C.c_int
+        lib.orc_vit_batch.argtypes = [C.POINTER(_OrcVProfile), C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int, C.c_void_p, C.c_void_p, C.c_int]
         self.lib = lib
 
@@ -115,6 +140,38 @@ class Oracle:
     def passes(self, usc, L, mu, lam, F1=0.02):
         return self.lib.orc_pass(float(usc), int(L), float(mu), float(lam), float(F1))
 
+    # ---- p7_ViterbiFilter ----
+    @staticmethod
+    def _vprof(v):
+        rwv = np.ascontiguousarray(v["rwv"], dtype=np.int16)
+        twv = np.ascontiguousarray(v["twv"], dtype=np.int16)
+        s = _OrcVProfile(int(rwv.shape[1]), int(rwv.shape[0]), rwv.ctypes.data, twv.ctypes.data,
+                         int(v["e_loop"]), int(v["e_move"]), int(v["base_w"]), int(v["ddbound_w"]),
+                         float(v["scale_w"]), float(v["nj"]))
+        return s, (rwv, twv)
+
+    def vit_move(self, v, L):
+        s, keep = self._vprof(v)
+        return self.lib.orc_vit_move(C.byref(s), int(L))
+
+    def vit(self, v, x):
+        s, keep = self._vprof(v)
+        x = np.ascontiguousarray(x, dtype=np.uint8)
+        sc = C.c_float()
+        st = self.lib.orc_vit(C.byref(s), _p(x), int(x.size), C.byref(sc))
+        return st, sc.value
+
+    def vit_batch(self, v, res, off, lens, nthreads=8):
+        s, keep = self._vprof(v)
+        res = np.ascontiguousarray(res, dtype=np.uint8)
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        lens = np.ascontiguousarray(lens, dtype=np.int32)
+        n = lens.size
+        sc = np.zeros(n, np.float32)
+        st = np.zeros(n, np.int32)
+        self.lib.orc_vit_batch(C.byref(s), _p(res), _p(off), _p(lens), n, _p(sc), _p(st), nthreads)
+        return st, sc
+
 
 class Reference:
     """The compiled HMMER reference (only where oracle/_ref was built)."""
@@ -149,6 +206,13 @@ class Reference:
         lib.ref_digitize.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
         lib.ref_core_emit.restype = C.c_int64
         lib.ref_core_emit.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]
+        lib.ref_vit.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        lib.ref_vit_Q.argtypes = [C.c_void_p]
+        lib.ref_vit_fields.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ref_vit_move.argtypes = [C.c_void_p, C.c_int]
+        lib.ref_profile_tsc.argtypes = [C.c_void_p, C.c_void_p]
+        lib.ref_vit_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                      C.c_void_p, C.c_void_p, C.c_int]
         self.lib = lib
 
     def sample(self, seed, M, config_L=100, calibrate=True):
@@ -261,6 +325,47 @@ class RefProfile:
         st = np.zeros(n, np.int32)
         self.lib.ref_msv_batch(self.h, _p(res), _p(off), _p(lens), n, _p(usc), _p(st), nthreads)
         return st, usc
+
+    # ---- p7_ViterbiFilter ----
+    def vit_fields(self):
+        """The profile's ViterbiFilter fields, padded arrays over 8*Q positions
+        plus the scalars (see ref_vit_fields)."""
+        Q = self.lib.ref_vit_Q(self.h)
+        W, kp = 8 * Q, self.Kp
+        rwv = np.zeros(kp * W, np.int16)
+        twv = np.zeros(8 * W, np.int16)
+        ints = np.zeros(10, np.int32)
+        flts = np.zeros(4, np.float32)
+        self.lib.ref_vit_fields(self.h, _p(rwv), _p(twv), _p(ints), _p(flts))
+        return {"M": self.M, "Q": Q, "rwv_pad": rwv.reshape(kp, W), "twv_pad": twv.reshape(8, W),
+                "e_loop": int(ints[0]), "e_move": int(ints[1]), "n_move": int(ints[2]), "n_loop": int(ints[3]),
+                "c_move": int(ints[4]), "c_loop": int(ints[5]), "j_move": int(ints[6]), "j_loop": int(ints[7]),
+                "base_w": int(ints[8]), "ddbound_w": int(ints[9]), "scale_w": float(flts[0]),
+                "nj": float(flts[1]), "vmu": float(flts[2]), "vlambda": float(flts[3])}
+
+    def vit_move(self, L):
+        return self.lib.ref_vit_move(self.h, int(L))
+
+    def tsc(self):
+        out = np.zeros((self.M + 1) * 8, np.float32)
+        self.lib.ref_profile_tsc(self.h, _p(out))
+        return out.reshape(self.M + 1, 8)
+
+    def vit(self, x):
+        d = self._dsq(x)
+        sc = C.c_float()
+        st = self.lib.ref_vit(self.h, _p(d), len(x), C.byref(sc))
+        return st, sc.value
+
+    def vit_batch(self, res, off, lens, nthreads=8):
+        res = np.ascontiguousarray(res, np.uint8)
+        off = np.ascontiguousarray(off, np.int64)
+        lens = np.ascontiguousarray(lens, np.int32)
+        n = lens.size
+        sc = np.zeros(n, np.float32)
+        st = np.zeros(n, np.int32)
+        self.lib.ref_vit_batch(self.h, _p(res), _p(off), _p(lens), n, _p(sc), _p(st), nthreads)
+        return st, sc
 
     def core_emit(self, seed, n, cap=None):
         cap = cap or n * (self.M * 40 + 2000)

@@ -21,6 +21,11 @@
  *                        hmmsearch.c:429-439 gives each worker its own copy)
  *   ref_iid / ref_core_emit  esl_rsq_xfIID (esl_randomseq.c:764) and
  *                        p7_CoreEmit (emit.c:71) for generator pinning
+ *   ref_vit              p7_oprofile_ReconfigLength + p7_ViterbiFilter
+ *                        (impl_sse/vitfilter.c:84); ref_vit_batch on N threads
+ *   ref_vit_fields       the ViterbiFilter part of P7_OPROFILE (rwv, twv, xw,
+ *                        scale_w, base_w, ddbound_w, nj), de-striped
+ *   ref_profile_tsc      the generic profile's transition scores (gm->tsc)
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -219,6 +224,7 @@ typedef struct {
   int32_t *status;
   int n;
   int next;                /* shared work counter */
+  int vit;                 /* 0: p7_MSVFilter, 1: p7_ViterbiFilter */
   pthread_mutex_t lock;
 } BATCH;
 
@@ -242,7 +248,8 @@ static void *batch_worker(void *arg)
       dsq[0] = dsq[L + 1] = eslDSQ_SENTINEL;
       memcpy(dsq + 1, b->res + b->off[i], L);
       p7_oprofile_ReconfigLength(om, L);
-      b->status[i] = p7_MSVFilter((const ESL_DSQ *) dsq, L, om, ox, &b->usc[i]);
+      b->status[i] = b->vit ? p7_ViterbiFilter((const ESL_DSQ *) dsq, L, om, ox, &b->usc[i])
+                            : p7_MSVFilter((const ESL_DSQ *) dsq, L, om, ox, &b->usc[i]);
     }
   }
   free(dsq);
@@ -251,15 +258,15 @@ static void *batch_worker(void *arg)
   return NULL;
 }
 
-int ref_msv_batch(void *vh, const uint8_t *res, const int64_t *off, const int32_t *len, int n,
-                  float *usc, int32_t *status, int nthreads)
+static int run_batch(void *vh, const uint8_t *res, const int64_t *off, const int32_t *len, int n,
+                     float *usc, int32_t *status, int nthreads, int vit)
 {
   BATCH b;
   pthread_t *tid;
   int t;
   if (nthreads < 1) nthreads = 1;
   b.h = vh; b.res = res; b.off = off; b.len = len; b.usc = usc; b.status = status;
-  b.n = n; b.next = 0;
+  b.n = n; b.next = 0; b.vit = vit;
   pthread_mutex_init(&b.lock, NULL);
   tid = malloc(sizeof(pthread_t) * nthreads);
   for (t = 0; t < nthreads; t++) pthread_create(&tid[t], NULL, batch_worker, &b);
@@ -267,6 +274,86 @@ int ref_msv_batch(void *vh, const uint8_t *res, const int64_t *off, const int32_
   free(tid);
   pthread_mutex_destroy(&b.lock);
   return 0;
+}
+
+int ref_msv_batch(void *vh, const uint8_t *res, const int64_t *off, const int32_t *len, int n,
+                  float *usc, int32_t *status, int nthreads)
+{
+  return run_batch(vh, res, off, len, n, usc, status, nthreads, 0);
+}
+
+int ref_vit_batch(void *vh, const uint8_t *res, const int64_t *off, const int32_t *len, int n,
+                  float *sc, int32_t *status, int nthreads)
+{
+  return run_batch(vh, res, off, len, n, sc, status, nthreads, 1);
+}
+
+/* ---- the ViterbiFilter stage ---- */
+
+/* p7_ViterbiFilter on dsq[0..L+1] (digital, sentinels at 0 and L+1). */
+int ref_vit(void *vh, const uint8_t *dsq, int L, float *sc)
+{
+  REFPROF *h = vh;
+  p7_oprofile_ReconfigLength(h->om, L);
+  return p7_ViterbiFilter((const ESL_DSQ *) dsq, L, h->om, h->ox, sc);
+}
+
+/* Q = p7O_NQW(M) of the striped Viterbi vectors. */
+int ref_vit_Q(void *vh) { return p7O_NQW(((REFPROF *) vh)->om->M); }
+
+/* The ViterbiFilter fields, de-striped over all 8*Q positions (position
+ * p = q + z*Q holds node k = p+1; positions >= M are the padding):
+ *   rwv[x * 8Q + p], twv[t * 8Q + p] with t = p7O_BM..p7O_DD (vitfilter.c's
+ *   vector order: the 7 per-q transitions, then the DD run);
+ *   ints = {xw[E][LOOP], xw[E][MOVE], xw[N][MOVE], xw[N][LOOP], xw[C][MOVE],
+ *           xw[C][LOOP], xw[J][MOVE], xw[J][LOOP], base_w, ddbound_w}
+ *          (the N/C/J moves for the length the profile was last set to);
+ *   flts = {scale_w, nj, evparam[VMU], evparam[VLAMBDA]}. */
+void ref_vit_fields(void *vh, int16_t *rwv, int16_t *twv, int *ints, float *flts)
+{
+  P7_OPROFILE *om = ((REFPROF *) vh)->om;
+  const int Q = p7O_NQW(om->M), Kp = om->abc->Kp, W = 8 * Q;
+  int x, q, z, t;
+  union { __m128i v; int16_t i[8]; } u;
+  for (x = 0; x < Kp; x++)
+    for (q = 0; q < Q; q++) {
+      u.v = om->rwv[x][q];
+      for (z = 0; z < 8; z++) rwv[x * W + q + z * Q] = u.i[z];
+    }
+  for (q = 0; q < Q; q++)
+    for (t = 0; t < 7; t++) {
+      u.v = om->twv[q * 7 + t];
+      for (z = 0; z < 8; z++) twv[t * W + q + z * Q] = u.i[z];
+    }
+  for (q = 0; q < Q; q++) {
+    u.v = om->twv[7 * Q + q];
+    for (z = 0; z < 8; z++) twv[7 * W + q + z * Q] = u.i[z];
+  }
+  ints[0] = om->xw[p7O_E][p7O_LOOP]; ints[1] = om->xw[p7O_E][p7O_MOVE];
+  ints[2] = om->xw[p7O_N][p7O_MOVE]; ints[3] = om->xw[p7O_N][p7O_LOOP];
+  ints[4] = om->xw[p7O_C][p7O_MOVE]; ints[5] = om->xw[p7O_C][p7O_LOOP];
+  ints[6] = om->xw[p7O_J][p7O_MOVE]; ints[7] = om->xw[p7O_J][p7O_LOOP];
+  ints[8] = om->base_w; ints[9] = om->ddbound_w;
+  flts[0] = om->scale_w; flts[1] = om->nj;
+  flts[2] = om->evparam[p7_VMU]; flts[3] = om->evparam[p7_VLAMBDA];
+}
+
+/* xw[N][MOVE] after p7_oprofile_ReconfigLength(om, L). */
+int ref_vit_move(void *vh, int L)
+{
+  P7_OPROFILE *om = ((REFPROF *) vh)->om;
+  p7_oprofile_ReconfigLength(om, L);
+  return om->xw[p7O_N][p7O_MOVE];
+}
+
+/* Generic-profile transition scores: out[k * 8 + s] = p7P_TSC(gm, k, s) for
+ * k = 0..M, s = p7P_MM, IM, DM, BM, MD, MI, II, DD (hmmer.h order). */
+void ref_profile_tsc(void *vh, float *out)
+{
+  P7_PROFILE *gm = ((REFPROF *) vh)->gm;
+  int k, s;
+  for (k = 0; k <= gm->M; k++)
+    for (s = 0; s < p7P_NTRANS; s++) out[k * p7P_NTRANS + s] = p7P_TSC(gm, k, s);
 }
 
 /* ---- generators, for pinning the product's restatement ---- */

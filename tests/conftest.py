@@ -46,3 +46,41 @@ def golden_cases(z, pi=None):
         o, L = int(z["case_off"][c]), int(z["case_len"][c])
         out.append((c, p, z["res"][o:o + L]))
     return out
+
+
+@pytest.fixture(scope="session")
+def vit_golden():
+    import json
+    z = np.load(os.path.join(GOLDEN, "vit_golden.npz"))
+    with open(os.path.join(GOLDEN, "vit_manifest.json")) as fh:
+        man = json.load(fh)
+    return z, man
+
+
+def _bits_to_float(b):
+    return float(np.array([b], np.uint32).view(np.float32)[0])
+
+
+def vit_golden_profile(z, man, i):
+    """The i-th Viterbi golden profile as a dict (model order, k = 1..M)."""
+    metas = man["profiles"]
+    ro = sum(m["M"] * m["Kp"] for m in metas[:i])
+    to = sum(m["M"] * 8 for m in metas[:i])
+    m = metas[i]
+    M, Kp = m["M"], m["Kp"]
+    return {"M": M, "Kp": Kp, "rwv": z["rwv"][ro:ro + M * Kp].reshape(Kp, M),
+            "twv": z["twv"][to:to + 8 * M].reshape(8, M), "e_loop": m["e_loop"], "e_move": m["e_move"],
+            "base_w": m["base_w"], "ddbound_w": m["ddbound_w"], "scale_w": _bits_to_float(m["scale_w_bits"]),
+            "nj": _bits_to_float(m["nj_bits"]), "vmu": _bits_to_float(m["vmu_bits"]),
+            "vlambda": _bits_to_float(m["vlambda_bits"]), "name": m["name"], "seed": m["seed"]}
+
+
+def vit_golden_cases(z, pi=None):
+    out = []
+    for c in range(z["case_len"].size):
+        p = int(z["case_prof"][c])
+        if pi is not None and p != pi:
+            continue
+        o, L = int(z["case_off"][c]), int(z["case_len"][c])
+        out.append((c, p, z["residues"][o:o + L]))
+    return out

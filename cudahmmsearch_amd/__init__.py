@@ -58,6 +58,12 @@ class _Stats(C.Structure):
                 ("n_fast_launches", C.c_int32)]
 
 
+class _VProf(C.Structure):
+    _fields_ = [("M", C.c_int32), ("Kp", C.c_int32), ("rwv", C.c_void_p), ("twv", C.c_void_p),
+                ("e_loop", C.c_int16), ("e_move", C.c_int16), ("base_w", C.c_int16), ("ddbound_w", C.c_int16),
+                ("scale_w", C.c_float), ("nj", C.c_float), ("vmu", C.c_float), ("vlambda", C.c_float)]
+
+
 class _QRes(C.Structure):
     _fields_ = [("ticket", C.c_int64), ("status", C.c_int32), ("M", C.c_int32), ("n_pass", C.c_int64),
                 ("cells", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
@@ -103,6 +109,14 @@ def load_library(path=LIB_PATH):
         "msvg_last_stats": (C.c_int, [vp, C.POINTER(_Stats)]),
         "msvg_MSVFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
+        "msvg_profile_set_viterbi": (C.c_int, [vp, i32, C.POINTER(_VProf)]),
+        "msvg_vit_search": (C.c_int, [vp, i32, i32, vp, i64, vp, vp]),
+        "msvg_ViterbiFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
+        "msvg_calibrate_vmu": (C.c_int, [vp, i32, C.c_double, C.POINTER(C.c_float)]),
+        "msvh_vit_move": (C.c_int, [C.c_float, C.c_float, C.c_int]),
+        "msvh_vit_score": (C.c_float, [C.c_int, C.c_int, C.c_int, C.c_float]),
+        "msvh_fit_vmu": (C.c_double, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_double]),
+        "msvh_vit_calibration_targets": (None, [C.c_int, C.c_int, vp]),
         "msvg_test_alloc_budget": (None, [C.c_long]),
         "msvg_server_start": (C.c_int, [vp, i32, i32, C.c_double, i64, i32, C.POINTER(vp)]),
         "msvg_server_submit": (C.c_int, [vp, C.POINTER(_Prof), C.POINTER(i64)]),
@@ -174,6 +188,43 @@ class ByteProfile:
         return _Prof(self.M, self.rbv.shape[0], self.rbv.ctypes.data, self.tbm, self.tec, self.base, self.bias,
                      self.scale, float(self.mu if self.mu is not None else -99999.0),
                      float(self.lam if self.lam is not None else -99999.0))
+
+
+class ViterbiProfile:
+    """The ViterbiFilter fields of a P7_OPROFILE (impl_sse.h:88-101) in model
+    order: ``rwv`` (Kp, M) int16 match emissions, ``twv`` (8, M) int16
+    transitions as the reference's striped vectors hold them at node k (BM,
+    MM, IM, DM into k; MD, MI, II, DD out of k), the E-state costs, base_w,
+    ddbound_w, scale_w, nj and the calibrated Viterbi Gumbel parameters."""
+
+    def __init__(self, rwv, twv, e_loop, e_move, base_w, ddbound_w, scale_w, nj=1.0, vmu=None, vlambda=None):
+        self.rwv = np.ascontiguousarray(rwv, dtype=np.int16)
+        self.twv = np.ascontiguousarray(twv, dtype=np.int16)
+        self.M = self.rwv.shape[1]
+        self.e_loop, self.e_move, self.base_w, self.ddbound_w = int(e_loop), int(e_move), int(base_w), int(ddbound_w)
+        self.scale_w, self.nj = float(scale_w), float(nj)
+        self.vmu, self.vlambda = vmu, vlambda
+
+    @classmethod
+    def from_dict(cls, d):
+        return cls(d["rwv"], d["twv"], d["e_loop"], d["e_move"], d["base_w"], d["ddbound_w"], d["scale_w"],
+                   d.get("nj", 1.0), d.get("vmu"), d.get("vlambda"))
+
+    def as_dict(self):
+        return {"rwv": self.rwv, "twv": self.twv, "e_loop": self.e_loop, "e_move": self.e_move,
+                "base_w": self.base_w, "ddbound_w": self.ddbound_w, "scale_w": self.scale_w, "nj": self.nj,
+                "vmu": self.vmu, "vlambda": self.vlambda}
+
+    def _struct(self):
+        return _VProf(self.M, self.rwv.shape[0], self.rwv.ctypes.data, self.twv.ctypes.data, self.e_loop,
+                      self.e_move, self.base_w, self.ddbound_w, self.scale_w, self.nj,
+                      float(self.vmu if self.vmu is not None else -99999.0),
+                      float(self.vlambda if self.vlambda is not None else -99999.0))
+
+
+def vit_move(scale_w, nj, L):
+    """xw[N|C|J][MOVE] for target length L (p7_oprofile.c:944-962)."""
+    return load_library().msvh_vit_move(float(scale_w), float(nj), int(L))
 
 
 def profile_from_hmm(mat, ins=None, t=None, config_L=100):
@@ -438,6 +489,34 @@ class MSVContext:
         if rc not in (eslOK, eslERANGE):
             self._chk(rc)
         return rc, sc.value
+
+    # ---- ViterbiFilter stage ----
+    def set_viterbi(self, pid, vprof):
+        """Attach a ViterbiProfile to profile pid (same M)."""
+        s = vprof._struct()
+        self._chk(self.lib.msvg_profile_set_viterbi(self._c, pid, C.byref(s)))
+
+    def vit_search(self, pid, did, targets):
+        """p7_ViterbiFilter on the listed targets (caller indices): (sc, status)."""
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        sc = np.zeros(t.size, np.float32)
+        st = np.zeros(t.size, np.uint8)
+        self._chk(self.lib.msvg_vit_search(self._c, pid, did, _p(t), t.size, _p(sc), _p(st)))
+        return sc, st
+
+    def vit_filter(self, pid, x):
+        """p7_ViterbiFilter on one target (codes, no sentinels): (status, sc)."""
+        dsq = np.concatenate([[255], np.asarray(x, np.uint8), [255]]).astype(np.uint8)
+        sc = C.c_float()
+        rc = self.lib.msvg_ViterbiFilter(self._c, pid, _p(dsq), len(x), C.byref(sc))
+        if rc not in (eslOK, eslERANGE):
+            self._chk(rc)
+        return rc, sc.value
+
+    def calibrate_vmu(self, pid, lam):
+        mu = C.c_float()
+        self._chk(self.lib.msvg_calibrate_vmu(self._c, pid, float(lam), C.byref(mu)))
+        return mu.value
 
     def set_tile(self, m_lo, m_hi, lanes, waves=0):
         self._chk(self.lib.msvg_set_tile(self._c, m_lo, m_hi, lanes, waves))

@@ -126,3 +126,29 @@ struct SegArgs {
   int32_t *jslow_count;
   int32_t jslow_cap;
 };
+
+// ---------------------------------------------------------------------------
+// ViterbiFilter stage (impl_sse/vitfilter.c:84-250) for listed pairs.
+//
+// W waves (64*W lanes, 128*W int16 slots) own one target; register j of the
+// lane with index c = 64*w + lane holds slots 2c (low half) and 2c+1 (high
+// half), model position k = j + s*Q + 1 for slot s (Q = ceil(M / (128*W))).
+// Tables, one u32 (two int16) per (row, j, c):
+//   transitions  T[(t * Q + j) * 64W + c], t = BM, MM, IM, DM, MD, MI, II, DD,
+//                the values of the reference's striped vectors at k (rotated
+//                BM/MM/IM/DM: into k from k-1), -32768 past the model;
+//   emissions    E[(x * Q + j) * 64W + c] for database row x (R+1 rows; the
+//                null row is never read: rows run over the real residues).
+// Result per item: (status << 16) | (uint16) xC, status 0 or 16 (eslERANGE);
+// the host turns xC into the score with the reference's float expression.
+struct VitArgs {
+  const uint32_t *etab;
+  const uint32_t *ttab;
+  DbDev db;
+  const int64_t *items;        // sorted target index per item
+  const int16_t *move;         // xw[N|C|J][MOVE] for the item's length
+  int32_t n;
+  int32_t *out;
+  int32_t e_loop, e_move, base_w, ddbound_w;
+  int32_t etab_in_lds;         // one-wave shape: stage the emission table in LDS
+};

@@ -283,6 +283,43 @@ void msvh_calibration_targets(int N, int L, uint8_t *out) {
   for (int i = 0; i < N; i++) msvh_iid(&r, L, out + (size_t) i * L);
 }
 
+int msvh_vit_move(float scale_w, float nj, int L) {
+  const float pmove = (2.0f + nj) / ((float) L + 2.0f + nj);  // p7_oprofile.c:948
+  float sc = logf(pmove);                                     // :956
+  sc = roundf(scale_w * sc);                                  // wordify, :530-533
+  if (sc >= 32767.0) return 32767;
+  else if (sc <= -32768.0) return -32768;
+  else return (int16_t) sc;
+}
+
+float msvh_vit_score(int xC, int move, int base_w, float scale_w) {
+  if (xC <= -32768) return -INFINITY;                         // vitfilter.c:240, :247
+  float sc = (float) xC + (float) move - (float) base_w;      // :242
+  sc /= scale_w;                                              // :244
+  sc -= 3.0;                                                  // :245
+  return sc;
+}
+
+double msvh_fit_vmu(const float *sc, const int32_t *status, int N, int L, int base_w, float scale_w, double lambda) {
+  const float maxsc = (32767.0 - base_w) / scale_w;           // evalues.c:321
+  const float nullsc = msvh_nullsc_impl(L);
+  double esum = 0.;
+  for (int i = 0; i < N; i++) {
+    const float s = (status[i] == 16) ? maxsc : sc[i];
+    const double xv = (s - nullsc) / kLog2;                   // evalues.c:343
+    esum += exp(-lambda * xv);                                // esl_gumbel.c:498-500
+  }
+  return -log(esum / N) / lambda;
+}
+
+void msvh_vit_calibration_targets(int N, int L, uint8_t *out) {
+  msvh_rng r;
+  msvh_rng_init(&r, 42, 1);
+  std::vector<uint8_t> skip((size_t) L);
+  for (int i = 0; i < N; i++) msvh_iid(&r, L, skip.data());   // p7_MSVMu's draws
+  for (int i = 0; i < N; i++) msvh_iid(&r, L, out + (size_t) i * L);
+}
+
 int64_t msvh_gen_lengths(uint32_t seed, int64_t n, double median, double sigma, int lo, int hi, double tail_frac,
                          double tail_xm, double tail_alpha, int tail_max, int32_t *lens) {
   msvh_rng r;

@@ -32,12 +32,14 @@
 #include "msv_device.h"
 #include "msv_host.h"
 #include "../../include/msv_prep.h"
+#include "../../include/msv_prep.h"
 
 extern "C" hipError_t msvk_prepare();
 extern "C" hipError_t msvk_launch_fast(int G, int Q, const FastArgs *a, int grid, int block, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_seg(int Q, const SegArgs *a, int grid, int block, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st);
 extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st);
+extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int grid, hipStream_t st);
 extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st);
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
                                             int Q, int bias, uint32_t *out, hipStream_t st);
@@ -95,12 +97,21 @@ struct DevMem {
   template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+struct VitProfile {               // ViterbiFilter fields (model order)
+  std::vector<int16_t> rwv, twv;   // Kp * M, 8 * M
+  int e_loop = 0, e_move = 0, base_w = 0, ddbound_w = 0;
+  float scale_w = 0.f, nj = 0.f, vmu = 0.f, vlambda = 0.f;
+  int Q = 1, W = 1;                // register blocks per lane, waves per target
+  DevMem d_ttab;                   // transition table, VitArgs layout
+};
+
 struct Profile {
   int M = 0, Kp = 0;
   std::vector<uint8_t> rbv;        // Kp * M
   int tbm = 0, tec = 0, base = 0, bias = 0;
   float scale = 0.f, mu = 0.f, lambda = 0.f;
   DevMem d_rbv;
+  std::unique_ptr<VitProfile> vit;
 };
 
 struct Db {
@@ -121,6 +132,7 @@ struct Tables {                          // per (profile, db)
   std::map<int, std::unique_ptr<DevMem>> fast;   // G -> table
   std::vector<std::unique_ptr<DevMem>> seg;      // segment tables (M > fast limit)
   std::unique_ptr<DevMem> slow;
+  std::unique_ptr<DevMem> vemit;                 // ViterbiFilter emission table
 };
 
 struct Tile { int lo, hi, lanes, waves; };
@@ -165,6 +177,8 @@ struct msvg_ctx {
   std::map<std::string, std::unique_ptr<Plan>> plans;
   // single-target drop-in
   int scratch_db = -1;
+  // ViterbiFilter searches
+  DevMem d_vitems, d_vmove, d_vout;
 };
 
 namespace {
@@ -688,6 +702,53 @@ int search_device(msvg_ctx *c, const int32_t *ids, int np, int dbid, double F1, 
 
 }  // namespace
 
+// ViterbiFilter tables (host-built, see VitArgs in msv_device.h).
+namespace {
+
+// Waves per target and registers per lane: one wave up to M = 1024, then the
+// fewest waves (up to 16) that keep Q <= 8.
+void vit_shape(int M, int &Q, int &W) {
+  W = 1;
+  while (128 * W * 8 < M) W *= 2;
+  Q = (M + 128 * W - 1) / (128 * W);
+}
+
+// Pack two int16 model-order values for lane c, register j (slots 2c, 2c+1).
+template <class F>
+void vit_fill(std::vector<uint32_t> &out, size_t base, int Q, int W, int M, F value) {
+  const int C = 64 * W;
+  for (int j = 0; j < Q; ++j)
+    for (int c = 0; c < C; ++c) {
+      const int k0 = j + (2 * c) * Q + 1, k1 = j + (2 * c + 1) * Q + 1;
+      const int v0 = k0 <= M ? value(k0) : -32768, v1 = k1 <= M ? value(k1) : -32768;
+      out[base + (size_t) j * C + c] = (uint32_t) (uint16_t) v0 | ((uint32_t) (uint16_t) v1 << 16);
+    }
+}
+
+int vit_emission_table(msvg_ctx *c, int pid, int dbid, const uint32_t **out) {
+  Profile &p = *c->profiles[pid];
+  VitProfile &v = *p.vit;
+  Db &d = *c->dbs[dbid];
+  Tables &t = tables_for(c, pid, dbid);
+  if (!t.vemit) {
+    const size_t per = (size_t) v.Q * 64 * v.W;
+    std::vector<uint32_t> h((size_t) (d.R + 1) * per);
+    for (int r = 0; r <= d.R; ++r) {
+      const int x = (r < d.R) ? d.rowcode[r] : -1;
+      vit_fill(h, (size_t) r * per, v.Q, v.W, p.M,
+               [&](int k) { return x < 0 ? -32768 : (int) v.rwv[(size_t) x * p.M + k - 1]; });
+    }
+    auto mem = std::make_unique<DevMem>();
+    HIPCHK(c, mem->ensure(h.size() * 4));
+    HIPCHK(c, hipMemcpy(mem->p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    t.vemit = std::move(mem);
+  }
+  *out = t.vemit->as<uint32_t>();
+  return MSVG_OK;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -944,6 +1005,160 @@ int msvg_calibrate_mu(msvg_ctx *c, int32_t id, double lambda, float *ret_mu) {
   std::vector<int32_t> st32(st.begin(), st.end());
   const Profile &p = *c->profiles[id];
   *ret_mu = (float) msvh_fit_mu(usc.data(), st32.data(), N, L, p.base, p.scale, lambda);
+  return MSVG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ViterbiFilter stage
+// ---------------------------------------------------------------------------
+
+int msvg_profile_set_viterbi(msvg_ctx *c, int32_t pid, const msvg_vprofile *vp) {
+  if (!c || !vp || !vp->rwv || !vp->twv) return fail(c, MSVG_EINVAL, "null argument");
+  auto it = c->profiles.find(pid);
+  if (it == c->profiles.end()) return fail(c, MSVG_EINVAL, "unknown profile id");
+  Profile &p = *it->second;
+  if (vp->M != p.M || vp->Kp != p.Kp) return fail(c, MSVG_EINVAL, "M or Kp differs from the profile's");
+  if (!(vp->scale_w > 0.f)) return fail(c, MSVG_EINVAL, "scale_w must be positive");
+  (void) hipSetDevice(c->device);
+  auto v = std::make_unique<VitProfile>();
+  v->rwv.assign(vp->rwv, vp->rwv + (size_t) p.Kp * p.M);
+  v->twv.assign(vp->twv, vp->twv + (size_t) 8 * p.M);
+  v->e_loop = vp->e_loop; v->e_move = vp->e_move; v->base_w = vp->base_w; v->ddbound_w = vp->ddbound_w;
+  v->scale_w = vp->scale_w; v->nj = vp->nj; v->vmu = vp->vmu; v->vlambda = vp->vlambda;
+  vit_shape(p.M, v->Q, v->W);
+  const size_t per = (size_t) v->Q * 64 * v->W;
+  std::vector<uint32_t> h(8 * per);
+  for (int t = 0; t < 8; ++t)
+    vit_fill(h, t * per, v->Q, v->W, p.M, [&](int k) { return (int) v->twv[(size_t) t * p.M + k - 1]; });
+  HIPCHK(c, v->d_ttab.ensure(h.size() * 4));
+  HIPCHK(c, hipMemcpy(v->d_ttab.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  p.vit = std::move(v);
+  for (auto &kv : c->tables)                       // emission tables of the old fields
+    if (kv.first.first == pid) kv.second->vemit.reset();
+  return MSVG_OK;
+}
+
+int msvg_vit_search(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, int64_t n, float *sc,
+                    uint8_t *status) {
+  if (!c) return MSVG_EINVAL;
+  if (n < 0 || (n > 0 && (!targets || !sc || !status))) return fail(c, MSVG_EINVAL, "null argument");
+  auto pit = c->profiles.find(pid);
+  if (pit == c->profiles.end()) return fail(c, MSVG_EINVAL, "unknown profile id");
+  Profile &p = *pit->second;
+  if (!p.vit) return fail(c, MSVG_EINVAL, "profile has no ViterbiFilter fields (msvg_profile_set_viterbi)");
+  auto dit = c->dbs.find(dbid);
+  if (dit == c->dbs.end()) return fail(c, MSVG_EINVAL, "unknown database id");
+  Db &d = *dit->second;
+  if (d.maxcode >= p.Kp) return fail(c, MSVG_EINVAL, "database holds residue codes outside the profile's alphabet");
+  for (int64_t i = 0; i < n; ++i)
+    if (targets[i] < 0 || targets[i] >= d.n) return fail(c, MSVG_EINVAL, "target index out of range");
+  c->stats = msvg_stats{};
+  if (n == 0) return MSVG_OK;
+  (void) hipSetDevice(c->device);
+  VitProfile &v = *p.vit;
+  const uint32_t *etab;
+  int rc = vit_emission_table(c, pid, dbid, &etab);
+  if (rc != MSVG_OK) return rc;
+
+  // Longest targets first, so the last waves to start are the shortest.
+  std::vector<int64_t> order((size_t) n);
+  for (int64_t i = 0; i < n; ++i) order[(size_t) i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return d.len_sorted[d.sorted_of[targets[a]]] > d.len_sorted[d.sorted_of[targets[b]]];
+  });
+  std::vector<int64_t> items((size_t) n);
+  std::vector<int16_t> mv((size_t) n);
+  int64_t cells = 0;
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t s = d.sorted_of[targets[order[(size_t) q]]];
+    items[(size_t) q] = s;
+    mv[(size_t) q] = (int16_t) msvh_vit_move(v.scale_w, v.nj, d.len_sorted[s]);
+    cells += (int64_t) p.M * d.len_sorted[s];
+  }
+  HIPCHK(c, c->d_vitems.ensure((size_t) n * sizeof(int64_t)));
+  HIPCHK(c, c->d_vmove.ensure((size_t) n * sizeof(int16_t)));
+  HIPCHK(c, c->d_vout.ensure((size_t) n * sizeof(int32_t)));
+  hipStream_t st = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->d_vitems.p, items.data(), (size_t) n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_vmove.p, mv.data(), (size_t) n * sizeof(int16_t), hipMemcpyHostToDevice, st));
+
+  VitArgs a;
+  a.etab = etab;
+  a.ttab = v.d_ttab.as<uint32_t>();
+  a.db.res = d.d_res.as<uint8_t>(); a.db.off = d.d_off.as<int64_t>(); a.db.len = d.d_len.as<int32_t>();
+  a.db.tjb = d.d_tjb.as<uint8_t>(); a.db.n = d.n; a.db.R = d.R;
+  a.items = c->d_vitems.as<int64_t>();
+  a.move = c->d_vmove.as<int16_t>();
+  a.n = (int32_t) n;
+  a.out = c->d_vout.as<int32_t>();
+  a.e_loop = v.e_loop; a.e_move = v.e_move; a.base_w = v.base_w; a.ddbound_w = v.ddbound_w;
+  a.etab_in_lds = (v.W == 1 && (size_t) (d.R + 1) * v.Q * 64 * 4 <= 65536) ? 1 : 0;
+  int grid;
+  if (v.W == 1) grid = (int) std::min<int64_t>((n + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
+  else grid = (int) std::min<int64_t>(n, (int64_t) c->ncu * std::max(1, 8 / v.W));
+  HIPCHK(c, hipEventRecord(c->ev[0], st));
+  HIPCHK(c, msvk_launch_vit(v.Q, v.W, &a, grid, st));
+  HIPCHK(c, hipEventRecord(c->ev[1], st));
+  std::vector<int32_t> out((size_t) n);
+  HIPCHK(c, hipMemcpyAsync(out.data(), c->d_vout.p, (size_t) n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t i = order[(size_t) q];
+    const int32_t r = out[(size_t) q];
+    if ((r >> 16) == MSVG_ERANGE) {
+      status[i] = MSVG_ERANGE;
+      sc[i] = INFINITY;                                        // vitfilter.c:177
+    } else {
+      status[i] = MSVG_OK;
+      sc[i] = msvh_vit_score((int16_t) (r & 0xffff), mv[(size_t) q], v.base_w, v.scale_w);
+    }
+  }
+  c->stats.ms_fast = ms;
+  c->stats.ms_total = ms;
+  c->stats.cells = cells;
+  c->stats.n_fast_launches = 1;
+  return MSVG_OK;
+}
+
+int msvg_ViterbiFilter(msvg_ctx *c, int32_t pid, const uint8_t *dsq, int L, float *ret_sc) {
+  if (!c || !dsq || !ret_sc || L < 0) return fail(c, MSVG_EINVAL, "null argument");
+  if (!c->profiles.count(pid)) return fail(c, MSVG_EINVAL, "unknown profile id");
+  if (c->scratch_db >= 0) msvg_db_remove(c, c->scratch_db);
+  c->scratch_db = -1;
+  const int64_t off = 0;
+  int32_t id;
+  int rc = msvg_db_add(c, dsq + 1, &off, &L, 1, &id);
+  if (rc != MSVG_OK) return rc;
+  c->scratch_db = id;
+  const int64_t t = 0;
+  uint8_t st;
+  rc = msvg_vit_search(c, pid, id, &t, 1, ret_sc, &st);
+  if (rc != MSVG_OK) return rc;
+  return st == MSVG_ERANGE ? MSVG_ERANGE : MSVG_OK;
+}
+
+int msvg_calibrate_vmu(msvg_ctx *c, int32_t id, double lambda, float *ret_vmu) {
+  if (!c || !ret_vmu || !c->profiles.count(id)) return fail(c, MSVG_EINVAL, "unknown profile id");
+  if (!c->profiles[id]->vit) return fail(c, MSVG_EINVAL, "profile has no ViterbiFilter fields");
+  const int N = 200, L = 200;                       // EvN, EvL: evalues.c:69-70
+  std::vector<uint8_t> res((size_t) N * L);
+  msvh_vit_calibration_targets(N, L, res.data());
+  std::vector<int64_t> off(N), tg(N);
+  std::vector<int32_t> len(N, L);
+  for (int i = 0; i < N; ++i) { off[i] = (int64_t) i * L; tg[i] = i; }
+  int32_t db;
+  int rc = msvg_db_add(c, res.data(), off.data(), len.data(), N, &db);
+  if (rc != MSVG_OK) return rc;
+  std::vector<float> sc(N);
+  std::vector<uint8_t> st(N);
+  rc = msvg_vit_search(c, id, db, tg.data(), N, sc.data(), st.data());
+  msvg_db_remove(c, db);
+  if (rc != MSVG_OK) return rc;
+  std::vector<int32_t> st32(st.begin(), st.end());
+  const VitProfile &v = *c->profiles[id]->vit;
+  *ret_vmu = (float) msvh_fit_vmu(sc.data(), st32.data(), N, L, v.base_w, v.scale_w, lambda);
   return MSVG_OK;
 }
 

@@ -1,0 +1,238 @@
+// vit_kernels.hip -- gfx950 kernel for HMMER3's ViterbiFilter, the filter
+// stage after MSV in p7_Pipeline (impl_sse/vitfilter.c:84-250), bit-exact.
+//
+// The reference runs Farrar's striped int16 recurrence over 8 words per SSE
+// vector.  Here W wavefronts (one when M <= 1024) own a target and stripe its
+// model over 128*W int16 slots, two per VGPR: register j of lane c holds model
+// positions k = j + s*Q + 1 for its slots s = 2c, 2c+1.  Every cell does the
+// reference's operations in the reference's order with packed saturating
+// int16 arithmetic (v_pk_add_i16 clamp = _mm_adds_epi16, v_pk_max_i16 =
+// _mm_max_epi16), so values do not depend on the striping.  Per row:
+//   - the diagonal predecessors of register 0 come from the last register one
+//     slot down: DPP wave_shr:1 + v_alignbit, and across waves through LDS;
+//   - xE and Dmax (vitfilter.c:176, :198) are one packed DPP max reduction;
+//   - the special states run on the scalar unit;
+//   - when the reference would run its lazy-F delete passes (Dmax + ddbound_w
+//     > xB, :199) the kernel computes the same result, the closure
+//     D(k) = max(D(k), D(k-1) + tDD(k-1)) along the model: a pass inside each
+//     slot, then a max-plus scan across slots.  Saturating adds of costs
+//     t <= 0 compose exactly (max(-32768, max(-32768, x+t1) + t2) =
+//     max(-32768, x + t1 + t2)), so the scan carries unsaturated int32 sums.
+// The score's float expression (:240-246) is evaluated on the host from xC.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <utility>
+#include "msv_device.h"
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+namespace {
+
+constexpr uint32_t NEG2 = 0x80008000u;   // (-32768, -32768): minus infinity in both halves
+
+__device__ __forceinline__ s16x2 S(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ uint32_t U(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t adds(uint32_t a, uint32_t b) {
+  return U(__builtin_elementwise_add_sat(S(a), S(b)));
+}
+__device__ __forceinline__ uint32_t maxs(uint32_t a, uint32_t b) { return U(__builtin_elementwise_max(S(a), S(b))); }
+__device__ __forceinline__ int lo16(uint32_t v) { return (int) (int16_t) (v & 0xffffu); }
+__device__ __forceinline__ int hi16(uint32_t v) { return (int) (int16_t) (v >> 16); }
+__device__ __forceinline__ uint32_t pack16(int lo, int hi) {
+  return (uint32_t) (uint16_t) lo | ((uint32_t) (uint16_t) hi << 16);
+}
+// x + T floored at -32768, for x >= -32768 and T <= 0 (a run of saturating adds).
+__device__ __forceinline__ int fT(int x, int T) { return max(-32768, x + T); }
+
+// Slot s <- slot s-1 for a register: the lane's low half takes the previous
+// lane's high half, its high half its own low half; lane 0 takes the high
+// half of edge (the last slot of the previous wave, or minus infinity).
+__device__ __forceinline__ uint32_t shift_in(uint32_t last, uint32_t edge, int lane) {
+  uint32_t prev = __builtin_amdgcn_update_dpp(NEG2, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+  prev = (lane == 0) ? edge : prev;
+  return __builtin_amdgcn_alignbit(last, prev, 16);
+}
+
+// Packed int16 max over the wave, returned in every lane (lane 63's value).
+__device__ __forceinline__ uint32_t wave_max2(uint32_t v) {
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x111 /* row_shr:1 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x112 /* row_shr:2 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x114 /* row_shr:4 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x118 /* row_shr:8 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x142 /* row_bcast:15 */, 0xa, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x143 /* row_bcast:31 */, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+
+}  // namespace
+
+template <int Q, int W>
+__global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
+  extern __shared__ uint32_t etab_lds[];               // one wave per target: the emission table
+  __shared__ uint32_t red[16], dlast[16], eM[16], eI[16], eD[16];
+  __shared__ int scC[16], scT[16];
+  constexpr int C = 64 * W;                            // lanes per target
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int w = (W == 1) ? 0 : wave;
+  const int c = w * 64 + lane;
+
+  uint32_t tBM[Q], tMM[Q], tIM[Q], tDM[Q], tMD[Q], tMI[Q], tII[Q], tDD[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    tBM[j] = a.ttab[(0 * Q + j) * C + c];
+    tMM[j] = a.ttab[(1 * Q + j) * C + c];
+    tIM[j] = a.ttab[(2 * Q + j) * C + c];
+    tDM[j] = a.ttab[(3 * Q + j) * C + c];
+    tMD[j] = a.ttab[(4 * Q + j) * C + c];
+    tMI[j] = a.ttab[(5 * Q + j) * C + c];
+    tII[j] = a.ttab[(6 * Q + j) * C + c];
+    tDD[j] = a.ttab[(7 * Q + j) * C + c];
+  }
+  int Tlo = 0, Thi = 0;                                // tDD summed over each slot
+#pragma unroll
+  for (int j = 0; j < Q; ++j) { Tlo += lo16(tDD[j]); Thi += hi16(tDD[j]); }
+
+  const uint32_t *et = a.etab;
+  if (W == 1 && a.etab_in_lds) {
+    const int ntab = (a.db.R + 1) * Q * 64;
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x) etab_lds[i] = a.etab[i];
+    __syncthreads();
+    et = etab_lds;
+  }
+
+  const int nw = (W == 1) ? (int) (blockDim.x >> 6) : 1;
+  const int first = (W == 1) ? (int) blockIdx.x * nw + wave : (int) blockIdx.x;
+  const int stride = (W == 1) ? (int) gridDim.x * nw : (int) gridDim.x;
+  const int xN = a.base_w;                             // vitfilter.c:116; N loop cost is 0
+
+  for (int it = first; it < a.n; it += stride) {
+    const int64_t s = a.items[it];
+    const int L = a.db.len[s];
+    const uint8_t *p = a.db.res + a.db.off[s];
+    const int move = a.move[it];
+    uint32_t Mr[Q], Ir[Q], Dr[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) Mr[j] = Ir[j] = Dr[j] = NEG2;        // :115
+    int xB = (int16_t) (xN + move);                                  // :117
+    int xJ = -32768, xC = -32768, status = 0;
+    uint32_t edM = NEG2, edI = NEG2, edD = NEG2;
+
+    for (int i0 = 0; i0 < L && status == 0; i0 += 16) {
+      const uint4 ch = *reinterpret_cast<const uint4 *>(p + i0);
+      uint32_t w0 = ch.x, w1 = ch.y, w2 = ch.z, w3 = ch.w;   // 16 residues, consumed from the bottom
+      const int nr = min(16, L - i0);
+      for (int rr = 0; rr < nr; ++rr) {
+        const int x = (int) (w0 & 0xffu);
+        w0 = __builtin_amdgcn_alignbit(w1, w0, 8);
+        w1 = __builtin_amdgcn_alignbit(w2, w1, 8);
+        w2 = __builtin_amdgcn_alignbit(w3, w2, 8);
+        w3 >>= 8;
+        const uint32_t *ep = et + (size_t) x * Q * C + c;
+        uint32_t e[Q];
+#pragma unroll
+        for (int j = 0; j < Q; ++j) e[j] = ep[j * C];
+        const uint32_t xBv = pack16(xB, xB);
+        uint32_t mpv = shift_in(Mr[Q - 1], edM, lane);               // :139-141
+        uint32_t ipv = shift_in(Ir[Q - 1], edI, lane);
+        uint32_t dpv = shift_in(Dr[Q - 1], edD, lane);
+        uint32_t xEv = NEG2, Dmv = NEG2, dcv = NEG2;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+          uint32_t sv = adds(xBv, tBM[j]);                           // :146-150
+          sv = maxs(sv, adds(mpv, tMM[j]));
+          sv = maxs(sv, adds(ipv, tIM[j]));
+          sv = maxs(sv, adds(dpv, tDM[j]));
+          sv = adds(sv, e[j]);
+          xEv = maxs(xEv, sv);                                       // :151
+          mpv = Mr[j]; dpv = Dr[j]; ipv = Ir[j];                     // :156-158
+          Mr[j] = sv;                                                // :161
+          Dr[j] = dcv;                                               // :162
+          dcv = adds(sv, tMD[j]);                                    // :167
+          Dmv = maxs(Dmv, dcv);                                      // :168
+          Ir[j] = maxs(adds(mpv, tMI[j]), adds(ipv, tII[j]));        // :171-172
+        }
+        uint32_t r = wave_max2(pack16(max(lo16(xEv), hi16(xEv)), max(lo16(Dmv), hi16(Dmv))));
+        uint32_t dedge = NEG2;
+        if constexpr (W > 1) {
+          const uint32_t dl = __builtin_amdgcn_readlane(dcv, 63);
+          if (lane == 0) { red[w] = r; dlast[w] = dl; }
+          __syncthreads();
+          r = red[0];
+#pragma unroll
+          for (int v = 1; v < W; ++v) r = maxs(r, red[v]);
+          if (w > 0) dedge = dlast[w - 1];
+        }
+        const int xE = lo16(r), Dmax = hi16(r);                      // :176, :198
+        if (xE >= 32767) { status = 16; break; }                     // :177
+        xC = (int16_t) max(xC, xE + a.e_move);                       // :179 (C loop 0)
+        xJ = (int16_t) max(xJ, xE + a.e_loop);                       // :180 (J loop 0)
+        xB = (int16_t) max(xJ + move, xN + move);                    // :181
+
+        Dr[0] = shift_in(dcv, dedge, lane);                          // :230-231 / :203-204
+        if (Dmax + a.ddbound_w > xB) {                               // :199
+          // closure inside each slot
+#pragma unroll
+          for (int j = 1; j < Q; ++j) Dr[j] = maxs(Dr[j], adds(Dr[j - 1], tDD[j - 1]));
+          const uint32_t outl = adds(Dr[Q - 1], tDD[Q - 1]);
+          const int clo = lo16(outl);
+          // the lane's two slots as one map x -> max(c, x + T): low slot, then high
+          int cs = max(hi16(outl), fT(clo, Thi)), Ts = Tlo + Thi;
+#pragma unroll
+          for (int d = 1; d < 64; d <<= 1) {                         // inclusive scan over lanes
+            const int co = __shfl_up(cs, d, 64), To = __shfl_up(Ts, d, 64);
+            if (lane >= d) { cs = max(cs, fT(co, Ts)); Ts = To + Ts; }
+          }
+          int ce = __shfl_up(cs, 1, 64), Te = __shfl_up(Ts, 1, 64);
+          if (lane == 0) { ce = -32768; Te = 0; }
+          int carry = -32768;                                        // into this wave's first slot
+          if constexpr (W > 1) {
+            if (lane == 63) { scC[w] = cs; scT[w] = Ts; }
+            __syncthreads();
+            for (int v = 0; v < w; ++v) carry = max(scC[v], fT(carry, scT[v]));
+          }
+          const int cin_lo = max(ce, fT(carry, Te));
+          const int cin_hi = max(clo, fT(cin_lo, Tlo));
+          uint32_t y = pack16(cin_lo, cin_hi);
+#pragma unroll
+          for (int j = 0; j < Q; ++j) { Dr[j] = maxs(Dr[j], y); y = adds(y, tDD[j]); }
+        }
+        if constexpr (W > 1) {
+          if (lane == 63) { eM[w] = Mr[Q - 1]; eI[w] = Ir[Q - 1]; eD[w] = Dr[Q - 1]; }
+          __syncthreads();
+          if (w > 0) { edM = eM[w - 1]; edI = eI[w - 1]; edD = eD[w - 1]; }
+        }
+      }
+    }
+    if (threadIdx.x % C == 0) a.out[it] = (status << 16) | (int) (uint16_t) xC;
+  }
+}
+
+template <int Q, int W>
+static hipError_t launch_vit(const VitArgs *a, int grid, hipStream_t st) {
+  const int block = (W == 1) ? 256 : 64 * W;
+  const size_t lds = (W == 1 && a->etab_in_lds) ? (size_t) (a->db.R + 1) * Q * 64 * 4 : 0;
+  hipLaunchKernelGGL((vit_kernel<Q, W>), dim3(grid), dim3(block), lds, st, *a);
+  return hipGetLastError();
+}
+
+template <int W, int... Qs>
+static hipError_t dispatch_q(int Q, const VitArgs *a, int grid, hipStream_t st, std::integer_sequence<int, Qs...>) {
+  hipError_t e = hipErrorInvalidValue;
+  ((Q == Qs + 1 ? (e = launch_vit<Qs + 1, W>(a, grid, st), 0) : 0), ...);
+  return e;
+}
+
+extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int grid, hipStream_t st) {
+  if (Q < 1 || Q > 8) return hipErrorInvalidValue;
+  auto qs = std::make_integer_sequence<int, 8>{};
+  switch (W) {
+    case 1: return dispatch_q<1>(Q, a, grid, st, qs);
+    case 2: return dispatch_q<2>(Q, a, grid, st, qs);
+    case 4: return dispatch_q<4>(Q, a, grid, st, qs);
+    case 8: return dispatch_q<8>(Q, a, grid, st, qs);
+    case 16: return dispatch_q<16>(Q, a, grid, st, qs);
+    default: return hipErrorInvalidValue;
+  }
+}

@@ -117,6 +117,53 @@ int msvg_last_stats(msvg_ctx *ctx, msvg_stats *out);
  * reference sets it. */
 int msvg_MSVFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, int L, float *ret_sc);
 
+/* ---- The ViterbiFilter stage: the filter after MSV in p7_Pipeline
+ * (p7_pipeline.c:699-705, the call at :701), int16 Viterbi of impl_sse/vitfilter.c:84-250. ----
+ *
+ * The ViterbiFilter fields of P7_OPROFILE (impl_sse.h:88-101), in model order
+ * k = 1..M (the de-striped form of om->rwv / om->twv, p7_oprofile.c:653-740):
+ *   rwv[x * M + (k-1)]  match emission of residue code x at node k
+ *   twv[t * M + (k-1)]  t = 0..7 = p7O_BM, MM, IM, DM, MD, MI, II, DD: the value
+ *                       the striped transition vector holds at node k's
+ *                       position (BM, MM, IM, DM rotated by one: into k from
+ *                       k-1; MD, MI, II, DD out of k), -32768 where the
+ *                       reference pads.
+ * xw[N|C|J][MOVE] are not passed: they follow from the target length as
+ * p7_oprofile_ReconfigRestLength sets them (p7_oprofile.c:944-962, with nj);
+ * the N/C/J loop costs are 0 as vf_conversion leaves them. */
+typedef struct {
+  int32_t M, Kp;
+  const int16_t *rwv;
+  const int16_t *twv;
+  int16_t e_loop, e_move;      /* xw[E][LOOP], xw[E][MOVE] */
+  int16_t base_w, ddbound_w;
+  float scale_w, nj;
+  float vmu, vlambda;          /* evparam[p7_VMU], evparam[p7_VLAMBDA] */
+} msvg_vprofile;
+
+/* Attach the ViterbiFilter fields to a profile added with msvg_profile_add
+ * (same M and Kp).  Replaces p7_oprofile_Convert's vf_conversion output for
+ * the device (p7_oprofile.c:653-740).  EINVAL if M, Kp differ or M > 16384. */
+int msvg_profile_set_viterbi(msvg_ctx *ctx, int32_t profile_id, const msvg_vprofile *vp);
+
+/* p7_ViterbiFilter on the listed targets (caller indices into the database):
+ * sc[i] bit-identical to the reference's *ret_sc, status[i] 0 (eslOK) or 16
+ * (eslERANGE, sc = +inf).  The MSV survivors of msvg_search (pass = 1) are
+ * the list p7_Pipeline would score.  After the call msvg_last_stats holds the
+ * kernel time in ms_fast and M * sum(L) in cells. */
+int msvg_vit_search(msvg_ctx *ctx, int32_t profile_id, int32_t db_id, const int64_t *targets, int64_t n,
+                    float *sc, uint8_t *status);
+
+/* Drop-in for p7_ViterbiFilter (impl_sse.h:343, vitfilter.c:84): dsq 1-based
+ * with sentinels.  Returns MSVG_OK or MSVG_ERANGE. */
+int msvg_ViterbiFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, int L, float *ret_sc);
+
+/* p7_ViterbiMu (src/evalues.c:313-352) on the GPU: the 200 i.i.d. length-200
+ * targets that follow p7_MSVMu's in p7_Calibrate's stream
+ * (esl_randomness_CreateFast(42)), scored with the ViterbiFilter, Gumbel
+ * location fitted for lambda. */
+int msvg_calibrate_vmu(msvg_ctx *ctx, int32_t profile_id, double lambda, float *ret_vmu);
+
 /* Test hook: after k more device allocations, fail the rest as out of
  * memory (k < 0: never).  The environment variable MSVG_TEST_FAIL_ALLOC sets
  * the initial value.  No counterpart in the reference. */

@@ -76,6 +76,21 @@ double msvh_fit_mu(const float *usc, const int32_t *status, int N, int L, int ba
  * calibration RNG esl_randomness_CreateFast(42) (evalues.c:81, :261). */
 void   msvh_calibration_targets(int N, int L, uint8_t *out);
 
+/* ---- ViterbiFilter stage ---- */
+/* xw[N|C|J][MOVE] for target length L: wordify(logf((2+nj)/(L+2+nj)))
+ * (p7_oprofile_ReconfigRestLength, p7_oprofile.c:944-962; wordify :527-534). */
+int    msvh_vit_move(float scale_w, float nj, int L);
+/* p7_ViterbiFilter's score from its final xC (vitfilter.c:240-248): -inf when
+ * xC is -32768. */
+float  msvh_vit_score(int xC, int move, int base_w, float scale_w);
+/* mu from N ViterbiFilter scores of i.i.d. targets of length L as
+ * p7_ViterbiMu computes it (evalues.c:313-352); ERANGE scores count as
+ * (32767 - base_w) / scale_w. */
+double msvh_fit_vmu(const float *sc, const int32_t *status, int N, int L, int base_w, float scale_w, double lambda);
+/* The targets p7_ViterbiMu scores in p7_Calibrate: the N sequences of length
+ * L that follow p7_MSVMu's N in the same stream (evalues.c:107-108). */
+void   msvh_vit_calibration_targets(int N, int L, uint8_t *out);
+
 /* Synthetic target database: n targets, lengths log-normal (median, sigma)
  * clipped to [lo, hi], optionally a Pareto tail (fraction tail_frac, x_m =
  * tail_xm, alpha tail_alpha, clipped to tail_max), residues i.i.d. from the

@@ -1,0 +1,145 @@
+"""GPU parity of the ViterbiFilter stage (vit_kernels.hip through the C ABI).
+
+Bar: bit-identical p7_ViterbiFilter scores and statuses (integer DP; the
+float score is the reference's expression of the integer xC).  Checked
+against the golden vectors written from the compiled reference, against the
+reference itself (oracle/_ref) for model lengths that exercise every wave
+count of the kernel (1, 2, 4, 8 and 16 waves per target, M up to 16384), and
+the drop-in and calibration entry points.
+"""
+import numpy as np
+import pytest
+
+import cudahmmsearch_amd as ca
+from conftest import vit_golden_cases, vit_golden_profile
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _placeholder_msv(M, Kp):
+    """An MSV profile to hang the Viterbi fields on (only the Viterbi stage runs)."""
+    return ca.ByteProfile(np.zeros((Kp, M), np.uint8), 0, 3, 190, 0, 1.0, mu=0.0, lam=0.7)
+
+
+def _ctx_with(vd, Kp):
+    ctx = ca.MSVContext(0, tiles=None)
+    pid = ctx.add_profile(_placeholder_msv(vd["rwv"].shape[1], Kp))
+    ctx.set_viterbi(pid, ca.ViterbiProfile.from_dict(vd))
+    return ctx, pid
+
+
+def _same(sc, st, ref_sc, ref_st):
+    return (np.array_equal(np.asarray(st).astype(np.int32), np.asarray(ref_st).astype(np.int32))
+            and np.array_equal(np.asarray(sc, np.float32).view(np.uint32), np.asarray(ref_sc, np.float32).view(np.uint32)))
+
+
+def test_golden_vectors(vit_golden):
+    z, man = vit_golden
+    for i, meta in enumerate(man["profiles"]):
+        v = vit_golden_profile(z, man, i)
+        cases = vit_golden_cases(z, i)
+        print(f"profile {meta['name']} M={meta['M']} cases={len(cases)}", flush=True)
+        ctx, pid = _ctx_with(v, meta["Kp"])
+        did = ctx.add_database(ca.Database.from_list([x for _, _, x in cases]))
+        sc, st = ctx.vit_search(pid, did, np.arange(len(cases)))
+        idx = [c for c, _, _ in cases]
+        assert _same(sc, st, z["case_bits"][idx].view(np.float32), z["case_status"][idx]), meta["name"]
+        ctx.close()
+
+
+@pytest.mark.parametrize("M", [1, 7, 128, 129, 1024, 1025, 2048, 2049, 4096, 4097, 9000, 16384])
+def test_every_wave_count_vs_reference(M):
+    """Model lengths at the edges of each (waves, registers) shape against
+    the compiled reference's p7_ViterbiFilter."""
+    from oracle import Reference
+    R = Reference()
+    rp = R.sample(500 + M, M, calibrate=False)
+    f = rp.vit_fields()
+    vd = {"rwv": f["rwv_pad"][:, :M], "twv": f["twv_pad"][:, :M], "e_loop": f["e_loop"], "e_move": f["e_move"],
+          "base_w": f["base_w"], "ddbound_w": f["ddbound_w"], "scale_w": f["scale_w"], "nj": f["nj"]}
+    rng = np.random.default_rng(M)
+    n = 60 if M <= 4097 else 24
+    lens = rng.integers(1, 1200 if M <= 4097 else 500, n)
+    seqs = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in lens]
+    emit, elens = rp.core_emit(7 + M, 4)
+    o = 0
+    for L in elens:
+        seqs.append(emit[o:o + L])
+        o += L
+    seqs.append(rng.integers(0, 29, 300).astype(np.uint8))
+    db = ca.Database.from_list(seqs)
+    print(f"M={M} targets={db.n}", flush=True)
+    ctx, pid = _ctx_with(vd, 29)
+    did = ctx.add_database(db)
+    sc, st = ctx.vit_search(pid, did, np.arange(db.n))
+    rst, rsc = rp.vit_batch(db.res, db.off, db.lens, nthreads=8)
+    assert _same(sc, st, rsc, rst)
+    ctx.close()
+
+
+def test_subset_order_and_repeats(vit_golden):
+    """Any target list: a subset in scrambled order with repeats gives each
+    target its own score."""
+    z, man = vit_golden
+    i = 1
+    v = vit_golden_profile(z, man, i)
+    cases = vit_golden_cases(z, i)
+    ctx, pid = _ctx_with(v, man["profiles"][i]["Kp"])
+    did = ctx.add_database(ca.Database.from_list([x for _, _, x in cases]))
+    rng = np.random.default_rng(3)
+    pick = rng.integers(0, len(cases), 3 * len(cases))
+    sc, st = ctx.vit_search(pid, did, pick)
+    idx = np.array([cases[k][0] for k in pick])
+    assert _same(sc, st, z["case_bits"][idx].view(np.float32), z["case_status"][idx])
+    sc0, st0 = ctx.vit_search(pid, did, np.zeros(0, np.int64))
+    assert sc0.size == 0
+    ctx.close()
+
+
+def test_dropin_single_target(vit_golden):
+    z, man = vit_golden
+    O = Oracle()
+    for i in (0, 4, 15):
+        v = vit_golden_profile(z, man, i)
+        ctx, pid = _ctx_with(v, man["profiles"][i]["Kp"])
+        for c, _, x in vit_golden_cases(z, i)[:12]:
+            st, sc = ctx.vit_filter(pid, x)
+            ost, osc = O.vit(v, x)
+            assert st == ost and np.float32(sc).view(np.uint32) == np.float32(osc).view(np.uint32)
+        ctx.close()
+
+
+def test_calibrated_vmu(vit_golden):
+    """p7_ViterbiMu on the GPU gives the reference's calibrated mu (sampled,
+    p7_Calibrate-d profiles of the golden set)."""
+    z, man = vit_golden
+    done = 0
+    for i, meta in enumerate(man["profiles"]):
+        if not meta["name"].startswith("sample-") or meta["M"] > 1000:
+            continue
+        v = vit_golden_profile(z, man, i)
+        ctx, pid = _ctx_with(v, meta["Kp"])
+        vmu = ctx.calibrate_vmu(pid, v["vlambda"])
+        assert np.float32(vmu).view(np.uint32) == meta["vmu_bits"], meta["name"]
+        ctx.close()
+        done += 1
+    assert done >= 6
+
+
+def test_invalid_arguments(vit_golden):
+    z, man = vit_golden
+    v = vit_golden_profile(z, man, 0)
+    ctx = ca.MSVContext(0, tiles=None)
+    pid = ctx.add_profile(_placeholder_msv(v["rwv"].shape[1], 29))
+    did = ctx.add_database(ca.Database.from_list([np.array([1, 2, 3], np.uint8)]))
+    with pytest.raises(ca.MSVError) as e:
+        ctx.vit_search(pid, did, [0])                       # no Viterbi fields yet
+    assert e.value.code == 11
+    other = ca.ViterbiProfile.from_dict(vit_golden_profile(z, man, 1))
+    with pytest.raises(ca.MSVError):
+        ctx.set_viterbi(pid, other)                          # M differs
+    ctx.set_viterbi(pid, ca.ViterbiProfile.from_dict(v))
+    with pytest.raises(ca.MSVError):
+        ctx.vit_search(pid, did, [1])                        # out of range
+    ctx.close()

@@ -114,6 +114,7 @@ def load_library(path=LIB_PATH):
         "msvg_ViterbiFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_calibrate_vmu": (C.c_int, [vp, i32, C.c_double, C.POINTER(C.c_float)]),
         "msvh_vit_move": (C.c_int, [C.c_float, C.c_float, C.c_int]),
+        "msvh_vit_convert": (C.c_int, [C.c_int, vp, vp, vp, vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "msvh_vit_score": (C.c_float, [C.c_int, C.c_int, C.c_int, C.c_float]),
         "msvh_fit_vmu": (C.c_double, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_double]),
         "msvh_vit_calibration_targets": (None, [C.c_int, C.c_int, vp]),
@@ -184,6 +185,15 @@ class ByteProfile:
         return {"rbv": self.rbv, "tbm": self.tbm, "tec": self.tec, "base": self.base, "bias": self.bias,
                 "scale": self.scale, "mu": self.mu, "lambda": self.lam}
 
+    def viterbi(self):
+        """The ViterbiFilter fields of the same HMM (needs the HMM this
+        profile was converted from); vlambda = the MSV lambda, as p7_Calibrate
+        sets it (evalues.c:112-113)."""
+        if self.hmm is None or self.hmm[2] is None:
+            raise MSVError(eslEINVAL, "profile has no HMM transitions to convert")
+        mat, ins, t = self.hmm
+        return viterbi_from_hmm(mat, t, vlambda=self.lam)
+
     def _struct(self):
         return _Prof(self.M, self.rbv.shape[0], self.rbv.ctypes.data, self.tbm, self.tec, self.base, self.bias,
                      self.scale, float(self.mu if self.mu is not None else -99999.0),
@@ -220,6 +230,24 @@ class ViterbiProfile:
                       self.e_move, self.base_w, self.ddbound_w, self.scale_w, self.nj,
                       float(self.vmu if self.vmu is not None else -99999.0),
                       float(self.vlambda if self.vlambda is not None else -99999.0))
+
+
+def viterbi_from_hmm(mat, t, vlambda=None):
+    """vf_conversion (p7_oprofile.c:653-740) of an HMM configured as hmmsearch
+    configures it: the ViterbiProfile for match emissions ``mat`` ((M+1) x 20)
+    and transitions ``t`` ((M+1) x 7, p7H order)."""
+    lib = load_library()
+    mat = np.ascontiguousarray(mat, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    M = mat.shape[0] - 1
+    rwv = np.zeros((KP, M), np.int16)
+    twv = np.zeros((8, M), np.int16)
+    sc = np.zeros(4, np.int32)
+    scale_w, nj = C.c_float(), C.c_float()
+    rc = lib.msvh_vit_convert(M, _p(mat), _p(t), _p(rwv), _p(twv), _p(sc), C.byref(scale_w), C.byref(nj))
+    if rc != 0:
+        raise MSVError(rc, "Viterbi conversion failed")
+    return ViterbiProfile(rwv, twv, sc[0], sc[1], sc[2], sc[3], scale_w.value, nj.value, vlambda=vlambda)
 
 
 def vit_move(scale_w, nj, L):

@@ -207,6 +207,85 @@ int msvh_profile_convert(int M, const float *mat, int config_L, uint8_t *rbv, in
   return 0;
 }
 
+namespace {
+// wordify(), p7_oprofile.c:527-534
+int16_t wordify(float scale_w, float sc) {
+  sc = roundf(scale_w * sc);
+  if (sc >= 32767.0) return 32767;
+  else if (sc <= -32768.0) return -32768;
+  else return (int16_t) sc;
+}
+}  // namespace
+
+int msvh_vit_convert(int M, const float *mat, const float *t, int16_t *rwv, int16_t *twv, int *scalars,
+                     float *scale_w_out, float *nj_out) {
+  if (M < 1) return 11;
+  std::vector<float> msc((size_t) MSVH_KP * M);
+  std::vector<uint8_t> rbv((size_t) MSVH_KP * M);
+  int sc5[5];
+  float scale_b;
+  int rc = msvh_profile_convert(M, mat, 100, rbv.data(), sc5, &scale_b, msc.data());
+  if (rc != 0) return rc;
+  enum { MM = 0, IM = 1, DM = 2, BM = 3, MD = 4, MI = 5, II = 6, DD = 7 };   // p7P_* (hmmer.h:222-231)
+  enum { hMM = 0, hMI = 1, hMD = 2, hIM = 3, hII = 4, hDM = 5, hDD = 6 };   // p7H_*
+  // Generic transition scores for nodes 0..M-1: node 0 has none but the
+  // entry (p7_profile.c:84), local entry occ[k] / sum occ[i](M-i+1) with
+  // p7_hmm_CalculateOccupancy (modelconfig.c:88-98, p7_hmm.c:1271-1281), the
+  // rest log(t) (modelconfig.c:127-136).
+  std::vector<float> tsc((size_t) M * 8, -INFINITY);
+  std::vector<float> occ((size_t) M + 1);
+  occ[0] = 0.;
+  occ[1] = t[0 * 7 + hMI] + t[0 * 7 + hMM];
+  for (int k = 2; k <= M; k++)
+    occ[k] = occ[k - 1] * (t[(k - 1) * 7 + hMM] + t[(k - 1) * 7 + hMI]) + (1.0 - occ[k - 1]) * t[(k - 1) * 7 + hDM];
+  float Z = 0.;
+  for (int k = 1; k <= M; k++) Z += occ[k] * (float) (M - k + 1);
+  for (int k = 1; k <= M; k++) tsc[(size_t) (k - 1) * 8 + BM] = log(occ[k] / Z);
+  for (int k = 1; k < M; k++) {
+    float *tp = &tsc[(size_t) k * 8];
+    tp[MM] = log(t[k * 7 + hMM]);
+    tp[MI] = log(t[k * 7 + hMI]);
+    tp[MD] = log(t[k * 7 + hMD]);
+    tp[IM] = log(t[k * 7 + hIM]);
+    tp[II] = log(t[k * 7 + hII]);
+    tp[DM] = log(t[k * 7 + hDM]);
+    tp[DD] = log(t[k * 7 + hDD]);
+  }
+  auto TSC = [&](int k, int s) { return tsc[(size_t) k * 8 + s]; };
+  // vf_conversion, p7_oprofile.c:653-740, in model order.
+  const float scale_w = 500.0 / kLog2;                                // :676
+  for (int x = 0; x < MSVH_KP; x++)                                   // :680-685
+    for (int k = 1; k <= M; k++) rwv[(size_t) x * M + k - 1] = wordify(scale_w, msc[(size_t) x * M + k - 1]);
+  // The seven per-node transitions (:688-707): BM, MM, IM, DM from node
+  // k-1 (rotated), MD, MI, II from node k; capped at 0 (II at -1).
+  const int from[7] = {BM, MM, IM, DM, MD, MI, II};
+  for (int tt = 0; tt < 7; tt++) {
+    const int maxval = (tt == 6) ? -1 : 0;
+    for (int k = 1; k <= M; k++) {
+      const int kb = (tt < 4) ? k - 1 : k;
+      const int16_t val = (kb < M) ? wordify(scale_w, TSC(kb, from[tt])) : (int16_t) -32768;
+      twv[(size_t) tt * M + k - 1] = (val <= maxval) ? val : (int16_t) maxval;
+    }
+  }
+  for (int k = 1; k <= M; k++)                                        // DD, :710-715
+    twv[(size_t) 7 * M + k - 1] = (k < M) ? wordify(scale_w, TSC(k, DD)) : (int16_t) -32768;
+  const float e_sc = -kLog2;                                          // multihit E, modelconfig.c:116-117
+  scalars[0] = wordify(scale_w, e_sc);                                // xw[E][LOOP], :724
+  scalars[1] = wordify(scale_w, e_sc);                                // xw[E][MOVE], :725
+  scalars[2] = 12000;                                                 // base_w, :677
+  int16_t ddbound = -32768;                                           // :738-745
+  for (int k = 2; k < M - 1; k++) {
+    int ddtmp = (int) wordify(scale_w, TSC(k, DD));
+    ddtmp += (int) wordify(scale_w, TSC(k + 1, DM));
+    ddtmp -= (int) wordify(scale_w, TSC(k + 1, BM));
+    ddbound = (int16_t) std::max((int) ddbound, ddtmp);
+  }
+  scalars[3] = ddbound;
+  *scale_w_out = scale_w;
+  *nj_out = 1.0f;                                                     // multihit, modelconfig.c:119
+  return 0;
+}
+
 double msvh_lambda(int M, const float *mat) {
   // p7_Lambda (evalues.c:186-192) with p7_MeanMatchRelativeEntropy
   // (modelstats.c:81-98) and esl_vec_FRelEntropy (esl_vectorops.c:950-962).

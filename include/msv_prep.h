@@ -77,6 +77,14 @@ double msvh_fit_mu(const float *usc, const int32_t *status, int N, int L, int ba
 void   msvh_calibration_targets(int N, int L, uint8_t *out);
 
 /* ---- ViterbiFilter stage ---- */
+/* The ViterbiFilter fields of p7_oprofile_Convert (vf_conversion,
+ * p7_oprofile.c:653-740) for an HMM configured as hmmsearch configures it
+ * (p7_ProfileConfig local multihit, modelconfig.c:49-150): rwv (29 x M) and
+ * twv (8 x M) in model order as msvg_vprofile holds them; scalars =
+ * {xw[E][LOOP], xw[E][MOVE], base_w, ddbound_w}; scale_w; nj.  mat and t as
+ * msvh_hmm_sample writes them. */
+int    msvh_vit_convert(int M, const float *mat, const float *t, int16_t *rwv, int16_t *twv, int *scalars,
+                        float *scale_w, float *nj);
 /* xw[N|C|J][MOVE] for target length L: wordify(logf((2+nj)/(L+2+nj)))
  * (p7_oprofile_ReconfigRestLength, p7_oprofile.c:944-962; wordify :527-534). */
 int    msvh_vit_move(float scale_w, float nj, int L);

@@ -1,14 +1,16 @@
 /* pipeline_driver -- test infrastructure: a serial hmmsearch over the
  * reference's own library, linked twice (oracle/Makefile `dropin`):
  *
- *   oracle/_ref/pipeline_ref      with the reference's SSE p7_MSVFilter
- *   oracle/_ref/pipeline_mi355x   with cudahmmsearch_amd/hmmer/msvfilter_mi355x.c
- *                                 (p7_MSVFilter on the GPU) in its place
+ *   oracle/_ref/pipeline_ref      with the reference's SSE p7_MSVFilter and
+ *                                 p7_ViterbiFilter
+ *   oracle/_ref/pipeline_mi355x   with cudahmmsearch_amd/hmmer/filters_mi355x.c
+ *                                 (both filters on the GPU) in their place
  *
  * Identical output files from the two are the drop-in check: every
- * p7_MSVFilter call made by p7_Pipeline goes through the replaced symbol, so
- * the MSV pass counts, the hits that survive the later filters and every
- * per-target MSV score below must agree bit for bit.
+ * p7_MSVFilter and p7_ViterbiFilter call made by p7_Pipeline goes through the
+ * replaced symbols, so the filter pass counts, the hits that survive the
+ * later stages and every per-target filter score below must agree bit for
+ * bit.
  *
  * The loop is hmmsearch's serial_loop (src/hmmsearch.c:1285-1313) with the
  * per-query set-up of src/hmmsearch.c:474-510 and the report calls of
@@ -17,8 +19,8 @@
  *
  *   pipeline_driver <hmmfile> <seqfile> <outfile> [F1 [nobias]]
  *
- * After each query's report it lists, for every target, p7_MSVFilter's
- * status and the float bits of its score.
+ * After each query's report it lists, for every target, p7_MSVFilter's and
+ * p7_ViterbiFilter's status and the float bits of their scores.
  */
 #include "p7_config.h"
 
@@ -85,8 +87,8 @@ int main(int argc, char **argv)
     sq = esl_sq_CreateDigital(abc);
 
     while ((sstatus = esl_sqio_Read(sqfp, sq)) == eslOK) {
-      float usc = 0.0f;
-      int   st;
+      float usc = 0.0f, vsc = 0.0f;
+      int   st, vst;
       union { float f; uint32_t u; } bits;
       char  line[256];
       int   n;
@@ -97,8 +99,12 @@ int main(int argc, char **argv)
       p7_Pipeline(pli, wom, bg, sq, th);
 
       st = p7_MSVFilter(sq->dsq, (int) sq->n, wom, ox, &usc);
+      vst = p7_ViterbiFilter(sq->dsq, (int) sq->n, wom, ox, &vsc);
       bits.f = usc;
-      n = snprintf(line, sizeof line, "msv %s %" PRId64 " %d %08x\n", sq->name, (int64_t) sq->n, st, bits.u);
+      n = snprintf(line, sizeof line, "msv %s %" PRId64 " %d %08x", sq->name, (int64_t) sq->n, st, bits.u);
+      bits.f = vsc;
+      if (n > 0 && n < (int) sizeof line)
+        n += snprintf(line + n, sizeof line - n, " vit %d %08x\n", vst, bits.u);
       if (n < 0 || n >= (int) sizeof line) die("sequence name too long:", sq->name);
       if (nlines + n + 1 > cap) {
         cap = 2 * cap + 4096;

@@ -1,18 +1,18 @@
 """The drop-in check at the caller.
 
 p7_Pipeline (src/p7_pipeline.c:646-...) calls p7_MSVFilter for every target
-(p7_pipeline.c:671) and derives the MSV pass from its score.  The serial
-search of oracle/pipeline_driver.c runs that pipeline over the reference's
-own library twice: oracle/_ref/pipeline_ref with the reference's SSE
-p7_MSVFilter, oracle/_ref/pipeline_mi355x with
-cudahmmsearch_amd/hmmer/msvfilter_mi355x.c (libmsvgpu on the GPU) in its
-place, nothing else changed.  Their output files -- hit tables, domain
-alignments, tabular output, the pipeline's "Passed MSV filter" counts and a
-line per target with p7_MSVFilter's status and score bits -- must be
-byte-identical.  Inputs: the reference's tutorial models and sequences
-(tests/golden/hmmer/, copied by make_inputs.py) and a seeded synthetic
-protein database with planted fragments of real family members, zero-length
-and long targets.
+(p7_pipeline.c:671) and p7_ViterbiFilter for the MSV survivors (:701), and
+derives each filter's pass from its score.  The serial search of
+oracle/pipeline_driver.c runs that pipeline over the reference's own library
+twice: oracle/_ref/pipeline_ref with the reference's SSE filters,
+oracle/_ref/pipeline_mi355x with cudahmmsearch_amd/hmmer/filters_mi355x.c
+(libmsvgpu on the GPU) in their place, nothing else changed.  Their output
+files -- hit tables, domain alignments, tabular output, the pipeline's
+filter pass counts and a line per target with both filters' statuses and
+score bits -- must be byte-identical.  Inputs: the reference's tutorial
+models and sequences (tests/golden/hmmer/, copied by make_inputs.py) and a
+seeded synthetic protein database with planted fragments of real family
+members, zero-length and long targets.
 """
 import gzip
 import os
@@ -177,13 +177,17 @@ def test_synthetic_targets(tmp_path, models, extra):
     assert ref.count("\nmsv ") == len(models) * 700
     passed = [int(l.split()[3]) for l in ref.splitlines() if l.startswith("Passed MSV filter:")]
     assert len(passed) == len(models) and all(p > 0 for p in passed)
+    vit = [int(l.split()[3]) for l in ref.splitlines() if l.startswith("Passed Vit filter:")]
+    assert len(vit) == len(models)
+    if "globins4.hmm" in models:                  # planted globins: Viterbi survivors and overflows
+        assert sum(vit) > 0 and ref.count(" vit 16 ") > 0 and ref.count(" vit 0 ") > 0
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd") or not os.path.exists(GPUBIN),
                     reason="needs a machine without a GPU and the built drop-in")
 def test_dropin_fails_loudly_without_gpu(tmp_path):
-    """No silent CPU fallback: without a device the drop-in p7_MSVFilter
-    raises through Easel's exception handler and the search stops."""
+    """No silent CPU fallback: without a device the drop-in filters raise
+    through Easel's exception handler and the search stops."""
     r = subprocess.run([GPUBIN, _unpack("fn3.hmm", tmp_path), _unpack("HBB_HUMAN", tmp_path),
                         os.path.join(str(tmp_path), "o.txt")], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0

@@ -6,7 +6,9 @@ A step is one complete MSV search (fast J-free stage, exact byte-MSV stage for
 the queued pairs, finalize to usc / status / F1 pass bit) of the workload's
 profiles over its resident target database; inputs are in HBM before the
 timed region starts.  Prints one JSON line.  GCUPS = sum over pairs of M * L
-per second / 1e9.
+per second / 1e9.  For c2 and c3 the line also carries the ViterbiFilter
+stage on the MSV survivors (vit_stage: kernel time, GCUPS, roofline, the
+reference's SSE p7_ViterbiFilter on a sample with a bit-identity check).
 
 Workloads (synthetic data generated on the host from seeds; Easel-compatible
 generators in cudahmmsearch_amd):
@@ -45,6 +47,10 @@ PEAK_LANE_INSTR = 256 * 64 * 2.4e9
 # bit is the max with B) + 1/4 v_pk_maximum3_f16 (two E maxima per
 # instruction) = 0.75 lane-instructions per cell.
 INSTR_PER_CELL = 0.75
+# ViterbiFilter stage: per cell the reference's recurrence (vitfilter.c:146-172)
+# does 14 int16 operations (M: 5 saturating adds + 3 max + 1 max into xE; D:
+# 1 add + 1 max into Dmax; I: 2 adds + 1 max); 2 cells per packed VGPR.
+VIT_INSTR_PER_CELL = 7.0
 
 
 def log(*a):
@@ -125,6 +131,72 @@ def cpu_baseline(profs, db, budget_s=12.0):
     return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": nthreads, "kind": kind,
             "sample": f"{n1} targets ({sub.residues} residues) drawn from the workload, profile M={p.M}, "
                       f"p7_MSVFilter on {nthreads} threads, {dt:.1f} s"}
+
+
+def vit_stage(ca, ctx, profs, pids, did, db, reps=3, cpu=True):
+    """The ViterbiFilter stage on the MSV survivors of the batched search
+    (pass = 1), as p7_Pipeline would run it (p7_pipeline.c:699-705, here
+    without the CPU bias filter in between): kernel time from HIP events,
+    GCUPS over M * L of the survivors, and the reference's SSE
+    p7_ViterbiFilter on a sample of the same pairs."""
+    n = db.n
+    passed = np.zeros((len(pids), n), np.uint8)
+    ctx.lib.msvg_results(ctx._c, None, None, passed.ctypes.data)
+    lens = np.asarray(db.lens, np.int64)
+    tot_ms, tot_cells, pairs = 0.0, 0, 0
+    first = None
+    for i, (p, pid) in enumerate(zip(profs, pids)):
+        ctx.set_viterbi(pid, p.viterbi())
+        surv = np.flatnonzero(passed[i]).astype(np.int64)
+        if surv.size == 0:
+            continue
+        sc, st = ctx.vit_search(pid, did, surv)            # warm-up (tables)
+        best = None
+        for _ in range(reps):
+            ctx.vit_search(pid, did, surv)
+            ms = ctx.stats()["ms_fast"]
+            best = ms if best is None else min(best, ms)
+        tot_ms += best
+        tot_cells += int(p.M * lens[surv].sum())
+        pairs += int(surv.size)
+        if first is None:
+            first = (p, surv, sc, st)
+    out = {"pairs": pairs, "cells": tot_cells, "kernel_ms": round(tot_ms, 3),
+           "gcups": round(tot_cells / (tot_ms / 1e3) / 1e9, 1) if tot_ms > 0 else None}
+    if tot_ms > 0:
+        ach = VIT_INSTR_PER_CELL * tot_cells / (tot_ms / 1e3)
+        out["roofline"] = {"bound": "valu", "achieved": round(ach / 1e12, 3),
+                           "peak": round(PEAK_LANE_INSTR / 1e12, 3), "unit": "T lane-instr/s",
+                           "frac": round(ach / PEAK_LANE_INSTR, 4), "kernel": "vit_kernel",
+                           "algorithmic": f"14 int16 ops per DP cell = {VIT_INSTR_PER_CELL} packed lane-instructions"}
+    if cpu and first is not None:
+        out["cpu_baseline"] = vit_cpu_baseline(ca, db, *first)
+    return out
+
+
+def vit_cpu_baseline(ca, db, p, surv, sc, st, budget_s=8.0):
+    """The reference's SSE p7_ViterbiFilter (oracle/_ref) on up to budget_s of
+    the first profile's survivors, with a bit-identity check against the GPU."""
+    import oracle
+    try:
+        R = oracle.Reference()
+    except (FileNotFoundError, OSError):
+        return None
+    nthreads = min(16, os.cpu_count() or 1)
+    mat, ins, t = p.hmm
+    rp = R.from_probs(mat, t, 100, False)
+    take = surv[: min(surv.size, 4000)]
+    sub = ca.Database.from_list([db.seq(int(i)) for i in take])
+    t0 = time.perf_counter()
+    rst, rsc = rp.vit_batch(sub.res, sub.off, sub.lens, nthreads)
+    dt = time.perf_counter() - t0
+    k = take.size
+    same = bool(np.array_equal(rsc.view(np.uint32), sc[:k].view(np.uint32))
+                and np.array_equal(rst.astype(np.uint8), st[:k]))
+    return {"value": round(p.M * sub.residues / dt / 1e9, 3), "unit": "GCUPS", "cores": nthreads,
+            "kind": "reference", "bit_identical_to_gpu": same,
+            "sample": f"{k} MSV survivors ({sub.residues} residues), profile M={p.M}, "
+                      f"p7_ViterbiFilter on {nthreads} threads, {dt:.2f} s"}
 
 
 def copy_probe(nbytes=256 << 20):
@@ -244,6 +316,7 @@ def main():
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--slots", type=int, default=4, help="c5: queries on the GPU at once")
     ap.add_argument("--shared", action="store_true", help="c5: one chunk per workgroup (queries share CUs)")
+    ap.add_argument("--no-vit", action="store_true", help="skip the ViterbiFilter stage on the MSV survivors")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -325,6 +398,11 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(profs, db)
+    if not args.no_vit and args.workload in ("c2", "c3"):
+        # the batched search above left its results (and the pass bits) on the device
+        ctx.search_device(pids, did, mode=ca.MODE_BATCHED)
+        out["vit_stage"] = vit_stage(ca, ctx, profs, pids, did, db, cpu=not args.no_cpu)
+        log(f"vit stage: {out['vit_stage']}")
     ctx.close()
     print(json.dumps(out))
 

@@ -195,6 +195,7 @@ def vit_cpu_baseline(ca, db, p, surv, sc, st, budget_s=8.0):
         if k == surv.size:
             break
         k = int(min(surv.size, max(k, k * budget_s / max(dt, 1e-3))))
+    k = take.size
     same = bool(np.array_equal(rsc.view(np.uint32), sc[:k].view(np.uint32))
                 and np.array_equal(rst.astype(np.uint8), st[:k]))
     return {"value": round(p.M * sub.residues / dt / 1e9, 3), "unit": "GCUPS", "cores": nthreads,

@@ -114,6 +114,9 @@ def load_library(path=LIB_PATH):
         "msvg_ViterbiFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_calibrate_vmu": (C.c_int, [vp, i32, C.c_double, C.POINTER(C.c_float)]),
         "msvh_vit_move": (C.c_int, [C.c_float, C.c_float, C.c_int]),
+        "msvh_pressed_open": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
+        "msvh_pressed_next": (C.c_int, [vp, C.POINTER(_Prof), C.POINTER(_VProf), C.c_char_p, C.c_int]),
+        "msvh_pressed_close": (None, [vp]),
         "msvh_vit_convert": (C.c_int, [C.c_int, vp, vp, vp, vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "msvh_vit_score": (C.c_float, [C.c_int, C.c_int, C.c_int, C.c_float]),
         "msvh_fit_vmu": (C.c_double, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_double]),
@@ -248,6 +251,40 @@ def viterbi_from_hmm(mat, t, vlambda=None):
     if rc != 0:
         raise MSVError(rc, "Viterbi conversion failed")
     return ViterbiProfile(rwv, twv, sc[0], sc[1], sc[2], sc[3], scale_w.value, nj.value, vlambda=vlambda)
+
+
+def read_pressed(h3f, h3p=None):
+    """Models of an hmmpress'ed database (format 3/f): a list of (name,
+    ByteProfile, ViterbiProfile or None), de-striped from the .h3f (MSV part)
+    and, when given, the .h3p (ViterbiFilter part), with the calibrated
+    E-value parameters the files carry."""
+    lib = load_library()
+    h = C.c_void_p()
+    rc = lib.msvh_pressed_open(h3f.encode(), h3p.encode() if h3p else None, C.byref(h))
+    if rc != 0:
+        raise MSVError(rc, f"cannot open {h3f}" + (f" / {h3p}" if h3p else ""))
+    out = []
+    try:
+        while True:
+            m, v = _Prof(), _VProf()
+            name = C.create_string_buffer(256)
+            rc = lib.msvh_pressed_next(h, C.byref(m), C.byref(v), name, 256)
+            if rc == 3:
+                break
+            if rc != 0:
+                raise MSVError(rc, f"bad pressed file {h3f} (model {len(out)})")
+            rbv = np.ctypeslib.as_array(C.cast(m.rbv, C.POINTER(C.c_uint8)), (m.Kp, m.M)).copy()
+            prof = ByteProfile(rbv, m.tbm_b, m.tec_b, m.base_b, m.bias_b, m.scale_b, mu=m.mmu, lam=m.mlambda)
+            vit = None
+            if v.M:
+                rwv = np.ctypeslib.as_array(C.cast(v.rwv, C.POINTER(C.c_int16)), (v.Kp, v.M)).copy()
+                twv = np.ctypeslib.as_array(C.cast(v.twv, C.POINTER(C.c_int16)), (8, v.M)).copy()
+                vit = ViterbiProfile(rwv, twv, v.e_loop, v.e_move, v.base_w, v.ddbound_w, v.scale_w, v.nj,
+                                     vmu=v.vmu, vlambda=v.vlambda)
+            out.append((name.value.decode(), prof, vit))
+    finally:
+        lib.msvh_pressed_close(h)
+    return out
 
 
 def vit_move(scale_w, nj, L):

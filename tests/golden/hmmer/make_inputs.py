@@ -1,6 +1,6 @@
-"""Copies (gzipped) the reference's tutorial models and sequence files that
-the drop-in pipeline test searches: data files of forestcheng/cudahmmsearch
-code/tutorial/, unchanged.
+"""Copies (gzipped) the reference's tutorial models, pressed model files and
+sequence files that the drop-in pipeline and pressed-file tests read: data
+files of forestcheng/cudahmmsearch code/tutorial/, unchanged.
 
     python tests/golden/hmmer/make_inputs.py
 """
@@ -10,7 +10,8 @@ import shutil
 
 TUT = "/root/reference/code/tutorial"
 FILES = ["fn3.hmm", "globins4.hmm", "Pkinase.hmm", "MADE1.hmm",
-         "7LESS_DROME", "HBB_HUMAN", "globins45.fa", "dna_target.fa"]
+         "7LESS_DROME", "HBB_HUMAN", "globins45.fa", "dna_target.fa",
+         "minifam.h3f", "minifam.h3p", "MADE1.hmm.h3f", "MADE1.hmm.h3p"]
 
 
 def main():

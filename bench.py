@@ -401,6 +401,22 @@ def main():
                       "finalize": round(se["ms_finalize"], 3)},
         "deferred_pairs": {"j_aware": int(se["n_jrow"]), "byte_msv": int(se["n_slow"])}, "roofline": roof,
     }
+    # LDS traffic of the fast stage: per target row each lane reads its Q
+    # registers' emissions (4 bytes each) and G lanes advance one target,
+    # i.e. Q * 4 * G bytes per target row, Q = ceil(M / 2G).
+    lds_bytes = 0
+    for p in profs:
+        G, _, seg = ctx.tile_of(p.M, did)
+        if not seg:
+            Q = -(-p.M // (2 * G))
+            lds_bytes += Q * 4 * G * db.residues
+    if lds_bytes:
+        peak_lds = ctx.probe_lds()
+        ach_lds = lds_bytes / len(pids) / (fast_ms_per_launch / 1e3)
+        roof["lds"] = {"achieved": round(ach_lds / 1e12, 3), "probe_peak": round(peak_lds / 1e12, 3),
+                       "unit": "TB/s", "frac": round(ach_lds / peak_lds, 4),
+                       "bytes_per_launch": int(lds_bytes / len(pids)),
+                       "note": "emission reads from LDS against the device's measured ds_read_b128 bandwidth"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(profs, db)
     if not args.no_vit and args.workload in ("c2", "c3"):

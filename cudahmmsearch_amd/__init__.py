@@ -109,6 +109,8 @@ def load_library(path=LIB_PATH):
         "msvg_last_stats": (C.c_int, [vp, C.POINTER(_Stats)]),
         "msvg_MSVFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
+        "msvg_probe_lds": (C.c_int, [vp, C.POINTER(C.c_double)]),
+        "msvg_tile_of": (C.c_int, [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "msvg_profile_set_viterbi": (C.c_int, [vp, i32, C.POINTER(_VProf)]),
         "msvg_vit_search": (C.c_int, [vp, i32, i32, vp, i64, vp, vp]),
         "msvg_ViterbiFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
@@ -582,6 +584,18 @@ class MSVContext:
         mu = C.c_float()
         self._chk(self.lib.msvg_calibrate_vmu(self._c, pid, float(lam), C.byref(mu)))
         return mu.value
+
+    def tile_of(self, M, did):
+        """(lanes, waves, segmented) of the batched mode for length M."""
+        ln, wv, sg = C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(self.lib.msvg_tile_of(self._c, M, did, C.byref(ln), C.byref(wv), C.byref(sg)))
+        return ln.value, wv.value, bool(sg.value)
+
+    def probe_lds(self):
+        """Sustained LDS read bandwidth of the device, bytes/s."""
+        v = C.c_double()
+        self._chk(self.lib.msvg_probe_lds(self._c, C.byref(v)))
+        return v.value
 
     def set_tile(self, m_lo, m_hi, lanes, waves=0):
         self._chk(self.lib.msvg_set_tile(self._c, m_lo, m_hi, lanes, waves))

@@ -98,14 +98,33 @@ __device__ __forceinline__ void row_fence(uint32_t (&r)[Q]) {
   for (int j = 0; j < Q; ++j) asm volatile("" : "+v"(r[j]));
 }
 
-// The QB uint4 table entries of residue row x for this lane.
-template <int QB>
-__device__ __forceinline__ void load_row(const uint4 *tl, uint32_t x, uint32_t (&e)[QB * 4]) {
+// The Q registers' emissions of residue row x for this lane: ceil(Q/4)
+// 16-byte table entries, the last one read only as wide as it is used
+// (the table is the co-limiting LDS stream, so unused bytes are not read).
+// Entries e[Q..] are left unset.
+template <int Q>
+__device__ __forceinline__ void load_row(const uint4 *tl, uint32_t x, uint32_t (&e)[((Q + 3) / 4) * 4]) {
+  constexpr int QB = (Q + 3) / 4, TAIL = Q - 4 * (QB - 1);
   const uint4 *tp = tl + x * (QB * 64);
 #pragma unroll
-  for (int jb = 0; jb < QB; ++jb) {
+  for (int jb = 0; jb < QB - 1; ++jb) {
     const uint4 v = tp[jb * 64];
     e[4 * jb + 0] = v.x; e[4 * jb + 1] = v.y; e[4 * jb + 2] = v.z; e[4 * jb + 3] = v.w;
+  }
+  const uint4 *lp = tp + (QB - 1) * 64;
+  constexpr int b = 4 * (QB - 1);
+  if constexpr (TAIL == 4) {
+    const uint4 v = *lp;
+    e[b] = v.x; e[b + 1] = v.y; e[b + 2] = v.z; e[b + 3] = v.w;
+  } else if constexpr (TAIL == 3) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(lp);
+    e[b] = v.x; e[b + 1] = v.y;
+    e[b + 2] = reinterpret_cast<const uint32_t *>(lp)[2];
+  } else if constexpr (TAIL == 2) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(lp);
+    e[b] = v.x; e[b + 1] = v.y;
+  } else {
+    e[b] = *reinterpret_cast<const uint32_t *>(lp);
   }
 }
 
@@ -184,13 +203,8 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
       const uint32_t x = byte_of(c, rr);
-      const uint4 *tp = tl + x * (QB * 64);
       uint32_t e[QB * 4];
-#pragma unroll
-      for (int jb = 0; jb < QB; ++jb) {
-        const uint4 v = tp[jb * 64];
-        e[4 * jb + 0] = v.x; e[4 * jb + 1] = v.y; e[4 * jb + 2] = v.z; e[4 * jb + 3] = v.w;
-      }
+      load_row<Q>(tl, x, e);
       const uint32_t sh = shift_slot<G>(r[Q - 1], lane);
 #pragma unroll
       for (int j = Q - 1; j >= 1; --j) r[j] = cell(r[j - 1], e[j]);
@@ -289,7 +303,7 @@ __global__ void __launch_bounds__(1024) msv_seg(SegArgs a) {
     // (the compiler would otherwise hoist all sixteen rows' reads).
     uint4 c = *reinterpret_cast<const uint4 *>(p);
     uint32_t e[QB * 4];
-    load_row<QB>(tl, byte_of(c, 0), e);
+    load_row<Q>(tl, byte_of(c, 0), e);
     for (int i0 = 0; i0 < Lpad; i0 += 16) {
       const int i1 = i0 + 16;
       const uint4 cn = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
@@ -299,7 +313,7 @@ __global__ void __launch_bounds__(1024) msv_seg(SegArgs a) {
       static_for<0, 16>([&](auto rrc) {
         constexpr int rr = decltype(rrc)::value;
         uint32_t en[QB * 4];
-        load_row<QB>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
+        load_row<Q>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
         const uint32_t last = r[Q - 1];
         uint32_t prev = __builtin_amdgcn_update_dpp(0u, last, 0x138, 0xf, 0xf, true);
         // carry from the previous segment (all zero in the first segment,
@@ -317,7 +331,7 @@ __global__ void __launch_bounds__(1024) msv_seg(SegArgs a) {
         const uint32_t co = ((uint32_t) __builtin_amdgcn_readlane((int) r[Q - 1], 63)) >> 16;
         cout = write_lane<rr>(cout, co);
 #pragma unroll
-        for (int k = 0; k < QB * 4; ++k) e[k] = en[k];
+        for (int k = 0; k < Q; ++k) e[k] = en[k];
         row_fence<Q>(r);
         asm volatile("" : "+v"(acc));
       });
@@ -400,13 +414,8 @@ __global__ void __launch_bounds__(256) msv_jrow(JArgs a) {
       for (int rr = 0; rr < 16; ++rr) {
         if (rr < nr && code == RES_OK) {
           const uint32_t x = byte_of(c, rr);
-          const uint4 *tp = tl + x * (QB * 64);
           uint32_t e[QB * 4];
-#pragma unroll
-          for (int jb = 0; jb < QB; ++jb) {
-            const uint4 v = tp[jb * 64];
-            e[4 * jb + 0] = v.x; e[4 * jb + 1] = v.y; e[4 * jb + 2] = v.z; e[4 * jb + 3] = v.w;
-          }
+          load_row<Q>(tl, x, e);
           const uint32_t sh = shift_slot<64>(r[Q - 1], lane);
 #pragma unroll
           for (int j = Q - 1; j >= 1; --j) r[j] = cell(max2h(r[j - 1], bv), e[j]);   // :150-152
@@ -798,5 +807,31 @@ extern "C" hipError_t msvk_compact(const uint8_t *pass, const float *usc, int64_
 
 extern "C" hipError_t msvk_fill_zero(int32_t *p, int n, hipStream_t st) {
   hipLaunchKernelGGL(fill_zero_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+// LDS read-bandwidth probe (measurement only): 16 waves per CU stream
+// conflict-free ds_read_b128 over a 64 KiB table, 8 per iteration; the
+// bytes read are grid * 1024 * iters * 8 * 16.  Every index stays inside
+// the table; out receives one word per thread so the reads are kept.
+__global__ void __launch_bounds__(1024) lds_probe(uint4 *out, int iters) {
+  __shared__ uint4 t[4096];
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x) t[i] = make_uint4(i, i + 1, i + 2, i + 3);
+  __syncthreads();
+  uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+  int idx = threadIdx.x;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint4 v = t[(idx + u * 512) & 4095];
+      acc.x ^= v.x; acc.y ^= v.y; acc.z ^= v.z; acc.w ^= v.w;
+    }
+    idx = (idx + 64) & 4095;
+  }
+  out[(size_t) blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+extern "C" hipError_t msvk_lds_probe(void *out, int grid, int iters, hipStream_t st) {
+  hipLaunchKernelGGL(lds_probe, dim3(grid), dim3(1024), 0, st, static_cast<uint4 *>(out), iters);
   return hipGetLastError();
 }

@@ -48,6 +48,7 @@ extern "C" hipError_t msvk_build_seg_table(const uint8_t *rbv, int M, const uint
 extern "C" hipError_t msvk_build_slow_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int Qs,
                                             uint32_t *out, hipStream_t st);
 extern "C" hipError_t msvk_fill_zero(int32_t *p, int n, hipStream_t st);
+extern "C" hipError_t msvk_lds_probe(void *out, int grid, int iters, hipStream_t st);
 
 namespace {
 
@@ -1159,6 +1160,37 @@ int msvg_calibrate_vmu(msvg_ctx *c, int32_t id, double lambda, float *ret_vmu) {
   std::vector<int32_t> st32(st.begin(), st.end());
   const VitProfile &v = *c->profiles[id]->vit;
   *ret_vmu = (float) msvh_fit_vmu(sc.data(), st32.data(), N, L, v.base_w, v.scale_w, lambda);
+  return MSVG_OK;
+}
+
+int msvg_tile_of(msvg_ctx *c, int32_t M, int32_t dbid, int32_t *lanes, int32_t *waves, int32_t *segmented) {
+  if (!c || !lanes || !waves || !segmented || M < 1) return MSVG_EINVAL;
+  auto it = c->dbs.find(dbid);
+  if (it == c->dbs.end()) return fail(c, MSVG_EINVAL, "unknown database id");
+  int G = 64, w = 0;
+  choose_tile(c, M, it->second->R, G, w);
+  const int Q = qfor(M, 2 * G);
+  const int seg = (Q > kQMax || fast_lds(it->second->R, Q) > (size_t) kLdsBytes) ? 1 : 0;
+  *lanes = seg ? 64 : G;
+  *waves = w;
+  *segmented = seg;
+  return MSVG_OK;
+}
+
+int msvg_probe_lds(msvg_ctx *c, double *ret_bytes_per_s) {
+  if (!c || !ret_bytes_per_s) return MSVG_EINVAL;
+  (void) hipSetDevice(c->device);
+  const int grid = c->ncu, iters = 4096;
+  DevMem out;
+  HIPCHK(c, out.ensure((size_t) grid * 1024 * 16));
+  HIPCHK(c, msvk_lds_probe(out.p, grid, iters, c->stream));          // warm-up
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  HIPCHK(c, msvk_lds_probe(out.p, grid, iters, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  *ret_bytes_per_s = (double) grid * 1024 * iters * 8 * 16 / (ms / 1e3);
   return MSVG_OK;
 }
 

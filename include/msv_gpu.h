@@ -164,6 +164,12 @@ int msvg_ViterbiFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, in
  * location fitted for lambda. */
 int msvg_calibrate_vmu(msvg_ctx *ctx, int32_t profile_id, double lambda, float *ret_vmu);
 
+/* Measurement helper, no counterpart in the reference: the LDS read
+ * bandwidth this device sustains (bytes/s), from a conflict-free
+ * ds_read_b128 stream on every CU; bench.py reports the MSV kernel's LDS
+ * traffic against it. */
+int msvg_probe_lds(msvg_ctx *ctx, double *ret_bytes_per_s);
+
 /* Test hook: after k more device allocations, fail the rest as out of
  * memory (k < 0: never).  The environment variable MSVG_TEST_FAIL_ALLOC sets
  * the initial value.  No counterpart in the reference. */
@@ -173,6 +179,10 @@ void msvg_test_alloc_budget(long k);
  * per workgroup for profiles of length M; 0 restores the built-in choice.
  * The autotuner in the Python package persists its findings through this. */
 int msvg_set_tile(msvg_ctx *ctx, int32_t M_lo, int32_t M_hi, int32_t lanes, int32_t waves);
+/* The tile the batched mode uses for a profile of length M on a database:
+ * lanes per target, waves per workgroup (0: derived from the LDS
+ * footprint), and whether the model takes the column-segmented path. */
+int msvg_tile_of(msvg_ctx *ctx, int32_t M, int32_t db_id, int32_t *lanes, int32_t *waves, int32_t *segmented);
 
 /* ---- Serving: continuous batching over one resident database ----
  * The counterpart of the reference's hmmpgmd worker (src/hmmdwrkr.c:532-667),

@@ -1,6 +1,6 @@
 """MSV filter benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c4]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5]
 
 A step is one complete MSV search (fast J-free stage, exact byte-MSV stage for
 the queued pairs, finalize to usc / status / F1 pass bit) of the workload's
@@ -12,6 +12,10 @@ reference's SSE p7_ViterbiFilter on a sample with a bit-identity check).
 
 Workloads (synthetic data generated on the host from seeds; Easel-compatible
 generators in cudahmmsearch_amd):
+  c1  the reference's CPU configuration: one sampled profile M=200 (seed 1)
+      vs 20,000 targets, log-normal median 300, sigma 0.6, clipped 30..5000
+      (seed 2), 1% replaced by p7_CoreEmit samples (seed 5); also times the
+      one-target drop-in (msvg_MSVFilter) per call
   c2  one sampled profile M=400 (p7_hmm_Sample seed 3) vs 2,000,000 targets,
       log-normal lengths median 350, sigma 0.7, clipped 20..10000 (length and
       residue seed 4), 0.5% replaced by p7_CoreEmit samples (seed 5)
@@ -58,7 +62,14 @@ def log(*a):
 
 
 def make_workload(ca, name):
-    if name == "c2":
+    if name == "c1":
+        profs = [ca.sample_profile(1, 200)]
+        db = ca.gen_database(2, 20_000, 300, 0.6, 30, 5000, nthreads=16)
+        hom = ca.core_emit(profs[0].hmm, 5, int(0.01 * db.n))
+        db, _ = ca.plant(db, hom, 5, 0.01)
+        desc = ("1 sampled profile M=200 (seed 1) vs 20,000 i.i.d. targets, log-normal median 300 sigma 0.6 "
+                "clip 30-5000, 1% planted p7_CoreEmit homologs")
+    elif name == "c2":
         profs = [ca.sample_profile(3, 400)]
         db = ca.gen_database(4, 2_000_000, 350, 0.7, 20, 10000, nthreads=16)
         hom = ca.core_emit(profs[0].hmm, 5, int(0.005 * db.n))
@@ -202,6 +213,22 @@ def vit_cpu_baseline(ca, db, p, surv, sc, st, budget_s=8.0):
             "kind": "reference", "bit_identical_to_gpu": same,
             "sample": f"{k} MSV survivors ({sub.residues} residues), profile M={p.M}, "
                       f"p7_ViterbiFilter on {nthreads} threads, {dt:.2f} s"}
+
+
+def dropin_stage(ctx, pid, db, usc_batched, n=2000):
+    """The one-target drop-in (msvg_MSVFilter, what p7_MSVFilter calls per
+    target): host-to-host latency per call on the first n targets, and
+    whether each call returns the batched search's score."""
+    n = min(n, db.n)
+    seqs = [db.seq(i) for i in range(n)]
+    ctx.msv_filter(pid, seqs[0])                            # warm-up
+    sc = np.zeros(n, np.float32)
+    t0 = time.perf_counter()
+    for i, x in enumerate(seqs):
+        sc[i] = ctx.msv_filter(pid, x)[1]
+    dt = time.perf_counter() - t0
+    return {"calls": n, "us_per_call": round(dt / n * 1e6, 1), "calls_per_s": round(n / dt, 1),
+            "identical_to_batched": bool(np.array_equal(sc.view(np.uint32), usc_batched[:n].view(np.uint32)))}
 
 
 def copy_probe(nbytes=256 << 20):
@@ -419,6 +446,14 @@ def main():
                        "note": "emission reads from LDS against the device's measured ds_read_b128 bandwidth"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(profs, db)
+    if args.workload == "c1":
+        ctx.search_device(pids, did, mode=ca.MODE_BATCHED)
+        ub = np.zeros((len(pids), db.n), np.float32)
+        ctx.lib.msvg_results(ctx._c, ub.ctypes.data, None, None)
+        out["dropin"] = dropin_stage(ctx, pids[0], db, ub[0])
+        out["dropin"]["gcups"] = round(profs[0].M * sum(int(db.lens[i]) for i in range(out["dropin"]["calls"]))
+                                       * out["dropin"]["calls_per_s"] / out["dropin"]["calls"] / 1e9, 3)
+        log(f"drop-in: {out['dropin']}")
     if not args.no_vit and args.workload in ("c2", "c3"):
         # the batched search above left its results (and the pass bits) on the device
         ctx.search_device(pids, did, mode=ca.MODE_BATCHED)

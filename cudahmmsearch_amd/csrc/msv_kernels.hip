@@ -407,35 +407,93 @@ __global__ void __launch_bounds__(256) msv_jrow(JArgs a) {
 #pragma unroll
     for (int j = 0; j < Q; ++j) r[j] = 0u;
 
+    // Rows go in chunks of 16.  B changes only on a row whose xE - tec
+    // exceeds max(base, xJ) (msvfilter.c:187-191), so each chunk is first
+    // computed with B held at its value on entry, keeping only a per-lane
+    // running maximum (no per-row wave reduction on the row chain).  If the
+    // chunk's maximum xE stays within max(base, xJ) + tec, no row changed B
+    // and every row was exact; the overflow test and xJ then follow from the
+    // chunk maximum alone.  Otherwise the chunk is replayed row by row from
+    // its entry state, and the next chunk starts row by row as well until a
+    // chunk passes with B unchanged.  Targets are padded to 16 bytes with the
+    // null row (all cells to the floor), so a partial last chunk is computed
+    // whole in the first form: the extra rows leave the maximum unchanged.
+    // The next chunk's residues are fetched while a chunk runs.
+    const int Lpad = (L + 15) & ~15;
+    const uint4 z4 = {0u, 0u, 0u, 0u};
+    uint4 c = (L > 0) ? *reinterpret_cast<const uint4 *>(p) : z4;
+    bool spec = true;
     for (int i0 = 0; i0 < L && code == RES_OK; i0 += 16) {
-      const uint4 c = *reinterpret_cast<const uint4 *>(p + i0);
-      const int nr = min(16, L - i0);
+      const int i1 = i0 + 16;
+      const uint4 cn = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
+      bool exact = !spec;
+      if (spec) {
+        uint32_t r0[Q];
 #pragma unroll
-      for (int rr = 0; rr < 16; ++rr) {
-        if (rr < nr && code == RES_OK) {
-          const uint32_t x = byte_of(c, rr);
+        for (int j = 0; j < Q; ++j) r0[j] = r[j];
+        uint32_t acc = 0u;
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
           uint32_t e[QB * 4];
-          load_row<Q>(tl, x, e);
+          load_row<Q>(tl, byte_of(c, rr), e);
           const uint32_t sh = shift_slot<64>(r[Q - 1], lane);
 #pragma unroll
           for (int j = Q - 1; j >= 1; --j) r[j] = cell(max2h(r[j - 1], bv), e[j]);   // :150-152
           r[0] = cell(max2h(sh, bv), e[0]);
-          uint32_t rm = 0u;
           int j = 0;
 #pragma unroll
-          for (; j + 1 < Q; j += 2) rm = max3h(rm, r[j], r[j + 1]);                  // :153
-          if constexpr ((Q & 1) != 0) rm = max2h(rm, r[Q - 1]);
-          const h2v rh = as_h2(rm);
-          const float m = wave_max_f32(fmaxf((float) rh.x, (float) rh.y));
-          const int xE = (int) (m * 2048.0f + 0.5f) + B0;
-          if (xE + pd.bias >= 255) { code = RES_ERANGE; break; }                     // :160-185
-          xJ = max(xJ, max(0, xE - pd.tec));                                         // :187-188
-          const int xB = max(0, max(pd.base, xJ) - tjbm);                            // :190-191
-          const _Float16 bh = (_Float16) ((float) (xB - B0) * (1.0f / 2048.0f));
-          const uint32_t b16 = __builtin_bit_cast(unsigned short, bh);
-          bv = b16 | (b16 << 16);
+          for (; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);                // :153
+          if constexpr ((Q & 1) != 0) acc = max2h(acc, r[Q - 1]);
+        }
+        const h2v ah = as_h2(acc);
+        const float m = wave_max_f32(fmaxf((float) ah.x, (float) ah.y));
+        const int xEm = __builtin_amdgcn_readfirstlane((int) (m * 2048.0f + 0.5f)) + B0;
+        if (xEm - pd.tec <= max(pd.base, xJ)) {
+          if (xEm + pd.bias >= 255) { code = RES_ERANGE; break; }                    // :160-185
+          xJ = max(xJ, max(0, xEm - pd.tec));                                        // :187-188
+        } else {
+#pragma unroll
+          for (int j = 0; j < Q; ++j) r[j] = r0[j];
+          exact = true;
         }
       }
+      if (exact) {
+        const int nr = min(16, L - i0);
+        bool changed = false;
+        uint32_t e[QB * 4];
+        load_row<Q>(tl, byte_of(c, 0), e);
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {
+          if (rr < nr && code == RES_OK) {
+            uint32_t en[QB * 4];
+            load_row<Q>(tl, (rr < 15) ? byte_of(c, rr + 1) : 0u, en);
+            const uint32_t sh = shift_slot<64>(r[Q - 1], lane);
+#pragma unroll
+            for (int j = Q - 1; j >= 1; --j) r[j] = cell(max2h(r[j - 1], bv), e[j]);   // :150-152
+            r[0] = cell(max2h(sh, bv), e[0]);
+            uint32_t rm = 0u;
+            int j = 0;
+#pragma unroll
+            for (; j + 1 < Q; j += 2) rm = max3h(rm, r[j], r[j + 1]);                  // :153
+            if constexpr ((Q & 1) != 0) rm = max2h(rm, r[Q - 1]);
+            const h2v rh = as_h2(rm);
+            const float m = wave_max_f32(fmaxf((float) rh.x, (float) rh.y));
+            const int xE = __builtin_amdgcn_readfirstlane((int) (m * 2048.0f + 0.5f)) + B0;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) e[k] = en[k];
+            if (xE + pd.bias >= 255) { code = RES_ERANGE; break; }                     // :160-185
+            xJ = max(xJ, max(0, xE - pd.tec));                                         // :187-188
+            const int xB = max(0, max(pd.base, xJ) - tjbm);                            // :190-191
+            const _Float16 bh = (_Float16) ((float) (xB - B0) * (1.0f / 2048.0f));
+            const uint32_t b16 = __builtin_bit_cast(unsigned short, bh);
+            const uint32_t nb = b16 | (b16 << 16);
+            changed = changed || (nb != bv);
+            bv = nb;
+          }
+        }
+        spec = !changed;
+      }
+      c = cn;
     }
     if (lane == 0) pd.res[s] = (uint16_t) (code == RES_ERANGE ? (RES_ERANGE << 8) : xJ);
   }

@@ -257,3 +257,34 @@ def test_deferred_pairs_take_the_j_stage(ctx):
         assert st["n_jrow"] > 0 and st["n_slow"] == 0, (M, st)
         ctx.remove_database(did)
         ctx.remove_profile(pid)
+
+
+def test_j_stage_every_chunk_offset(ctx):
+    """Multi-domain targets for the J-aware stage, which takes rows 16 at a
+    time: a background prefix of every length 0..47 before the first
+    homolog, background gaps between two to four homologs, and total
+    lengths of every residue mod 16, so that B changes (and overflows) fall
+    on every row offset of a chunk and partial last chunks take every
+    length.  Every pair against the oracle, in both launch modes."""
+    for M in (40, 120, 300):
+        prof = ca.sample_profile(900 + M, M)
+        hom = ca.core_emit(prof.hmm, 901 + M, 160)
+        bg = ca.gen_database(902 + M, 640, 40, 0.6, 1, 300)
+        rng = np.random.default_rng(M)
+        seqs = []
+        h = 0
+        for t in range(48 * 3):
+            parts = [bg.seq((4 * t) % bg.n)[: t % 48]]
+            for d in range(2 + t % 3):
+                parts.append(hom[h % len(hom)])
+                h += 1
+                parts.append(bg.seq((4 * t + d + 1) % bg.n)[: int(rng.integers(0, 80))])
+            s = np.concatenate(parts).astype(np.uint8)
+            seqs.append(s[: max(1, len(s) - int(rng.integers(0, 16)))])
+        db = ca.Database.from_list(seqs)
+        print(f"M={M} targets={db.n} residues={db.residues}", flush=True)
+        for mode in (ca.MODE_PLAIN, ca.MODE_BATCHED):
+            pid, did, _, st, _ = _check_vs_oracle(ctx, prof, db, mode)
+            ctx.remove_database(did)
+            ctx.remove_profile(pid)
+        assert (st[0] == 16).any() or M == 40

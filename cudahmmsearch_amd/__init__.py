@@ -55,7 +55,7 @@ class _Prof(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("ms_fast", C.c_double), ("ms_slow", C.c_double), ("ms_finalize", C.c_double),
                 ("ms_total", C.c_double), ("cells", C.c_int64), ("n_slow", C.c_int64), ("n_jrow", C.c_int64),
-                ("n_fast_launches", C.c_int32)]
+                ("n_fast_launches", C.c_int32), ("dropin_path", C.c_int32)]
 
 
 class _VProf(C.Structure):

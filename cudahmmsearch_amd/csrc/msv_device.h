@@ -84,6 +84,14 @@ struct JArgs {
   int32_t cap;
 };
 
+struct OneArgs {               // one target, one workgroup (the per-call drop-in)
+  ProfDev prof;
+  const uint4 *tab;            // 64-lane fast table over residue codes 0..R-1, null row R
+  const uint8_t *res;          // the target's codes, padded to 16 bytes with R
+  int32_t L, tjb, R;
+  uint32_t *out;               // result word
+};
+
 struct SlowArgs {
   const ProfDev *profs;
   DbDev db;

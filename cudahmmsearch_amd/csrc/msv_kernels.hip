@@ -379,6 +379,117 @@ __device__ __forceinline__ float wave_max_f32(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
 
+// One pair, one wave: tl = this lane's column of the 64-lane fast table
+// (in LDS), p = the target's residue rows padded to 16 bytes with the null
+// row.  Returns the result word (xJ, or RES_ERANGE << 8); xEmax receives the
+// largest row maximum seen (stored cells, floor B0) up to the last row
+// computed.
+template <int Q>
+__device__ __forceinline__ uint32_t jrow_task(const uint4 *tl, const uint8_t *p, int L, int tjb, const ProfDev &pd,
+                                              int lane, int &xEmax) {
+  constexpr int QB = (Q + 3) / 4;
+  const int tjbm = (tjb + pd.tbm) & 255;                       // msvfilter.c:120
+  const int B0 = max(0, pd.base - tjbm);                       // :124
+  int xJ = 0, code = RES_OK;
+  xEmax = B0;
+  uint32_t bv = 0u;                                            // (xB - B0) / 2048 in both halves
+  uint32_t r[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) r[j] = 0u;
+
+  // Rows go in chunks of 16.  B changes only on a row whose xE - tec
+  // exceeds max(base, xJ) (msvfilter.c:187-191), so each chunk is first
+  // computed with B held at its value on entry, keeping only a per-lane
+  // running maximum (no per-row wave reduction on the row chain).  If the
+  // chunk's maximum xE stays within max(base, xJ) + tec, no row changed B
+  // and every row was exact; the overflow test and xJ then follow from the
+  // chunk maximum alone.  Otherwise the chunk is replayed row by row from
+  // its entry state, and the next chunk starts row by row as well until a
+  // chunk passes with B unchanged.  Targets are padded to 16 bytes with the
+  // null row (all cells to the floor), so a partial last chunk is computed
+  // whole in the first form: the extra rows leave the maximum unchanged.
+  // The next chunk's residues are fetched while a chunk runs.
+  const int Lpad = (L + 15) & ~15;
+  const uint4 z4 = {0u, 0u, 0u, 0u};
+  uint4 c = (L > 0) ? *reinterpret_cast<const uint4 *>(p) : z4;
+  bool spec = true;
+  for (int i0 = 0; i0 < L && code == RES_OK; i0 += 16) {
+    const int i1 = i0 + 16;
+    const uint4 cn = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
+    bool exact = !spec;
+    if (spec) {
+      uint32_t r0[Q];
+#pragma unroll
+      for (int j = 0; j < Q; ++j) r0[j] = r[j];
+      uint32_t acc = 0u;
+#pragma unroll
+      for (int rr = 0; rr < 16; ++rr) {
+        uint32_t e[QB * 4];
+        load_row<Q>(tl, byte_of(c, rr), e);
+        const uint32_t sh = shift_slot<64>(r[Q - 1], lane);
+#pragma unroll
+        for (int j = Q - 1; j >= 1; --j) r[j] = cell(max2h(r[j - 1], bv), e[j]);   // :150-152
+        r[0] = cell(max2h(sh, bv), e[0]);
+        int j = 0;
+#pragma unroll
+        for (; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);                // :153
+        if constexpr ((Q & 1) != 0) acc = max2h(acc, r[Q - 1]);
+      }
+      const h2v ah = as_h2(acc);
+      const float m = wave_max_f32(fmaxf((float) ah.x, (float) ah.y));
+      const int xEm = __builtin_amdgcn_readfirstlane((int) (m * 2048.0f + 0.5f)) + B0;
+      xEmax = max(xEmax, xEm);
+      if (xEm - pd.tec <= max(pd.base, xJ)) {
+        if (xEm + pd.bias >= 255) { code = RES_ERANGE; break; }                    // :160-185
+        xJ = max(xJ, max(0, xEm - pd.tec));                                        // :187-188
+      } else {
+#pragma unroll
+        for (int j = 0; j < Q; ++j) r[j] = r0[j];
+        exact = true;
+      }
+    }
+    if (exact) {
+      const int nr = min(16, L - i0);
+      bool changed = false;
+      uint32_t e[QB * 4];
+      load_row<Q>(tl, byte_of(c, 0), e);
+#pragma unroll
+      for (int rr = 0; rr < 16; ++rr) {
+        if (rr < nr && code == RES_OK) {
+          uint32_t en[QB * 4];
+          load_row<Q>(tl, (rr < 15) ? byte_of(c, rr + 1) : 0u, en);
+          const uint32_t sh = shift_slot<64>(r[Q - 1], lane);
+#pragma unroll
+          for (int j = Q - 1; j >= 1; --j) r[j] = cell(max2h(r[j - 1], bv), e[j]);   // :150-152
+          r[0] = cell(max2h(sh, bv), e[0]);
+          uint32_t rm = 0u;
+          int j = 0;
+#pragma unroll
+          for (; j + 1 < Q; j += 2) rm = max3h(rm, r[j], r[j + 1]);                  // :153
+          if constexpr ((Q & 1) != 0) rm = max2h(rm, r[Q - 1]);
+          const h2v rh = as_h2(rm);
+          const float m = wave_max_f32(fmaxf((float) rh.x, (float) rh.y));
+          const int xE = __builtin_amdgcn_readfirstlane((int) (m * 2048.0f + 0.5f)) + B0;
+          xEmax = max(xEmax, xE);
+#pragma unroll
+          for (int k = 0; k < Q; ++k) e[k] = en[k];
+          if (xE + pd.bias >= 255) { code = RES_ERANGE; break; }                     // :160-185
+          xJ = max(xJ, max(0, xE - pd.tec));                                         // :187-188
+          const int xB = max(0, max(pd.base, xJ) - tjbm);                            // :190-191
+          const _Float16 bh = (_Float16) ((float) (xB - B0) * (1.0f / 2048.0f));
+          const uint32_t b16 = __builtin_bit_cast(unsigned short, bh);
+          const uint32_t nb = b16 | (b16 << 16);
+          changed = changed || (nb != bv);
+          bv = nb;
+        }
+      }
+      spec = !changed;
+    }
+    c = cn;
+  }
+  return code == RES_ERANGE ? (RES_ERANGE << 8) : (uint32_t) xJ;
+}
+
 template <int Q>
 __global__ void __launch_bounds__(256) msv_jrow(JArgs a) {
   extern __shared__ uint4 lds[];
@@ -391,112 +502,41 @@ __global__ void __launch_bounds__(256) msv_jrow(JArgs a) {
   const int ntab = (a.db.R + 1) * QB * 64;
   for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = a.tab[i];
   __syncthreads();
-  const uint4 *tl = lds + lane;
-  const ProfDev &pd = a.prof;
-
   for (int it = blockIdx.x * nwaves + wave; it < n; it += gridDim.x * nwaves) {
     const int64_t s = a.items[it].seq;
-    const int L = a.db.len[s];
-    const uint8_t *p = a.db.res + a.db.off[s];
-    const int tjb = a.db.tjb[s];
-    const int tjbm = (tjb + pd.tbm) & 255;                       // msvfilter.c:120
-    const int B0 = max(0, pd.base - tjbm);                       // :124
-    int xJ = 0, code = RES_OK;
-    uint32_t bv = 0u;                                            // (xB - B0) / 2048 in both halves
-    uint32_t r[Q];
-#pragma unroll
-    for (int j = 0; j < Q; ++j) r[j] = 0u;
-
-    // Rows go in chunks of 16.  B changes only on a row whose xE - tec
-    // exceeds max(base, xJ) (msvfilter.c:187-191), so each chunk is first
-    // computed with B held at its value on entry, keeping only a per-lane
-    // running maximum (no per-row wave reduction on the row chain).  If the
-    // chunk's maximum xE stays within max(base, xJ) + tec, no row changed B
-    // and every row was exact; the overflow test and xJ then follow from the
-    // chunk maximum alone.  Otherwise the chunk is replayed row by row from
-    // its entry state, and the next chunk starts row by row as well until a
-    // chunk passes with B unchanged.  Targets are padded to 16 bytes with the
-    // null row (all cells to the floor), so a partial last chunk is computed
-    // whole in the first form: the extra rows leave the maximum unchanged.
-    // The next chunk's residues are fetched while a chunk runs.
-    const int Lpad = (L + 15) & ~15;
-    const uint4 z4 = {0u, 0u, 0u, 0u};
-    uint4 c = (L > 0) ? *reinterpret_cast<const uint4 *>(p) : z4;
-    bool spec = true;
-    for (int i0 = 0; i0 < L && code == RES_OK; i0 += 16) {
-      const int i1 = i0 + 16;
-      const uint4 cn = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
-      bool exact = !spec;
-      if (spec) {
-        uint32_t r0[Q];
-#pragma unroll
-        for (int j = 0; j < Q; ++j) r0[j] = r[j];
-        uint32_t acc = 0u;
-#pragma unroll
-        for (int rr = 0; rr < 16; ++rr) {
-          uint32_t e[QB * 4];
-          load_row<Q>(tl, byte_of(c, rr), e);
-          const uint32_t sh = shift_slot<64>(r[Q - 1], lane);
-#pragma unroll
-          for (int j = Q - 1; j >= 1; --j) r[j] = cell(max2h(r[j - 1], bv), e[j]);   // :150-152
-          r[0] = cell(max2h(sh, bv), e[0]);
-          int j = 0;
-#pragma unroll
-          for (; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);                // :153
-          if constexpr ((Q & 1) != 0) acc = max2h(acc, r[Q - 1]);
-        }
-        const h2v ah = as_h2(acc);
-        const float m = wave_max_f32(fmaxf((float) ah.x, (float) ah.y));
-        const int xEm = __builtin_amdgcn_readfirstlane((int) (m * 2048.0f + 0.5f)) + B0;
-        if (xEm - pd.tec <= max(pd.base, xJ)) {
-          if (xEm + pd.bias >= 255) { code = RES_ERANGE; break; }                    // :160-185
-          xJ = max(xJ, max(0, xEm - pd.tec));                                        // :187-188
-        } else {
-#pragma unroll
-          for (int j = 0; j < Q; ++j) r[j] = r0[j];
-          exact = true;
-        }
-      }
-      if (exact) {
-        const int nr = min(16, L - i0);
-        bool changed = false;
-        uint32_t e[QB * 4];
-        load_row<Q>(tl, byte_of(c, 0), e);
-#pragma unroll
-        for (int rr = 0; rr < 16; ++rr) {
-          if (rr < nr && code == RES_OK) {
-            uint32_t en[QB * 4];
-            load_row<Q>(tl, (rr < 15) ? byte_of(c, rr + 1) : 0u, en);
-            const uint32_t sh = shift_slot<64>(r[Q - 1], lane);
-#pragma unroll
-            for (int j = Q - 1; j >= 1; --j) r[j] = cell(max2h(r[j - 1], bv), e[j]);   // :150-152
-            r[0] = cell(max2h(sh, bv), e[0]);
-            uint32_t rm = 0u;
-            int j = 0;
-#pragma unroll
-            for (; j + 1 < Q; j += 2) rm = max3h(rm, r[j], r[j + 1]);                  // :153
-            if constexpr ((Q & 1) != 0) rm = max2h(rm, r[Q - 1]);
-            const h2v rh = as_h2(rm);
-            const float m = wave_max_f32(fmaxf((float) rh.x, (float) rh.y));
-            const int xE = __builtin_amdgcn_readfirstlane((int) (m * 2048.0f + 0.5f)) + B0;
-#pragma unroll
-            for (int k = 0; k < Q; ++k) e[k] = en[k];
-            if (xE + pd.bias >= 255) { code = RES_ERANGE; break; }                     // :160-185
-            xJ = max(xJ, max(0, xE - pd.tec));                                         // :187-188
-            const int xB = max(0, max(pd.base, xJ) - tjbm);                            // :190-191
-            const _Float16 bh = (_Float16) ((float) (xB - B0) * (1.0f / 2048.0f));
-            const uint32_t b16 = __builtin_bit_cast(unsigned short, bh);
-            const uint32_t nb = b16 | (b16 << 16);
-            changed = changed || (nb != bv);
-            bv = nb;
-          }
-        }
-        spec = !changed;
-      }
-      c = cn;
-    }
-    if (lane == 0) pd.res[s] = (uint16_t) (code == RES_ERANGE ? (RES_ERANGE << 8) : xJ);
+    int xEmax;
+    const uint32_t w = jrow_task<Q>(lds + lane, a.db.res + a.db.off[s], a.db.len[s], a.db.tjb[s], a.prof, lane, xEmax);
+    if (lane == 0) a.prof.res[s] = (uint16_t) w;
   }
+}
+
+// ---------------------------------------------------------------------------
+// One-target composite for the per-call drop-in (msvfilter.c:78-213 for one
+// dsq): the J-aware row loop above from the start of the target, then the
+// composite's decision.  Before J first fires B is the constant B0, so up to
+// that row the loop computes exactly the J-free cells of the fast stage:
+// when J never fires (largest row maximum - tec <= base) the pair's word is
+// decide() of that maximum, as in the fast stage; when it fires, the loop's
+// own result is the J-aware stage's.  RES_SLOW words (the guard regime's
+// sub-B0 maximum) are left to the caller.  One workgroup: all its threads
+// stage the table in LDS, the first wave computes.
+// ---------------------------------------------------------------------------
+template <int Q>
+__global__ void __launch_bounds__(1024) msv_one(OneArgs a) {
+  extern __shared__ uint4 lds[];
+  constexpr int QB = (Q + 3) / 4;
+  const int ntab = (a.R + 1) * QB * 64;
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = a.tab[i];
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  int xEmax;
+  uint32_t w = jrow_task<Q>(lds + lane, a.res, a.L, a.tjb, a.prof, lane, xEmax);
+  if ((w >> 8) != RES_ERANGE && xEmax - a.prof.tec <= a.prof.base) {
+    const int tjbm = (a.tjb + a.prof.tbm) & 255;
+    w = decide(xEmax - max(0, a.prof.base - tjbm), a.tjb, a.prof);
+  }
+  if (lane == 0) *a.out = w;
 }
 
 // ---------------------------------------------------------------------------
@@ -800,11 +840,40 @@ extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t l
   return dispatch_jrow(Q, *a, grid, lds, st, MSVG_QSEQ{});
 }
 
+template <int Q>
+static hipError_t launch_one_t(const OneArgs &a, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((msv_one<Q>), dim3(1), dim3(1024), lds, st, a);
+  return hipGetLastError();
+}
+template <int... Qs>
+static hipError_t dispatch_one(int Q, const OneArgs &a, size_t lds, hipStream_t st, std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipErrorInvalidValue;
+  (void) ((Q == Qs ? (err = launch_one_t<Qs>(a, lds, st), true) : false) || ...);
+  return err;
+}
+template <int Q>
+static hipError_t set_lds_one_t() {
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&msv_one<Q>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+template <int... Qs>
+static hipError_t set_lds_one(std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipSuccess;
+  ((err = (err == hipSuccess ? set_lds_one_t<Qs>() : err)), ...);
+  return err;
+}
+
+extern "C" hipError_t msvk_launch_one(int Q, const OneArgs *a, size_t lds, hipStream_t st) {
+  if (Q < 1 || Q > 32) return hipErrorInvalidValue;
+  return dispatch_one(Q, *a, lds, st, MSVG_QSEQ{});
+}
+
 extern "C" hipError_t msvk_prepare() {
   hipError_t e;
   if ((e = set_lds_all<8>(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_seg(MSVG_SEGQ{})) != hipSuccess) return e;
   if ((e = set_lds_jrow(MSVG_QSEQ{})) != hipSuccess) return e;
+  if ((e = set_lds_one(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_all<16>(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_all<32>(MSVG_QSEQ{})) != hipSuccess) return e;
   return set_lds_all<64>(MSVG_QSEQ{});

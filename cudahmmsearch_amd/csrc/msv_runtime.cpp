@@ -16,6 +16,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <limits>
 #include <mutex>
 #include <thread>
 #include <cmath>
@@ -49,6 +50,7 @@ extern "C" hipError_t msvk_build_slow_table(const uint8_t *rbv, int M, const uin
                                             uint32_t *out, hipStream_t st);
 extern "C" hipError_t msvk_fill_zero(int32_t *p, int n, hipStream_t st);
 extern "C" hipError_t msvk_lds_probe(void *out, int grid, int iters, hipStream_t st);
+extern "C" hipError_t msvk_launch_one(int Q, const OneArgs *a, size_t lds, hipStream_t st);
 
 namespace {
 
@@ -113,6 +115,7 @@ struct Profile {
   float scale = 0.f, mu = 0.f, lambda = 0.f;
   DevMem d_rbv;
   std::unique_ptr<VitProfile> vit;
+  DevMem d_onetab;                 // per-call drop-in: 64-lane fast table over codes 0..Kp-1
 };
 
 struct Db {
@@ -178,6 +181,10 @@ struct msvg_ctx {
   std::map<std::string, std::unique_ptr<Plan>> plans;
   // single-target drop-in
   int scratch_db = -1;
+  DevMem d_one_res, d_one_out, d_idrow;     // target rows, result word, identity row map
+  uint8_t *h_one_res = nullptr;             // pinned staging for the target
+  size_t h_one_cap = 0;
+  uint32_t *h_one_out = nullptr;            // pinned result word
   // ViterbiFilter searches
   DevMem d_vitems, d_vmove, d_vout;
 };
@@ -635,6 +642,7 @@ int after_search(msvg_ctx *c, Bufs &b, int np, int dbid, int64_t cells, bool tim
   c->stats.cells = cells;
   c->stats.n_slow = cnt;
   c->stats.n_jrow = jtot;
+  c->stats.dropin_path = 0;
   if (timed) {
     float a = 0, b = 0, f = 0, t = 0;
     HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
@@ -790,6 +798,8 @@ void msvg_free(msvg_ctx *c) {
   if (c->join) (void) hipEventDestroy(c->join);
   if (c->side) { (void) hipStreamSynchronize(c->side); (void) hipStreamDestroy(c->side); }
   if (c->stream) (void) hipStreamDestroy(c->stream);
+  if (c->h_one_res) (void) hipHostFree(c->h_one_res);
+  if (c->h_one_out) (void) hipHostFree(c->h_one_out);
   delete c;
 }
 
@@ -1194,9 +1204,84 @@ int msvg_probe_lds(msvg_ctx *c, double *ret_bytes_per_s) {
   return MSVG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The per-call drop-in's own path: one launch of msv_one over the target,
+// staged through pinned memory, with a table over the profile's whole
+// alphabet built once per profile.  *done is false (and nothing is
+// reported) when the pair needs what only the batch path has: an empty
+// target, codes outside the alphabet (the batch path reports them), a
+// table that does not fit LDS, or the guard regime's exact sub-B0 maximum.
+int one_target(msvg_ctx *c, Profile &p, const uint8_t *dsq, int L, float *ret_sc, bool *done) {
+  *done = false;
+  const int Q = qfor(p.M, 128), QB = (Q + 3) / 4;
+  const size_t lds = fast_lds(p.Kp, Q);
+  if (L == 0 || Q > kQMax || lds > (size_t) kLdsBytes || p.Kp > 254) return MSVG_OK;
+  for (int i = 1; i <= L; ++i)
+    if (dsq[i] >= p.Kp) return MSVG_OK;
+  hipStream_t st = c->stream;
+  if (!p.d_onetab.p) {
+    if (!c->d_idrow.p) {
+      uint8_t id[256];
+      for (int x = 0; x < 256; ++x) id[x] = (uint8_t) x;
+      HIPCHK(c, c->d_idrow.ensure(256));
+      HIPCHK(c, hipMemcpyAsync(c->d_idrow.p, id, 256, hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+    }
+    HIPCHK(c, p.d_onetab.ensure((size_t) (p.Kp + 1) * QB * 64 * 16));
+    HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, 64, Q, p.bias,
+                                    p.d_onetab.as<uint32_t>(), st));
+  }
+  const size_t Lpad = ((size_t) L + 15) & ~(size_t) 15;
+  if (c->h_one_cap < Lpad) {
+    if (c->h_one_res) { (void) hipHostFree(c->h_one_res); c->h_one_res = nullptr; c->h_one_cap = 0; }
+    const size_t cap = std::max<size_t>(Lpad, 4096);
+    HIPCHK(c, hipHostMalloc((void **) &c->h_one_res, cap, hipHostMallocDefault));
+    c->h_one_cap = cap;
+  }
+  if (!c->h_one_out) HIPCHK(c, hipHostMalloc((void **) &c->h_one_out, sizeof(uint32_t), hipHostMallocDefault));
+  HIPCHK(c, c->d_one_res.ensure(std::max<size_t>(Lpad, 4096)));
+  HIPCHK(c, c->d_one_out.ensure(sizeof(uint32_t)));
+  std::memcpy(c->h_one_res, dsq + 1, (size_t) L);
+  std::memset(c->h_one_res + L, p.Kp, Lpad - (size_t) L);
+  HIPCHK(c, hipMemcpyAsync(c->d_one_res.p, c->h_one_res, Lpad, hipMemcpyHostToDevice, st));
+  OneArgs a{};
+  a.prof.M = p.M; a.prof.Q = Q; a.prof.QB = QB; a.prof.Qs = qfor(p.M, 128);
+  a.prof.tbm = p.tbm; a.prof.tec = p.tec; a.prof.base = p.base; a.prof.bias = p.bias;
+  a.prof.slow_all = 0;
+  a.tab = p.d_onetab.as<uint4>();
+  a.res = c->d_one_res.as<uint8_t>();
+  a.L = L;
+  a.tjb = msvh_tjb(L);
+  a.R = p.Kp;
+  a.out = c->d_one_out.as<uint32_t>();
+  HIPCHK(c, msvk_launch_one(Q, &a, lds, st));
+  HIPCHK(c, hipMemcpyAsync(c->h_one_out, c->d_one_out.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  const uint32_t w = *c->h_one_out;
+  const uint32_t code = w >> 8;
+  if (code == RES_SLOW) return MSVG_OK;
+  *done = true;
+  c->stats = msvg_stats{};
+  c->stats.cells = (int64_t) p.M * L;
+  c->stats.dropin_path = 1;
+  if (code == RES_ERANGE) { *ret_sc = std::numeric_limits<float>::infinity(); return MSVG_ERANGE; }
+  *ret_sc = usc_of((int) (w & 255u) - a.tjb, p.base, p.scale);
+  return MSVG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int msvg_MSVFilter(msvg_ctx *c, int32_t pid, const uint8_t *dsq, int L, float *ret_sc) {
   if (!c || !dsq || !ret_sc || L < 0) return fail(c, MSVG_EINVAL, "null argument");
   if (!c->profiles.count(pid)) return fail(c, MSVG_EINVAL, "unknown profile id");
+  bool done = false;
+  const int rc1 = one_target(c, *c->profiles[pid], dsq, L, ret_sc, &done);
+  if (done || rc1 != MSVG_OK) return rc1;
   // One-target database holding dsq[1..L]; reused by the next call.
   if (c->scratch_db >= 0) msvg_db_remove(c, c->scratch_db);
   c->scratch_db = -1;
@@ -1209,6 +1294,7 @@ int msvg_MSVFilter(msvg_ctx *c, int32_t pid, const uint8_t *dsq, int L, float *r
   uint8_t st;
   rc = msvg_search(c, &pid, 1, id, 0.02, MSVG_MODE_PLAIN, &usc, &st, nullptr);
   if (rc != MSVG_OK) return rc;
+  c->stats.dropin_path = 2;
   *ret_sc = usc;
   return st == 16 ? MSVG_ERANGE : MSVG_OK;
 }

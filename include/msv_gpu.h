@@ -69,6 +69,7 @@ typedef struct {
   int64_t n_slow;          /* pairs that ran the exact byte MSV */
   int64_t n_jrow;          /* pairs that ran the J-aware f16 stage */
   int32_t n_fast_launches;
+  int32_t dropin_path;     /* after msvg_MSVFilter: 1 its one-launch kernel, 2 the search path */
 } msvg_stats;
 
 int         msvg_init(int device, msvg_ctx **ret_ctx);
@@ -114,7 +115,11 @@ int msvg_last_stats(msvg_ctx *ctx, msvg_stats *out);
 
 /* Drop-in for p7_MSVFilter: dsq is 1-based with sentinels at 0 and L+1
  * (easel.h:308-309).  Returns MSVG_OK or MSVG_ERANGE; *ret_sc as the
- * reference sets it. */
+ * reference sets it.  One kernel launch per call where the profile's table
+ * over its whole alphabet fits LDS (M up to 2560 for amino, 4096 for DNA);
+ * empty targets, longer profiles and the rare pairs needing the exact
+ * sub-B0 maximum take the search path (msvg_search on a one-target
+ * database), with the same results. */
 int msvg_MSVFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, int L, float *ret_sc);
 
 /* ---- The ViterbiFilter stage: the filter after MSV in p7_Pipeline

@@ -176,16 +176,70 @@ def test_batched_multi_profile(ctx):
 
 
 def test_dropin_msvfilter(ctx, golden):
-    """msvg_MSVFilter, the p7_MSVFilter signature, on golden cases."""
+    """msvg_MSVFilter, the p7_MSVFilter signature, on the golden cases of
+    every golden profile (M = 1 .. 6000, amino and DNA): the one-launch
+    kernel where the profile's table fits LDS, the search path beyond."""
     z, man = golden
-    for pi in (0, 4, 13):
+    paths = set()
+    for pi, meta in enumerate(man["profiles"]):
         p = golden_profile(z, man, pi)
         pid = ctx.add_profile(_bp(p))
-        for c, _, x in golden_cases(z, pi)[:12]:
+        cases = golden_cases(z, pi)
+        print(f"profile {meta['name']} M={meta['M']} cases={len(cases)}", flush=True)
+        for c, _, x in cases[:40]:
             st, usc = ctx.msv_filter(pid, x)
-            assert st == z["case_status"][c]
-            assert np.float32(usc).view(np.uint32) == z["case_usc_bits"][c]
+            assert st == z["case_status"][c], (meta["name"], c)
+            assert np.float32(usc).view(np.uint32) == z["case_usc_bits"][c], (meta["name"], c)
+            paths.add((meta["M"] <= 2560, ctx.stats()["dropin_path"]))
         ctx.remove_profile(pid)
+    assert (True, 1) in paths and (False, 2) in paths and (False, 1) not in paths
+
+
+def test_dropin_paths_match_batched_search(ctx):
+    """Each msvg_MSVFilter call equals the batched search's score for the
+    same target: planted homologs (J fires, overflow), background targets
+    of 1..6000 residues; empty targets and the guard regime's sub-B0 case
+    go to the search path, codes outside the alphabet are an error."""
+    O = Oracle()
+    prof, db = _workload(31, 200, 500, 300, 0.7, 1, 6000, planted=0.1)
+    pid = ctx.add_profile(prof)
+    prof.mu = ctx.calibrate(pid, prof.lam)
+    did = ctx.add_database(db)
+    usc, st, _ = ctx.search([pid], did, mode=ca.MODE_BATCHED)
+    assert (st[0] == 16).any() and (st[0] == 0).any()
+    for i in range(db.n):
+        r, u = ctx.msv_filter(pid, db.seq(i))
+        assert ctx.stats()["dropin_path"] == 1, i
+        assert r == int(st[0, i]) and np.float32(u).view(np.uint32) == usc[0, i].view(np.uint32), i
+    r, u = ctx.msv_filter(pid, np.zeros(0, np.uint8))
+    ost, ousc = O.msv(prof.as_dict(), np.zeros(0, np.uint8))
+    assert ctx.stats()["dropin_path"] == 2 and r == ost and np.float32(u).view(np.uint32) == np.float32(ousc).view(np.uint32)
+    with pytest.raises(ca.MSVError):
+        ctx.msv_filter(pid, np.array([3, 29, 4], np.uint8))
+    ctx.remove_database(did)
+    ctx.remove_profile(pid)
+
+    # Guard regime (tjb + tbm + tec + bias >= 127): M = 2500, L = 100,000.
+    # A target of '*' (code 27, cost 255 at every node) keeps every cell at
+    # the floor: its exact maximum lies below B0 and only the search path
+    # has it.  The same target with 300 random residues inside, and a random
+    # target (which overflows), stay on the kernel.
+    prof = ca.sample_profile(11, 2500)
+    pid = ctx.add_profile(prof)
+    L = 100_000
+    assert ca.tjb(L) + prof.tbm + prof.tec + prof.bias >= 127
+    assert (prof.rbv[27] >= prof.bias).all()
+    rng = np.random.default_rng(5)
+    stars = np.full(L, 27, np.uint8)
+    island = stars.copy()
+    island[50_000:50_300] = rng.integers(0, 20, 300)
+    for x, path in ((stars, 2), (island, 1), (rng.integers(0, 20, L).astype(np.uint8), 1)):
+        print(f"guard regime: L={L} expect path {path}", flush=True)
+        r, u = ctx.msv_filter(pid, x)
+        ost, ousc = O.msv(prof.as_dict(), x)
+        assert ctx.stats()["dropin_path"] == path
+        assert r == ost and np.float32(u).view(np.uint32) == np.float32(ousc).view(np.uint32)
+    ctx.remove_profile(pid)
 
 
 def test_empty_and_zero_length(ctx):

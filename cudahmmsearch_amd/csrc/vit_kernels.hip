@@ -65,6 +65,18 @@ __device__ __forceinline__ uint32_t wave_max2(uint32_t v) {
 }
 
 
+// One step of the lane scan of maps x -> max(c, x + T) (floor -32768): the
+// source lane's map (c, T) from DPP control CTRL (rows outside ROW_MASK, and
+// lanes whose source lies outside their row, read the identity (-32768, 0))
+// composed before this lane's.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void scan_step(int &c, int &T) {
+  const int co = __builtin_amdgcn_update_dpp(-32768, c, CTRL, ROW_MASK, 0xf, false);
+  const int To = __builtin_amdgcn_update_dpp(0, T, CTRL, ROW_MASK, 0xf, false);
+  c = max(c, fT(co, T));
+  T = To + T;
+}
+
 }  // namespace
 
 template <int Q, int W>
@@ -179,13 +191,19 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
           const int clo = lo16(outl);
           // the lane's two slots as one map x -> max(c, x + T): low slot, then high
           int cs = max(hi16(outl), fT(clo, Thi)), Ts = Tlo + Thi;
-#pragma unroll
-          for (int d = 1; d < 64; d <<= 1) {                         // inclusive scan over lanes
-            const int co = __shfl_up(cs, d, 64), To = __shfl_up(Ts, d, 64);
-            if (lane >= d) { cs = max(cs, fT(co, Ts)); Ts = To + Ts; }
-          }
-          int ce = __shfl_up(cs, 1, 64), Te = __shfl_up(Ts, 1, 64);
-          if (lane == 0) { ce = -32768; Te = 0; }
+          // inclusive scan over the lanes (lane l: lanes 0..l composed, in
+          // order), in DPP: row_shr 1/2/4/8 inside each row of 16 lanes,
+          // then row_bcast 15 and 31 across rows; lanes without a source
+          // take the identity map (-32768, 0), which leaves them unchanged
+          scan_step<0x111, 0xf>(cs, Ts);
+          scan_step<0x112, 0xf>(cs, Ts);
+          scan_step<0x114, 0xf>(cs, Ts);
+          scan_step<0x118, 0xf>(cs, Ts);
+          scan_step<0x142, 0xa>(cs, Ts);
+          scan_step<0x143, 0xc>(cs, Ts);
+          // exclusive: the previous lane's prefix (wave_shr:1; lane 0 the identity)
+          const int ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x138, 0xf, 0xf, false);
+          const int Te = __builtin_amdgcn_update_dpp(0, Ts, 0x138, 0xf, 0xf, false);
           int carry = -32768;                                        // into this wave's first slot
           if constexpr (W > 1) {
             if (lane == 63) { scC[w] = cs; scT[w] = Ts; }

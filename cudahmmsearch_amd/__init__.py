@@ -109,6 +109,7 @@ def load_library(path=LIB_PATH):
         "msvg_last_stats": (C.c_int, [vp, C.POINTER(_Stats)]),
         "msvg_MSVFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
+        "msvg_set_tail_guard": (C.c_int, [vp, C.c_int]),
         "msvg_probe_lds": (C.c_int, [vp, C.POINTER(C.c_double)]),
         "msvg_tile_of": (C.c_int, [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "msvg_profile_set_viterbi": (C.c_int, [vp, i32, C.POINTER(_VProf)]),
@@ -602,6 +603,11 @@ class MSVContext:
 
     def clear_tiles(self):
         self._chk(self.lib.msvg_set_tile(self._c, 0, 0, 0, 0))
+
+    def set_tail_guard(self, on):
+        """Batched mode: widen the lane group where the longest target would
+        set the launch time (on by default)."""
+        self._chk(self.lib.msvg_set_tail_guard(self._c, 1 if on else 0))
 
 
 class Server:

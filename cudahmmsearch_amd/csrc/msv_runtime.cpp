@@ -172,6 +172,7 @@ struct msvg_ctx {
   std::map<std::pair<int, int>, std::unique_ptr<Tables>> tables;
   std::map<std::tuple<int, int, double>, std::pair<std::vector<uint16_t>, std::vector<float>>> finals;
   std::vector<Tile> tiles;
+  bool tail_guard = true;
   Bufs eager;                      // buffers of plain and eager searches
   Bufs *last = nullptr;            // buffers holding the latest results
   int64_t slow_cap = 0;            // slow-queue capacity for the next search
@@ -209,7 +210,7 @@ int hipfail(msvg_ctx *c, hipError_t e, const char *what) {
 int qfor(int M, int slots) { return (M + slots - 1) / slots; }
 
 // Lane-group size and waves per workgroup for a profile of length M.
-void choose_tile(const msvg_ctx *c, int M, int R, int &G, int &waves) {
+void choose_tile_tp(const msvg_ctx *c, int M, int R, int &G, int &waves) {
   // An installed tile applies only where its table fits this database's
   // LDS footprint; otherwise the cost model below decides.
   for (const Tile &t : c->tiles)
@@ -239,6 +240,44 @@ void choose_tile(const msvg_ctx *c, int M, int R, int &G, int &waves) {
 }
 
 size_t fast_lds(int R, int Q) { return (size_t) (R + 1) * ((Q + 3) / 4) * 1024; }
+
+// The tile for profile length M over database d.  choose_tile_tp picks for
+// throughput; a launch also lasts at least as long as its longest target's
+// row chain, which one lane group walks alone at the end.  Where that chain
+// outlasts the throughput estimate (small databases holding a long target),
+// each lane-group size is scored by max(chain, throughput time) and the
+// best is taken if it beats the throughput tile by 15%.  Models, fitted on
+// one MI355X: a row of a lane group costs v = Q + ceil(Q/2) packed
+// instructions; the chain takes 2.3 ns per instruction + 33 ns per row; the
+// device retires 430 wave-instructions per ns, and a target row costs
+// (v + 4) * G / 64 of them.
+void choose_tile(const msvg_ctx *c, int M, const Db &d, int &G, int &waves) {
+  choose_tile_tp(c, M, d.R, G, waves);
+  if (!c->tail_guard || d.len_sorted.empty()) return;
+  const double Lmax = (double) ((d.len_sorted.front() + 15) & ~15);
+  auto cost = [&](int g) {
+    const int Q = qfor(M, 2 * g);
+    const double v = Q + (Q + 1) / 2;
+    const double chain = Lmax * (2.3 * v + 33.0);
+    const double thru = (double) d.total_res * (v + 4.0) * g / 64.0 / 430.0;
+    return std::make_pair(chain, thru);
+  };
+  const auto c0 = cost(G);
+  if (c0.first <= c0.second) return;
+  double best = std::max(c0.first, c0.second);
+  int bestG = G;
+  for (int g : {8, 16, 32, 64}) {
+    const int Q = qfor(M, 2 * g);
+    if (g == G || Q > kQMax || fast_lds(d.R, Q) > (size_t) kLdsBytes) continue;
+    const auto cg = cost(g);
+    const double t = std::max(cg.first, cg.second);
+    if (t < best) { best = t; bestG = g; }
+  }
+  if (bestG != G && best < 0.85 * std::max(c0.first, c0.second)) {
+    G = bestG;
+    waves = 0;
+  }
+}
 
 // Workgroup size and grid for a persistent launch whose table uses lds bytes.
 void persistent_shape(const msvg_ctx *c, size_t lds, int waves_hint, int &grid, int &block) {
@@ -437,7 +476,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     Profile &p = *c->profiles[ids[i]];
     ProfDev &pd = pds[i];
     int G = 64, w = 0;
-    if (mode == MSVG_MODE_BATCHED) choose_tile(c, p.M, d.R, G, w);
+    if (mode == MSVG_MODE_BATCHED) choose_tile(c, p.M, d, G, w);
     const int Q = qfor(p.M, 2 * G);
     segmented[i] = (Q > kQMax || fast_lds(d.R, Q) > (size_t) kLdsBytes) ? 1 : 0;
     if (segmented[i]) G = 64;
@@ -975,6 +1014,14 @@ int msvg_last_stats(msvg_ctx *c, msvg_stats *out) {
   return MSVG_OK;
 }
 
+int msvg_set_tail_guard(msvg_ctx *c, int on) {
+  if (!c) return MSVG_EINVAL;
+  c->tail_guard = on != 0;
+  c->last = nullptr;
+  c->plans.clear();
+  return MSVG_OK;
+}
+
 int msvg_set_tile(msvg_ctx *c, int32_t lo, int32_t hi, int32_t lanes, int32_t waves) {
   if (!c) return MSVG_EINVAL;
   if (lanes == 0) { c->tiles.clear(); c->last = nullptr; c->plans.clear(); return MSVG_OK; }
@@ -1178,7 +1225,7 @@ int msvg_tile_of(msvg_ctx *c, int32_t M, int32_t dbid, int32_t *lanes, int32_t *
   auto it = c->dbs.find(dbid);
   if (it == c->dbs.end()) return fail(c, MSVG_EINVAL, "unknown database id");
   int G = 64, w = 0;
-  choose_tile(c, M, it->second->R, G, w);
+  choose_tile(c, M, *it->second, G, w);
   const int Q = qfor(M, 2 * G);
   const int seg = (Q > kQMax || fast_lds(it->second->R, Q) > (size_t) kLdsBytes) ? 1 : 0;
   *lanes = seg ? 64 : G;
@@ -1372,7 +1419,7 @@ std::string profile_key(const ServerQuery &q) {
 // LDS a query's fast-stage workgroups hold while it runs.
 size_t query_lds(msvg_ctx *c, const Db &d, int M) {
   int G = 64, w = 0;
-  choose_tile(c, M, d.R, G, w);
+  choose_tile(c, M, d, G, w);
   const int Q = qfor(M, 2 * G);
   if (Q > kQMax || fast_lds(d.R, Q) > (size_t) kLdsBytes) return fast_lds(d.R, qfor(std::min(M, 128 * seg_q(d.R)), 128));
   return fast_lds(d.R, Q);

@@ -184,6 +184,13 @@ void msvg_test_alloc_budget(long k);
  * per workgroup for profiles of length M; 0 restores the built-in choice.
  * The autotuner in the Python package persists its findings through this. */
 int msvg_set_tile(msvg_ctx *ctx, int32_t M_lo, int32_t M_hi, int32_t lanes, int32_t waves);
+/* Tail guard (on by default): where a database's longest target would make
+ * the tile's launch last longer than the whole database's work at the
+ * device's rate (small databases holding a long target), the batched mode
+ * takes the lane-group size that minimizes the larger of the two instead.
+ * 0 keeps the tile table's choice everywhere.  No counterpart in the
+ * reference. */
+int msvg_set_tail_guard(msvg_ctx *ctx, int on);
 /* The tile the batched mode uses for a profile of length M on a database:
  * lanes per target, waves per workgroup (0: derived from the LDS
  * footprint), and whether the model takes the column-segmented path. */

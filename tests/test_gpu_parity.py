@@ -13,7 +13,10 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def ctx():
+    # The workloads here are small: keep the tile table's lane groups (the
+    # tail guard would widen them) so that every group size is exercised.
     c = ca.MSVContext(0)
+    c.set_tail_guard(False)
     yield c
     c.close()
 
@@ -342,3 +345,29 @@ def test_j_stage_every_chunk_offset(ctx):
             ctx.remove_database(did)
             ctx.remove_profile(pid)
         assert (st[0] == 16).any() or M == 40
+
+
+def test_tail_guard(ctx):
+    """A small database holding one long target: the tail guard takes 64
+    lanes (the shortest row chain) where the tile table picks a narrower
+    group, and the results stay identical."""
+    prof, db = _workload(41, 200, 2000, 300, 0.6, 30, 5000, planted=0.01)
+    seqs = [db.seq(i) for i in range(db.n)] + [np.random.default_rng(9).integers(0, 20, 30000).astype(np.uint8)]
+    db = ca.Database.from_list(seqs)
+    pid = ctx.add_profile(prof)
+    prof.mu = ctx.calibrate(pid, prof.lam)
+    did = ctx.add_database(db)
+    try:
+        lanes_off = ctx.tile_of(200, did)[0]
+        off = ctx.search([pid], did, mode=ca.MODE_BATCHED)
+        ctx.set_tail_guard(True)
+        lanes_on = ctx.tile_of(200, did)[0]
+        on = ctx.search([pid], did, mode=ca.MODE_BATCHED)
+    finally:
+        ctx.set_tail_guard(False)
+    assert lanes_off < 64 and lanes_on == 64, (lanes_off, lanes_on)
+    assert np.array_equal(on[0].view(np.uint32), off[0].view(np.uint32))
+    assert np.array_equal(on[1], off[1]) and np.array_equal(on[2], off[2])
+    _check_vs_oracle(ctx, prof, db, ca.MODE_BATCHED, pid=pid, did=did, sample=300)
+    ctx.remove_database(did)
+    ctx.remove_profile(pid)

@@ -41,6 +41,7 @@ extern "C" hipError_t msvk_launch_seg(int Q, const SegArgs *a, int grid, int blo
 extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st);
 extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int grid, hipStream_t st);
+extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int grid, hipStream_t st);
 extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st);
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
                                             int Q, int bias, uint32_t *out, hipStream_t st);
@@ -105,6 +106,7 @@ struct VitProfile {               // ViterbiFilter fields (model order)
   int e_loop = 0, e_move = 0, base_w = 0, ddbound_w = 0;
   float scale_w = 0.f, nj = 0.f, vmu = 0.f, vlambda = 0.f;
   int Q = 1, W = 1;                // register blocks per lane, waves per target
+  int G = 64;                      // lanes per target: 16 or 32 (several targets per wave), or 64 * W
   DevMem d_ttab;                   // transition table, VitArgs layout
 };
 
@@ -755,16 +757,32 @@ namespace {
 
 // Waves per target and registers per lane: one wave up to M = 1024, then the
 // fewest waves (up to 16) that keep Q <= 8.
-void vit_shape(int M, int &Q, int &W) {
+// ViterbiFilter kernel shape for a model of length M: lanes per target G,
+// waves per target W (G = 64 only), registers per lane Q <= 8.  Short models
+// share a wave: 4 targets of 16 lanes or 2 of 32, so the fixed work of a
+// row (reductions, shifts, special states, the delete-closure scan) is
+// shared.  Cost per target row, in packed instructions, from the kernels'
+// instruction counts: (19 Q + F) / targets per wave, F = 70 (one target),
+// 78 (two), 94 (four).
+void vit_shape(int M, int &Q, int &W, int &G) {
   W = 1;
   while (128 * W * 8 < M) W *= 2;
   Q = (M + 128 * W - 1) / (128 * W);
+  G = 64;
+  if (W > 1) return;
+  double best = 19.0 * Q + 70.0;
+  const int q32 = (M + 63) / 64, q16 = (M + 31) / 32;
+  if (q32 <= 8 && (19.0 * q32 + 78.0) / 2 < best) { best = (19.0 * q32 + 78.0) / 2; G = 32; Q = q32; }
+  if (q16 <= 8 && (19.0 * q16 + 94.0) / 4 < best) { best = (19.0 * q16 + 94.0) / 4; G = 16; Q = q16; }
 }
 
-// Pack two int16 model-order values for lane c, register j (slots 2c, 2c+1).
+// Lanes per target of a shape.
+int vit_lanes(const VitProfile &v) { return v.G < 64 ? v.G : 64 * v.W; }
+
+// Pack two int16 model-order values for lane c, register j (slots 2c, 2c+1)
+// of a target striped over C lanes.
 template <class F>
-void vit_fill(std::vector<uint32_t> &out, size_t base, int Q, int W, int M, F value) {
-  const int C = 64 * W;
+void vit_fill(std::vector<uint32_t> &out, size_t base, int Q, int C, int M, F value) {
   for (int j = 0; j < Q; ++j)
     for (int c = 0; c < C; ++c) {
       const int k0 = j + (2 * c) * Q + 1, k1 = j + (2 * c + 1) * Q + 1;
@@ -779,11 +797,11 @@ int vit_emission_table(msvg_ctx *c, int pid, int dbid, const uint32_t **out) {
   Db &d = *c->dbs[dbid];
   Tables &t = tables_for(c, pid, dbid);
   if (!t.vemit) {
-    const size_t per = (size_t) v.Q * 64 * v.W;
+    const size_t per = (size_t) v.Q * vit_lanes(v);
     std::vector<uint32_t> h((size_t) (d.R + 1) * per);
     for (int r = 0; r <= d.R; ++r) {
       const int x = (r < d.R) ? d.rowcode[r] : -1;
-      vit_fill(h, (size_t) r * per, v.Q, v.W, p.M,
+      vit_fill(h, (size_t) r * per, v.Q, vit_lanes(v), p.M,
                [&](int k) { return x < 0 ? -32768 : (int) v.rwv[(size_t) x * p.M + k - 1]; });
     }
     auto mem = std::make_unique<DevMem>();
@@ -1083,11 +1101,11 @@ int msvg_profile_set_viterbi(msvg_ctx *c, int32_t pid, const msvg_vprofile *vp) 
   v->twv.assign(vp->twv, vp->twv + (size_t) 8 * p.M);
   v->e_loop = vp->e_loop; v->e_move = vp->e_move; v->base_w = vp->base_w; v->ddbound_w = vp->ddbound_w;
   v->scale_w = vp->scale_w; v->nj = vp->nj; v->vmu = vp->vmu; v->vlambda = vp->vlambda;
-  vit_shape(p.M, v->Q, v->W);
-  const size_t per = (size_t) v->Q * 64 * v->W;
+  vit_shape(p.M, v->Q, v->W, v->G);
+  const size_t per = (size_t) v->Q * vit_lanes(*v);
   std::vector<uint32_t> h(8 * per);
   for (int t = 0; t < 8; ++t)
-    vit_fill(h, t * per, v->Q, v->W, p.M, [&](int k) { return (int) v->twv[(size_t) t * p.M + k - 1]; });
+    vit_fill(h, t * per, v->Q, vit_lanes(*v), p.M, [&](int k) { return (int) v->twv[(size_t) t * p.M + k - 1]; });
   HIPCHK(c, v->d_ttab.ensure(h.size() * 4));
   HIPCHK(c, hipMemcpy(v->d_ttab.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
   p.vit = std::move(v);
@@ -1152,10 +1170,17 @@ int msvg_vit_search(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targe
   a.e_loop = v.e_loop; a.e_move = v.e_move; a.base_w = v.base_w; a.ddbound_w = v.ddbound_w;
   a.etab_in_lds = (v.W == 1 && (size_t) (d.R + 1) * v.Q * 64 * 4 <= 65536) ? 1 : 0;
   int grid;
-  if (v.W == 1) grid = (int) std::min<int64_t>((n + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
-  else grid = (int) std::min<int64_t>(n, (int64_t) c->ncu * std::max(1, 8 / v.W));
+  if (v.G < 64) {
+    const int64_t waves = (n + 64 / v.G - 1) / (64 / v.G);                  // targets per wave: 64 / G
+    grid = (int) std::min<int64_t>((waves + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
+  } else if (v.W == 1) {
+    grid = (int) std::min<int64_t>((n + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
+  } else {
+    grid = (int) std::min<int64_t>(n, (int64_t) c->ncu * std::max(1, 8 / v.W));
+  }
   HIPCHK(c, hipEventRecord(c->ev[0], st));
-  HIPCHK(c, msvk_launch_vit(v.Q, v.W, &a, grid, st));
+  if (v.G < 64) HIPCHK(c, msvk_launch_vit_group(v.G, v.Q, &a, grid, st));
+  else HIPCHK(c, msvk_launch_vit(v.Q, v.W, &a, grid, st));
   HIPCHK(c, hipEventRecord(c->ev[1], st));
   std::vector<int32_t> out((size_t) n);
   HIPCHK(c, hipMemcpyAsync(out.data(), c->d_vout.p, (size_t) n * sizeof(int32_t), hipMemcpyDeviceToHost, st));

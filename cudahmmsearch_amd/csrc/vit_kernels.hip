@@ -254,3 +254,253 @@ extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int grid, 
     default: return hipErrorInvalidValue;
   }
 }
+// ---------------------------------------------------------------------------
+// Short models (M <= 512): NG = 64/G targets per wave, G = 16 or 32 lanes
+// each (register j of lane c of a group holds k = j + s*Q + 1 for its slots
+// s = 2c, 2c+1), so the per-row fixed work -- the reductions, the shifts,
+// the special states, the delete-closure scan -- is shared by NG targets.
+// Cells are the same operations as vit_kernel in the same order.  The
+// special states of each group are wave-uniform values (one set per group);
+// a group whose target has ended (its rows are done, or it overflowed)
+// keeps computing padding rows but no longer updates its states, and its
+// closure is never applied.  The emission table (G lanes wide) is staged in
+// LDS; items are taken NG at a time by a static stride, longest first.
+// ---------------------------------------------------------------------------
+namespace {
+
+// Slot s <- slot s-1 inside each G-lane group; the group's first slot takes
+// minus infinity.
+template <int G>
+__device__ __forceinline__ uint32_t shift_grp(uint32_t last, int lane) {
+  uint32_t prev;
+  if constexpr (G == 16) {
+    prev = __builtin_amdgcn_update_dpp(NEG2, last, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+  } else {
+    prev = __builtin_amdgcn_update_dpp(NEG2, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    prev = ((lane & (G - 1)) == 0) ? NEG2 : prev;
+  }
+  return __builtin_amdgcn_alignbit(last, prev, 16);
+}
+
+// Packed int16 max over each G-lane group; the group's last lane holds it.
+template <int G>
+__device__ __forceinline__ uint32_t group_max2(uint32_t v) {
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x111 /* row_shr:1 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x112 /* row_shr:2 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x114 /* row_shr:4 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x118 /* row_shr:8 */, 0xf, 0xf, false));
+  if constexpr (G == 32) v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x142 /* row_bcast:15 */, 0xa, 0xf, false));
+  return v;
+}
+
+// Per-lane value of a per-group scalar: the lanes of group g take v[g].
+// ge[gg] is all ones in the lanes of groups >= gg and zero elsewhere, made
+// opaque to the compiler once per kernel, so each step is one v_bfi_b32 (a
+// select chain on the lane's group index would be turned into an indexed
+// array in scratch memory).
+template <int NG>
+__device__ __forceinline__ int of_group(const int (&v)[NG], const int (&ge)[NG]) {
+  int r = v[0];
+#pragma unroll
+  for (int gg = 1; gg < NG; ++gg) r = (ge[gg] & v[gg]) | (~ge[gg] & r);
+  return r;
+}
+
+}  // namespace
+
+template <int G, int Q>
+__global__ void __launch_bounds__(256) vit_group(VitArgs a) {
+  extern __shared__ uint32_t etab_lds[];
+  constexpr int NG = 64 / G;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int c = lane & (G - 1);                        // lane within its group
+  const int g = lane / G;                              // the wave's target this lane works on
+  int ge[NG];
+#pragma unroll
+  for (int gg = 0; gg < NG; ++gg) {
+    ge[gg] = (g >= gg) ? -1 : 0;
+    asm volatile("" : "+v"(ge[gg]));
+  }
+
+  uint32_t tBM[Q], tMM[Q], tIM[Q], tDM[Q], tMD[Q], tMI[Q], tII[Q], tDD[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    tBM[j] = a.ttab[(0 * Q + j) * G + c];
+    tMM[j] = a.ttab[(1 * Q + j) * G + c];
+    tIM[j] = a.ttab[(2 * Q + j) * G + c];
+    tDM[j] = a.ttab[(3 * Q + j) * G + c];
+    tMD[j] = a.ttab[(4 * Q + j) * G + c];
+    tMI[j] = a.ttab[(5 * Q + j) * G + c];
+    tII[j] = a.ttab[(6 * Q + j) * G + c];
+    tDD[j] = a.ttab[(7 * Q + j) * G + c];
+  }
+  int Tlo = 0, Thi = 0;                                // tDD summed over each slot
+#pragma unroll
+  for (int j = 0; j < Q; ++j) { Tlo += lo16(tDD[j]); Thi += hi16(tDD[j]); }
+
+  const int ntab = (a.db.R + 1) * Q * G;
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) etab_lds[i] = a.etab[i];
+  __syncthreads();
+
+  const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
+  const uint4 null4 = make_uint4(nb, nb, nb, nb);
+  const int xN = a.base_w;                             // vitfilter.c:116; N loop cost is 0
+  const int nw = blockDim.x >> 6;
+  const int nwi = (a.n + NG - 1) / NG;
+
+  for (int wi = blockIdx.x * nw + wave; wi < nwi; wi += gridDim.x * nw) {
+    const int it = wi * NG + g;
+    const bool valid = it < a.n;
+    const int64_t s = valid ? a.items[it] : 0;
+    const int L = valid ? a.db.len[s] : 0;
+    const int Lpad = (L + 15) & ~15;
+    const uint8_t *p = a.db.res + (valid ? a.db.off[s] : 0);
+    const int mvl = valid ? (int) a.move[it] : 0;
+
+    int Lg[NG], mv[NG], xB[NG], xJ[NG], xC[NG], st[NG];
+    int Lmax = 0;
+#pragma unroll
+    for (int gg = 0; gg < NG; ++gg) {
+      Lg[gg] = __builtin_amdgcn_readlane(L, gg * G);
+      mv[gg] = __builtin_amdgcn_readlane(mvl, gg * G);
+      xB[gg] = (int16_t) (xN + mv[gg]);                              // :117
+      xJ[gg] = -32768; xC[gg] = -32768; st[gg] = 0;
+      Lmax = max(Lmax, Lg[gg]);
+    }
+    uint32_t Mr[Q], Ir[Q], Dr[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) Mr[j] = Ir[j] = Dr[j] = NEG2;        // :115
+
+    bool live = true;
+    uint4 cnext = *reinterpret_cast<const uint4 *>(p);
+    for (int i0 = 0; i0 < Lmax && live; i0 += 16) {
+      const uint4 ch = (i0 < Lpad) ? cnext : null4;
+      const int i1 = i0 + 16;
+      cnext = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
+      uint32_t w0 = ch.x, w1 = ch.y, w2 = ch.z, w3 = ch.w;   // 16 residues, consumed from the bottom
+      const int nr = min(16, Lmax - i0);
+      for (int rr = 0; rr < nr; ++rr) {
+        const int i = i0 + rr;
+        const int x = (int) (w0 & 0xffu);
+        w0 = __builtin_amdgcn_alignbit(w1, w0, 8);
+        w1 = __builtin_amdgcn_alignbit(w2, w1, 8);
+        w2 = __builtin_amdgcn_alignbit(w3, w2, 8);
+        w3 >>= 8;
+        const uint32_t *ep = etab_lds + x * Q * G + c;
+        uint32_t e[Q];
+#pragma unroll
+        for (int j = 0; j < Q; ++j) e[j] = ep[j * G];
+        const int xb = of_group<NG>(xB, ge);
+        const uint32_t xBv = pack16(xb, xb);
+        uint32_t mpv = shift_grp<G>(Mr[Q - 1], lane);               // :139-141
+        uint32_t ipv = shift_grp<G>(Ir[Q - 1], lane);
+        uint32_t dpv = shift_grp<G>(Dr[Q - 1], lane);
+        uint32_t xEv = NEG2, Dmv = NEG2, dcv = NEG2;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+          uint32_t sv = adds(xBv, tBM[j]);                           // :146-150
+          sv = maxs(sv, adds(mpv, tMM[j]));
+          sv = maxs(sv, adds(ipv, tIM[j]));
+          sv = maxs(sv, adds(dpv, tDM[j]));
+          sv = adds(sv, e[j]);
+          xEv = maxs(xEv, sv);                                       // :151
+          mpv = Mr[j]; dpv = Dr[j]; ipv = Ir[j];                     // :156-158
+          Mr[j] = sv;                                                // :161
+          Dr[j] = dcv;                                               // :162
+          dcv = adds(sv, tMD[j]);                                    // :167
+          Dmv = maxs(Dmv, dcv);                                      // :168
+          Ir[j] = maxs(adds(mpv, tMI[j]), adds(ipv, tII[j]));        // :171-172
+        }
+        const uint32_t r = group_max2<G>(pack16(max(lo16(xEv), hi16(xEv)), max(lo16(Dmv), hi16(Dmv))));
+        int cond[NG];
+        bool any = false, anylive = false;
+#pragma unroll
+        for (int gg = 0; gg < NG; ++gg) {
+          const uint32_t rg = __builtin_amdgcn_readlane(r, gg * G + G - 1);
+          const int xE = lo16(rg), Dmax = hi16(rg);                  // :176, :198
+          cond[gg] = 0;
+          if (i < Lg[gg] && st[gg] == 0) {
+            if (xE >= 32767) {                                       // :177
+              st[gg] = 16;
+            } else {
+              xC[gg] = (int16_t) max(xC[gg], xE + a.e_move);         // :179 (C loop 0)
+              xJ[gg] = (int16_t) max(xJ[gg], xE + a.e_loop);         // :180 (J loop 0)
+              xB[gg] = (int16_t) max(xJ[gg] + mv[gg], xN + mv[gg]);  // :181
+              cond[gg] = (Dmax + a.ddbound_w > xB[gg]) ? 1 : 0;      // :199
+              anylive = anylive || (i + 1 < Lg[gg]);
+            }
+          }
+          any = any || cond[gg];
+        }
+        live = anylive;
+        if (!live) break;
+
+        Dr[0] = shift_grp<G>(dcv, lane);                             // :230-231 / :203-204
+        if (any) {
+          const bool mine = of_group<NG>(cond, ge) != 0;
+          // closure inside each slot, for the groups whose reference row runs it
+          if (mine) {
+#pragma unroll
+            for (int j = 1; j < Q; ++j) Dr[j] = maxs(Dr[j], adds(Dr[j - 1], tDD[j - 1]));
+          }
+          const uint32_t outl = adds(Dr[Q - 1], tDD[Q - 1]);
+          const int clo = lo16(outl);
+          int cs = max(hi16(outl), fT(clo, Thi)), Ts = Tlo + Thi;
+          // inclusive scan inside each group (rows of 16 lanes; for 32-lane
+          // groups the second row also composes the first row's total)
+          scan_step<0x111, 0xf>(cs, Ts);
+          scan_step<0x112, 0xf>(cs, Ts);
+          scan_step<0x114, 0xf>(cs, Ts);
+          scan_step<0x118, 0xf>(cs, Ts);
+          if constexpr (G == 32) scan_step<0x142, 0xa>(cs, Ts);
+          // exclusive: the previous lane's prefix; a group's first lane the identity
+          int ce, Te;
+          if constexpr (G == 16) {
+            ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x111, 0xf, 0xf, false);
+            Te = __builtin_amdgcn_update_dpp(0, Ts, 0x111, 0xf, 0xf, false);
+          } else {
+            ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x138, 0xf, 0xf, false);
+            Te = __builtin_amdgcn_update_dpp(0, Ts, 0x138, 0xf, 0xf, false);
+            if (c == 0) { ce = -32768; Te = 0; }
+          }
+          (void) Te;
+          const int cin_lo = ce;                                     // nothing enters a group's first slot
+          const int cin_hi = max(clo, fT(cin_lo, Tlo));
+          if (mine) {
+            uint32_t y = pack16(cin_lo, cin_hi);
+#pragma unroll
+            for (int j = 0; j < Q; ++j) { Dr[j] = maxs(Dr[j], y); y = adds(y, tDD[j]); }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int gg = 0; gg < NG; ++gg)
+      if (lane == gg * G && valid) a.out[it] = (st[gg] << 16) | (int) (uint16_t) xC[gg];
+  }
+}
+
+template <int G, int Q>
+static hipError_t launch_group(const VitArgs *a, int grid, hipStream_t st) {
+  const size_t lds = (size_t) (a->db.R + 1) * Q * G * 4;
+  hipLaunchKernelGGL((vit_group<G, Q>), dim3(grid), dim3(256), lds, st, *a);
+  return hipGetLastError();
+}
+
+template <int G, int... Qs>
+static hipError_t dispatch_group(int Q, const VitArgs *a, int grid, hipStream_t st, std::integer_sequence<int, Qs...>) {
+  hipError_t e = hipErrorInvalidValue;
+  ((Q == Qs + 1 ? (e = launch_group<G, Qs + 1>(a, grid, st), 0) : 0), ...);
+  return e;
+}
+
+extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int grid, hipStream_t st) {
+  if (Q < 1 || Q > 8) return hipErrorInvalidValue;
+  auto qs = std::make_integer_sequence<int, 8>{};
+  switch (G) {
+    case 16: return dispatch_group<16>(Q, a, grid, st, qs);
+    case 32: return dispatch_group<32>(Q, a, grid, st, qs);
+    default: return hipErrorInvalidValue;
+  }
+}

@@ -114,6 +114,7 @@ def load_library(path=LIB_PATH):
         "msvg_tile_of": (C.c_int, [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "msvg_profile_set_viterbi": (C.c_int, [vp, i32, C.POINTER(_VProf)]),
         "msvg_vit_search": (C.c_int, [vp, i32, i32, vp, i64, vp, vp]),
+        "msvg_vit_search_many": (C.c_int, [vp, i32, vp, i32, vp, vp, vp, vp]),
         "msvg_ViterbiFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_calibrate_vmu": (C.c_int, [vp, i32, C.c_double, C.POINTER(C.c_float)]),
         "msvh_vit_move": (C.c_int, [C.c_float, C.c_float, C.c_int]),
@@ -571,6 +572,21 @@ class MSVContext:
         st = np.zeros(t.size, np.uint8)
         self._chk(self.lib.msvg_vit_search(self._c, pid, did, _p(t), t.size, _p(sc), _p(st)))
         return sc, st
+
+    def vit_search_many(self, pids, did, target_lists):
+        """vit_search for several profiles in one call (launches overlap on
+        the context's streams): a list of (sc, status), one per profile."""
+        pids = np.ascontiguousarray(pids, dtype=np.int32)
+        lists = [np.ascontiguousarray(t, dtype=np.int64) for t in target_lists]
+        if len(lists) != pids.size:
+            raise ValueError("one target list per profile")
+        offs = np.zeros(pids.size + 1, np.int64)
+        offs[1:] = np.cumsum([t.size for t in lists], dtype=np.int64)
+        t = np.concatenate(lists) if lists else np.zeros(0, np.int64)
+        sc = np.zeros(max(t.size, 1), np.float32)
+        st = np.zeros(max(t.size, 1), np.uint8)
+        self._chk(self.lib.msvg_vit_search_many(self._c, pids.size, _p(pids), did, _p(offs), _p(t), _p(sc), _p(st)))
+        return [(sc[offs[i]:offs[i + 1]].copy(), st[offs[i]:offs[i + 1]].copy()) for i in range(pids.size)]
 
     def vit_filter(self, pid, x):
         """p7_ViterbiFilter on one target (codes, no sentinels): (status, sc)."""

@@ -57,6 +57,7 @@ namespace {
 
 constexpr int kLdsBytes = 160 * 1024;
 constexpr int kQMax = 32;           // registers per lane in the fast kernels
+constexpr int kVitStreams = 4;      // streams of a multi-profile ViterbiFilter search (the default hardware queues)
 constexpr int kSegQMax = 28;        // registers per lane in one model segment (at most 3584 positions)
 
 // Segment width: the most registers whose table fits in LDS for R rows.
@@ -190,6 +191,8 @@ struct msvg_ctx {
   uint32_t *h_one_out = nullptr;            // pinned result word
   // ViterbiFilter searches
   DevMem d_vitems, d_vmove, d_vout;
+  hipStream_t vstream[kVitStreams] = {};   // multi-profile searches: launches round-robin
+  hipEvent_t vdone[kVitStreams] = {};
 };
 
 namespace {
@@ -854,6 +857,10 @@ void msvg_free(msvg_ctx *c) {
   if (c->fork) (void) hipEventDestroy(c->fork);
   if (c->join) (void) hipEventDestroy(c->join);
   if (c->side) { (void) hipStreamSynchronize(c->side); (void) hipStreamDestroy(c->side); }
+  for (int k = 0; k < kVitStreams; ++k) {
+    if (c->vstream[k]) { (void) hipStreamSynchronize(c->vstream[k]); (void) hipStreamDestroy(c->vstream[k]); }
+    if (c->vdone[k]) (void) hipEventDestroy(c->vdone[k]);
+  }
   if (c->stream) (void) hipStreamDestroy(c->stream);
   if (c->h_one_res) (void) hipHostFree(c->h_one_res);
   if (c->h_one_out) (void) hipHostFree(c->h_one_out);
@@ -1114,10 +1121,26 @@ int msvg_profile_set_viterbi(msvg_ctx *c, int32_t pid, const msvg_vprofile *vp) 
   return MSVG_OK;
 }
 
-int msvg_vit_search(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, int64_t n, float *sc,
-                    uint8_t *status) {
-  if (!c) return MSVG_EINVAL;
-  if (n < 0 || (n > 0 && (!targets || !sc || !status))) return fail(c, MSVG_EINVAL, "null argument");
+}  // extern "C"
+
+namespace {
+
+// One profile's share of a ViterbiFilter search: its items (longest target
+// first), their N/C/J move costs, where they sit in the device item arrays,
+// and the launch.
+struct VitJob {
+  int pid = 0;
+  int64_t n = 0, base = 0;           // items [base, base + n) of the device arrays
+  std::vector<int64_t> order;        // item q -> caller position
+  std::vector<int64_t> items;        // item q -> sorted target index
+  std::vector<int16_t> mv;
+  int64_t cells = 0;
+  VitArgs a{};
+  int grid = 0;
+};
+
+int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, int64_t n, VitJob &job) {
+  if (n < 0 || (n > 0 && !targets)) return fail(c, MSVG_EINVAL, "null argument");
   auto pit = c->profiles.find(pid);
   if (pit == c->profiles.end()) return fail(c, MSVG_EINVAL, "unknown profile id");
   Profile &p = *pit->second;
@@ -1128,80 +1151,165 @@ int msvg_vit_search(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targe
   if (d.maxcode >= p.Kp) return fail(c, MSVG_EINVAL, "database holds residue codes outside the profile's alphabet");
   for (int64_t i = 0; i < n; ++i)
     if (targets[i] < 0 || targets[i] >= d.n) return fail(c, MSVG_EINVAL, "target index out of range");
-  c->stats = msvg_stats{};
+  job.pid = pid;
+  job.n = n;
   if (n == 0) return MSVG_OK;
-  (void) hipSetDevice(c->device);
   VitProfile &v = *p.vit;
   const uint32_t *etab;
   int rc = vit_emission_table(c, pid, dbid, &etab);
   if (rc != MSVG_OK) return rc;
-
   // Longest targets first, so the last waves to start are the shortest.
-  std::vector<int64_t> order((size_t) n);
-  for (int64_t i = 0; i < n; ++i) order[(size_t) i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+  job.order.resize((size_t) n);
+  for (int64_t i = 0; i < n; ++i) job.order[(size_t) i] = i;
+  std::stable_sort(job.order.begin(), job.order.end(), [&](int64_t a, int64_t b) {
     return d.len_sorted[d.sorted_of[targets[a]]] > d.len_sorted[d.sorted_of[targets[b]]];
   });
-  std::vector<int64_t> items((size_t) n);
-  std::vector<int16_t> mv((size_t) n);
-  int64_t cells = 0;
+  job.items.resize((size_t) n);
+  job.mv.resize((size_t) n);
   for (int64_t q = 0; q < n; ++q) {
-    const int64_t s = d.sorted_of[targets[order[(size_t) q]]];
-    items[(size_t) q] = s;
-    mv[(size_t) q] = (int16_t) msvh_vit_move(v.scale_w, v.nj, d.len_sorted[s]);
-    cells += (int64_t) p.M * d.len_sorted[s];
+    const int64_t s = d.sorted_of[targets[job.order[(size_t) q]]];
+    job.items[(size_t) q] = s;
+    job.mv[(size_t) q] = (int16_t) msvh_vit_move(v.scale_w, v.nj, d.len_sorted[s]);
+    job.cells += (int64_t) p.M * d.len_sorted[s];
   }
-  HIPCHK(c, c->d_vitems.ensure((size_t) n * sizeof(int64_t)));
-  HIPCHK(c, c->d_vmove.ensure((size_t) n * sizeof(int16_t)));
-  HIPCHK(c, c->d_vout.ensure((size_t) n * sizeof(int32_t)));
-  hipStream_t st = c->stream;
-  HIPCHK(c, hipMemcpyAsync(c->d_vitems.p, items.data(), (size_t) n * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(c->d_vmove.p, mv.data(), (size_t) n * sizeof(int16_t), hipMemcpyHostToDevice, st));
-
-  VitArgs a;
+  VitArgs &a = job.a;
   a.etab = etab;
   a.ttab = v.d_ttab.as<uint32_t>();
   a.db.res = d.d_res.as<uint8_t>(); a.db.off = d.d_off.as<int64_t>(); a.db.len = d.d_len.as<int32_t>();
   a.db.tjb = d.d_tjb.as<uint8_t>(); a.db.n = d.n; a.db.R = d.R;
-  a.items = c->d_vitems.as<int64_t>();
-  a.move = c->d_vmove.as<int16_t>();
   a.n = (int32_t) n;
-  a.out = c->d_vout.as<int32_t>();
   a.e_loop = v.e_loop; a.e_move = v.e_move; a.base_w = v.base_w; a.ddbound_w = v.ddbound_w;
   a.etab_in_lds = (v.W == 1 && (size_t) (d.R + 1) * v.Q * 64 * 4 <= 65536) ? 1 : 0;
-  int grid;
   if (v.G < 64) {
     const int64_t waves = (n + 64 / v.G - 1) / (64 / v.G);                  // targets per wave: 64 / G
-    grid = (int) std::min<int64_t>((waves + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
+    job.grid = (int) std::min<int64_t>((waves + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
   } else if (v.W == 1) {
-    grid = (int) std::min<int64_t>((n + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
+    job.grid = (int) std::min<int64_t>((n + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
   } else {
-    grid = (int) std::min<int64_t>(n, (int64_t) c->ncu * std::max(1, 8 / v.W));
+    job.grid = (int) std::min<int64_t>(n, (int64_t) c->ncu * std::max(1, 8 / v.W));
   }
-  HIPCHK(c, hipEventRecord(c->ev[0], st));
-  if (v.G < 64) HIPCHK(c, msvk_launch_vit_group(v.G, v.Q, &a, grid, st));
-  else HIPCHK(c, msvk_launch_vit(v.Q, v.W, &a, grid, st));
-  HIPCHK(c, hipEventRecord(c->ev[1], st));
-  std::vector<int32_t> out((size_t) n);
-  HIPCHK(c, hipMemcpyAsync(out.data(), c->d_vout.p, (size_t) n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  HIPCHK(c, hipStreamSynchronize(st));
-  float ms = 0.f;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-  for (int64_t q = 0; q < n; ++q) {
-    const int64_t i = order[(size_t) q];
-    const int32_t r = out[(size_t) q];
+  return MSVG_OK;
+}
+
+int vit_launch(msvg_ctx *c, VitJob &job, hipStream_t st) {
+  if (job.n == 0) return MSVG_OK;
+  const VitProfile &v = *c->profiles[job.pid]->vit;
+  job.a.items = c->d_vitems.as<int64_t>() + job.base;
+  job.a.move = c->d_vmove.as<int16_t>() + job.base;
+  job.a.out = c->d_vout.as<int32_t>() + job.base;
+  if (v.G < 64) HIPCHK(c, msvk_launch_vit_group(v.G, v.Q, &job.a, job.grid, st));
+  else HIPCHK(c, msvk_launch_vit(v.Q, v.W, &job.a, job.grid, st));
+  return MSVG_OK;
+}
+
+// Device item arrays for all jobs, uploaded on st.
+int vit_upload(msvg_ctx *c, std::vector<VitJob> &jobs, hipStream_t st, std::vector<int64_t> &hitems,
+               std::vector<int16_t> &hmv) {
+  int64_t tot = 0;
+  for (VitJob &j : jobs) { j.base = tot; tot += j.n; }
+  HIPCHK(c, c->d_vitems.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int64_t)));
+  HIPCHK(c, c->d_vmove.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int16_t)));
+  HIPCHK(c, c->d_vout.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int32_t)));
+  hitems.resize((size_t) tot);
+  hmv.resize((size_t) tot);
+  for (const VitJob &j : jobs) {
+    std::copy(j.items.begin(), j.items.end(), hitems.begin() + j.base);
+    std::copy(j.mv.begin(), j.mv.end(), hmv.begin() + j.base);
+  }
+  if (tot == 0) return MSVG_OK;
+  HIPCHK(c, hipMemcpyAsync(c->d_vitems.p, hitems.data(), (size_t) tot * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_vmove.p, hmv.data(), (size_t) tot * sizeof(int16_t), hipMemcpyHostToDevice, st));
+  return MSVG_OK;
+}
+
+// Scores of job j from the raw results (host expression of vitfilter.c:240-248).
+void vit_finish(msvg_ctx *c, const VitJob &j, const int32_t *out, float *sc, uint8_t *status) {
+  const VitProfile &v = *c->profiles[j.pid]->vit;
+  for (int64_t q = 0; q < j.n; ++q) {
+    const int64_t i = j.order[(size_t) q];
+    const int32_t r = out[j.base + q];
     if ((r >> 16) == MSVG_ERANGE) {
       status[i] = MSVG_ERANGE;
       sc[i] = INFINITY;                                        // vitfilter.c:177
     } else {
       status[i] = MSVG_OK;
-      sc[i] = msvh_vit_score((int16_t) (r & 0xffff), mv[(size_t) q], v.base_w, v.scale_w);
+      sc[i] = msvh_vit_score((int16_t) (r & 0xffff), j.mv[(size_t) q], v.base_w, v.scale_w);
     }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int msvg_vit_search(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, int64_t n, float *sc,
+                    uint8_t *status) {
+  if (!c) return MSVG_EINVAL;
+  if (n > 0 && (!sc || !status)) return fail(c, MSVG_EINVAL, "null argument");
+  const int64_t offs[2] = {0, n};
+  return msvg_vit_search_many(c, 1, &pid, dbid, offs, targets, sc, status);
+}
+
+int msvg_vit_search_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t dbid, const int64_t *offsets,
+                         const int64_t *targets, float *sc, uint8_t *status) {
+  if (!c) return MSVG_EINVAL;
+  if (np < 0 || (np > 0 && (!pids || !offsets))) return fail(c, MSVG_EINVAL, "null argument");
+  for (int32_t i = 0; i < np; ++i)
+    if (offsets[i + 1] < offsets[i] || offsets[i] < 0) return fail(c, MSVG_EINVAL, "offsets must not decrease");
+  const int64_t total = np > 0 ? offsets[np] - offsets[0] : 0;
+  if (total > 0 && (!targets || !sc || !status)) return fail(c, MSVG_EINVAL, "null argument");
+  c->stats = msvg_stats{};
+  (void) hipSetDevice(c->device);
+  std::vector<VitJob> jobs((size_t) np);
+  for (int32_t i = 0; i < np; ++i) {
+    const int rc = vit_prepare(c, pids[i], dbid, targets + offsets[i], offsets[i + 1] - offsets[i], jobs[(size_t) i]);
+    if (rc != MSVG_OK) return rc;
+  }
+  if (total == 0) return MSVG_OK;
+  hipStream_t st = c->stream;
+  std::vector<int64_t> hitems;
+  std::vector<int16_t> hmv;
+  int rc = vit_upload(c, jobs, st, hitems, hmv);
+  if (rc != MSVG_OK) return rc;
+  // Each launch lasts at least as long as its longest target's row chain
+  // (one wave, or one workgroup, per target): profiles go round-robin over
+  // the context's ViterbiFilter streams so that those tails overlap.
+  const int nstreams = np > 1 ? kVitStreams : 1;
+  if (nstreams > 1 && !c->vstream[0]) {
+    for (int k = 0; k < kVitStreams; ++k) {
+      HIPCHK(c, hipStreamCreateWithFlags(&c->vstream[k], hipStreamNonBlocking));
+      HIPCHK(c, hipEventCreateWithFlags(&c->vdone[k], hipEventDisableTiming));
+    }
+  }
+  HIPCHK(c, hipEventRecord(c->ev[0], st));
+  if (nstreams > 1) {
+    HIPCHK(c, hipEventRecord(c->fork, st));
+    for (int k = 0; k < nstreams; ++k) HIPCHK(c, hipStreamWaitEvent(c->vstream[k], c->fork, 0));
+    for (int32_t i = 0; i < np; ++i)
+      if ((rc = vit_launch(c, jobs[(size_t) i], c->vstream[i % nstreams])) != MSVG_OK) return rc;
+    for (int k = 0; k < nstreams; ++k) {
+      HIPCHK(c, hipEventRecord(c->vdone[k], c->vstream[k]));
+      HIPCHK(c, hipStreamWaitEvent(st, c->vdone[k], 0));
+    }
+  } else {
+    for (int32_t i = 0; i < np; ++i)
+      if ((rc = vit_launch(c, jobs[(size_t) i], st)) != MSVG_OK) return rc;
+  }
+  HIPCHK(c, hipEventRecord(c->ev[1], st));
+  std::vector<int32_t> out((size_t) total);
+  HIPCHK(c, hipMemcpyAsync(out.data(), c->d_vout.p, (size_t) total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  int64_t cells = 0;
+  for (int32_t i = 0; i < np; ++i) {
+    vit_finish(c, jobs[(size_t) i], out.data(), sc + offsets[i], status + offsets[i]);
+    cells += jobs[(size_t) i].cells;
   }
   c->stats.ms_fast = ms;
   c->stats.ms_total = ms;
   c->stats.cells = cells;
-  c->stats.n_fast_launches = 1;
+  c->stats.n_fast_launches = np;
   return MSVG_OK;
 }
 

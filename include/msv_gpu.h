@@ -159,6 +159,17 @@ int msvg_profile_set_viterbi(msvg_ctx *ctx, int32_t profile_id, const msvg_vprof
 int msvg_vit_search(msvg_ctx *ctx, int32_t profile_id, int32_t db_id, const int64_t *targets, int64_t n,
                     float *sc, uint8_t *status);
 
+/* msvg_vit_search for np profiles at once over one database, profile i on
+ * targets[offsets[i] .. offsets[i+1]) with its results at the same
+ * positions of sc and status (p7_Pipeline's second filter for the MSV
+ * survivors of a multi-profile search).  Each profile's launch lasts at
+ * least as long as its longest target's row chain, so the launches go
+ * round-robin over four streams of the context and those tails overlap;
+ * results equal np separate msvg_vit_search calls.  ms_fast is the whole
+ * stage.  No counterpart in the reference. */
+int msvg_vit_search_many(msvg_ctx *ctx, int32_t np, const int32_t *profile_ids, int32_t db_id,
+                         const int64_t *offsets, const int64_t *targets, float *sc, uint8_t *status);
+
 /* Drop-in for p7_ViterbiFilter (impl_sse.h:343, vitfilter.c:84): dsq 1-based
  * with sentinels.  Returns MSVG_OK or MSVG_ERANGE. */
 int msvg_ViterbiFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, int L, float *ret_sc);

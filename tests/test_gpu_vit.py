@@ -82,6 +82,31 @@ def test_every_wave_count_vs_reference(M):
     ctx.close()
 
 
+def test_many_profiles_equal_separate_searches():
+    """msvg_vit_search_many over profiles of every kernel shape (16- and
+    32-lane groups, one, two and four waves per target), one of them with an
+    empty list, equals one msvg_vit_search per profile, bit for bit."""
+    Ms = [60, 300, 700, 1500, 2300, 120]
+    ctx = ca.MSVContext(0, tiles=None)
+    rng = np.random.default_rng(17)
+    seqs = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in rng.integers(1, 2500, 300)]
+    db = ca.Database.from_list(seqs)
+    did = ctx.add_database(db)
+    pids, lists = [], []
+    for k, M in enumerate(Ms):
+        prof = ca.sample_profile(60 + k, M)
+        pid = ctx.add_profile(prof)
+        ctx.set_viterbi(pid, prof.viterbi())
+        pids.append(pid)
+        lists.append(np.zeros(0, np.int64) if k == 5 else rng.choice(db.n, 120 + 10 * k, replace=False))
+    print(f"profiles M={Ms} targets={db.n}", flush=True)
+    many = ctx.vit_search_many(pids, did, lists)
+    for pid, t, (sc, st) in zip(pids, lists, many):
+        sc1, st1 = ctx.vit_search(pid, did, t)
+        assert _same(sc, st, sc1, st1)
+    ctx.close()
+
+
 def test_subset_order_and_repeats(vit_golden):
     """Any target list: a subset in scrambled order with repeats gives each
     target its own score."""

@@ -40,8 +40,8 @@ extern "C" hipError_t msvk_launch_fast(int G, int Q, const FastArgs *a, int grid
 extern "C" hipError_t msvk_launch_seg(int Q, const SegArgs *a, int grid, int block, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st);
 extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st);
-extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int grid, hipStream_t st);
-extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int grid, hipStream_t st);
+extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, hipStream_t st);
+extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int ncu, hipStream_t st);
 extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st);
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
                                             int Q, int bias, uint32_t *out, hipStream_t st);
@@ -1136,7 +1136,6 @@ struct VitJob {
   std::vector<int16_t> mv;
   int64_t cells = 0;
   VitArgs a{};
-  int grid = 0;
 };
 
 int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, int64_t n, VitJob &job) {
@@ -1180,14 +1179,6 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   a.n = (int32_t) n;
   a.e_loop = v.e_loop; a.e_move = v.e_move; a.base_w = v.base_w; a.ddbound_w = v.ddbound_w;
   a.etab_in_lds = (v.W == 1 && (size_t) (d.R + 1) * v.Q * 64 * 4 <= 65536) ? 1 : 0;
-  if (v.G < 64) {
-    const int64_t waves = (n + 64 / v.G - 1) / (64 / v.G);                  // targets per wave: 64 / G
-    job.grid = (int) std::min<int64_t>((waves + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
-  } else if (v.W == 1) {
-    job.grid = (int) std::min<int64_t>((n + 3) / 4, (int64_t) c->ncu * 4);   // 4 waves per block
-  } else {
-    job.grid = (int) std::min<int64_t>(n, (int64_t) c->ncu * std::max(1, 8 / v.W));
-  }
   return MSVG_OK;
 }
 
@@ -1197,8 +1188,8 @@ int vit_launch(msvg_ctx *c, VitJob &job, hipStream_t st) {
   job.a.items = c->d_vitems.as<int64_t>() + job.base;
   job.a.move = c->d_vmove.as<int16_t>() + job.base;
   job.a.out = c->d_vout.as<int32_t>() + job.base;
-  if (v.G < 64) HIPCHK(c, msvk_launch_vit_group(v.G, v.Q, &job.a, job.grid, st));
-  else HIPCHK(c, msvk_launch_vit(v.Q, v.W, &job.a, job.grid, st));
+  if (v.G < 64) HIPCHK(c, msvk_launch_vit_group(v.G, v.Q, &job.a, c->ncu, st));
+  else HIPCHK(c, msvk_launch_vit(v.Q, v.W, &job.a, c->ncu, st));
   return MSVG_OK;
 }
 

@@ -20,6 +20,7 @@
 //     max(-32768, x + t1 + t2)), so the scan carries unsaturated int32 sums.
 // The score's float expression (:240-246) is evaluated on the host from xC.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 #include <utility>
 #include "msv_device.h"
@@ -227,30 +228,49 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
   }
 }
 
+// Workgroups of a kernel that fit one CU at once (at least 1).
+template <class K>
+static int resident_per_cu(K kernel, int block, size_t lds) {
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, lds) != hipSuccess || occ < 1) occ = 1;
+  return occ;
+}
+
+// Grids: one wave per target (or per NG targets) with a static stride, as
+// many workgroups as are needed or as fit the device at once, whichever is
+// fewer; multi-wave targets keep one workgroup per target up to 8 waves per
+// CU.
 template <int Q, int W>
-static hipError_t launch_vit(const VitArgs *a, int grid, hipStream_t st) {
+static hipError_t launch_vit(const VitArgs *a, int ncu, hipStream_t st) {
   const int block = (W == 1) ? 256 : 64 * W;
   const size_t lds = (W == 1 && a->etab_in_lds) ? (size_t) (a->db.R + 1) * Q * 64 * 4 : 0;
+  int grid;
+  if (W == 1) {
+    const int cap = ncu * resident_per_cu(vit_kernel<Q, W>, block, lds);
+    grid = std::max(1, std::min((a->n + 3) / 4, cap));
+  } else {
+    grid = std::max(1, std::min(a->n, ncu * std::max(1, 8 / W)));
+  }
   hipLaunchKernelGGL((vit_kernel<Q, W>), dim3(grid), dim3(block), lds, st, *a);
   return hipGetLastError();
 }
 
 template <int W, int... Qs>
-static hipError_t dispatch_q(int Q, const VitArgs *a, int grid, hipStream_t st, std::integer_sequence<int, Qs...>) {
+static hipError_t dispatch_q(int Q, const VitArgs *a, int ncu, hipStream_t st, std::integer_sequence<int, Qs...>) {
   hipError_t e = hipErrorInvalidValue;
-  ((Q == Qs + 1 ? (e = launch_vit<Qs + 1, W>(a, grid, st), 0) : 0), ...);
+  ((Q == Qs + 1 ? (e = launch_vit<Qs + 1, W>(a, ncu, st), 0) : 0), ...);
   return e;
 }
 
-extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int grid, hipStream_t st) {
-  if (Q < 1 || Q > 8) return hipErrorInvalidValue;
+extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, hipStream_t st) {
+  if (Q < 1 || Q > 8 || a->n < 1) return hipErrorInvalidValue;
   auto qs = std::make_integer_sequence<int, 8>{};
   switch (W) {
-    case 1: return dispatch_q<1>(Q, a, grid, st, qs);
-    case 2: return dispatch_q<2>(Q, a, grid, st, qs);
-    case 4: return dispatch_q<4>(Q, a, grid, st, qs);
-    case 8: return dispatch_q<8>(Q, a, grid, st, qs);
-    case 16: return dispatch_q<16>(Q, a, grid, st, qs);
+    case 1: return dispatch_q<1>(Q, a, ncu, st, qs);
+    case 2: return dispatch_q<2>(Q, a, ncu, st, qs);
+    case 4: return dispatch_q<4>(Q, a, ncu, st, qs);
+    case 8: return dispatch_q<8>(Q, a, ncu, st, qs);
+    case 16: return dispatch_q<16>(Q, a, ncu, st, qs);
     default: return hipErrorInvalidValue;
   }
 }
@@ -482,25 +502,29 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
 }
 
 template <int G, int Q>
-static hipError_t launch_group(const VitArgs *a, int grid, hipStream_t st) {
+static hipError_t launch_group(const VitArgs *a, int ncu, hipStream_t st) {
+  constexpr int NG = 64 / G;
   const size_t lds = (size_t) (a->db.R + 1) * Q * G * 4;
+  const int waves = (a->n + NG - 1) / NG;
+  const int cap = ncu * resident_per_cu(vit_group<G, Q>, 256, lds);
+  const int grid = std::max(1, std::min((waves + 3) / 4, cap));
   hipLaunchKernelGGL((vit_group<G, Q>), dim3(grid), dim3(256), lds, st, *a);
   return hipGetLastError();
 }
 
 template <int G, int... Qs>
-static hipError_t dispatch_group(int Q, const VitArgs *a, int grid, hipStream_t st, std::integer_sequence<int, Qs...>) {
+static hipError_t dispatch_group(int Q, const VitArgs *a, int ncu, hipStream_t st, std::integer_sequence<int, Qs...>) {
   hipError_t e = hipErrorInvalidValue;
-  ((Q == Qs + 1 ? (e = launch_group<G, Qs + 1>(a, grid, st), 0) : 0), ...);
+  ((Q == Qs + 1 ? (e = launch_group<G, Qs + 1>(a, ncu, st), 0) : 0), ...);
   return e;
 }
 
-extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int grid, hipStream_t st) {
-  if (Q < 1 || Q > 8) return hipErrorInvalidValue;
+extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int ncu, hipStream_t st) {
+  if (Q < 1 || Q > 8 || a->n < 1) return hipErrorInvalidValue;
   auto qs = std::make_integer_sequence<int, 8>{};
   switch (G) {
-    case 16: return dispatch_group<16>(Q, a, grid, st, qs);
-    case 32: return dispatch_group<32>(Q, a, grid, st, qs);
+    case 16: return dispatch_group<16>(Q, a, ncu, st, qs);
+    case 32: return dispatch_group<32>(Q, a, ncu, st, qs);
     default: return hipErrorInvalidValue;
   }
 }

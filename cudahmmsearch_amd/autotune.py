@@ -44,7 +44,10 @@ def time_tile(ctx, pid, did, lanes, waves, reps=3):
 
 
 def tune(ctx, db, edges=BIN_EDGES, seed=900, log=print):
-    """Returns [{"lo", "hi", "lanes", "waves", "ms", "candidates"}] for the bins."""
+    """Returns [{"lo", "hi", "lanes", "waves", "ms", "candidates"}] for the bins.
+    The tail guard is off while timing, so each candidate runs the tile it is
+    labelled with."""
+    ctx.set_tail_guard(False)
     did = ctx.add_database(db)
     table = []
     for b in range(len(edges) - 1):
@@ -72,6 +75,7 @@ def tune(ctx, db, edges=BIN_EDGES, seed=900, log=print):
         ctx.remove_profile(pid)
     ctx.clear_tiles()
     ctx.remove_database(did)
+    ctx.set_tail_guard(True)
     return table
 
 

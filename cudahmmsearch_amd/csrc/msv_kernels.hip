@@ -194,17 +194,20 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
   uint32_t acc = 0u;
 
   // Residues arrive 16 at a time; the next group is loaded while the current
-  // one is processed.
+  // one is processed, and each row's emissions are read one row ahead (the
+  // LDS read of row i+1 is in flight while row i computes).
   uint4 cnext = *reinterpret_cast<const uint4 *>(p);
+  uint4 c = (0 < Lpad) ? cnext : null4;
+  uint32_t e[QB * 4];
+  load_row<Q>(tl, byte_of(c, 0), e);
   for (int i0 = 0; i0 < Lmax; i0 += 16) {
-    uint4 c = (i0 < Lpad) ? cnext : null4;
     const int i1 = i0 + 16;
     cnext = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
+    const uint4 cn = (i1 < Lpad) ? cnext : null4;
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
-      const uint32_t x = byte_of(c, rr);
-      uint32_t e[QB * 4];
-      load_row<Q>(tl, x, e);
+      uint32_t en[QB * 4];
+      load_row<Q>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
       const uint32_t sh = shift_slot<G>(r[Q - 1], lane);
 #pragma unroll
       for (int j = Q - 1; j >= 1; --j) r[j] = cell(r[j - 1], e[j]);
@@ -213,9 +216,12 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
 #pragma unroll
       for (; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);
       if constexpr ((Q & 1) != 0) acc = max2h(acc, r[Q - 1]);
+#pragma unroll
+      for (int k = 0; k < Q; ++k) e[k] = en[k];
       row_fence<Q>(r);
       asm volatile("" : "+v"(acc));
     }
+    c = cn;
   }
 
   // Group maximum of both f16 halves, then the reference's decision.

@@ -131,6 +131,14 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
     int xB = (int16_t) (xN + move);                                  // :117
     int xJ = -32768, xC = -32768, status = 0;
     uint32_t edM = NEG2, edI = NEG2, edD = NEG2;
+    // xB + tBM per register, the first term of every M cell: xB changes on
+    // few rows (xJ is a running maximum), so it is recomputed only then
+    // (except with 16 waves per target, where the 128-VGPR budget of a
+    // 1024-lane workgroup has no room for it).
+    constexpr bool kCacheBM = W < 16;
+    uint32_t bm[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xB, xB), tBM[j]);
 
     for (int i0 = 0; i0 < L && status == 0; i0 += 16) {
       const uint4 ch = *reinterpret_cast<const uint4 *>(p + i0);
@@ -146,14 +154,13 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
         uint32_t e[Q];
 #pragma unroll
         for (int j = 0; j < Q; ++j) e[j] = ep[j * C];
-        const uint32_t xBv = pack16(xB, xB);
         uint32_t mpv = shift_in(Mr[Q - 1], edM, lane);               // :139-141
         uint32_t ipv = shift_in(Ir[Q - 1], edI, lane);
         uint32_t dpv = shift_in(Dr[Q - 1], edD, lane);
         uint32_t xEv = NEG2, Dmv = NEG2, dcv = NEG2;
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
-          uint32_t sv = adds(xBv, tBM[j]);                           // :146-150
+          uint32_t sv = kCacheBM ? bm[j] : adds(pack16(xB, xB), tBM[j]);   // :146-150
           sv = maxs(sv, adds(mpv, tMM[j]));
           sv = maxs(sv, adds(ipv, tIM[j]));
           sv = maxs(sv, adds(dpv, tDM[j]));
@@ -181,7 +188,12 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
         if (xE >= 32767) { status = 16; break; }                     // :177
         xC = (int16_t) max(xC, xE + a.e_move);                       // :179 (C loop 0)
         xJ = (int16_t) max(xJ, xE + a.e_loop);                       // :180 (J loop 0)
-        xB = (int16_t) max(xJ + move, xN + move);                    // :181
+        const int xBn = (int16_t) max(xJ + move, xN + move);         // :181
+        if (kCacheBM && xBn != xB) {
+#pragma unroll
+          for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xBn, xBn), tBM[j]);
+        }
+        xB = xBn;
 
         Dr[0] = shift_in(dcv, dedge, lane);                          // :230-231 / :203-204
         if (Dmax + a.ddbound_w > xB) {                               // :199
@@ -391,6 +403,14 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
     uint32_t Mr[Q], Ir[Q], Dr[Q];
 #pragma unroll
     for (int j = 0; j < Q; ++j) Mr[j] = Ir[j] = Dr[j] = NEG2;        // :115
+    // xB + tBM per register (each lane with its group's xB), recomputed only
+    // on the rows where some group's xB changes
+    uint32_t bm[Q];
+    {
+      const int xb = of_group<NG>(xB, ge);
+#pragma unroll
+      for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xb, xb), tBM[j]);
+    }
 
     bool live = true;
     uint4 cnext = *reinterpret_cast<const uint4 *>(p);
@@ -411,15 +431,13 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
         uint32_t e[Q];
 #pragma unroll
         for (int j = 0; j < Q; ++j) e[j] = ep[j * G];
-        const int xb = of_group<NG>(xB, ge);
-        const uint32_t xBv = pack16(xb, xb);
         uint32_t mpv = shift_grp<G>(Mr[Q - 1], lane);               // :139-141
         uint32_t ipv = shift_grp<G>(Ir[Q - 1], lane);
         uint32_t dpv = shift_grp<G>(Dr[Q - 1], lane);
         uint32_t xEv = NEG2, Dmv = NEG2, dcv = NEG2;
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
-          uint32_t sv = adds(xBv, tBM[j]);                           // :146-150
+          uint32_t sv = bm[j];                                       // :146-150
           sv = maxs(sv, adds(mpv, tMM[j]));
           sv = maxs(sv, adds(ipv, tIM[j]));
           sv = maxs(sv, adds(dpv, tDM[j]));
@@ -434,7 +452,7 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
         }
         const uint32_t r = group_max2<G>(pack16(max(lo16(xEv), hi16(xEv)), max(lo16(Dmv), hi16(Dmv))));
         int cond[NG];
-        bool any = false, anylive = false;
+        bool any = false, anylive = false, xbchg = false;
 #pragma unroll
         for (int gg = 0; gg < NG; ++gg) {
           const uint32_t rg = __builtin_amdgcn_readlane(r, gg * G + G - 1);
@@ -446,7 +464,9 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
             } else {
               xC[gg] = (int16_t) max(xC[gg], xE + a.e_move);         // :179 (C loop 0)
               xJ[gg] = (int16_t) max(xJ[gg], xE + a.e_loop);         // :180 (J loop 0)
-              xB[gg] = (int16_t) max(xJ[gg] + mv[gg], xN + mv[gg]);  // :181
+              const int xBn = (int16_t) max(xJ[gg] + mv[gg], xN + mv[gg]);  // :181
+              xbchg = xbchg || (xBn != xB[gg]);
+              xB[gg] = xBn;
               cond[gg] = (Dmax + a.ddbound_w > xB[gg]) ? 1 : 0;      // :199
               anylive = anylive || (i + 1 < Lg[gg]);
             }
@@ -455,6 +475,11 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
         }
         live = anylive;
         if (!live) break;
+        if (xbchg) {
+          const int xb = of_group<NG>(xB, ge);
+#pragma unroll
+          for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xb, xb), tBM[j]);
+        }
 
         Dr[0] = shift_grp<G>(dcv, lane);                             // :230-231 / :203-204
         if (any) {

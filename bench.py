@@ -349,8 +349,9 @@ def main():
     ap.add_argument("--no-vit", action="store_true", help="skip the ViterbiFilter stage on the MSV survivors")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; across GPUs the search would shard the "
+                                   "database by residue count (one process per GPU, results gathered), "
+                                   "which is not launched here", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import cudahmmsearch_amd as ca

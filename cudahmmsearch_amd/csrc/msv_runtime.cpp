@@ -350,15 +350,25 @@ int build_chunks(msvg_ctx *c, Db &d, int G) {
   if (it != d.chunks.end()) return MSVG_OK;
   const int NG = 64 / G;
   const int64_t ntasks = (d.n + NG - 1) / NG;
-  // Chunks of roughly equal residue count; about 16 per wave of a full chip.
+  // Guided chunks: each holds about 1/(8 waves) of the residues not yet
+  // chunked (a full chip runs ncu * 16 waves), down to 2,048 residues, so
+  // chunk sizes shrink toward the end of the (longest-first) task list and
+  // the waves finish within a small chunk of each other.
   const int64_t waves = (int64_t) c->ncu * 16;
-  const int64_t target = std::max<int64_t>(4096, d.total_res / (waves * 16));
+  int64_t remaining = 0;
+  for (int64_t t = 0; t < ntasks; ++t) remaining += d.len_sorted[t * NG];
   std::vector<int32_t> cs;
   cs.push_back(0);
   int64_t acc = 0;
+  int64_t target = std::max<int64_t>(2048, remaining / (waves * 8));
   for (int64_t t = 0; t < ntasks; ++t) {
     acc += d.len_sorted[t * NG];     // the longest in the task sets its rows
-    if (acc >= target) { cs.push_back((int32_t) (t + 1)); acc = 0; }
+    if (acc >= target) {
+      cs.push_back((int32_t) (t + 1));
+      remaining -= acc;
+      acc = 0;
+      target = std::max<int64_t>(2048, remaining / (waves * 8));
+    }
   }
   if (cs.back() != ntasks) cs.push_back((int32_t) ntasks);
   auto mem = std::make_unique<DevMem>();

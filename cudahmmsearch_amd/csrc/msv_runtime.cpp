@@ -771,7 +771,8 @@ namespace {
 // Waves per target and registers per lane: one wave up to M = 1024, then the
 // fewest waves (up to 16) that keep Q <= 8.
 // ViterbiFilter kernel shape for a model of length M: lanes per target G,
-// waves per target W (G = 64 only), registers per lane Q <= 8.  Short models
+// waves per target W (G = 64 only: the fewest with Q <= 16 registers per
+// lane, W <= 8 for M <= 16384), registers per lane Q.  Short models
 // share a wave: 4 targets of 16 lanes or 2 of 32, so the fixed work of a
 // row (reductions, shifts, special states, the delete-closure scan) is
 // shared.  Cost per target row, in packed instructions, from the kernels'
@@ -779,7 +780,7 @@ namespace {
 // 78 (two), 94 (four).
 void vit_shape(int M, int &Q, int &W, int &G) {
   W = 1;
-  while (128 * W * 8 < M) W *= 2;
+  while (128 * W * 16 < M) W *= 2;
   Q = (M + 128 * W - 1) / (128 * W);
   G = 64;
   if (W > 1) return;

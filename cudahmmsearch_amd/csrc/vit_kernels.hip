@@ -132,10 +132,7 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
     int xJ = -32768, xC = -32768, status = 0;
     uint32_t edM = NEG2, edI = NEG2, edD = NEG2;
     // xB + tBM per register, the first term of every M cell: xB changes on
-    // few rows (xJ is a running maximum), so it is recomputed only then
-    // (except with 16 waves per target, where the 128-VGPR budget of a
-    // 1024-lane workgroup has no room for it).
-    constexpr bool kCacheBM = W < 16;
+    // few rows (xJ is a running maximum), so it is recomputed only then.
     uint32_t bm[Q];
 #pragma unroll
     for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xB, xB), tBM[j]);
@@ -160,7 +157,7 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
         uint32_t xEv = NEG2, Dmv = NEG2, dcv = NEG2;
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
-          uint32_t sv = kCacheBM ? bm[j] : adds(pack16(xB, xB), tBM[j]);   // :146-150
+          uint32_t sv = bm[j];                                       // :146-150
           sv = maxs(sv, adds(mpv, tMM[j]));
           sv = maxs(sv, adds(ipv, tIM[j]));
           sv = maxs(sv, adds(dpv, tDM[j]));
@@ -189,7 +186,7 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
         xC = (int16_t) max(xC, xE + a.e_move);                       // :179 (C loop 0)
         xJ = (int16_t) max(xJ, xE + a.e_loop);                       // :180 (J loop 0)
         const int xBn = (int16_t) max(xJ + move, xN + move);         // :181
-        if (kCacheBM && xBn != xB) {
+        if (xBn != xB) {
 #pragma unroll
           for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xBn, xBn), tBM[j]);
         }
@@ -274,15 +271,17 @@ static hipError_t dispatch_q(int Q, const VitArgs *a, int ncu, hipStream_t st, s
   return e;
 }
 
+// Q <= 16 registers per lane and W = 1, 2, 4 or 8 waves per target (M up to
+// 16384): the transitions of 16 registers still fit the 256 VGPRs a lane
+// has when a workgroup holds at most two waves per SIMD.
 extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, hipStream_t st) {
-  if (Q < 1 || Q > 8 || a->n < 1) return hipErrorInvalidValue;
-  auto qs = std::make_integer_sequence<int, 8>{};
+  if (Q < 1 || Q > 16 || a->n < 1) return hipErrorInvalidValue;
+  auto qs = std::make_integer_sequence<int, 16>{};
   switch (W) {
     case 1: return dispatch_q<1>(Q, a, ncu, st, qs);
     case 2: return dispatch_q<2>(Q, a, ncu, st, qs);
     case 4: return dispatch_q<4>(Q, a, ncu, st, qs);
     case 8: return dispatch_q<8>(Q, a, ncu, st, qs);
-    case 16: return dispatch_q<16>(Q, a, ncu, st, qs);
     default: return hipErrorInvalidValue;
   }
 }

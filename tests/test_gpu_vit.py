@@ -49,7 +49,7 @@ def test_golden_vectors(vit_golden):
         ctx.close()
 
 
-@pytest.mark.parametrize("M", [1, 7, 33, 128, 129, 200, 256, 257, 400, 512, 513, 1024, 1025, 2048, 2049,
+@pytest.mark.parametrize("M", [1, 7, 33, 128, 129, 200, 256, 257, 400, 480, 481, 512, 513, 1024, 1025, 2048, 2049,
                                4096, 4097, 8192, 8193, 9000, 16384])
 def test_every_wave_count_vs_reference(M):
     """Model lengths at the edges of each (lanes, waves, registers) shape

@@ -115,6 +115,7 @@ def load_library(path=LIB_PATH):
         "msvg_profile_set_viterbi": (C.c_int, [vp, i32, C.POINTER(_VProf)]),
         "msvg_vit_search": (C.c_int, [vp, i32, i32, vp, i64, vp, vp]),
         "msvg_vit_search_many": (C.c_int, [vp, i32, vp, i32, vp, vp, vp, vp]),
+        "msvg_scan": (C.c_int, [vp, i32, vp, vp, C.c_int, C.c_double, vp, vp, vp]),
         "msvg_ViterbiFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_calibrate_vmu": (C.c_int, [vp, i32, C.c_double, C.POINTER(C.c_float)]),
         "msvh_vit_move": (C.c_int, [C.c_float, C.c_float, C.c_int]),
@@ -521,7 +522,10 @@ class MSVContext:
 
     def add_database(self, db):
         did = C.c_int32()
-        self._chk(self.lib.msvg_db_add(self._c, _p(db.res) if db.res.size else None, _p(db.off), _p(db.lens),
+        # a database of empty targets has no residues; the ABI still takes a
+        # residue pointer when n > 0
+        res = db.res if db.res.size else np.zeros(1, np.uint8)
+        self._chk(self.lib.msvg_db_add(self._c, _p(res) if db.n else None, _p(db.off), _p(db.lens),
                                        db.n, C.byref(did)))
         return did.value
 
@@ -572,6 +576,18 @@ class MSVContext:
         st = np.zeros(t.size, np.uint8)
         self._chk(self.lib.msvg_vit_search(self._c, pid, did, _p(t), t.size, _p(sc), _p(st)))
         return sc, st
+
+    def scan(self, pids, x, F1=0.02):
+        """hmmscan direction: the profiles pids against one query x (codes, no
+        sentinels): (usc, status, pass), one entry per profile."""
+        pids = np.ascontiguousarray(pids, dtype=np.int32)
+        dsq = np.concatenate([[255], np.asarray(x, np.uint8), [255]]).astype(np.uint8)
+        n = max(pids.size, 1)
+        usc = np.zeros(n, np.float32)
+        st = np.zeros(n, np.uint8)
+        ps = np.zeros(n, np.uint8)
+        self._chk(self.lib.msvg_scan(self._c, pids.size, _p(pids), _p(dsq), len(x), float(F1), _p(usc), _p(st), _p(ps)))
+        return usc[:pids.size], st[:pids.size], ps[:pids.size]
 
     def vit_search_many(self, pids, did, target_lists):
         """vit_search for several profiles in one call (launches overlap on

@@ -92,6 +92,18 @@ struct OneArgs {               // one target, one workgroup (the per-call drop-i
   uint32_t *out;               // result word
 };
 
+struct ScanProf {              // one profile of a scan set (many profiles, one query)
+  const uint4 *tab;            // its 64-lane table over residue codes 0..R-1, null row R
+  ProfDev prof;
+};
+
+struct ScanArgs {              // one workgroup per profile
+  const ScanProf *profs;
+  const uint8_t *res;          // the query's codes, padded to 16 bytes with R
+  int32_t L, tjb, R;
+  uint32_t *out;               // result word per profile
+};
+
 struct SlowArgs {
   const ProfDev *profs;
   DbDev db;

@@ -546,6 +546,40 @@ __global__ void __launch_bounds__(1024) msv_one(OneArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Scan direction (hmmscan: many profiles against one query): the one-target
+// composite above with one workgroup per profile.  Its four waves stage the
+// profile's table in LDS (four 16-byte reads in flight per lane), three of
+// them leave, and the first computes the query's rows.  Workgroups are
+// independent, so the hardware keeps as many resident per CU as their LDS
+// allows (five for tables of 30 KiB) and starts the next as one ends.
+// ---------------------------------------------------------------------------
+template <int Q>
+__global__ void __launch_bounds__(256) msv_scan(ScanArgs a) {
+  extern __shared__ uint4 lds[];
+  constexpr int QB = (Q + 3) / 4;
+  const ScanProf *sp = a.profs + blockIdx.x;
+  const uint4 *tab = sp->tab;
+  const int ntab = (a.R + 1) * QB * 64;
+  int i = threadIdx.x;
+  for (; i + 768 < ntab; i += 1024) {
+    const uint4 v0 = tab[i], v1 = tab[i + 256], v2 = tab[i + 512], v3 = tab[i + 768];
+    lds[i] = v0; lds[i + 256] = v1; lds[i + 512] = v2; lds[i + 768] = v3;
+  }
+  for (; i < ntab; i += 256) lds[i] = tab[i];
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  const ProfDev pd = sp->prof;
+  int xEmax;
+  uint32_t w = jrow_task<Q>(lds + lane, a.res, a.L, a.tjb, pd, lane, xEmax);
+  if ((w >> 8) != RES_ERANGE && xEmax - pd.tec <= pd.base) {
+    const int tjbm = (a.tjb + pd.tbm) & 255;
+    w = decide(xEmax - max(0, pd.base - tjbm), a.tjb, pd);
+  }
+  if (lane == 0) a.out[blockIdx.x] = w;
+}
+
+// ---------------------------------------------------------------------------
 // Stage 2: exact byte MSV (msvfilter.c:112-210) for queued pairs.
 // One wave per pair; the DP row lives in LDS, one dword column per lane.
 // ---------------------------------------------------------------------------
@@ -874,12 +908,42 @@ extern "C" hipError_t msvk_launch_one(int Q, const OneArgs *a, size_t lds, hipSt
   return dispatch_one(Q, *a, lds, st, MSVG_QSEQ{});
 }
 
+// Register classes of the scan kernel: a profile's Q = ceil(M/128) is
+// rounded up to the next class (the extra registers hold padding positions).
+#define MSVG_SCANQ std::integer_sequence<int, 2, 4, 8, 12, 16, 20, 24, 28, 32>
+template <int Q>
+static hipError_t launch_scan_t(const ScanArgs &a, int grid, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((msv_scan<Q>), dim3(grid), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
+template <int... Qs>
+static hipError_t dispatch_scan(int Q, const ScanArgs &a, int grid, size_t lds, hipStream_t st,
+                                std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipErrorInvalidValue;
+  (void) ((Q == Qs ? (err = launch_scan_t<Qs>(a, grid, lds, st), true) : false) || ...);
+  return err;
+}
+template <int... Qs>
+static hipError_t set_lds_scan(std::integer_sequence<int, Qs...>) {
+  hipError_t err = hipSuccess;
+  ((err = (err == hipSuccess ? hipFuncSetAttribute(reinterpret_cast<const void *>(&msv_scan<Qs>),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)
+                             : err)), ...);
+  return err;
+}
+
+extern "C" hipError_t msvk_launch_scan(int Q, const ScanArgs *a, int grid, size_t lds, hipStream_t st) {
+  if (grid < 1) return hipErrorInvalidValue;
+  return dispatch_scan(Q, *a, grid, lds, st, MSVG_SCANQ{});
+}
+
 extern "C" hipError_t msvk_prepare() {
   hipError_t e;
   if ((e = set_lds_all<8>(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_seg(MSVG_SEGQ{})) != hipSuccess) return e;
   if ((e = set_lds_jrow(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_one(MSVG_QSEQ{})) != hipSuccess) return e;
+  if ((e = set_lds_scan(MSVG_SCANQ{})) != hipSuccess) return e;
   if ((e = set_lds_all<16>(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_all<32>(MSVG_QSEQ{})) != hipSuccess) return e;
   return set_lds_all<64>(MSVG_QSEQ{});

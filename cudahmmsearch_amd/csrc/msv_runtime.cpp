@@ -52,6 +52,7 @@ extern "C" hipError_t msvk_build_slow_table(const uint8_t *rbv, int M, const uin
 extern "C" hipError_t msvk_fill_zero(int32_t *p, int n, hipStream_t st);
 extern "C" hipError_t msvk_lds_probe(void *out, int grid, int iters, hipStream_t st);
 extern "C" hipError_t msvk_launch_one(int Q, const OneArgs *a, size_t lds, hipStream_t st);
+extern "C" hipError_t msvk_launch_scan(int Q, const ScanArgs *a, int grid, size_t lds, hipStream_t st);
 
 namespace {
 
@@ -119,6 +120,21 @@ struct Profile {
   DevMem d_rbv;
   std::unique_ptr<VitProfile> vit;
   DevMem d_onetab;                 // per-call drop-in: 64-lane fast table over codes 0..Kp-1
+  DevMem d_scantab;                // scan direction: the same table at the profile's register class
+  int scanQ = 0;                   // that class (0: not built)
+};
+
+struct ScanSet {                   // the profiles of msvg_scan calls, grouped by register class
+  std::vector<int32_t> pids;
+  int Kp = 0;
+  struct Cls {
+    int Q = 0;
+    std::vector<int32_t> members;  // positions in pids
+    DevMem d_profs;                // ScanProf per member
+  };
+  std::vector<std::unique_ptr<Cls>> cls;
+  std::vector<int32_t> fallback;   // positions whose table does not fit LDS
+  int64_t sumM = 0;
 };
 
 struct Db {
@@ -189,6 +205,11 @@ struct msvg_ctx {
   uint8_t *h_one_res = nullptr;             // pinned staging for the target
   size_t h_one_cap = 0;
   uint32_t *h_one_out = nullptr;            // pinned result word
+  // scan direction (many profiles, one query)
+  std::unique_ptr<ScanSet> scanset;
+  DevMem d_scanout;
+  uint32_t *h_scanout = nullptr;            // pinned result words
+  size_t h_scancap = 0;
   // ViterbiFilter searches
   DevMem d_vitems, d_vmove, d_vout;
   hipStream_t vstream[kVitStreams] = {};   // multi-profile searches: launches round-robin
@@ -875,6 +896,8 @@ void msvg_free(msvg_ctx *c) {
   if (c->stream) (void) hipStreamDestroy(c->stream);
   if (c->h_one_res) (void) hipHostFree(c->h_one_res);
   if (c->h_one_out) (void) hipHostFree(c->h_one_out);
+  if (c->h_scanout) (void) hipHostFree(c->h_scanout);
+  c->scanset.reset();
   delete c;
 }
 
@@ -911,6 +934,7 @@ int msvg_profile_remove(msvg_ctx *c, int32_t id) {
     it = (std::get<0>(it->first) == id) ? c->finals.erase(it) : std::next(it);
   c->last = nullptr;
   c->plans.clear();
+  c->scanset.reset();
   c->profiles.erase(id);
   return MSVG_OK;
 }
@@ -1383,6 +1407,188 @@ int msvg_probe_lds(msvg_ctx *c, double *ret_bytes_per_s) {
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   *ret_bytes_per_s = (double) grid * 1024 * iters * 8 * 16 / (ms / 1e3);
+  return MSVG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Register class of the scan kernel for Q = ceil(M/128).
+int scan_class(int Q) {
+  for (int q : {2, 4, 8, 12, 16, 20, 24, 28, 32})
+    if (Q <= q) return q;
+  return 0;
+}
+
+// The scan set for pids: classes, per-profile tables at their class, and the
+// device descriptors; reused while the same profiles are scanned.
+int scan_prepare(msvg_ctx *c, const int32_t *pids, int32_t np, ScanSet **ret) {
+  if (c->scanset && (int32_t) c->scanset->pids.size() == np &&
+      std::equal(pids, pids + np, c->scanset->pids.begin())) {
+    *ret = c->scanset.get();
+    return MSVG_OK;
+  }
+  c->scanset.reset();
+  auto set = std::make_unique<ScanSet>();
+  set->pids.assign(pids, pids + np);
+  set->Kp = c->profiles[pids[0]]->Kp;
+  std::map<int, ScanSet::Cls *> by_q;
+  hipStream_t st = c->stream;
+  if (!c->d_idrow.p) {
+    uint8_t id[256];
+    for (int x = 0; x < 256; ++x) id[x] = (uint8_t) x;
+    HIPCHK(c, c->d_idrow.ensure(256));
+    HIPCHK(c, hipMemcpy(c->d_idrow.p, id, 256, hipMemcpyHostToDevice));
+  }
+  for (int32_t i = 0; i < np; ++i) {
+    Profile &p = *c->profiles[pids[i]];
+    set->sumM += p.M;
+    const int Q = scan_class(qfor(p.M, 128));
+    if (Q == 0 || fast_lds(p.Kp, Q) > (size_t) kLdsBytes || p.Kp > 254) { set->fallback.push_back(i); continue; }
+    if (p.scanQ != Q) {
+      HIPCHK(c, p.d_scantab.ensure((size_t) (p.Kp + 1) * ((Q + 3) / 4) * 64 * 16));
+      HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, 64, Q, p.bias,
+                                      p.d_scantab.as<uint32_t>(), st));
+      p.scanQ = Q;
+    }
+    auto it = by_q.find(Q);
+    if (it == by_q.end()) {
+      set->cls.push_back(std::make_unique<ScanSet::Cls>());
+      set->cls.back()->Q = Q;
+      it = by_q.emplace(Q, set->cls.back().get()).first;
+    }
+    it->second->members.push_back(i);
+  }
+  for (auto &cl : set->cls) {
+    std::vector<ScanProf> h(cl->members.size());
+    for (size_t k = 0; k < cl->members.size(); ++k) {
+      const Profile &p = *c->profiles[pids[cl->members[k]]];
+      ScanProf &sp = h[k];
+      std::memset(&sp, 0, sizeof sp);
+      sp.tab = p.d_scantab.as<uint4>();
+      sp.prof.M = p.M; sp.prof.Q = cl->Q; sp.prof.QB = (cl->Q + 3) / 4; sp.prof.Qs = qfor(p.M, 128);
+      sp.prof.tbm = p.tbm; sp.prof.tec = p.tec; sp.prof.base = p.base; sp.prof.bias = p.bias;
+      sp.prof.slow_all = 0;
+    }
+    HIPCHK(c, cl->d_profs.ensure(h.size() * sizeof(ScanProf)));
+    HIPCHK(c, hipMemcpyAsync(cl->d_profs.p, h.data(), h.size() * sizeof(ScanProf), hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(c, hipStreamSynchronize(st));
+  c->scanset = std::move(set);
+  *ret = c->scanset.get();
+  return MSVG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int msvg_scan(msvg_ctx *c, int32_t np, const int32_t *pids, const uint8_t *dsq, int L, double F1, float *usc,
+              uint8_t *status, uint8_t *pass) {
+  if (!c) return MSVG_EINVAL;
+  if (np < 0 || L < 0 || (np > 0 && (!pids || !dsq || !usc || !status || !pass)))
+    return fail(c, MSVG_EINVAL, "null argument");
+  c->stats = msvg_stats{};
+  if (np == 0) return MSVG_OK;
+  for (int32_t i = 0; i < np; ++i)
+    if (!c->profiles.count(pids[i])) return fail(c, MSVG_EINVAL, "unknown profile id");
+  const int Kp = c->profiles[pids[0]]->Kp;
+  for (int32_t i = 1; i < np; ++i)
+    if (c->profiles[pids[i]]->Kp != Kp) return fail(c, MSVG_EINVAL, "scanned profiles must share one alphabet");
+  for (int i = 1; i <= L; ++i)
+    if (dsq[i] >= Kp) return fail(c, MSVG_EINVAL, "query holds residue codes outside the profiles' alphabet");
+  (void) hipSetDevice(c->device);
+  ScanSet *set;
+  int rc = scan_prepare(c, pids, np, &set);
+  if (rc != MSVG_OK) return rc;
+  hipStream_t st = c->stream;
+  const int tjb = msvh_tjb(L);
+  std::vector<int32_t> slow(set->fallback);
+  if (L == 0) {
+    slow.clear();
+    for (int32_t i = 0; i < np; ++i) slow.push_back(i);
+  } else {
+    // stage the query (padded to 16 bytes with the null row Kp)
+    const size_t Lpad = ((size_t) L + 15) & ~(size_t) 15;
+    if (c->h_one_cap < Lpad) {
+      if (c->h_one_res) { (void) hipHostFree(c->h_one_res); c->h_one_res = nullptr; c->h_one_cap = 0; }
+      const size_t cap = std::max<size_t>(Lpad, 4096);
+      HIPCHK(c, hipHostMalloc((void **) &c->h_one_res, cap, hipHostMallocDefault));
+      c->h_one_cap = cap;
+    }
+    if (c->h_scancap < (size_t) np) {
+      if (c->h_scanout) { (void) hipHostFree(c->h_scanout); c->h_scanout = nullptr; c->h_scancap = 0; }
+      HIPCHK(c, hipHostMalloc((void **) &c->h_scanout, (size_t) np * sizeof(uint32_t), hipHostMallocDefault));
+      c->h_scancap = (size_t) np;
+    }
+    HIPCHK(c, c->d_one_res.ensure(std::max<size_t>(Lpad, 4096)));
+    HIPCHK(c, c->d_scanout.ensure((size_t) np * sizeof(uint32_t)));
+    std::memcpy(c->h_one_res, dsq + 1, (size_t) L);
+    std::memset(c->h_one_res + L, Kp, Lpad - (size_t) L);
+    HIPCHK(c, hipMemcpyAsync(c->d_one_res.p, c->h_one_res, Lpad, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipEventRecord(c->ev[0], st));
+    int64_t base = 0;
+    for (auto &cl : set->cls) {
+      ScanArgs a{};
+      a.profs = cl->d_profs.as<ScanProf>();
+      a.res = c->d_one_res.as<uint8_t>();
+      a.L = L; a.tjb = tjb; a.R = Kp;
+      a.out = c->d_scanout.as<uint32_t>() + base;
+      HIPCHK(c, msvk_launch_scan(cl->Q, &a, (int) cl->members.size(), fast_lds(Kp, cl->Q), st));
+      base += (int64_t) cl->members.size();
+    }
+    HIPCHK(c, hipEventRecord(c->ev[1], st));
+    if (base > 0)
+      HIPCHK(c, hipMemcpyAsync(c->h_scanout, c->d_scanout.p, (size_t) base * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->stats.ms_fast = ms;
+    c->stats.ms_total = ms;
+    base = 0;
+    for (auto &cl : set->cls) {
+      for (size_t k = 0; k < cl->members.size(); ++k) {
+        const int32_t i = cl->members[k];
+        const uint32_t w = c->h_scanout[base + (int64_t) k];
+        const uint32_t code = w >> 8;
+        const Profile &p = *c->profiles[pids[i]];
+        if (code == RES_SLOW) { slow.push_back(i); continue; }
+        if (code == RES_ERANGE) {
+          usc[i] = std::numeric_limits<float>::infinity(); status[i] = MSVG_ERANGE; pass[i] = 1;
+        } else {
+          usc[i] = usc_of((int) (w & 255u) - tjb, p.base, p.scale);
+          status[i] = MSVG_OK;
+          pass[i] = passes(usc[i], L, p.mu, p.lambda, F1) ? 1 : 0;
+        }
+      }
+      base += (int64_t) cl->members.size();
+    }
+  }
+  c->stats.cells = set->sumM * (int64_t) L;
+  c->stats.n_slow = (int64_t) slow.size();
+  if (!slow.empty()) {
+    // the search path on a one-target database for what the kernel leaves
+    std::sort(slow.begin(), slow.end());
+    if (c->scratch_db >= 0) msvg_db_remove(c, c->scratch_db);
+    c->scratch_db = -1;
+    const int64_t off = 0;
+    int32_t id;
+    if ((rc = msvg_db_add(c, dsq + 1, &off, &L, 1, &id)) != MSVG_OK) return rc;
+    c->scratch_db = id;
+    std::vector<int32_t> ids(slow.size());
+    for (size_t k = 0; k < slow.size(); ++k) ids[k] = pids[slow[k]];
+    std::vector<float> u(slow.size());
+    std::vector<uint8_t> s2(slow.size()), p2(slow.size());
+    const msvg_stats keep = c->stats;
+    if ((rc = msvg_search(c, ids.data(), (int32_t) ids.size(), id, F1, MSVG_MODE_PLAIN, u.data(), s2.data(),
+                          p2.data())) != MSVG_OK)
+      return rc;
+    c->stats = keep;
+    for (size_t k = 0; k < slow.size(); ++k) {
+      usc[slow[k]] = u[k]; status[slow[k]] = s2[k]; pass[slow[k]] = p2[k];
+    }
+  }
   return MSVG_OK;
 }
 

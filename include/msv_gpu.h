@@ -122,6 +122,19 @@ int msvg_last_stats(msvg_ctx *ctx, msvg_stats *out);
  * database), with the same results. */
 int msvg_MSVFilter(msvg_ctx *ctx, int32_t profile_id, const uint8_t *dsq, int L, float *ret_sc);
 
+/* hmmscan's direction: many profiles against one query (dsq 1-based with
+ * sentinels, as for msvg_MSVFilter).  For profile i: usc[i], status[i]
+ * and pass[i] (F1 rule of p7_pipeline.c:662-677 with its mu, lambda) as
+ * msvg_search would give them for a one-target database — replaces the
+ * per-model p7_MSVFilter calls that p7_Pipeline makes in hmmscan's loop over
+ * a pressed database (src/hmmscan.c:1176-1190, serial_loop).  One launch per register class,
+ * one workgroup per profile; the profiles' tables are built on the first
+ * call and kept while the same profile list is scanned.  All profiles must
+ * share one alphabet (EINVAL otherwise).  ms_fast in msvg_last_stats is the
+ * kernels' time, cells M * L summed over the profiles. */
+int msvg_scan(msvg_ctx *ctx, int32_t np, const int32_t *profile_ids, const uint8_t *dsq, int L, double F1,
+              float *usc, uint8_t *status, uint8_t *pass);
+
 /* ---- The ViterbiFilter stage: the filter after MSV in p7_Pipeline
  * (p7_pipeline.c:699-705, the call at :701), int16 Viterbi of impl_sse/vitfilter.c:84-250. ----
  *

@@ -371,3 +371,60 @@ def test_tail_guard(ctx):
     _check_vs_oracle(ctx, prof, db, ca.MODE_BATCHED, pid=pid, did=did, sample=300)
     ctx.remove_database(did)
     ctx.remove_profile(pid)
+
+
+def test_scan_direction_equals_search(ctx):
+    """msvg_scan (many profiles, one query; one workgroup per profile) gives
+    each profile what msvg_search gives on a one-target database: profiles
+    of every register class (M = 1 .. 3000, beyond LDS at 2,561 and up,
+    which take the search path), against random queries, planted homologs
+    (J fires, overflow), a one-residue and an empty query."""
+    rng = np.random.default_rng(77)
+    Ms = [1, 2, 7, 60, 128, 129, 200, 256, 257, 400, 512, 513, 1000, 1024, 1025, 1536, 2048, 2049, 2560, 2561,
+          3000] + [int(m) for m in rng.integers(20, 700, 40)]
+    profs = [ca.sample_profile(5000 + i, M) for i, M in enumerate(Ms)]
+    pids = []
+    for p in profs:
+        pid = ctx.add_profile(p)
+        p.mu = ctx.calibrate(pid, p.lam)
+        pids.append(pid)
+    queries = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in (350, 1200, 37)]
+    queries += ca.core_emit(profs[9].hmm, 78, 2) + ca.core_emit(profs[3].hmm, 79, 1)
+    queries += [np.array([5], np.uint8), np.zeros(0, np.uint8)]
+    for q in queries:
+        print(f"query L={q.size} profiles={len(pids)}", flush=True)
+        u, s, ps = ctx.scan(pids, q)
+        did = ctx.add_database(ca.Database.from_list([q]))
+        ru, rs, rp = ctx.search(pids, did, mode=ca.MODE_PLAIN)
+        ctx.remove_database(did)
+        assert np.array_equal(u.view(np.uint32), ru[:, 0].view(np.uint32))
+        assert np.array_equal(s, rs[:, 0]) and np.array_equal(ps, rp[:, 0])
+    st = ctx.stats()
+    assert st["cells"] == 0 or st["cells"] > 0
+    with pytest.raises(ca.MSVError):
+        ctx.scan(pids, np.array([1, 29], np.uint8))              # code outside the alphabet
+    for pid in pids:
+        ctx.remove_profile(pid)
+
+
+def test_scan_dna_and_mixed_alphabets(ctx, golden):
+    """A DNA profile set (the tutorial's MADE1) scans like the amino sets;
+    mixing alphabets in one scan is an error."""
+    z, man = golden
+    names = [m["name"] for m in man["profiles"]]
+    dna = golden_profile(z, man, names.index("MADE1"))
+    amino = golden_profile(z, man, names.index("fn3"))
+    pd_ = ctx.add_profile(_bp(dna))
+    pa = ctx.add_profile(_bp(amino))
+    rng = np.random.default_rng(3)
+    for L in (1, 80, 500):
+        q = rng.integers(0, 4, L).astype(np.uint8)
+        u, s, ps = ctx.scan([pd_], q)
+        did = ctx.add_database(ca.Database.from_list([q]))
+        ru, rs, rp = ctx.search([pd_], did, mode=ca.MODE_PLAIN)
+        ctx.remove_database(did)
+        assert np.array_equal(u.view(np.uint32), ru[:, 0].view(np.uint32)) and np.array_equal(s, rs[:, 0])
+    with pytest.raises(ca.MSVError):
+        ctx.scan([pd_, pa], np.array([1, 2], np.uint8))
+    ctx.remove_profile(pd_)
+    ctx.remove_profile(pa)

@@ -1,6 +1,6 @@
 """MSV filter benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5|scan]
 
 A step is one complete MSV search (fast J-free stage, exact byte-MSV stage for
 the queued pairs, finalize to usc / status / F1 pass bit) of the workload's
@@ -32,6 +32,10 @@ generators in cudahmmsearch_amd):
       their STATS lines); Poisson arrivals at --rate req/s for --duration s.
       Also measures the service capacity (closed loop, every slot busy) and
       p50/p99 at 70% of it, where the queue is stable.
+  scan  hmmscan direction: --scan-profiles sampled profiles (M log-normal
+      median 250, sigma 0.6, clipped 30..2560, seeds 100000..) against
+      --scan-queries query sequences (log-normal median 350, sigma 0.7,
+      seed 6), one msvg_scan call per query
 """
 import argparse
 import json
@@ -229,6 +233,86 @@ def dropin_stage(ctx, pid, db, usc_batched, n=2000):
             "identical_to_batched": bool(np.array_equal(sc.view(np.uint32), usc_batched[:n].view(np.uint32)))}
 
 
+def scan_workload(ca, args):
+    """hmmscan direction: a database of sampled profiles (M log-normal median
+    250, sigma 0.6, clipped 30..2560, seeds 100000+i) against query sequences
+    (log-normal median 350, sigma 0.7, clipped 20..10000, seed 6), one
+    msvg_scan call per query (host to host: query upload, launches, results,
+    the F1 rule).  Also: the search path on the same pairs (msvg_search over
+    a one-target database per query) and the reference's SSE p7_MSVFilter on
+    a sample of the profiles."""
+    nprof, nq = args.scan_profiles, args.scan_queries
+    rs = np.random.default_rng(100)
+    Ms = np.clip(np.round(250 * np.exp(0.6 * rs.standard_normal(nprof))), 30, 2560).astype(int)
+    t0 = time.perf_counter()
+    profs = [ca.sample_profile(100000 + i, int(M)) for i, M in enumerate(Ms)]
+    qdb = ca.gen_database(6, nq, 350, 0.7, 20, 10000, nthreads=16)
+    log(f"scan workload: {nprof} profiles, {nq} queries ({qdb.residues} residues) in {time.perf_counter() - t0:.1f} s")
+    ctx = ca.MSVContext(0)
+    pids = np.array([ctx.add_profile(p) for p in profs], np.int32)
+    queries = [qdb.seq(i) for i in range(nq)]
+    sumM = int(Ms.sum())
+    cells = sumM * qdb.residues
+    for q in queries[:args.warmup + 1]:                        # table builds, warm-up
+        ctx.scan(pids, q)
+    res = []
+    ms_k = 0.0
+    t0 = time.perf_counter()
+    for q in queries:
+        res.append(ctx.scan(pids, q))
+        ms_k += ctx.stats()["ms_fast"]
+    wall = time.perf_counter() - t0
+    # the search path on the first queries, identity checked
+    nsp = min(2, nq)
+    same = True
+    t1 = time.perf_counter()
+    for k in range(nsp):
+        did = ctx.add_database(ca.Database.from_list([queries[k]]))
+        u, st, ps = ctx.search(pids, did, mode=ca.MODE_PLAIN)
+        ctx.remove_database(did)
+        same = same and np.array_equal(u[:, 0].view(np.uint32), res[k][0].view(np.uint32)) \
+            and np.array_equal(st[:, 0], res[k][1]) and np.array_equal(ps[:, 0], res[k][2])
+    search_s = (time.perf_counter() - t1) / nsp
+    out = {"metric": "MSV scan (hmmscan direction: profiles x one query) GCUPS on 1 MI355X", "value": round(cells / wall / 1e9, 1),
+           "unit": "GCUPS", "n_gpus": 1, "steps": nq, "warmup": args.warmup + 1,
+           "ms_per_step": round(wall / nq * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "u8", "data": "synthetic",
+           "config": {"workload": f"scan: {nprof} sampled profiles (M log-normal median 250, sigma 0.6, clip 30-2560; "
+                                  f"sum M {sumM}) vs {nq} queries (log-normal median 350, sigma 0.7)",
+                      "profiles": nprof, "queries": nq, "query_residues": qdb.residues, "cells": cells},
+           "queries_per_s": round(nq / wall, 1), "kernel_ms_per_query": round(ms_k / nq, 3),
+           "kernel_gcups": round(cells / (ms_k / 1e3) / 1e9, 1),
+           "search_path_ms_per_query": round(search_s * 1e3, 1),
+           "scan_over_search": round(search_s / (wall / nq), 1), "identical_to_search": bool(same)}
+    if not args.no_cpu:
+        import oracle
+        try:
+            R = oracle.Reference()
+            nthreads = min(16, os.cpu_count() or 1)
+            take = rs.choice(nprof, min(nprof, 400), replace=False)
+            rps = []
+            for i in take:                                   # profile conversion, not timed
+                mat, ins, t = profs[i].hmm
+                rps.append(R.from_probs(mat, t, 100, False))
+            t2 = time.perf_counter()
+            ccells = 0
+            for i, rp in zip(take, rps):
+                rp.msv_batch(qdb.res, qdb.off, qdb.lens, nthreads)
+                ccells += int(Ms[i]) * qdb.residues
+            dt = time.perf_counter() - t2
+            out["cpu_baseline"] = {"value": round(ccells / dt / 1e9, 3), "unit": "GCUPS", "cores": nthreads,
+                                   "kind": "reference",
+                                   "sample": f"{take.size} of the profiles against all {nq} queries, p7_MSVFilter "
+                                             f"on {nthreads} threads over the queries, one profile at a time, "
+                                             f"{dt:.2f} s"}
+        except (FileNotFoundError, OSError):
+            out["cpu_baseline"] = None
+    log(f"scan: {out['queries_per_s']} queries/s, {out['value']} GCUPS host to host, kernels {out['kernel_gcups']} GCUPS; "
+        f"search path {out['search_path_ms_per_query']} ms per query")
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def copy_probe(nbytes=256 << 20):
     """One device-to-device copy of a known size, for scaling FETCH_SIZE /
     WRITE_SIZE counters (the copy reads and writes nbytes each)."""
@@ -347,6 +431,8 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="c5: queries on the GPU at once")
     ap.add_argument("--shared", action="store_true", help="c5: one chunk per workgroup (queries share CUs)")
     ap.add_argument("--no-vit", action="store_true", help="skip the ViterbiFilter stage on the MSV survivors")
+    ap.add_argument("--scan-profiles", type=int, default=10000, help="scan: profiles in the scanned database")
+    ap.add_argument("--scan-queries", type=int, default=64, help="scan: query sequences")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; across GPUs the search would shard the "
@@ -357,6 +443,9 @@ def main():
     import cudahmmsearch_amd as ca
     if args.workload == "c5":
         serving(ca, args)
+        return
+    if args.workload == "scan":
+        scan_workload(ca, args)
         return
     if args.copy_probe:
         copy_probe()

@@ -331,6 +331,30 @@ bool passes(float usc, int L, float mu, float lambda, double F1) {
   return !(P > F1);
 }
 
+// passes() for many scores of one target length: nullsc(L) computed once,
+// and the Gumbel survival evaluated only near its F1 crossing.  P falls
+// strictly with y = lambda (seq_score - mu), so the rule !(P > F1) is y >= y*
+// with y* = -log(-log(1 - F1)); scores more than 1e-6 from y* are decided by
+// the comparison (P's double evaluation is accurate to ~1e-16 there), the
+// rest by the exact expression.
+struct PassRule {
+  float nullsc;
+  double F1, ystar;
+  bool fast;
+  PassRule(int L, double f1) : nullsc(msvh_nullsc(L)), F1(f1), ystar(0), fast(f1 > 0 && f1 < 1) {
+    if (fast) ystar = -std::log(-std::log1p(-f1));
+  }
+  bool operator()(float usc, float mu, float lambda) const {
+    const float seq_score = (usc - nullsc) / 0.69314718055994529;
+    const double y = (double) lambda * ((double) seq_score - (double) mu);
+    if (fast && y > ystar + 1e-6) return true;
+    if (fast && y < ystar - 1e-6) return false;
+    const double ey = -std::exp(-y);
+    const double P = (std::fabs(ey) < 5e-9) ? -ey : 1 - std::exp(ey);
+    return !(P > F1);
+  }
+};
+
 const std::pair<std::vector<uint16_t>, std::vector<float>> &final_tables(msvg_ctx *c, int pid, int dbid, double F1) {
   auto key = std::make_tuple(pid, dbid, F1);
   auto it = c->finals.find(key);
@@ -1460,6 +1484,8 @@ int scan_prepare(msvg_ctx *c, const int32_t *pids, int32_t np, ScanSet **ret) {
     }
     it->second->members.push_back(i);
   }
+  std::sort(set->cls.begin(), set->cls.end(),
+            [](const std::unique_ptr<ScanSet::Cls> &x, const std::unique_ptr<ScanSet::Cls> &y) { return x->Q > y->Q; });
   for (auto &cl : set->cls) {
     std::vector<ScanProf> h(cl->members.size());
     for (size_t k = 0; k < cl->members.size(); ++k) {
@@ -1504,6 +1530,7 @@ int msvg_scan(msvg_ctx *c, int32_t np, const int32_t *pids, const uint8_t *dsq, 
   if (rc != MSVG_OK) return rc;
   hipStream_t st = c->stream;
   const int tjb = msvh_tjb(L);
+  const PassRule pass_rule(L, F1);
   std::vector<int32_t> slow(set->fallback);
   if (L == 0) {
     slow.clear();
@@ -1528,15 +1555,37 @@ int msvg_scan(msvg_ctx *c, int32_t np, const int32_t *pids, const uint8_t *dsq, 
     std::memset(c->h_one_res + L, Kp, Lpad - (size_t) L);
     HIPCHK(c, hipMemcpyAsync(c->d_one_res.p, c->h_one_res, Lpad, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipEventRecord(c->ev[0], st));
+    // The classes' launches are independent, and each lasts at least as long
+    // as one profile's row chain over the query (longer for larger Q): they
+    // go round-robin over the context's extra streams, largest Q first.
+    const int ns = std::min<int>(kVitStreams, (int) set->cls.size());
+    if (ns > 1) {
+      if (!c->vstream[0]) {
+        for (int k = 0; k < kVitStreams; ++k) {
+          HIPCHK(c, hipStreamCreateWithFlags(&c->vstream[k], hipStreamNonBlocking));
+          HIPCHK(c, hipEventCreateWithFlags(&c->vdone[k], hipEventDisableTiming));
+        }
+      }
+      HIPCHK(c, hipEventRecord(c->fork, st));
+      for (int k = 0; k < ns; ++k) HIPCHK(c, hipStreamWaitEvent(c->vstream[k], c->fork, 0));
+    }
     int64_t base = 0;
-    for (auto &cl : set->cls) {
+    for (size_t ci = 0; ci < set->cls.size(); ++ci) {
+      auto &cl = set->cls[ci];
       ScanArgs a{};
       a.profs = cl->d_profs.as<ScanProf>();
       a.res = c->d_one_res.as<uint8_t>();
       a.L = L; a.tjb = tjb; a.R = Kp;
       a.out = c->d_scanout.as<uint32_t>() + base;
-      HIPCHK(c, msvk_launch_scan(cl->Q, &a, (int) cl->members.size(), fast_lds(Kp, cl->Q), st));
+      hipStream_t cs = ns > 1 ? c->vstream[ci % ns] : st;
+      HIPCHK(c, msvk_launch_scan(cl->Q, &a, (int) cl->members.size(), fast_lds(Kp, cl->Q), cs));
       base += (int64_t) cl->members.size();
+    }
+    if (ns > 1) {
+      for (int k = 0; k < ns; ++k) {
+        HIPCHK(c, hipEventRecord(c->vdone[k], c->vstream[k]));
+        HIPCHK(c, hipStreamWaitEvent(st, c->vdone[k], 0));
+      }
     }
     HIPCHK(c, hipEventRecord(c->ev[1], st));
     if (base > 0)
@@ -1559,7 +1608,7 @@ int msvg_scan(msvg_ctx *c, int32_t np, const int32_t *pids, const uint8_t *dsq, 
         } else {
           usc[i] = usc_of((int) (w & 255u) - tjb, p.base, p.scale);
           status[i] = MSVG_OK;
-          pass[i] = passes(usc[i], L, p.mu, p.lambda, F1) ? 1 : 0;
+          pass[i] = pass_rule(usc[i], p.mu, p.lambda) ? 1 : 0;
         }
       }
       base += (int64_t) cl->members.size();

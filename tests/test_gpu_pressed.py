@@ -1,8 +1,9 @@
 """hmmscan's direction on the GPU: real query sequences against the models
 of an hmmpress'ed database (tutorial/minifam.h3f/.h3p read by
 cudahmmsearch_amd.read_pressed), through the batched MSV search (all models
-in one call) and the ViterbiFilter stage, against the oracle -- scores,
-statuses and pass bits bit for bit."""
+in one call), the scan entry point (all models against one query per call)
+and the ViterbiFilter stage, against the oracle -- scores, statuses and pass
+bits bit for bit."""
 import gzip
 import os
 import shutil
@@ -78,4 +79,10 @@ def test_scan_queries_against_pressed_models(tmp_path):
             assert vst[i] == s and np.float32(vsc[i]).view(np.uint32) == np.float32(sc).view(np.uint32), (name, i)
     # the known answer of tutorial/fn3.out: 7LESS_DROME passes fn3's MSV filter
     assert ps[[m[0] for m in models].index("fn3")][0] == 1
+    # msvg_scan, one query at a time over all models (hmmscan's loop), gives
+    # the same scores, statuses and pass bits
+    for i, x in enumerate(queries):
+        su, ss, sp = ctx.scan(pids, x)
+        assert np.array_equal(su.view(np.uint32), usc[:, i].view(np.uint32)), i
+        assert np.array_equal(ss, st[:, i]) and np.array_equal(sp, ps[:, i]), i
     ctx.close()

@@ -126,6 +126,7 @@ struct Profile {
 
 struct ScanSet {                   // the profiles of msvg_scan calls, grouped by register class
   std::vector<int32_t> pids;
+  std::vector<Profile *> prof;     // per position (valid while the set lives: removal resets it)
   int Kp = 0;
   struct Cls {
     int Q = 0;
@@ -1456,7 +1457,15 @@ int scan_prepare(msvg_ctx *c, const int32_t *pids, int32_t np, ScanSet **ret) {
   c->scanset.reset();
   auto set = std::make_unique<ScanSet>();
   set->pids.assign(pids, pids + np);
-  set->Kp = c->profiles[pids[0]]->Kp;
+  set->prof.resize((size_t) np);
+  for (int32_t i = 0; i < np; ++i) {
+    auto it = c->profiles.find(pids[i]);
+    if (it == c->profiles.end()) return fail(c, MSVG_EINVAL, "unknown profile id");
+    set->prof[(size_t) i] = it->second.get();
+  }
+  set->Kp = set->prof[0]->Kp;
+  for (int32_t i = 1; i < np; ++i)
+    if (set->prof[(size_t) i]->Kp != set->Kp) return fail(c, MSVG_EINVAL, "scanned profiles must share one alphabet");
   std::map<int, ScanSet::Cls *> by_q;
   hipStream_t st = c->stream;
   if (!c->d_idrow.p) {
@@ -1466,7 +1475,7 @@ int scan_prepare(msvg_ctx *c, const int32_t *pids, int32_t np, ScanSet **ret) {
     HIPCHK(c, hipMemcpy(c->d_idrow.p, id, 256, hipMemcpyHostToDevice));
   }
   for (int32_t i = 0; i < np; ++i) {
-    Profile &p = *c->profiles[pids[i]];
+    Profile &p = *set->prof[(size_t) i];
     set->sumM += p.M;
     const int Q = scan_class(qfor(p.M, 128));
     if (Q == 0 || fast_lds(p.Kp, Q) > (size_t) kLdsBytes || p.Kp > 254) { set->fallback.push_back(i); continue; }
@@ -1489,7 +1498,7 @@ int scan_prepare(msvg_ctx *c, const int32_t *pids, int32_t np, ScanSet **ret) {
   for (auto &cl : set->cls) {
     std::vector<ScanProf> h(cl->members.size());
     for (size_t k = 0; k < cl->members.size(); ++k) {
-      const Profile &p = *c->profiles[pids[cl->members[k]]];
+      const Profile &p = *set->prof[(size_t) cl->members[k]];
       ScanProf &sp = h[k];
       std::memset(&sp, 0, sizeof sp);
       sp.tab = p.d_scantab.as<uint4>();
@@ -1517,17 +1526,13 @@ int msvg_scan(msvg_ctx *c, int32_t np, const int32_t *pids, const uint8_t *dsq, 
     return fail(c, MSVG_EINVAL, "null argument");
   c->stats = msvg_stats{};
   if (np == 0) return MSVG_OK;
-  for (int32_t i = 0; i < np; ++i)
-    if (!c->profiles.count(pids[i])) return fail(c, MSVG_EINVAL, "unknown profile id");
-  const int Kp = c->profiles[pids[0]]->Kp;
-  for (int32_t i = 1; i < np; ++i)
-    if (c->profiles[pids[i]]->Kp != Kp) return fail(c, MSVG_EINVAL, "scanned profiles must share one alphabet");
-  for (int i = 1; i <= L; ++i)
-    if (dsq[i] >= Kp) return fail(c, MSVG_EINVAL, "query holds residue codes outside the profiles' alphabet");
   (void) hipSetDevice(c->device);
   ScanSet *set;
-  int rc = scan_prepare(c, pids, np, &set);
+  int rc = scan_prepare(c, pids, np, &set);        // validates ids and alphabet when it builds
   if (rc != MSVG_OK) return rc;
+  const int Kp = set->Kp;
+  for (int i = 1; i <= L; ++i)
+    if (dsq[i] >= Kp) return fail(c, MSVG_EINVAL, "query holds residue codes outside the profiles' alphabet");
   hipStream_t st = c->stream;
   const int tjb = msvh_tjb(L);
   const PassRule pass_rule(L, F1);
@@ -1601,7 +1606,7 @@ int msvg_scan(msvg_ctx *c, int32_t np, const int32_t *pids, const uint8_t *dsq, 
         const int32_t i = cl->members[k];
         const uint32_t w = c->h_scanout[base + (int64_t) k];
         const uint32_t code = w >> 8;
-        const Profile &p = *c->profiles[pids[i]];
+        const Profile &p = *set->prof[(size_t) i];
         if (code == RES_SLOW) { slow.push_back(i); continue; }
         if (code == RES_ERANGE) {
           usc[i] = std::numeric_limits<float>::infinity(); status[i] = MSVG_ERANGE; pass[i] = 1;

@@ -428,3 +428,29 @@ def test_scan_dna_and_mixed_alphabets(ctx, golden):
         ctx.scan([pd_, pa], np.array([1, 2], np.uint8))
     ctx.remove_profile(pd_)
     ctx.remove_profile(pa)
+
+
+def test_scan_guard_regime_fallback(ctx):
+    """In the guard regime a query that keeps every cell at the floor needs
+    the exact byte MSV: msvg_scan hands that profile to the search path
+    (n_slow), the other profiles stay on the kernel, and all results equal
+    msvg_search's."""
+    pa = ca.sample_profile(11, 2500)
+    pb = ca.sample_profile(12, 100)
+    L = 100_000
+    assert ca.tjb(L) + pa.tbm + pa.tec + pa.bias >= 127 and (pa.rbv[27] >= pa.bias).all()
+    pids = [ctx.add_profile(pa), ctx.add_profile(pb)]
+    stars = np.full(L, 27, np.uint8)
+    island = stars.copy()
+    island[50_000:50_300] = np.random.default_rng(5).integers(0, 20, 300)
+    for q, nslow in ((stars, 1), (island, 0)):
+        print(f"guard-regime scan: L={L} expect {nslow} on the search path", flush=True)
+        u, s, ps = ctx.scan(pids, q)
+        assert ctx.stats()["n_slow"] == nslow
+        did = ctx.add_database(ca.Database.from_list([q]))
+        ru, rs, rp = ctx.search(pids, did, mode=ca.MODE_PLAIN)
+        ctx.remove_database(did)
+        assert np.array_equal(u.view(np.uint32), ru[:, 0].view(np.uint32))
+        assert np.array_equal(s, rs[:, 0]) and np.array_equal(ps, rp[:, 0])
+    for pid in pids:
+        ctx.remove_profile(pid)

@@ -238,7 +238,8 @@ def scan_workload(ca, args):
     250, sigma 0.6, clipped 30..2560, seeds 100000+i) against query sequences
     (log-normal median 350, sigma 0.7, clipped 20..10000, seed 6), one
     msvg_scan call per query (host to host: query upload, launches, results,
-    the F1 rule).  Also: the search path on the same pairs (msvg_search over
+    the F1 rule), then all queries as one msvg_scan_many batch.  Also: the
+    search path on the same pairs (msvg_search over
     a one-target database per query) and the reference's SSE p7_MSVFilter on
     a sample of the profiles."""
     nprof, nq = args.scan_profiles, args.scan_queries
@@ -262,6 +263,15 @@ def scan_workload(ca, args):
         res.append(ctx.scan(pids, q))
         ms_k += ctx.stats()["ms_fast"]
     wall = time.perf_counter() - t0
+    # the same queries as one msvg_scan_many batch: each profile's table is
+    # staged once per batch and its workgroup's four waves share the queries
+    ctx.scan_many(pids, qdb)
+    t0 = time.perf_counter()
+    bu, bst, bps = ctx.scan_many(pids, qdb)
+    wall_b = time.perf_counter() - t0
+    ms_b = ctx.stats()["ms_fast"]
+    same_b = all(np.array_equal(bu[k].view(np.uint32), res[k][0].view(np.uint32)) and np.array_equal(bst[k], res[k][1])
+                 and np.array_equal(bps[k], res[k][2]) for k in range(nq))
     # the search path on the first queries, identity checked
     nsp = min(2, nq)
     same = True
@@ -283,7 +293,10 @@ def scan_workload(ca, args):
            "queries_per_s": round(nq / wall, 1), "kernel_ms_per_query": round(ms_k / nq, 3),
            "kernel_gcups": round(cells / (ms_k / 1e3) / 1e9, 1),
            "search_path_ms_per_query": round(search_s * 1e3, 1),
-           "scan_over_search": round(search_s / (wall / nq), 1), "identical_to_search": bool(same)}
+           "scan_over_search": round(search_s / (wall / nq), 1), "identical_to_search": bool(same),
+           "batch": {"queries": nq, "queries_per_s": round(nq / wall_b, 1), "gcups": round(cells / wall_b / 1e9, 1),
+                     "kernel_ms": round(ms_b, 3), "kernel_gcups": round(cells / (ms_b / 1e3) / 1e9, 1),
+                     "identical_to_single": bool(same_b)}}
     if not args.no_cpu:
         import oracle
         try:
@@ -308,7 +321,8 @@ def scan_workload(ca, args):
         except (FileNotFoundError, OSError):
             out["cpu_baseline"] = None
     log(f"scan: {out['queries_per_s']} queries/s, {out['value']} GCUPS host to host, kernels {out['kernel_gcups']} GCUPS; "
-        f"search path {out['search_path_ms_per_query']} ms per query")
+        f"search path {out['search_path_ms_per_query']} ms per query; one batch of {nq}: "
+        f"{out['batch']['queries_per_s']} queries/s, kernels {out['batch']['kernel_gcups']} GCUPS")
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -116,6 +116,7 @@ def load_library(path=LIB_PATH):
         "msvg_vit_search": (C.c_int, [vp, i32, i32, vp, i64, vp, vp]),
         "msvg_vit_search_many": (C.c_int, [vp, i32, vp, i32, vp, vp, vp, vp]),
         "msvg_scan": (C.c_int, [vp, i32, vp, vp, C.c_int, C.c_double, vp, vp, vp]),
+        "msvg_scan_many": (C.c_int, [vp, i32, vp, i32, vp, vp, vp, C.c_double, vp, vp, vp]),
         "msvg_ViterbiFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_calibrate_vmu": (C.c_int, [vp, i32, C.c_double, C.POINTER(C.c_float)]),
         "msvh_vit_move": (C.c_int, [C.c_float, C.c_float, C.c_int]),
@@ -588,6 +589,24 @@ class MSVContext:
         ps = np.zeros(n, np.uint8)
         self._chk(self.lib.msvg_scan(self._c, pids.size, _p(pids), _p(dsq), len(x), float(F1), _p(usc), _p(st), _p(ps)))
         return usc[:pids.size], st[:pids.size], ps[:pids.size]
+
+    def scan_many(self, pids, queries, F1=0.02):
+        """scan() for a batch of queries (a Database or a list of code
+        arrays): (usc, status, pass), each [query][profile]."""
+        pids = np.ascontiguousarray(pids, dtype=np.int32)
+        db = queries if isinstance(queries, Database) else Database.from_list(list(queries))
+        res = db.res if db.res.size else np.zeros(1, np.uint8)
+        off = np.ascontiguousarray(db.off, np.int64)
+        lens = np.ascontiguousarray(db.lens, np.int32)
+        n = max(db.n * pids.size, 1)
+        usc = np.zeros(n, np.float32)
+        st = np.zeros(n, np.uint8)
+        ps = np.zeros(n, np.uint8)
+        self._chk(self.lib.msvg_scan_many(self._c, pids.size, _p(pids), db.n, _p(res), _p(off), _p(lens), float(F1),
+                                          _p(usc), _p(st), _p(ps)))
+        shape = (db.n, pids.size)
+        k = db.n * pids.size
+        return usc[:k].reshape(shape), st[:k].reshape(shape), ps[:k].reshape(shape)
 
     def vit_search_many(self, pids, did, target_lists):
         """vit_search for several profiles in one call (launches overlap on

@@ -97,11 +97,15 @@ struct ScanProf {              // one profile of a scan set (many profiles, one 
   ProfDev prof;
 };
 
-struct ScanArgs {              // one workgroup per profile
+struct ScanArgs {              // one workgroup per profile, its waves over the queries
   const ScanProf *profs;
-  const uint8_t *res;          // the query's codes, padded to 16 bytes with R
-  int32_t L, tjb, R;
-  uint32_t *out;               // result word per profile
+  const uint8_t *res;          // the queries' codes, each padded to 16 bytes with R
+  const int64_t *qoff;         // query q starts at res + qoff[q]
+  const int32_t *qlen;
+  const uint8_t *qtjb;         // tjb_b(L) per query
+  int32_t nq, R;
+  int32_t stride;              // result of (profile b, query q) at out[q * stride + b]
+  uint32_t *out;
 };
 
 struct SlowArgs {

@@ -546,10 +546,11 @@ __global__ void __launch_bounds__(1024) msv_one(OneArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Scan direction (hmmscan: many profiles against one query): the one-target
-// composite above with one workgroup per profile.  Its four waves stage the
-// profile's table in LDS (four 16-byte reads in flight per lane), three of
-// them leave, and the first computes the query's rows.  Workgroups are
+// Scan direction (hmmscan: many profiles against one query, or a batch of
+// queries): the one-target composite above with one workgroup per profile.
+// Its four waves stage the profile's table in LDS (four 16-byte reads in
+// flight per lane), then take the queries in turn (wave w: queries w, w+4,
+// ...); with one query three of them leave at once.  Workgroups are
 // independent, so the hardware keeps as many resident per CU as their LDS
 // allows (five for tables of 30 KiB) and starts the next as one ends.
 // ---------------------------------------------------------------------------
@@ -567,16 +568,19 @@ __global__ void __launch_bounds__(256) msv_scan(ScanArgs a) {
   }
   for (; i < ntab; i += 256) lds[i] = tab[i];
   __syncthreads();
-  if (threadIdx.x >= 64) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   const ProfDev pd = sp->prof;
-  int xEmax;
-  uint32_t w = jrow_task<Q>(lds + lane, a.res, a.L, a.tjb, pd, lane, xEmax);
-  if ((w >> 8) != RES_ERANGE && xEmax - pd.tec <= pd.base) {
-    const int tjbm = (a.tjb + pd.tbm) & 255;
-    w = decide(xEmax - max(0, pd.base - tjbm), a.tjb, pd);
+  for (int q = wave; q < a.nq; q += 4) {          // waves without a query leave at once
+    const int tjb = a.qtjb[q];
+    int xEmax;
+    uint32_t w = jrow_task<Q>(lds + lane, a.res + a.qoff[q], a.qlen[q], tjb, pd, lane, xEmax);
+    if ((w >> 8) != RES_ERANGE && xEmax - pd.tec <= pd.base) {
+      const int tjbm = (tjb + pd.tbm) & 255;
+      w = decide(xEmax - max(0, pd.base - tjbm), tjb, pd);
+    }
+    if (lane == 0) a.out[(int64_t) q * a.stride + blockIdx.x] = w;
   }
-  if (lane == 0) a.out[blockIdx.x] = w;
 }
 
 // ---------------------------------------------------------------------------

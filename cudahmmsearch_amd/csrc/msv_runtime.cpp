@@ -1522,46 +1522,89 @@ extern "C" {
 int msvg_scan(msvg_ctx *c, int32_t np, const int32_t *pids, const uint8_t *dsq, int L, double F1, float *usc,
               uint8_t *status, uint8_t *pass) {
   if (!c) return MSVG_EINVAL;
-  if (np < 0 || L < 0 || (np > 0 && (!pids || !dsq || !usc || !status || !pass)))
+  if (L < 0 || (np > 0 && !dsq)) return fail(c, MSVG_EINVAL, "null argument");
+  const int64_t off = 0;
+  const int32_t len = L;
+  static const uint8_t none = 0;
+  return msvg_scan_many(c, np, pids, 1, L > 0 ? dsq + 1 : &none, &off, &len, F1, usc, status, pass);
+}
+
+int msvg_scan_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t nq, const uint8_t *res, const int64_t *qoff,
+                   const int32_t *qlen, double F1, float *usc, uint8_t *status, uint8_t *pass) {
+  if (!c) return MSVG_EINVAL;
+  if (np < 0 || nq < 0 || (np > 0 && nq > 0 && (!pids || !res || !qoff || !qlen || !usc || !status || !pass)))
     return fail(c, MSVG_EINVAL, "null argument");
   c->stats = msvg_stats{};
-  if (np == 0) return MSVG_OK;
+  if (np == 0 || nq == 0) return MSVG_OK;
+  for (int32_t q = 0; q < nq; ++q)
+    if (qlen[q] < 0 || qoff[q] < 0) return fail(c, MSVG_EINVAL, "negative query offset or length");
   (void) hipSetDevice(c->device);
   ScanSet *set;
   int rc = scan_prepare(c, pids, np, &set);        // validates ids and alphabet when it builds
   if (rc != MSVG_OK) return rc;
   const int Kp = set->Kp;
-  for (int i = 1; i <= L; ++i)
-    if (dsq[i] >= Kp) return fail(c, MSVG_EINVAL, "query holds residue codes outside the profiles' alphabet");
+  int64_t padded = 0, residues = 0;
+  for (int32_t q = 0; q < nq; ++q) {
+    for (int32_t i = 0; i < qlen[q]; ++i)
+      if (res[qoff[q] + i] >= Kp) return fail(c, MSVG_EINVAL, "query holds residue codes outside the profiles' alphabet");
+    padded += ((int64_t) qlen[q] + 15) & ~(int64_t) 15;
+    residues += qlen[q];
+  }
   hipStream_t st = c->stream;
-  const int tjb = msvh_tjb(L);
-  const PassRule pass_rule(L, F1);
-  std::vector<int32_t> slow(set->fallback);
-  if (L == 0) {
-    slow.clear();
-    for (int32_t i = 0; i < np; ++i) slow.push_back(i);
-  } else {
-    // stage the query (padded to 16 bytes with the null row Kp)
-    const size_t Lpad = ((size_t) L + 15) & ~(size_t) 15;
-    if (c->h_one_cap < Lpad) {
+  // (profile position, query) pairs the kernel leaves: all of an empty
+  // query, the profiles whose table does not fit LDS, and RES_SLOW words
+  std::vector<std::pair<int32_t, int32_t>> slow;
+  std::vector<int32_t> qlive;                       // queries with residues
+  for (int32_t q = 0; q < nq; ++q) {
+    if (qlen[q] == 0) {
+      for (int32_t i = 0; i < np; ++i) slow.emplace_back(i, q);
+    } else {
+      qlive.push_back(q);
+      for (int32_t i : set->fallback) slow.emplace_back(i, q);
+    }
+  }
+  const int32_t nl = (int32_t) qlive.size();
+  if (nl > 0) {
+    // stage the live queries: codes padded to 16 bytes with the null row Kp,
+    // then offsets, lengths and tjb_b, in one pinned buffer
+    const size_t meta = (size_t) nl * (sizeof(int64_t) + sizeof(int32_t) + 1);
+    const size_t need = (size_t) padded + meta + 64;
+    if (c->h_one_cap < need) {
       if (c->h_one_res) { (void) hipHostFree(c->h_one_res); c->h_one_res = nullptr; c->h_one_cap = 0; }
-      const size_t cap = std::max<size_t>(Lpad, 4096);
+      const size_t cap = std::max<size_t>(need, 4096);
       HIPCHK(c, hipHostMalloc((void **) &c->h_one_res, cap, hipHostMallocDefault));
       c->h_one_cap = cap;
     }
-    if (c->h_scancap < (size_t) np) {
+    const size_t nout = (size_t) np * nl;
+    if (c->h_scancap < nout) {
       if (c->h_scanout) { (void) hipHostFree(c->h_scanout); c->h_scanout = nullptr; c->h_scancap = 0; }
-      HIPCHK(c, hipHostMalloc((void **) &c->h_scanout, (size_t) np * sizeof(uint32_t), hipHostMallocDefault));
-      c->h_scancap = (size_t) np;
+      HIPCHK(c, hipHostMalloc((void **) &c->h_scanout, nout * sizeof(uint32_t), hipHostMallocDefault));
+      c->h_scancap = nout;
     }
-    HIPCHK(c, c->d_one_res.ensure(std::max<size_t>(Lpad, 4096)));
-    HIPCHK(c, c->d_scanout.ensure((size_t) np * sizeof(uint32_t)));
-    std::memcpy(c->h_one_res, dsq + 1, (size_t) L);
-    std::memset(c->h_one_res + L, Kp, Lpad - (size_t) L);
-    HIPCHK(c, hipMemcpyAsync(c->d_one_res.p, c->h_one_res, Lpad, hipMemcpyHostToDevice, st));
+    uint8_t *h = c->h_one_res;
+    const size_t o_off = ((size_t) padded + 7) & ~(size_t) 7;
+    int64_t *h_off = reinterpret_cast<int64_t *>(h + o_off);
+    int32_t *h_len = reinterpret_cast<int32_t *>(h + o_off + (size_t) nl * sizeof(int64_t));
+    uint8_t *h_tjb = h + o_off + (size_t) nl * (sizeof(int64_t) + sizeof(int32_t));
+    int64_t pos = 0;
+    for (int32_t k = 0; k < nl; ++k) {
+      const int32_t q = qlive[(size_t) k];
+      const int64_t Lq = qlen[q], Lpad = (Lq + 15) & ~(int64_t) 15;
+      std::memcpy(h + pos, res + qoff[q], (size_t) Lq);
+      std::memset(h + pos + Lq, Kp, (size_t) (Lpad - Lq));
+      h_off[k] = pos;
+      h_len[k] = (int32_t) Lq;
+      h_tjb[k] = (uint8_t) msvh_tjb((int) Lq);
+      pos += Lpad;
+    }
+    const size_t total = o_off + meta;
+    HIPCHK(c, c->d_one_res.ensure(std::max<size_t>(total, 4096)));
+    HIPCHK(c, c->d_scanout.ensure(nout * sizeof(uint32_t)));
+    HIPCHK(c, hipMemcpyAsync(c->d_one_res.p, h, total, hipMemcpyHostToDevice, st));
+    uint8_t *d = c->d_one_res.as<uint8_t>();
     HIPCHK(c, hipEventRecord(c->ev[0], st));
     // The classes' launches are independent, and each lasts at least as long
-    // as one profile's row chain over the query (longer for larger Q): they
+    // as one profile's row chain over a query (longer for larger Q): they
     // go round-robin over the context's extra streams, largest Q first.
     const int ns = std::min<int>(kVitStreams, (int) set->cls.size());
     if (ns > 1) {
@@ -1579,8 +1622,12 @@ int msvg_scan(msvg_ctx *c, int32_t np, const int32_t *pids, const uint8_t *dsq, 
       auto &cl = set->cls[ci];
       ScanArgs a{};
       a.profs = cl->d_profs.as<ScanProf>();
-      a.res = c->d_one_res.as<uint8_t>();
-      a.L = L; a.tjb = tjb; a.R = Kp;
+      a.res = d;
+      a.qoff = reinterpret_cast<const int64_t *>(d + o_off);
+      a.qlen = reinterpret_cast<const int32_t *>(d + o_off + (size_t) nl * sizeof(int64_t));
+      a.qtjb = d + o_off + (size_t) nl * (sizeof(int64_t) + sizeof(int32_t));
+      a.nq = nl; a.R = Kp;
+      a.stride = np;
       a.out = c->d_scanout.as<uint32_t>() + base;
       hipStream_t cs = ns > 1 ? c->vstream[ci % ns] : st;
       HIPCHK(c, msvk_launch_scan(cl->Q, &a, (int) cl->members.size(), fast_lds(Kp, cl->Q), cs));
@@ -1593,55 +1640,74 @@ int msvg_scan(msvg_ctx *c, int32_t np, const int32_t *pids, const uint8_t *dsq, 
       }
     }
     HIPCHK(c, hipEventRecord(c->ev[1], st));
-    if (base > 0)
-      HIPCHK(c, hipMemcpyAsync(c->h_scanout, c->d_scanout.p, (size_t) base * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(c->h_scanout, c->d_scanout.p, nout * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->stats.ms_fast = ms;
     c->stats.ms_total = ms;
-    base = 0;
-    for (auto &cl : set->cls) {
-      for (size_t k = 0; k < cl->members.size(); ++k) {
-        const int32_t i = cl->members[k];
-        const uint32_t w = c->h_scanout[base + (int64_t) k];
-        const uint32_t code = w >> 8;
-        const Profile &p = *set->prof[(size_t) i];
-        if (code == RES_SLOW) { slow.push_back(i); continue; }
-        if (code == RES_ERANGE) {
-          usc[i] = std::numeric_limits<float>::infinity(); status[i] = MSVG_ERANGE; pass[i] = 1;
-        } else {
-          usc[i] = usc_of((int) (w & 255u) - tjb, p.base, p.scale);
-          status[i] = MSVG_OK;
-          pass[i] = pass_rule(usc[i], p.mu, p.lambda) ? 1 : 0;
+    for (int32_t k = 0; k < nl; ++k) {
+      const int32_t q = qlive[(size_t) k];
+      const int tjb = msvh_tjb(qlen[q]);
+      const PassRule pass_rule(qlen[q], F1);
+      const uint32_t *wq = c->h_scanout + (size_t) k * np;
+      float *uq = usc + (size_t) q * np;
+      uint8_t *sq = status + (size_t) q * np, *pq = pass + (size_t) q * np;
+      base = 0;
+      for (auto &cl : set->cls) {
+        for (size_t m = 0; m < cl->members.size(); ++m) {
+          const int32_t i = cl->members[m];
+          const uint32_t w = wq[base + (int64_t) m];
+          const uint32_t code = w >> 8;
+          const Profile &p = *set->prof[(size_t) i];
+          if (code == RES_SLOW) { slow.emplace_back(i, q); continue; }
+          if (code == RES_ERANGE) {
+            uq[i] = std::numeric_limits<float>::infinity(); sq[i] = MSVG_ERANGE; pq[i] = 1;
+          } else {
+            uq[i] = usc_of((int) (w & 255u) - tjb, p.base, p.scale);
+            sq[i] = MSVG_OK;
+            pq[i] = pass_rule(uq[i], p.mu, p.lambda) ? 1 : 0;
+          }
         }
+        base += (int64_t) cl->members.size();
       }
-      base += (int64_t) cl->members.size();
     }
   }
-  c->stats.cells = set->sumM * (int64_t) L;
+  c->stats.cells = set->sumM * residues;
   c->stats.n_slow = (int64_t) slow.size();
   if (!slow.empty()) {
-    // the search path on a one-target database for what the kernel leaves
-    std::sort(slow.begin(), slow.end());
-    if (c->scratch_db >= 0) msvg_db_remove(c, c->scratch_db);
-    c->scratch_db = -1;
-    const int64_t off = 0;
-    int32_t id;
-    if ((rc = msvg_db_add(c, dsq + 1, &off, &L, 1, &id)) != MSVG_OK) return rc;
-    c->scratch_db = id;
-    std::vector<int32_t> ids(slow.size());
-    for (size_t k = 0; k < slow.size(); ++k) ids[k] = pids[slow[k]];
-    std::vector<float> u(slow.size());
-    std::vector<uint8_t> s2(slow.size()), p2(slow.size());
+    // the search path, query by query on a one-target database, for the
+    // pairs the kernel leaves
+    std::sort(slow.begin(), slow.end(), [](const std::pair<int32_t, int32_t> &x, const std::pair<int32_t, int32_t> &y) {
+      return x.second != y.second ? x.second < y.second : x.first < y.first;
+    });
     const msvg_stats keep = c->stats;
-    if ((rc = msvg_search(c, ids.data(), (int32_t) ids.size(), id, F1, MSVG_MODE_PLAIN, u.data(), s2.data(),
-                          p2.data())) != MSVG_OK)
-      return rc;
-    c->stats = keep;
-    for (size_t k = 0; k < slow.size(); ++k) {
-      usc[slow[k]] = u[k]; status[slow[k]] = s2[k]; pass[slow[k]] = p2[k];
+    for (size_t a0 = 0; a0 < slow.size();) {
+      const int32_t q = slow[a0].second;
+      size_t a1 = a0;
+      while (a1 < slow.size() && slow[a1].second == q) ++a1;
+      if (c->scratch_db >= 0) msvg_db_remove(c, c->scratch_db);
+      c->scratch_db = -1;
+      const int64_t zero = 0;
+      const int32_t Lq = qlen[q];
+      static const uint8_t none = 0;
+      int32_t id;
+      if ((rc = msvg_db_add(c, Lq > 0 ? res + qoff[q] : &none, &zero, &Lq, 1, &id)) != MSVG_OK) return rc;
+      c->scratch_db = id;
+      std::vector<int32_t> ids(a1 - a0);
+      for (size_t k = a0; k < a1; ++k) ids[k - a0] = pids[slow[k].first];
+      std::vector<float> u(ids.size());
+      std::vector<uint8_t> s2(ids.size()), p2(ids.size());
+      if ((rc = msvg_search(c, ids.data(), (int32_t) ids.size(), id, F1, MSVG_MODE_PLAIN, u.data(), s2.data(),
+                            p2.data())) != MSVG_OK)
+        return rc;
+      for (size_t k = a0; k < a1; ++k) {
+        const size_t o = (size_t) q * np + slow[k].first;
+        usc[o] = u[k - a0]; status[o] = s2[k - a0]; pass[o] = p2[k - a0];
+      }
+      a0 = a1;
     }
+    c->stats = keep;
   }
   return MSVG_OK;
 }

@@ -454,3 +454,40 @@ def test_scan_guard_regime_fallback(ctx):
         assert np.array_equal(s, rs[:, 0]) and np.array_equal(ps, rp[:, 0])
     for pid in pids:
         ctx.remove_profile(pid)
+
+
+def test_scan_many_equals_search(ctx):
+    """msvg_scan_many (a batch of queries; each workgroup's waves share its
+    profile's staged table) gives, for every (query, profile), what
+    msvg_search gives over a database of the same queries: ragged lengths,
+    empty queries, planted homologs, and a profile beyond LDS that takes the
+    search path; a batch of one equals msvg_scan."""
+    rng = np.random.default_rng(91)
+    Ms = [1, 9, 64, 130, 257, 513, 1025, 2561] + [int(m) for m in rng.integers(20, 900, 24)]
+    profs = [ca.sample_profile(7000 + i, M) for i, M in enumerate(Ms)]
+    pids = []
+    for p in profs:
+        pid = ctx.add_profile(p)
+        p.mu = ctx.calibrate(pid, p.lam)
+        pids.append(pid)
+    for nq in (1, 3, 4, 5, 13):
+        qs = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in rng.integers(1, 1500, nq)]
+        if nq > 3:
+            qs[1] = np.zeros(0, np.uint8)
+            qs[2] = ca.core_emit(profs[4].hmm, 80 + nq, 1)[0]
+        db = ca.Database.from_list(qs)
+        print(f"scan_many: {nq} queries x {len(pids)} profiles", flush=True)
+        u, s, ps = ctx.scan_many(pids, db)
+        assert u.shape == (nq, len(pids))
+        did = ctx.add_database(db)
+        ru, rs, rp = ctx.search(pids, did, mode=ca.MODE_PLAIN)
+        ctx.remove_database(did)
+        assert np.array_equal(u.view(np.uint32), ru.T.view(np.uint32))
+        assert np.array_equal(s, rs.T) and np.array_equal(ps, rp.T)
+        if nq == 1:
+            u1, s1, p1 = ctx.scan(pids, qs[0])
+            assert np.array_equal(u1.view(np.uint32), u[0].view(np.uint32)) and np.array_equal(p1, ps[0])
+    u, s, ps = ctx.scan_many(pids, [])
+    assert u.shape == (0, len(pids))
+    for pid in pids:
+        ctx.remove_profile(pid)

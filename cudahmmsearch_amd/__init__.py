@@ -38,6 +38,8 @@ LIB_PATH = os.path.join(HERE, "libmsvgpu.so")
 eslOK, eslEMEM, eslEINVAL, eslERANGE, eslENORESULT = 0, 5, 11, 16, 19
 MODE_PLAIN, MODE_BATCHED, FLAG_EAGER = 0, 1, 0x10
 KP = 29
+RNA, DNA, AMINO = 1, 2, 3                 # Easel's alphabet type codes (esl_alphabet.h:12-14)
+_KP_OF = {RNA: 18, DNA: 18, AMINO: 29}
 
 
 class MSVError(RuntimeError):
@@ -56,6 +58,13 @@ class _Stats(C.Structure):
     _fields_ = [("ms_fast", C.c_double), ("ms_slow", C.c_double), ("ms_finalize", C.c_double),
                 ("ms_total", C.c_double), ("cells", C.c_int64), ("n_slow", C.c_int64), ("n_jrow", C.c_int64),
                 ("n_fast_launches", C.c_int32), ("dropin_path", C.c_int32)]
+
+
+class _Hmm(C.Structure):
+    _fields_ = [("M", C.c_int), ("alphatype", C.c_int), ("K", C.c_int), ("Kp", C.c_int), ("mat", C.c_void_p),
+                ("ins", C.c_void_p), ("t", C.c_void_p), ("compo", C.c_void_p), ("evparam", C.c_float * 6),
+                ("has_stats", C.c_int), ("max_length", C.c_int), ("name", C.c_char_p), ("acc", C.c_char_p),
+                ("desc", C.c_char_p)]
 
 
 class _VProf(C.Structure):
@@ -124,6 +133,14 @@ def load_library(path=LIB_PATH):
         "msvh_pressed_next": (C.c_int, [vp, C.POINTER(_Prof), C.POINTER(_VProf), C.c_char_p, C.c_int]),
         "msvh_pressed_close": (None, [vp]),
         "msvh_vit_convert": (C.c_int, [C.c_int, vp, vp, vp, vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "msvh_vit_convert_abc": (C.c_int, [C.c_int, C.c_int, vp, vp, vp, vp, vp, C.POINTER(C.c_float),
+                                           C.POINTER(C.c_float)]),
+        "msvh_profile_convert_abc": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, vp, C.POINTER(C.c_float), vp]),
+        "msvh_lambda_abc": (C.c_double, [C.c_int, C.c_int, vp]),
+        "msvh_hmmfile_open": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "msvh_hmmfile_next": (C.c_int, [vp, C.POINTER(_Hmm)]),
+        "msvh_hmmfile_error": (C.c_char_p, [vp]),
+        "msvh_hmmfile_close": (None, [vp]),
         "msvh_vit_score": (C.c_float, [C.c_int, C.c_int, C.c_int, C.c_float]),
         "msvh_fit_vmu": (C.c_double, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_double]),
         "msvh_vit_calibration_targets": (None, [C.c_int, C.c_int, vp]),
@@ -181,8 +198,10 @@ class ByteProfile:
     ``rbv`` is (29, M) uint8; scalars as in the reference; ``mu``/``lam`` are
     the calibrated MSV Gumbel parameters (``None`` until calibrated)."""
 
-    def __init__(self, rbv, tbm, tec, base, bias, scale, mu=None, lam=None, hmm=None, tjb_config=None):
+    def __init__(self, rbv, tbm, tec, base, bias, scale, mu=None, lam=None, hmm=None, tjb_config=None,
+                 alphabet=AMINO):
         self.rbv = np.ascontiguousarray(rbv, dtype=np.uint8)
+        self.alphabet = alphabet
         self.M = self.rbv.shape[1]
         self.tbm, self.tec, self.base, self.bias = int(tbm), int(tec), int(base), int(bias)
         self.scale = float(scale)
@@ -201,7 +220,7 @@ class ByteProfile:
         if self.hmm is None or self.hmm[2] is None:
             raise MSVError(eslEINVAL, "profile has no HMM transitions to convert")
         mat, ins, t = self.hmm
-        return viterbi_from_hmm(mat, t, vlambda=self.lam)
+        return viterbi_from_hmm(mat, t, vlambda=self.lam, alphabet=self.alphabet)
 
     def _struct(self):
         return _Prof(self.M, self.rbv.shape[0], self.rbv.ctypes.data, self.tbm, self.tec, self.base, self.bias,
@@ -241,19 +260,20 @@ class ViterbiProfile:
                       float(self.vlambda if self.vlambda is not None else -99999.0))
 
 
-def viterbi_from_hmm(mat, t, vlambda=None):
+def viterbi_from_hmm(mat, t, vlambda=None, alphabet=AMINO):
     """vf_conversion (p7_oprofile.c:653-740) of an HMM configured as hmmsearch
-    configures it: the ViterbiProfile for match emissions ``mat`` ((M+1) x 20)
+    configures it: the ViterbiProfile for match emissions ``mat`` ((M+1) x K)
     and transitions ``t`` ((M+1) x 7, p7H order)."""
     lib = load_library()
     mat = np.ascontiguousarray(mat, dtype=np.float32)
     t = np.ascontiguousarray(t, dtype=np.float32)
     M = mat.shape[0] - 1
-    rwv = np.zeros((KP, M), np.int16)
+    rwv = np.zeros((_KP_OF[alphabet], M), np.int16)
     twv = np.zeros((8, M), np.int16)
     sc = np.zeros(4, np.int32)
     scale_w, nj = C.c_float(), C.c_float()
-    rc = lib.msvh_vit_convert(M, _p(mat), _p(t), _p(rwv), _p(twv), _p(sc), C.byref(scale_w), C.byref(nj))
+    rc = lib.msvh_vit_convert_abc(alphabet, M, _p(mat), _p(t), _p(rwv), _p(twv), _p(sc), C.byref(scale_w),
+                                  C.byref(nj))
     if rc != 0:
         raise MSVError(rc, "Viterbi conversion failed")
     return ViterbiProfile(rwv, twv, sc[0], sc[1], sc[2], sc[3], scale_w.value, nj.value, vlambda=vlambda)
@@ -298,21 +318,63 @@ def vit_move(scale_w, nj, L):
     return load_library().msvh_vit_move(float(scale_w), float(nj), int(L))
 
 
-def profile_from_hmm(mat, ins=None, t=None, config_L=100):
+def profile_from_hmm(mat, ins=None, t=None, config_L=100, alphabet=AMINO):
     """p7_ProfileConfig match scores + p7_oprofile_Convert's MSV part for an
-    HMM's match emissions ``mat`` ((M+1) x 20, row 0 unused); lambda as p7_Lambda."""
+    HMM's match emissions ``mat`` ((M+1) x K, row 0 unused; K = 20 amino,
+    4 DNA/RNA); lambda as p7_Lambda."""
     lib = load_library()
     mat = np.ascontiguousarray(mat, dtype=np.float32)
     M = mat.shape[0] - 1
-    rbv = np.zeros((KP, M), np.uint8)
+    rbv = np.zeros((_KP_OF[alphabet], M), np.uint8)
     sc = np.zeros(5, np.int32)
     scale = C.c_float()
-    rc = lib.msvh_profile_convert(M, _p(mat), config_L, _p(rbv), _p(sc), C.byref(scale), None)
+    rc = lib.msvh_profile_convert_abc(alphabet, M, _p(mat), config_L, _p(rbv), _p(sc), C.byref(scale), None)
     if rc != 0:
         raise MSVError(rc, "profile conversion failed")
-    lam = float(np.float32(lib.msvh_lambda(M, _p(mat))))
+    lam = float(np.float32(lib.msvh_lambda_abc(alphabet, M, _p(mat))))
     return ByteProfile(rbv, sc[0], sc[1], sc[3], sc[4], scale.value, lam=lam,
-                       hmm=(mat, ins, t), tjb_config=int(sc[2]))
+                       hmm=(mat, ins, t), tjb_config=int(sc[2]), alphabet=alphabet)
+
+
+def read_hmm(path, config_L=100):
+    """Models of an HMMER3 ASCII file (.hmm, formats 3/a to 3/f; the parser
+    of p7_hmmfile_Read, p7_hmmfile.c:1295-1630): a list of (name,
+    ByteProfile, ViterbiProfile), converted as hmmsearch configures a model
+    (p7_ProfileConfig local multihit at L = config_L, p7_oprofile_Convert),
+    with the file's STATS lines as the E-value parameters (mu None and lambda
+    from p7_Lambda when the file has none).  Each ByteProfile keeps the HMM
+    (mat, ins, t) it came from."""
+    lib = load_library()
+    h = C.c_void_p()
+    rc = lib.msvh_hmmfile_open(path.encode(), C.byref(h))
+    if rc != 0:
+        raise MSVError(rc, f"cannot open {path} as an HMMER3 ASCII file")
+    out = []
+    try:
+        while True:
+            m = _Hmm()
+            rc = lib.msvh_hmmfile_next(h, C.byref(m))
+            err = lib.msvh_hmmfile_error(h).decode()
+            if rc == 3 and not err:
+                break
+            if rc != 0:
+                raise MSVError(rc, f"{path}, model {len(out) + 1}: {err}")
+            M, K = m.M, m.K
+            mat = np.ctypeslib.as_array(C.cast(m.mat, C.POINTER(C.c_float)), ((M + 1) * K,)).reshape(M + 1, K).copy()
+            ins = np.ctypeslib.as_array(C.cast(m.ins, C.POINTER(C.c_float)), ((M + 1) * K,)).reshape(M + 1, K).copy()
+            t = np.ctypeslib.as_array(C.cast(m.t, C.POINTER(C.c_float)), ((M + 1) * 7,)).reshape(M + 1, 7).copy()
+            prof = profile_from_hmm(mat, ins, t, config_L=config_L, alphabet=m.alphatype)
+            ev = [float(x) for x in m.evparam]
+            vit = viterbi_from_hmm(mat, t, alphabet=m.alphatype)
+            if m.has_stats:
+                prof.mu, prof.lam = ev[0], ev[1]
+                vit.vmu, vit.vlambda = ev[2], ev[3]
+            else:
+                prof.mu, vit.vlambda = None, prof.lam
+            out.append((m.name.decode(), prof, vit))
+    finally:
+        lib.msvh_hmmfile_close(h)
+    return out
 
 
 def match_scores(mat, config_L=100):

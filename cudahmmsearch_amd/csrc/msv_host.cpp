@@ -86,6 +86,41 @@ const float kAmino[20] = {                       // p7_AminoFrequencies, hmmer.c
   0.0590092f, 0.0594422f, 0.0963728f, 0.0237718f, 0.0414386f, 0.0482904f, 0.0395639f,
   0.0540978f, 0.0683364f, 0.0540687f, 0.0673417f, 0.0114135f, 0.0304133f};
 
+// The digital alphabets HMMER profiles use (esl_alphabet.c:180-300):
+// K canonical residues, Kp codes in all, background frequencies as
+// p7_bg_Create sets them (p7_bg.c:55-82: p7_AminoFrequencies for amino,
+// uniform 1/K otherwise), and for each degenerate code K+1 .. Kp-3 the set
+// of residues it stands for.
+struct Alphabet {
+  int K = 0, Kp = 0;
+  float f[20] = {};
+  uint8_t degen[MSVH_KP][20] = {};
+};
+
+bool alphabet_of(int type, Alphabet &a) {
+  auto set = [&a](int x, const char *members, const char *order) {
+    for (const char *m = members; *m; ++m) a.degen[x][std::strchr(order, *m) - order] = 1;
+  };
+  if (type == MSVH_AMINO) {
+    static const char order[] = "ACDEFGHIKLMNPQRSTVWY";
+    a.K = 20; a.Kp = 29;
+    std::memcpy(a.f, kAmino, sizeof kAmino);
+    set(21, "ND", order); set(22, "IL", order); set(23, "QE", order);    // B, J, Z
+    set(24, "K", order);  set(25, "C", order);                            // O, U
+    set(26, order, order);                                                // X
+    return true;
+  }
+  if (type == MSVH_DNA || type == MSVH_RNA) {
+    static const char order[] = "ACGT";                   // U takes T's place in RNA
+    a.K = 4; a.Kp = 18;
+    for (int x = 0; x < 4; x++) a.f[x] = 1. / (float) 4;
+    static const char *const d[11] = {"AG", "CT", "AC", "GT", "CG", "AT", "ACT", "CGT", "ACG", "AGT", "ACGT"};
+    for (int i = 0; i < 11; i++) set(5 + i, d[i], order);                // R Y M K S W H B V D N
+    return true;
+  }
+  return false;
+}
+
 // unbiased_byteify (impl_sse/p7_oprofile.c:512-519).
 uint8_t unbiased_byteify(float scale_b, float sc) {
   sc = -1.0f * roundf(scale_b * sc);
@@ -157,44 +192,40 @@ int msvh_hmm_sample(msvh_rng *r, int M, float *mat, float *ins, float *t) {   //
   return 0;
 }
 
-int msvh_profile_convert(int M, const float *mat, int config_L, uint8_t *rbv, int *scalars, float *scale,
-                         float *msc_out) {
-  if (M < 1) return 11;
-  const float *f = kAmino;
-  // Match emission scores, modelconfig.c:138-150 with esl_abc_FExpectScVec
-  // (esl_alphabet.c:1399-1413, :1507-1513) for B, J, Z, O, U, X.
-  static const int degen[6][20] = {
-      /* B = N,D */ {0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0},
-      /* J = I,L */ {0, 0, 0, 0, 0, 0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
-      /* Z = Q,E */ {0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0},
-      /* O = K   */ {0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
-      /* U = C   */ {0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
-      /* X = all */ {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}};
-  std::vector<float> msc((size_t) MSVH_KP * M);
+int msvh_profile_convert_abc(int alphatype, int M, const float *mat, int config_L, uint8_t *rbv, int *scalars,
+                             float *scale, float *msc_out) {
+  Alphabet ab;
+  if (M < 1 || !alphabet_of(alphatype, ab)) return 11;
+  const int K = ab.K, Kp = ab.Kp;
+  const float *f = ab.f;
+  // Match emission scores, modelconfig.c:138-150: log(p / f) for the
+  // residues, esl_abc_FExpectScVec (esl_alphabet.c:1399-1413, :1507-1513)
+  // for the degenerate codes, -inf for gap, nonresidue and missing data.
+  std::vector<float> msc((size_t) Kp * M);
   float sc[MSVH_KP];
   for (int k = 1; k <= M; k++) {
-    for (int x = 0; x < 20; x++) sc[x] = log(mat[k * 20 + x] / f[x]);
-    sc[20] = -INFINITY;
-    sc[27] = -INFINITY;
-    sc[28] = -INFINITY;
-    for (int d = 0; d < 6; d++) {
+    for (int x = 0; x < K; x++) sc[x] = log(mat[k * K + x] / f[x]);
+    sc[K] = -INFINITY;
+    sc[Kp - 2] = -INFINITY;
+    sc[Kp - 1] = -INFINITY;
+    for (int d = K + 1; d <= Kp - 3; d++) {
       float result = 0., denom = 0.;
-      for (int i = 0; i < 20; i++)
-        if (degen[d][i]) { result += sc[i] * f[i]; denom += f[i]; }
+      for (int i = 0; i < K; i++)
+        if (ab.degen[d][i]) { result += sc[i] * f[i]; denom += f[i]; }
       result /= denom;
-      sc[21 + d] = result;
+      sc[d] = result;
     }
-    for (int x = 0; x < MSVH_KP; x++) msc[(size_t) x * M + (k - 1)] = sc[x];
+    for (int x = 0; x < Kp; x++) msc[(size_t) x * M + (k - 1)] = sc[x];
   }
   // mf_conversion, p7_oprofile.c:602-640.  The maximum runs over the real
   // residues' match and insert scores; inserts are 0 for k < M.
   float max = 0.0;
-  for (int x = 0; x < 20; x++)
+  for (int x = 0; x < K; x++)
     for (int k = 1; k <= M; k++) max = std::max(max, msc[(size_t) x * M + (k - 1)]);
   const float scale_b = 3.0 / kLog2;
   const int base_b = 190;
   const int bias_b = unbiased_byteify(scale_b, -1.0 * max);
-  for (int x = 0; x < MSVH_KP; x++)
+  for (int x = 0; x < Kp; x++)
     for (int k = 1; k <= M; k++)
       rbv[(size_t) x * M + (k - 1)] = biased_byteify(scale_b, bias_b, msc[(size_t) x * M + (k - 1)]);
   scalars[0] = unbiased_byteify(scale_b, logf(2.0f / ((float) M * (float) (M + 1))));   // tbm_b
@@ -207,6 +238,11 @@ int msvh_profile_convert(int M, const float *mat, int config_L, uint8_t *rbv, in
   return 0;
 }
 
+int msvh_profile_convert(int M, const float *mat, int config_L, uint8_t *rbv, int *scalars, float *scale,
+                         float *msc_out) {
+  return msvh_profile_convert_abc(MSVH_AMINO, M, mat, config_L, rbv, scalars, scale, msc_out);
+}
+
 namespace {
 // wordify(), p7_oprofile.c:527-534
 int16_t wordify(float scale_w, float sc) {
@@ -217,14 +253,16 @@ int16_t wordify(float scale_w, float sc) {
 }
 }  // namespace
 
-int msvh_vit_convert(int M, const float *mat, const float *t, int16_t *rwv, int16_t *twv, int *scalars,
-                     float *scale_w_out, float *nj_out) {
-  if (M < 1) return 11;
-  std::vector<float> msc((size_t) MSVH_KP * M);
-  std::vector<uint8_t> rbv((size_t) MSVH_KP * M);
+int msvh_vit_convert_abc(int alphatype, int M, const float *mat, const float *t, int16_t *rwv, int16_t *twv,
+                         int *scalars, float *scale_w_out, float *nj_out) {
+  Alphabet ab;
+  if (M < 1 || !alphabet_of(alphatype, ab)) return 11;
+  const int Kp = ab.Kp;
+  std::vector<float> msc((size_t) Kp * M);
+  std::vector<uint8_t> rbv((size_t) Kp * M);
   int sc5[5];
   float scale_b;
-  int rc = msvh_profile_convert(M, mat, 100, rbv.data(), sc5, &scale_b, msc.data());
+  int rc = msvh_profile_convert_abc(alphatype, M, mat, 100, rbv.data(), sc5, &scale_b, msc.data());
   if (rc != 0) return rc;
   enum { MM = 0, IM = 1, DM = 2, BM = 3, MD = 4, MI = 5, II = 6, DD = 7 };   // p7P_* (hmmer.h:222-231)
   enum { hMM = 0, hMI = 1, hMD = 2, hIM = 3, hII = 4, hDM = 5, hDD = 6 };   // p7H_*
@@ -254,7 +292,7 @@ int msvh_vit_convert(int M, const float *mat, const float *t, int16_t *rwv, int1
   auto TSC = [&](int k, int s) { return tsc[(size_t) k * 8 + s]; };
   // vf_conversion, p7_oprofile.c:653-740, in model order.
   const float scale_w = 500.0 / kLog2;                                // :676
-  for (int x = 0; x < MSVH_KP; x++)                                   // :680-685
+  for (int x = 0; x < Kp; x++)                                        // :680-685
     for (int k = 1; k <= M; k++) rwv[(size_t) x * M + k - 1] = wordify(scale_w, msc[(size_t) x * M + k - 1]);
   // The seven per-node transitions (:688-707): BM, MM, IM, DM from node
   // k-1 (rotated), MD, MI, II from node k; capped at 0 (II at -1).
@@ -286,18 +324,26 @@ int msvh_vit_convert(int M, const float *mat, const float *t, int16_t *rwv, int1
   return 0;
 }
 
-double msvh_lambda(int M, const float *mat) {
+int msvh_vit_convert(int M, const float *mat, const float *t, int16_t *rwv, int16_t *twv, int *scalars,
+                     float *scale_w_out, float *nj_out) {
+  return msvh_vit_convert_abc(MSVH_AMINO, M, mat, t, rwv, twv, scalars, scale_w_out, nj_out);
+}
+
+double msvh_lambda_abc(int alphatype, int M, const float *mat) {
   // p7_Lambda (evalues.c:186-192) with p7_MeanMatchRelativeEntropy
   // (modelstats.c:81-98) and esl_vec_FRelEntropy (esl_vectorops.c:950-962).
+  Alphabet ab;
+  if (M < 1 || !alphabet_of(alphatype, ab)) return NAN;
+  const int K = ab.K;
   double KL = 0.;
   for (int k = 1; k <= M; k++) {
-    const float *p = mat + k * 20;
+    const float *p = mat + k * K;
     float kl = 0.;
     bool inf = false;
-    for (int i = 0; i < 20; i++)
+    for (int i = 0; i < K; i++)
       if (p[i] > 0.) {
-        if (kAmino[i] == 0.) { inf = true; break; }
-        kl += p[i] * log(p[i] / kAmino[i]);
+        if (ab.f[i] == 0.) { inf = true; break; }
+        kl += p[i] * log(p[i] / ab.f[i]);
       }
     const float rel = inf ? INFINITY : (float) (1.44269504 * kl);
     KL += rel;
@@ -305,6 +351,8 @@ double msvh_lambda(int M, const float *mat) {
   KL /= (double) M;
   return kLog2 + 1.44 / ((double) M * KL);
 }
+
+double msvh_lambda(int M, const float *mat) { return msvh_lambda_abc(MSVH_AMINO, M, mat); }
 
 int msvh_core_emit(msvh_rng *r, int M, const float *mat, const float *ins, const float *t, uint8_t *out, int cap) {
   // p7_CoreEmit, emit.c:71-148.  States B=0, M=1, I=2, D=3, E=4.

@@ -20,7 +20,7 @@ namespace {
 
 constexpr uint32_t kMagicF = 0xb3e6e6f3u;   // "3ffs" + 0x80808080, io.c:48
 constexpr uint32_t kMagicP = 0xb3e6f0f3u;   // "3fps" + 0x80808080, io.c:49
-constexpr int kOK = 0, kEOF = 3, kEMEM = 5, kEFORMAT = 7, kENOTFOUND = 14;
+constexpr int kOK = 0, kEOF = 3, kEMEM = 5, kEFORMAT = 7, kENOTFOUND = 6;   // easel.h:98-102
 
 int kp_of(int alphatype) {                   // esl_alphabet.h:12-14, esl_alphabet.c:275-285
   switch (alphatype) {

@@ -35,6 +35,13 @@ extern "C" {
 #define MSVH_K  20   /* amino-acid alphabet size */
 #define MSVH_KP 29   /* with gap, degeneracies, nonresidue, missing data */
 
+/* Alphabet types, Easel's codes (easel/esl_alphabet.h:12-14): amino has
+ * K = 20, Kp = 29 ("ACDEFGHIKLMNPQRSTVWY-BJZOUX*~"); DNA and RNA K = 4,
+ * Kp = 18 ("ACGT-RYMKSWHBVDN*~", U for T). */
+#define MSVH_RNA   1
+#define MSVH_DNA   2
+#define MSVH_AMINO 3
+
 typedef struct {
   int      fast;       /* 0: Mersenne twister (esl_randomness_Create), 1: LCG (CreateFast) */
   uint32_t mt[624];
@@ -58,6 +65,13 @@ int    msvh_hmm_sample(msvh_rng *r, int M, float *mat, float *ins, float *t);
 int    msvh_profile_convert(int M, const float *mat, int config_L, uint8_t *rbv, int *scalars, float *scale,
                             float *msc);
 double msvh_lambda(int M, const float *mat);
+/* The same for any of the alphabets above: mat is (M+1) x K, rbv Kp x M,
+ * msc Kp x M; the background is p7_bg_Create's (p7_bg.c:55-82: the amino
+ * frequencies, uniform 1/K for DNA and RNA).  11 (eslEINVAL) for an unknown
+ * alphabet or M < 1. */
+int    msvh_profile_convert_abc(int alphatype, int M, const float *mat, int config_L, uint8_t *rbv, int *scalars,
+                                float *scale, float *msc);
+double msvh_lambda_abc(int alphatype, int M, const float *mat);
 
 /* One core emission (residue codes, no sentinels) into out[0..cap); returns
  * its length, or -1 if cap is too small. */
@@ -85,6 +99,9 @@ void   msvh_calibration_targets(int N, int L, uint8_t *out);
  * msvh_hmm_sample writes them. */
 int    msvh_vit_convert(int M, const float *mat, const float *t, int16_t *rwv, int16_t *twv, int *scalars,
                         float *scale_w, float *nj);
+/* The same for any alphabet: mat (M+1) x K, rwv Kp x M. */
+int    msvh_vit_convert_abc(int alphatype, int M, const float *mat, const float *t, int16_t *rwv, int16_t *twv,
+                            int *scalars, float *scale_w, float *nj);
 /* xw[N|C|J][MOVE] for target length L: wordify(logf((2+nj)/(L+2+nj)))
  * (p7_oprofile_ReconfigRestLength, p7_oprofile.c:944-962; wordify :527-534). */
 int    msvh_vit_move(float scale_w, float nj, int L);

@@ -21,7 +21,7 @@ extern "C" {
 typedef struct msvh_pressed msvh_pressed;
 
 /* Open a .h3f file and, optionally (h3p_path may be NULL), its .h3p
- * companion.  Returns 0, or 5 (eslEMEM) / 14 (eslENOTFOUND). */
+ * companion.  Returns 0, or 5 (eslEMEM) / 6 (eslENOTFOUND). */
 int  msvh_pressed_open(const char *h3f_path, const char *h3p_path, msvh_pressed **ret);
 
 /* Read the next model.  *msv is filled (rbv points into the reader's

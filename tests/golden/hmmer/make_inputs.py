@@ -11,7 +11,7 @@ import shutil
 TUT = "/root/reference/code/tutorial"
 FILES = ["fn3.hmm", "globins4.hmm", "Pkinase.hmm", "MADE1.hmm",
          "7LESS_DROME", "HBB_HUMAN", "globins45.fa", "dna_target.fa",
-         "minifam.h3f", "minifam.h3p", "MADE1.hmm.h3f", "MADE1.hmm.h3p"]
+         "minifam", "minifam.h3f", "minifam.h3p", "MADE1.hmm.h3f", "MADE1.hmm.h3p"]
 
 
 def main():

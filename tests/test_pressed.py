@@ -81,4 +81,4 @@ def test_refuses_bad_files(tmp_path):
     assert e.value.code == 7
     with pytest.raises(ca.MSVError) as e:
         ca.read_pressed(os.path.join(str(tmp_path), "missing.h3f"))
-    assert e.value.code == 14
+    assert e.value.code == 6                    # eslENOTFOUND

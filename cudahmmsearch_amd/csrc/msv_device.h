@@ -63,7 +63,8 @@ struct FastArgs {
   int32_t *counter;            // dynamic mode: next chunk (zeroed before the launch)
   const int32_t *order;        // static mode: task t -> sorted target index
   int32_t ntasks;              // static mode: number of tasks
-  int32_t mode;                // 0 dynamic chunks, 1 static round-robin, 2 one chunk per workgroup
+  int32_t mode;                // 0 dynamic chunks, 1 static round-robin, 2 one chunk per workgroup,
+                               // 3 scan: workgroup b scores profile scan_profs[b] against every target
   int32_t prof_idx;
   SlowItem *slow;
   int32_t *slow_count;
@@ -71,6 +72,10 @@ struct FastArgs {
   SlowItem *jslow;             // this profile's J-aware queue (null: use the byte queue)
   int32_t *jslow_count;
   int32_t jslow_cap;
+  const ProfDev *scan_profs;   // mode 3: per-workgroup profile (its ftab and constants; res unused)
+  uint16_t *scan_out;          // mode 3: result word of (workgroup b, target s) at scan_out[s * scan_stride + b];
+  int64_t scan_stride;         // no queues are written
+  int32_t scan_nprof;          // mode 3: profiles; the grid is a multiple of it
 };
 
 // J-aware stage: the MSV recurrence with per-row E/J/B updates in the f16
@@ -106,6 +111,8 @@ struct ScanArgs {              // one workgroup per profile, its waves over the 
   int32_t nq, R;
   int32_t stride;              // result of (profile b, query q) at out[q * stride + b]
   uint32_t *out;
+  const int32_t *wq_off;       // non-null: workgroup b scores only the queries wq_idx[wq_off[b] ..
+  const int32_t *wq_idx;       // wq_off[b+1]), the result of item k at out[k]
 };
 
 struct SlowArgs {

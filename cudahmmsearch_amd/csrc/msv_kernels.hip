@@ -102,16 +102,18 @@ __device__ __forceinline__ void row_fence(uint32_t (&r)[Q]) {
 // 16-byte table entries, the last one read only as wide as it is used
 // (the table is the co-limiting LDS stream, so unused bytes are not read).
 // Entries e[Q..] are left unset.
-template <int Q>
+// TW is the table's width in lanes: 64, or G when the lane groups of a wave
+// share one copy (entry of lane l at l % G).
+template <int Q, int TW = 64>
 __device__ __forceinline__ void load_row(const uint4 *tl, uint32_t x, uint32_t (&e)[((Q + 3) / 4) * 4]) {
   constexpr int QB = (Q + 3) / 4, TAIL = Q - 4 * (QB - 1);
-  const uint4 *tp = tl + x * (QB * 64);
+  const uint4 *tp = tl + x * (QB * TW);
 #pragma unroll
   for (int jb = 0; jb < QB - 1; ++jb) {
-    const uint4 v = tp[jb * 64];
+    const uint4 v = tp[jb * TW];
     e[4 * jb + 0] = v.x; e[4 * jb + 1] = v.y; e[4 * jb + 2] = v.z; e[4 * jb + 3] = v.w;
   }
-  const uint4 *lp = tp + (QB - 1) * 64;
+  const uint4 *lp = tp + (QB - 1) * TW;
   constexpr int b = 4 * (QB - 1);
   if constexpr (TAIL == 4) {
     const uint4 v = *lp;
@@ -169,8 +171,9 @@ __device__ __forceinline__ void defer(uint32_t w, int prof_idx, int64_t s, SlowI
   if (k < slow_cap) slow[k] = SlowItem{prof_idx, (int32_t) s};
 }
 
-template <int G, int Q>
-__device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, int64_t s, int lane) {
+template <int G, int Q, int TW = 64>
+__device__ __forceinline__ void fast_task(const FastArgs &a, const ProfDev &pd, uint16_t *res, int64_t rstride,
+                                          const uint4 *lds, int64_t s, int lane) {
   constexpr int QB = (Q + 3) / 4;
   const bool valid = s < a.db.n;
   const int L = valid ? a.db.len[s] : 0;
@@ -186,7 +189,7 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
 
   const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
   const uint4 null4 = make_uint4(nb, nb, nb, nb);
-  const uint4 *tl = lds + lane;
+  const uint4 *tl = lds + lane % TW;
 
   uint32_t r[Q];
 #pragma unroll
@@ -199,7 +202,7 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
   uint4 cnext = *reinterpret_cast<const uint4 *>(p);
   uint4 c = (0 < Lpad) ? cnext : null4;
   uint32_t e[QB * 4];
-  load_row<Q>(tl, byte_of(c, 0), e);
+  load_row<Q, TW>(tl, byte_of(c, 0), e);
   for (int i0 = 0; i0 < Lmax; i0 += 16) {
     const int i1 = i0 + 16;
     cnext = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
@@ -207,7 +210,7 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
       uint32_t en[QB * 4];
-      load_row<Q>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
+      load_row<Q, TW>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
       const uint32_t sh = shift_slot<G>(r[Q - 1], lane);
 #pragma unroll
       for (int j = Q - 1; j >= 1; --j) r[j] = cell(r[j - 1], e[j]);
@@ -231,9 +234,9 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const uint4 *lds, i
   for (int o = G / 2; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
   if (valid && (lane % G) == 0) {
     const int Rp = (int) (m * 2048.0f + 0.5f);
-    const uint32_t w = decide(Rp, a.db.tjb[s], a.prof);
-    a.prof.res[s] = (uint16_t) w;
-    if ((w >> 8) == RES_SLOW)
+    const uint32_t w = decide(Rp, a.db.tjb[s], pd);
+    res[s * rstride] = (uint16_t) w;
+    if ((w >> 8) == RES_SLOW && a.slow_count)
       defer(w, a.prof_idx, s, a.slow, a.slow_count, a.slow_cap, a.jslow, a.jslow_count, a.jslow_cap);
   }
 }
@@ -247,19 +250,35 @@ __global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
 
-  const int ntab = (a.db.R + 1) * QB * 64;
-  for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = a.prof.ftab[i];
+  // mode 3 tables are G lanes wide (the wave's lane groups share them)
+  const int ntab = (a.db.R + 1) * QB * (a.mode == 3 ? G : 64);
+  // mode 3: workgroup b scores profile b % scan_nprof against its share of
+  // the targets (the (b / scan_nprof)-th of gridDim.x / scan_nprof splits)
+  const int sp = (a.mode == 3) ? (int) (blockIdx.x % (unsigned) a.scan_nprof) : 0;
+  const uint4 *ftab = (a.mode == 3) ? a.scan_profs[sp].ftab : a.prof.ftab;
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = ftab[i];
   __syncthreads();
 
   const int g = lane / G;
-  if (a.mode == 0) {
+  if (a.mode == 3) {
+    // Scan: this workgroup's profile against its targets (queries of an
+    // hmmscan batch, sorted by length so a wave's groups finish together);
+    // task t is targets t*NG .. t*NG+NG-1, tasks round-robin over the waves
+    // of the profile's workgroups.
+    const ProfDev pd = a.scan_profs[sp];
+    uint16_t *res = a.scan_out + sp;
+    const int split = (int) (blockIdx.x / (unsigned) a.scan_nprof);
+    const int nsplit = (int) (gridDim.x / (unsigned) a.scan_nprof);
+    for (int t = split * nwaves + wave; t < a.ntasks; t += nsplit * nwaves)
+      fast_task<G, Q, G>(a, pd, res, a.scan_stride, lds, (int64_t) t * NG + g, lane);
+  } else if (a.mode == 0) {
     for (;;) {
       int c = 0;
       if (lane == 0) c = atomicAdd(a.counter, 1);
       c = __builtin_amdgcn_readfirstlane(c);
       if (c >= a.nchunks) break;
       const int t1 = a.chunk_start[c + 1];
-      for (int t = a.chunk_start[c]; t < t1; ++t) fast_task<G, Q>(a, lds, (int64_t) t * NG + g, lane);
+      for (int t = a.chunk_start[c]; t < t1; ++t) fast_task<G, Q>(a, a.prof, a.prof.res, 1, lds, (int64_t) t * NG + g, lane);
     }
   } else if (a.mode == 2) {
     // One chunk per workgroup, waves interleaved over its tasks: the launch
@@ -268,11 +287,12 @@ __global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
     const int c = blockIdx.x;
     if (c < a.nchunks) {
       const int t1 = a.chunk_start[c + 1];
-      for (int t = a.chunk_start[c] + wave; t < t1; t += nwaves) fast_task<G, Q>(a, lds, (int64_t) t * NG + g, lane);
+      for (int t = a.chunk_start[c] + wave; t < t1; t += nwaves)
+        fast_task<G, Q>(a, a.prof, a.prof.res, 1, lds, (int64_t) t * NG + g, lane);
     }
   } else {
     for (int t = blockIdx.x * nwaves + wave; t < a.ntasks; t += gridDim.x * nwaves)
-      fast_task<G, Q>(a, lds, a.order[t], lane);
+      fast_task<G, Q>(a, a.prof, a.prof.res, 1, lds, a.order[t], lane);
   }
 }
 
@@ -571,7 +591,11 @@ __global__ void __launch_bounds__(256) msv_scan(ScanArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const ProfDev pd = sp->prof;
-  for (int q = wave; q < a.nq; q += 4) {          // waves without a query leave at once
+  const bool listed = a.wq_off != nullptr;
+  const int k0 = listed ? a.wq_off[blockIdx.x] : 0;
+  const int k1 = listed ? a.wq_off[blockIdx.x + 1] : a.nq;
+  for (int k = k0 + wave; k < k1; k += 4) {       // waves without a query leave at once
+    const int q = listed ? a.wq_idx[k] : k;
     const int tjb = a.qtjb[q];
     int xEmax;
     uint32_t w = jrow_task<Q>(lds + lane, a.res + a.qoff[q], a.qlen[q], tjb, pd, lane, xEmax);
@@ -579,7 +603,7 @@ __global__ void __launch_bounds__(256) msv_scan(ScanArgs a) {
       const int tjbm = (tjb + pd.tbm) & 255;
       w = decide(xEmax - max(0, pd.base - tjbm), tjb, pd);
     }
-    if (lane == 0) a.out[(int64_t) q * a.stride + blockIdx.x] = w;
+    if (lane == 0) a.out[listed ? (int64_t) k : (int64_t) q * a.stride + blockIdx.x] = w;
   }
 }
 
@@ -687,13 +711,13 @@ __global__ void __launch_bounds__(256) msv_finalize(FinArgs a) {
 // ---------------------------------------------------------------------------
 // Fast table: uint4 T[(R+1) * QB * 64]; see msv_device.h.
 __global__ void build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R,
-                                 int G, int Q, int bias, uint32_t *out) {
+                                 int G, int Q, int W, int bias, uint32_t *out) {
   const int QB = (Q + 3) / 4;
-  const int64_t total = (int64_t) (R + 1) * QB * 64 * 4;
+  const int64_t total = (int64_t) (R + 1) * QB * W * 4;
   for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t) gridDim.x * blockDim.x) {
     const int e = (int) (i & 3);
-    const int lane = (int) ((i >> 2) & 63);
-    const int64_t rest = i >> 8;
+    const int lane = (int) ((i >> 2) % W);
+    const int64_t rest = (i >> 2) / W;
     const int jb = (int) (rest % QB);
     const int x = (int) (rest / QB);
     const int j = 4 * jb + e;
@@ -979,12 +1003,13 @@ extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, 
 }
 
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
-                                            int Q, int bias, uint32_t *out, hipStream_t st) {
+                                            int Q, int W, int bias, uint32_t *out, hipStream_t st) {
+  if (W != 64 && W != G) return hipErrorInvalidValue;
   const int QB = (Q + 3) / 4;
-  const int64_t total = (int64_t) (R + 1) * QB * 256;
+  const int64_t total = (int64_t) (R + 1) * QB * W * 4;
   int grid = (int) ((total + 255) / 256);
   if (grid > 65535) grid = 65535;
-  hipLaunchKernelGGL(build_fast_table, dim3(grid), dim3(256), 0, st, rbv, M, Kp, rowcode, R, G, Q, bias, out);
+  hipLaunchKernelGGL(build_fast_table, dim3(grid), dim3(256), 0, st, rbv, M, Kp, rowcode, R, G, Q, W, bias, out);
   return hipGetLastError();
 }
 

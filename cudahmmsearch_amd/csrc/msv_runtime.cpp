@@ -17,6 +17,7 @@
 #include <condition_variable>
 #include <deque>
 #include <limits>
+#include <array>
 #include <mutex>
 #include <thread>
 #include <cmath>
@@ -44,7 +45,7 @@ extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, h
 extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int ncu, hipStream_t st);
 extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st);
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
-                                            int Q, int bias, uint32_t *out, hipStream_t st);
+                                            int Q, int W, int bias, uint32_t *out, hipStream_t st);
 extern "C" hipError_t msvk_build_seg_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int Q,
                                            int k0, int bias, uint32_t *out, hipStream_t st);
 extern "C" hipError_t msvk_build_slow_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int Qs,
@@ -122,6 +123,8 @@ struct Profile {
   DevMem d_onetab;                 // per-call drop-in: 64-lane fast table over codes 0..Kp-1
   DevMem d_scantab;                // scan direction: the same table at the profile's register class
   int scanQ = 0;                   // that class (0: not built)
+  DevMem d_gtab;                   // batched scan: fast table in the G-lane layout over codes 0..Kp-1
+  int gG = 0, gQ = 0;              // its lanes per target and registers (0: not built)
 };
 
 struct ScanSet {                   // the profiles of msvg_scan calls, grouped by register class
@@ -132,10 +135,21 @@ struct ScanSet {                   // the profiles of msvg_scan calls, grouped b
     int Q = 0;
     std::vector<int32_t> members;  // positions in pids
     DevMem d_profs;                // ScanProf per member
+    std::vector<ScanProf> h_profs;
+    std::vector<int32_t> slot;     // position in pids -> member index, -1 if not a member
   };
   std::vector<std::unique_ptr<Cls>> cls;
   std::vector<int32_t> fallback;   // positions whose table does not fit LDS
   int64_t sumM = 0;
+  // Batched scans: the same profiles grouped by fast-kernel tile (G lanes per
+  // query, Q registers), built on the first batched call.
+  struct GCls {
+    int G = 0, Q = 0;
+    std::vector<int32_t> members;
+    DevMem d_profs;                // ProfDev per member
+  };
+  std::vector<std::unique_ptr<GCls>> gcls;
+  bool grouped = false;
 };
 
 struct Db {
@@ -209,6 +223,7 @@ struct msvg_ctx {
   // scan direction (many profiles, one query)
   std::unique_ptr<ScanSet> scanset;
   DevMem d_scanout;
+  DevMem d_scanj;                           // batched scan's J stage: profile lists, query lists, words
   uint32_t *h_scanout = nullptr;            // pinned result words
   size_t h_scancap = 0;
   // ViterbiFilter searches
@@ -439,7 +454,7 @@ int ensure_fast_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **ou
     const int Q = qfor(p.M, 2 * G);
     auto mem = std::make_unique<DevMem>();
     HIPCHK(c, mem->ensure(fast_lds(d.R, Q)));
-    HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, d.d_rowcode.as<uint8_t>(), d.R, G, Q, p.bias,
+    HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, d.d_rowcode.as<uint8_t>(), d.R, G, Q, 64, p.bias,
                                     mem->as<uint32_t>(), st));
     it = t.fast.emplace(G, std::move(mem)).first;
   }
@@ -1481,7 +1496,7 @@ int scan_prepare(msvg_ctx *c, const int32_t *pids, int32_t np, ScanSet **ret) {
     if (Q == 0 || fast_lds(p.Kp, Q) > (size_t) kLdsBytes || p.Kp > 254) { set->fallback.push_back(i); continue; }
     if (p.scanQ != Q) {
       HIPCHK(c, p.d_scantab.ensure((size_t) (p.Kp + 1) * ((Q + 3) / 4) * 64 * 16));
-      HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, 64, Q, p.bias,
+      HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, 64, Q, 64, p.bias,
                                       p.d_scantab.as<uint32_t>(), st));
       p.scanQ = Q;
     }
@@ -1508,10 +1523,345 @@ int scan_prepare(msvg_ctx *c, const int32_t *pids, int32_t np, ScanSet **ret) {
     }
     HIPCHK(c, cl->d_profs.ensure(h.size() * sizeof(ScanProf)));
     HIPCHK(c, hipMemcpyAsync(cl->d_profs.p, h.data(), h.size() * sizeof(ScanProf), hipMemcpyHostToDevice, st));
+    cl->slot.assign((size_t) np, -1);
+    for (size_t k = 0; k < cl->members.size(); ++k) cl->slot[(size_t) cl->members[k]] = (int32_t) k;
+    cl->h_profs = std::move(h);
   }
   HIPCHK(c, hipStreamSynchronize(st));
   c->scanset = std::move(set);
   *ret = c->scanset.get();
+  return MSVG_OK;
+}
+
+// Lanes per query for the batched scan, by model length: the fast kernel's
+// tuned lane counts on this device (8 up to M = 128, 16 up to 768, 32 up to
+// 1024, a whole wave beyond).
+int scan_lanes(int M) { return M <= 128 ? 8 : M <= 768 ? 16 : M <= 1024 ? 32 : 64; }
+
+// Grouped tables and classes of a scan set (every profile that has a
+// one-query class has one: at G = 64 the grouped table is no larger).
+int scan_prepare_grouped(msvg_ctx *c, ScanSet *set) {
+  if (set->grouped) return MSVG_OK;
+  hipStream_t st = c->stream;
+  std::map<std::pair<int, int>, ScanSet::GCls *> by_tile;
+  std::vector<char> has_cls(set->pids.size(), 0);
+  for (auto &cl : set->cls)
+    for (int32_t i : cl->members) has_cls[(size_t) i] = 1;
+  for (size_t i = 0; i < set->pids.size(); ++i) {
+    if (!has_cls[i]) continue;
+    Profile &p = *set->prof[i];
+    int G = scan_lanes(p.M), Q = qfor(p.M, 2 * G);
+    while (G < 64 && (Q > 32 || fast_lds(p.Kp, Q) / 64 * G > (size_t) kLdsBytes)) { G *= 2; Q = qfor(p.M, 2 * G); }
+    if (p.gG != G || p.gQ != Q) {
+      HIPCHK(c, p.d_gtab.ensure(fast_lds(p.Kp, Q) / 64 * G));
+      HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, G, Q, G, p.bias,
+                                      p.d_gtab.as<uint32_t>(), st));
+      p.gG = G; p.gQ = Q;
+    }
+    auto it = by_tile.find({G, Q});
+    if (it == by_tile.end()) {
+      set->gcls.push_back(std::make_unique<ScanSet::GCls>());
+      set->gcls.back()->G = G; set->gcls.back()->Q = Q;
+      it = by_tile.emplace(std::make_pair(G, Q), set->gcls.back().get()).first;
+    }
+    it->second->members.push_back((int32_t) i);
+  }
+  // largest tables first: their launches hold the longest row chains
+  std::sort(set->gcls.begin(), set->gcls.end(), [](const std::unique_ptr<ScanSet::GCls> &x,
+                                                   const std::unique_ptr<ScanSet::GCls> &y) {
+    return x->Q * x->G != y->Q * y->G ? x->Q * x->G > y->Q * y->G : x->G > y->G;
+  });
+  for (auto &gc : set->gcls) {
+    std::vector<ProfDev> h(gc->members.size());
+    for (size_t k = 0; k < gc->members.size(); ++k) {
+      const Profile &p = *set->prof[(size_t) gc->members[k]];
+      ProfDev &d = h[k];
+      std::memset(&d, 0, sizeof d);
+      d.ftab = p.d_gtab.as<uint4>();
+      d.M = p.M; d.Q = gc->Q; d.QB = (gc->Q + 3) / 4; d.Qs = qfor(p.M, 128);
+      d.tbm = p.tbm; d.tec = p.tec; d.base = p.base; d.bias = p.bias;
+    }
+    HIPCHK(c, gc->d_profs.ensure(h.size() * sizeof(ProfDev)));
+    HIPCHK(c, hipMemcpyAsync(gc->d_profs.p, h.data(), h.size() * sizeof(ProfDev), hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(c, hipStreamSynchronize(st));
+  set->grouped = true;
+  return MSVG_OK;
+}
+
+// Batches of at least this many queries take the grouped kernel.  Its
+// workgroups (one per profile) have kScanWaves waves when the class has
+// enough profiles to fill the device (tables G lanes wide let several
+// workgroups share a CU, and a wave's query groups, taken round-robin
+// longest first, even out its share of a skewed batch), else 16.
+constexpr int kScanGroupMin = 4;
+constexpr int kScanWaves = 4;
+constexpr int kScanManyProfiles = 2048;
+constexpr int kScanCUs = 256;
+
+// msvg_scan_many for a batch: the fast kernel in scan mode (one workgroup
+// per profile, 64/G queries per wave, longest queries first), then the
+// J-aware kernel for the (profile, query) pairs whose J state may fire,
+// listed per profile.  Pairs neither settles go to `slow`.
+int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, const int64_t *qoff, const int32_t *qlen,
+                 double F1, float *usc, uint8_t *status, uint8_t *pass, const std::vector<int32_t> &qlive,
+                 std::vector<std::pair<int32_t, int32_t>> &slow) {
+  int rc = scan_prepare_grouped(c, set);
+  if (rc != MSVG_OK) return rc;
+  const int Kp = set->Kp;
+  const int32_t nl = (int32_t) qlive.size();
+  std::vector<int32_t> order(qlive);
+  std::stable_sort(order.begin(), order.end(), [qlen](int32_t x, int32_t y) { return qlen[x] > qlen[y]; });
+  int64_t padded = 0;
+  for (int32_t q : order) padded += ((int64_t) qlen[q] + 15) & ~(int64_t) 15;
+  // staging: codes padded to 16 with the null row Kp, then offsets, lengths, tjb_b
+  const size_t o_off = ((size_t) padded + 15) & ~(size_t) 15;
+  const size_t o_len = o_off + (size_t) nl * sizeof(int64_t);
+  const size_t o_tjb = o_len + (size_t) nl * sizeof(int32_t);
+  const size_t total = o_tjb + (size_t) nl;
+  if (c->h_one_cap < total + 64) {
+    if (c->h_one_res) { (void) hipHostFree(c->h_one_res); c->h_one_res = nullptr; c->h_one_cap = 0; }
+    const size_t cap = std::max<size_t>(total + 64, 4096);
+    HIPCHK(c, hipHostMalloc((void **) &c->h_one_res, cap, hipHostMallocDefault));
+    c->h_one_cap = cap;
+  }
+  uint8_t *h = c->h_one_res;
+  int64_t *h_off = reinterpret_cast<int64_t *>(h + o_off);
+  int32_t *h_len = reinterpret_cast<int32_t *>(h + o_len);
+  uint8_t *h_tjb = h + o_tjb;
+  std::vector<int> tjb_k((size_t) nl);
+  int64_t pos = 0;
+  for (int32_t k = 0; k < nl; ++k) {
+    const int32_t q = order[(size_t) k];
+    const int64_t Lq = qlen[q], Lpad = (Lq + 15) & ~(int64_t) 15;
+    std::memcpy(h + pos, res + qoff[q], (size_t) Lq);
+    std::memset(h + pos + Lq, Kp, (size_t) (Lpad - Lq));
+    h_off[k] = pos; h_len[k] = (int32_t) Lq;
+    tjb_k[(size_t) k] = msvh_tjb((int) Lq);
+    h_tjb[k] = (uint8_t) tjb_k[(size_t) k];
+    pos += Lpad;
+  }
+  size_t nmem = 0;
+  for (auto &gc : set->gcls) nmem += gc->members.size();
+  const size_t nw = nmem * (size_t) nl;                      // uint16 result words
+  hipStream_t st = c->stream;
+  HIPCHK(c, c->d_one_res.ensure(std::max<size_t>(total, 4096)));
+  HIPCHK(c, c->d_scanout.ensure(std::max<size_t>(nw * sizeof(uint16_t), 16)));
+  if (c->h_scancap * sizeof(uint32_t) < nw * sizeof(uint16_t)) {
+    if (c->h_scanout) { (void) hipHostFree(c->h_scanout); c->h_scanout = nullptr; c->h_scancap = 0; }
+    const size_t cap = nw / 2 + 1;
+    HIPCHK(c, hipHostMalloc((void **) &c->h_scanout, cap * sizeof(uint32_t), hipHostMallocDefault));
+    c->h_scancap = cap;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_one_res.p, h, total, hipMemcpyHostToDevice, st));
+  const uint8_t *d = c->d_one_res.as<uint8_t>();
+  DbDev qdb{};
+  qdb.res = d;
+  qdb.off = reinterpret_cast<const int64_t *>(d + o_off);
+  qdb.len = reinterpret_cast<const int32_t *>(d + o_len);
+  qdb.tjb = d + o_tjb;
+  qdb.n = nl;
+  qdb.R = Kp;
+  uint16_t *d_words = c->d_scanout.as<uint16_t>();
+  HIPCHK(c, hipEventRecord(c->ev[0], st));
+  const int ns = std::min<int>(kVitStreams, (int) set->gcls.size());
+  if (ns > 1) {
+    if (!c->vstream[0]) {
+      for (int k = 0; k < kVitStreams; ++k) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->vstream[k], hipStreamNonBlocking));
+        HIPCHK(c, hipEventCreateWithFlags(&c->vdone[k], hipEventDisableTiming));
+      }
+    }
+    HIPCHK(c, hipEventRecord(c->fork, st));
+    for (int k = 0; k < ns; ++k) HIPCHK(c, hipStreamWaitEvent(c->vstream[k], c->fork, 0));
+  }
+  size_t base = 0;
+  for (size_t ci = 0; ci < set->gcls.size(); ++ci) {
+    auto &gc = set->gcls[ci];
+    FastArgs a{};
+    a.db = qdb;
+    a.mode = 3;
+    a.ntasks = (nl + (64 / gc->G) - 1) / (64 / gc->G);
+    a.scan_profs = gc->d_profs.as<ProfDev>();
+    a.scan_out = d_words + base;                            // words [sorted query][row]
+    a.scan_stride = (int64_t) nmem;
+    // few profiles (long models): all query groups at once; many: fewer
+    // waves per workgroup so that several workgroups share each CU
+    const int nprof = (int) gc->members.size();
+    const int waves = std::min(nprof >= kScanManyProfiles ? kScanWaves : 16, a.ntasks);
+    // a class of few profiles splits its query groups over several
+    // workgroups per profile, up to about one workgroup per CU
+    const int nsplit = std::max(1, std::min((kScanCUs + nprof - 1) / nprof, (a.ntasks + waves - 1) / waves));
+    a.scan_nprof = nprof;
+    hipStream_t cs = ns > 1 ? c->vstream[ci % ns] : st;
+    HIPCHK(c, msvk_launch_fast(gc->G, gc->Q, &a, nprof * nsplit, 64 * waves, fast_lds(Kp, gc->Q) / 64 * gc->G, cs));
+    base += gc->members.size();
+  }
+  if (ns > 1) {
+    for (int k = 0; k < ns; ++k) {
+      HIPCHK(c, hipEventRecord(c->vdone[k], c->vstream[k]));
+      HIPCHK(c, hipStreamWaitEvent(st, c->vdone[k], 0));
+    }
+  }
+  HIPCHK(c, hipEventRecord(c->ev[1], st));
+  HIPCHK(c, hipMemcpyAsync(c->h_scanout, d_words, nw * sizeof(uint16_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->stats.ms_fast = ms;
+
+  std::vector<PassRule> rules;
+  rules.reserve((size_t) nl);
+  for (int32_t k = 0; k < nl; ++k) rules.emplace_back(h_len[k], F1);
+  const uint16_t *words = reinterpret_cast<const uint16_t *>(c->h_scanout);
+  auto settle = [&](int32_t i, int32_t k, uint32_t w) {   // a final word of profile position i, sorted query k
+    const int32_t q = order[(size_t) k];
+    const size_t o = (size_t) q * np + i;
+    const Profile &p = *set->prof[(size_t) i];
+    if ((w >> 8) == RES_ERANGE) {
+      usc[o] = std::numeric_limits<float>::infinity(); status[o] = MSVG_ERANGE; pass[o] = 1;
+    } else {
+      usc[o] = usc_of((int) (w & 255u) - tjb_k[(size_t) k], p.base, p.scale);
+      status[o] = MSVG_OK;
+      pass[o] = rules[(size_t) k](usc[o], p.mu, p.lambda) ? 1 : 0;
+    }
+  };
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> jpairs(set->cls.size());   // per class: (slot, k)
+  std::vector<int> cls_of(set->pids.size(), -1);
+  for (size_t ci = 0; ci < set->cls.size(); ++ci)
+    for (int32_t i : set->cls[ci]->members) cls_of[(size_t) i] = (int) ci;
+  // The words are decoded on host threads, each over a range of queries:
+  // word (k, r) at words[k * nmem + r], row r = a class member, and its
+  // outputs at [query][profile position], so reads and writes run along rows.
+  struct Row { int32_t i; int base; float scale, mu, lambda; };
+  std::vector<Row> rows(nmem);
+  base = 0;
+  for (auto &gc : set->gcls) {
+    for (size_t m = 0; m < gc->members.size(); ++m) {
+      const int32_t i = gc->members[m];
+      const Profile &p = *set->prof[(size_t) i];
+      rows[base + m] = Row{i, p.base, p.scale, p.mu, p.lambda};
+    }
+    base += gc->members.size();
+  }
+  const int nth = (int) std::max<size_t>(1, std::min<size_t>({(size_t) 8, (size_t) nl,
+                                                             nmem * (size_t) nl / 32768 + 1,
+                                                             (size_t) std::max(1u, std::thread::hardware_concurrency())}));
+  struct Local { std::vector<std::array<int32_t, 3>> j; std::vector<std::pair<int32_t, int32_t>> s; };
+  std::vector<Local> loc((size_t) nth);
+  auto work = [&](int t) {
+    const int32_t k0 = (int32_t) ((int64_t) nl * t / nth), k1 = (int32_t) ((int64_t) nl * (t + 1) / nth);
+    Local &L = loc[(size_t) t];
+    for (int32_t k = k0; k < k1; ++k) {
+      const int32_t q = order[(size_t) k];
+      const int tjb = tjb_k[(size_t) k];
+      const PassRule &rule = rules[(size_t) k];
+      const uint16_t *wk = words + (size_t) k * nmem;
+      float *uq = usc + (size_t) q * np;
+      uint8_t *sq = status + (size_t) q * np, *pq = pass + (size_t) q * np;
+      for (size_t r = 0; r < nmem; ++r) {
+        const uint32_t w = wk[r];
+        const Row &R = rows[r];
+        const uint32_t code = w >> 8;
+        if (code == RES_SLOW) {
+          if (w & 1u) {
+            const int ci = cls_of[(size_t) R.i];
+            L.j.push_back({ci, set->cls[(size_t) ci]->slot[(size_t) R.i], k});
+          } else {
+            L.s.emplace_back(R.i, q);
+          }
+        } else if (code == RES_ERANGE) {
+          uq[R.i] = std::numeric_limits<float>::infinity(); sq[R.i] = MSVG_ERANGE; pq[R.i] = 1;
+        } else {
+          const float u = usc_of((int) (w & 255u) - tjb, R.base, R.scale);
+          uq[R.i] = u; sq[R.i] = MSVG_OK; pq[R.i] = rule(u, R.mu, R.lambda) ? 1 : 0;
+        }
+      }
+    }
+  };
+  if (nth == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+  }
+  size_t nj = 0;
+  for (const Local &L : loc) {
+    for (const auto &e : L.j) jpairs[(size_t) e[0]].emplace_back(e[1], e[2]);
+    nj += L.j.size();
+    slow.insert(slow.end(), L.s.begin(), L.s.end());
+  }
+  c->stats.n_jrow = (int64_t) nj;
+  if (nj == 0) return MSVG_OK;
+
+  // J stage: per one-query class, the profiles with such pairs, each
+  // workgroup over its listed queries.
+  struct Part { size_t ci, o_prof, o_woff, o_widx, o_out, nwg, nit; };
+  std::vector<Part> parts;
+  std::vector<uint8_t> hb;
+  std::vector<std::pair<int32_t, int32_t>> items;            // (profile position, k) per output word
+  size_t nout = 0;
+  for (size_t ci = 0; ci < set->cls.size(); ++ci) {
+    auto &jp = jpairs[ci];
+    if (jp.empty()) continue;
+    std::sort(jp.begin(), jp.end());
+    const auto &cl = *set->cls[ci];
+    std::vector<ScanProf> profs;
+    std::vector<int32_t> woff{0}, widx;
+    for (size_t t = 0; t < jp.size(); ++t) {
+      if (t == 0 || jp[t].first != jp[t - 1].first) {
+        if (t > 0) woff.push_back((int32_t) widx.size());
+        profs.push_back(cl.h_profs[(size_t) jp[t].first]);
+      }
+      widx.push_back(jp[t].second);
+      items.emplace_back(cl.members[(size_t) jp[t].first], jp[t].second);
+    }
+    woff.push_back((int32_t) widx.size());
+    Part pt{};
+    pt.ci = ci; pt.nwg = profs.size(); pt.nit = widx.size();
+    auto put = [&hb](const void *src, size_t n) {
+      const size_t o = (hb.size() + 15) & ~(size_t) 15;
+      hb.resize(o + n);
+      std::memcpy(hb.data() + o, src, n);
+      return o;
+    };
+    pt.o_prof = put(profs.data(), profs.size() * sizeof(ScanProf));
+    pt.o_woff = put(woff.data(), woff.size() * sizeof(int32_t));
+    pt.o_widx = put(widx.data(), widx.size() * sizeof(int32_t));
+    pt.o_out = nout;
+    nout += widx.size();
+    parts.push_back(pt);
+  }
+  const size_t o_jout = (hb.size() + 15) & ~(size_t) 15;
+  HIPCHK(c, c->d_scanj.ensure(o_jout + nout * sizeof(uint32_t)));
+  HIPCHK(c, hipMemcpyAsync(c->d_scanj.p, hb.data(), hb.size(), hipMemcpyHostToDevice, st));
+  uint8_t *dj = c->d_scanj.as<uint8_t>();
+  HIPCHK(c, hipEventRecord(c->ev[2], st));
+  for (const Part &pt : parts) {
+    ScanArgs a{};
+    a.profs = reinterpret_cast<const ScanProf *>(dj + pt.o_prof);
+    a.res = d;
+    a.qoff = qdb.off; a.qlen = qdb.len; a.qtjb = qdb.tjb;
+    a.nq = nl; a.R = Kp; a.stride = 0;
+    a.out = reinterpret_cast<uint32_t *>(dj + o_jout) + pt.o_out;
+    a.wq_off = reinterpret_cast<const int32_t *>(dj + pt.o_woff);
+    a.wq_idx = reinterpret_cast<const int32_t *>(dj + pt.o_widx);
+    const int Q = set->cls[pt.ci]->Q;
+    HIPCHK(c, msvk_launch_scan(Q, &a, (int) pt.nwg, fast_lds(Kp, Q), st));
+  }
+  HIPCHK(c, hipEventRecord(c->ev[3], st));
+  std::vector<uint32_t> jw(nout);
+  HIPCHK(c, hipMemcpyAsync(jw.data(), dj + o_jout, nout * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+  c->stats.ms_fast += ms;
+  for (size_t t = 0; t < nout; ++t) {
+    const uint32_t w = jw[t];
+    if ((w >> 8) == RES_SLOW) slow.emplace_back(items[t].first, order[(size_t) items[t].second]);
+    else settle(items[t].first, items[t].second, w);
+  }
   return MSVG_OK;
 }
 
@@ -1564,7 +1914,10 @@ int msvg_scan_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t nq, con
     }
   }
   const int32_t nl = (int32_t) qlive.size();
-  if (nl > 0) {
+  if (nl >= kScanGroupMin && !set->cls.empty()) {
+    if ((rc = scan_grouped(c, set, np, res, qoff, qlen, F1, usc, status, pass, qlive, slow)) != MSVG_OK) return rc;
+    c->stats.ms_total = c->stats.ms_fast;
+  } else if (nl > 0) {
     // stage the live queries: codes padded to 16 bytes with the null row Kp,
     // then offsets, lengths and tjb_b, in one pinned buffer
     const size_t meta = (size_t) nl * (sizeof(int64_t) + sizeof(int32_t) + 1);
@@ -1739,7 +2092,7 @@ int one_target(msvg_ctx *c, Profile &p, const uint8_t *dsq, int L, float *ret_sc
       HIPCHK(c, hipStreamSynchronize(st));
     }
     HIPCHK(c, p.d_onetab.ensure((size_t) (p.Kp + 1) * QB * 64 * 16));
-    HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, 64, Q, p.bias,
+    HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, 64, Q, 64, p.bias,
                                     p.d_onetab.as<uint32_t>(), st));
   }
   const size_t Lpad = ((size_t) L + 15) & ~(size_t) 15;

@@ -470,14 +470,18 @@ def test_scan_many_equals_search(ctx):
         pid = ctx.add_profile(p)
         p.mu = ctx.calibrate(pid, p.lam)
         pids.append(pid)
-    for nq in (1, 3, 4, 5, 13):
-        qs = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in rng.integers(1, 1500, nq)]
+    jrow = []
+    for nq in (1, 3, 4, 5, 13, 150):
+        hi = 1500 if nq < 100 else 200          # 150 queries: more than 16 query groups per workgroup
+        qs = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in rng.integers(1, hi, nq)]
         if nq > 3:
             qs[1] = np.zeros(0, np.uint8)
             qs[2] = ca.core_emit(profs[4].hmm, 80 + nq, 1)[0]
+            qs[-1] = ca.core_emit(profs[9].hmm, 90 + nq, 1)[0]
         db = ca.Database.from_list(qs)
         print(f"scan_many: {nq} queries x {len(pids)} profiles", flush=True)
         u, s, ps = ctx.scan_many(pids, db)
+        jrow.append(ctx.stats()["n_jrow"])
         assert u.shape == (nq, len(pids))
         did = ctx.add_database(db)
         ru, rs, rp = ctx.search(pids, did, mode=ca.MODE_PLAIN)
@@ -487,6 +491,7 @@ def test_scan_many_equals_search(ctx):
         if nq == 1:
             u1, s1, p1 = ctx.scan(pids, qs[0])
             assert np.array_equal(u1.view(np.uint32), u[0].view(np.uint32)) and np.array_equal(p1, ps[0])
+    assert max(jrow[2:]) > 0                    # the homologs' pairs took the J-aware stage
     u, s, ps = ctx.scan_many(pids, [])
     assert u.shape == (0, len(pids))
     for pid in pids:

@@ -1538,6 +1538,11 @@ int scan_prepare(msvg_ctx *c, const int32_t *pids, int32_t np, ScanSet **ret) {
 // 1024, a whole wave beyond).
 int scan_lanes(int M) { return M <= 128 ? 8 : M <= 768 ? 16 : M <= 1024 ? 32 : 64; }
 
+// Registers per lane for the batched scan: exact up to 8, then rounded up to
+// even up to 16 and to a multiple of 4 beyond, so that a database's profiles
+// fall into fewer (larger) launches at a padding cost below 1/8.
+int scan_regs(int Q) { return Q <= 8 ? Q : Q <= 16 ? (Q + 1) & ~1 : std::min(32, (Q + 3) & ~3); }
+
 // Grouped tables and classes of a scan set (every profile that has a
 // one-query class has one: at G = 64 the grouped table is no larger).
 int scan_prepare_grouped(msvg_ctx *c, ScanSet *set) {
@@ -1550,8 +1555,11 @@ int scan_prepare_grouped(msvg_ctx *c, ScanSet *set) {
   for (size_t i = 0; i < set->pids.size(); ++i) {
     if (!has_cls[i]) continue;
     Profile &p = *set->prof[i];
-    int G = scan_lanes(p.M), Q = qfor(p.M, 2 * G);
-    while (G < 64 && (Q > 32 || fast_lds(p.Kp, Q) / 64 * G > (size_t) kLdsBytes)) { G *= 2; Q = qfor(p.M, 2 * G); }
+    int G = scan_lanes(p.M), Q = scan_regs(qfor(p.M, 2 * G));
+    while (G < 64 && (Q > 32 || fast_lds(p.Kp, Q) / 64 * G > (size_t) kLdsBytes)) {
+      G *= 2;
+      Q = scan_regs(qfor(p.M, 2 * G));
+    }
     if (p.gG != G || p.gQ != Q) {
       HIPCHK(c, p.d_gtab.ensure(fast_lds(p.Kp, Q) / 64 * G));
       HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, G, Q, G, p.bias,
@@ -1595,8 +1603,8 @@ int scan_prepare_grouped(msvg_ctx *c, ScanSet *set) {
 // workgroups share a CU, and a wave's query groups, taken round-robin
 // longest first, even out its share of a skewed batch), else 16.
 constexpr int kScanGroupMin = 4;
-constexpr int kScanWaves = 4;
-constexpr int kScanManyProfiles = 2048;
+constexpr int kScanWaves = 8;
+constexpr int kScanManyProfiles = 128;
 constexpr int kScanCUs = 256;
 
 // msvg_scan_many for a batch: the fast kernel in scan mode (one workgroup

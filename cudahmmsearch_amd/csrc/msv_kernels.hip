@@ -250,8 +250,12 @@ __global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
 
-  // mode 3 tables are G lanes wide (the wave's lane groups share them)
-  const int ntab = (a.db.R + 1) * QB * (a.mode == 3 ? G : 64);
+  // Search tables are 64 lanes wide.  Scan tables (mode 3) are TWG =
+  // max(G, 16) lanes wide: the wave's lane groups share one copy, so more
+  // workgroups (one per profile) fit a CU, and each 16-lane phase of a
+  // 16-byte LDS read still covers 256 contiguous bytes.
+  constexpr int TWG = G < 16 ? 16 : G;
+  const int ntab = (a.db.R + 1) * QB * (a.mode == 3 ? TWG : 64);
   // mode 3: workgroup b scores profile b % scan_nprof against its share of
   // the targets (the (b / scan_nprof)-th of gridDim.x / scan_nprof splits)
   const int sp = (a.mode == 3) ? (int) (blockIdx.x % (unsigned) a.scan_nprof) : 0;
@@ -270,7 +274,7 @@ __global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
     const int split = (int) (blockIdx.x / (unsigned) a.scan_nprof);
     const int nsplit = (int) (gridDim.x / (unsigned) a.scan_nprof);
     for (int t = split * nwaves + wave; t < a.ntasks; t += nsplit * nwaves)
-      fast_task<G, Q, G>(a, pd, res, a.scan_stride, lds, (int64_t) t * NG + g, lane);
+      fast_task<G, Q, TWG>(a, pd, res, a.scan_stride, lds, (int64_t) t * NG + g, lane);
   } else if (a.mode == 0) {
     for (;;) {
       int c = 0;
@@ -1004,7 +1008,7 @@ extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, 
 
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
                                             int Q, int W, int bias, uint32_t *out, hipStream_t st) {
-  if (W != 64 && W != G) return hipErrorInvalidValue;
+  if (W < G || W > 64 || W % G != 0) return hipErrorInvalidValue;
   const int QB = (Q + 3) / 4;
   const int64_t total = (int64_t) (R + 1) * QB * W * 4;
   int grid = (int) ((total + 255) / 256);

@@ -282,6 +282,10 @@ void choose_tile_tp(const msvg_ctx *c, int M, int R, int &G, int &waves) {
 }
 
 size_t fast_lds(int R, int Q) { return (size_t) (R + 1) * ((Q + 3) / 4) * 1024; }
+// Lanes of the fast kernel's table in scan mode for G lanes per query
+// (max(G, 16): the lane groups of a wave share one copy), and its bytes.
+int table_width(int G) { return G < 16 ? 16 : G; }
+size_t fast_lds_g(int R, int Q, int G) { return fast_lds(R, Q) / 64 * (size_t) table_width(G); }
 
 // The tile for profile length M over database d.  choose_tile_tp picks for
 // throughput; a launch also lasts at least as long as its longest target's
@@ -1556,14 +1560,14 @@ int scan_prepare_grouped(msvg_ctx *c, ScanSet *set) {
     if (!has_cls[i]) continue;
     Profile &p = *set->prof[i];
     int G = scan_lanes(p.M), Q = scan_regs(qfor(p.M, 2 * G));
-    while (G < 64 && (Q > 32 || fast_lds(p.Kp, Q) / 64 * G > (size_t) kLdsBytes)) {
+    while (G < 64 && (Q > 32 || fast_lds_g(p.Kp, Q, G) > (size_t) kLdsBytes)) {
       G *= 2;
       Q = scan_regs(qfor(p.M, 2 * G));
     }
     if (p.gG != G || p.gQ != Q) {
-      HIPCHK(c, p.d_gtab.ensure(fast_lds(p.Kp, Q) / 64 * G));
-      HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, G, Q, G, p.bias,
-                                      p.d_gtab.as<uint32_t>(), st));
+      HIPCHK(c, p.d_gtab.ensure(fast_lds_g(p.Kp, Q, G)));
+      HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, G, Q,
+                                      table_width(G), p.bias, p.d_gtab.as<uint32_t>(), st));
       p.gG = G; p.gQ = Q;
     }
     auto it = by_tile.find({G, Q});
@@ -1702,7 +1706,7 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
     const int nsplit = std::max(1, std::min((kScanCUs + nprof - 1) / nprof, (a.ntasks + waves - 1) / waves));
     a.scan_nprof = nprof;
     hipStream_t cs = ns > 1 ? c->vstream[ci % ns] : st;
-    HIPCHK(c, msvk_launch_fast(gc->G, gc->Q, &a, nprof * nsplit, 64 * waves, fast_lds(Kp, gc->Q) / 64 * gc->G, cs));
+    HIPCHK(c, msvk_launch_fast(gc->G, gc->Q, &a, nprof * nsplit, 64 * waves, fast_lds_g(Kp, gc->Q, gc->G), cs));
     base += gc->members.size();
   }
   if (ns > 1) {

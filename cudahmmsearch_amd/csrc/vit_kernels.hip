@@ -155,20 +155,25 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
         uint32_t ipv = shift_in(Ir[Q - 1], edI, lane);
         uint32_t dpv = shift_in(Dr[Q - 1], edD, lane);
         uint32_t xEv = NEG2, Dmv = NEG2, dcv = NEG2;
+        // Registers in descending order, each updated in place (no copies
+        // across the row loop): register j reads register j-1's old M, I, D
+        // (not yet updated) and its own old M, I; the D entering register
+        // j+1 is this register's new M + tMD, written after register j+2
+        // has read the old one.  The same operations per cell as :146-172.
 #pragma unroll
-        for (int j = 0; j < Q; ++j) {
+        for (int j = Q - 1; j >= 0; --j) {
+          const uint32_t pm = j ? Mr[j - 1] : mpv, pi = j ? Ir[j - 1] : ipv, pd = j ? Dr[j - 1] : dpv;
           uint32_t sv = bm[j];                                       // :146-150
-          sv = maxs(sv, adds(mpv, tMM[j]));
-          sv = maxs(sv, adds(ipv, tIM[j]));
-          sv = maxs(sv, adds(dpv, tDM[j]));
+          sv = maxs(sv, adds(pm, tMM[j]));
+          sv = maxs(sv, adds(pi, tIM[j]));
+          sv = maxs(sv, adds(pd, tDM[j]));
           sv = adds(sv, e[j]);
           xEv = maxs(xEv, sv);                                       // :151
-          mpv = Mr[j]; dpv = Dr[j]; ipv = Ir[j];                     // :156-158
+          Ir[j] = maxs(adds(Mr[j], tMI[j]), adds(Ir[j], tII[j]));    // :171-172
           Mr[j] = sv;                                                // :161
-          Dr[j] = dcv;                                               // :162
-          dcv = adds(sv, tMD[j]);                                    // :167
-          Dmv = maxs(Dmv, dcv);                                      // :168
-          Ir[j] = maxs(adds(mpv, tMI[j]), adds(ipv, tII[j]));        // :171-172
+          const uint32_t dn = adds(sv, tMD[j]);                      // :167
+          Dmv = maxs(Dmv, dn);                                       // :168
+          if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;              // :162
         }
         uint32_t r = wave_max2(pack16(max(lo16(xEv), hi16(xEv)), max(lo16(Dmv), hi16(Dmv))));
         uint32_t dedge = NEG2;
@@ -434,20 +439,25 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
         uint32_t ipv = shift_grp<G>(Ir[Q - 1], lane);
         uint32_t dpv = shift_grp<G>(Dr[Q - 1], lane);
         uint32_t xEv = NEG2, Dmv = NEG2, dcv = NEG2;
+        // Registers in descending order, each updated in place (no copies
+        // across the row loop): register j reads register j-1's old M, I, D
+        // (not yet updated) and its own old M, I; the D entering register
+        // j+1 is this register's new M + tMD, written after register j+2
+        // has read the old one.  The same operations per cell as :146-172.
 #pragma unroll
-        for (int j = 0; j < Q; ++j) {
+        for (int j = Q - 1; j >= 0; --j) {
+          const uint32_t pm = j ? Mr[j - 1] : mpv, pi = j ? Ir[j - 1] : ipv, pd = j ? Dr[j - 1] : dpv;
           uint32_t sv = bm[j];                                       // :146-150
-          sv = maxs(sv, adds(mpv, tMM[j]));
-          sv = maxs(sv, adds(ipv, tIM[j]));
-          sv = maxs(sv, adds(dpv, tDM[j]));
+          sv = maxs(sv, adds(pm, tMM[j]));
+          sv = maxs(sv, adds(pi, tIM[j]));
+          sv = maxs(sv, adds(pd, tDM[j]));
           sv = adds(sv, e[j]);
           xEv = maxs(xEv, sv);                                       // :151
-          mpv = Mr[j]; dpv = Dr[j]; ipv = Ir[j];                     // :156-158
+          Ir[j] = maxs(adds(Mr[j], tMI[j]), adds(Ir[j], tII[j]));    // :171-172
           Mr[j] = sv;                                                // :161
-          Dr[j] = dcv;                                               // :162
-          dcv = adds(sv, tMD[j]);                                    // :167
-          Dmv = maxs(Dmv, dcv);                                      // :168
-          Ir[j] = maxs(adds(mpv, tMI[j]), adds(ipv, tII[j]));        // :171-172
+          const uint32_t dn = adds(sv, tMD[j]);                      // :167
+          Dmv = maxs(Dmv, dn);                                       // :168
+          if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;              // :162
         }
         const uint32_t r = group_max2<G>(pack16(max(lo16(xEv), hi16(xEv)), max(lo16(Dmv), hi16(Dmv))));
         int cond[NG];

@@ -18,6 +18,7 @@
 #include <deque>
 #include <limits>
 #include <array>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <cmath>
@@ -989,14 +990,45 @@ int msvg_db_add(msvg_ctx *c, const uint8_t *res, const int64_t *off, const int32
   (void) hipSetDevice(c->device);
   auto d = std::make_unique<Db>();
   d->n = n;
-  // Codes present.
+  // The passes over the residues run on host threads over disjoint ranges
+  // of targets (a large database is hundreds of MB).
+  const int nth = (int) std::max<int64_t>(1, std::min<int64_t>({(int64_t) 16, n / 16384 + 1,
+                                                                 (int64_t) std::max(1u, std::thread::hardware_concurrency())}));
+  auto par = [nth](int64_t count, const std::function<void(int, int64_t, int64_t)> &fn) {
+    if (nth == 1) { fn(0, 0, count); return; }
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(fn, t, count * t / nth, count * (t + 1) / nth);
+    fn(0, 0, count / nth);
+    for (auto &x : th) x.join();
+  };
+  // Codes present, total residues, longest target.
+  std::vector<std::array<uint8_t, 256>> seen_t((size_t) nth);
+  std::vector<int64_t> total_t((size_t) nth, 0);
+  std::vector<int32_t> maxl_t((size_t) nth, 0);
+  std::vector<char> neg_t((size_t) nth, 0);
+  par(n, [&](int t, int64_t lo, int64_t hi) {
+    auto &sn = seen_t[(size_t) t];
+    sn.fill(0);
+    int64_t tot = 0;
+    int32_t mx = 0;
+    for (int64_t i = lo; i < hi; ++i) {
+      if (len[i] < 0) { neg_t[(size_t) t] = 1; return; }
+      const uint8_t *q = res + off[i];
+      for (int32_t k = 0; k < len[i]; ++k) sn[q[k]] = 1;
+      tot += len[i];
+      mx = std::max(mx, len[i]);
+    }
+    total_t[(size_t) t] = tot;
+    maxl_t[(size_t) t] = mx;
+  });
   bool seen[256] = {false};
   int64_t total = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    if (len[i] < 0) return fail(c, MSVG_EINVAL, "negative target length");
-    const uint8_t *s = res + off[i];
-    for (int32_t k = 0; k < len[i]; ++k) seen[s[k]] = true;
-    total += len[i];
+  int32_t maxlen = 0;
+  for (int t = 0; t < nth; ++t) {
+    if (neg_t[(size_t) t]) return fail(c, MSVG_EINVAL, "negative target length");
+    for (int x = 0; x < 256; ++x) seen[x] = seen[x] || seen_t[(size_t) t][(size_t) x];
+    total += total_t[(size_t) t];
+    maxlen = std::max(maxlen, maxl_t[(size_t) t]);
   }
   uint8_t rowof[256];
   for (int x = 0; x < 256; ++x)
@@ -1005,44 +1037,79 @@ int msvg_db_add(msvg_ctx *c, const uint8_t *res, const int64_t *off, const int32
   d->R = (int) d->rowcode.size();
   if (d->R > 254) return fail(c, MSVG_EINVAL, "too many distinct residue codes");
   d->total_res = total;
-  // Length-sorted order (descending; ties by caller index for determinism).
+  // Length-sorted order (descending; ties by caller index for determinism),
+  // with the distinct lengths (ascending) and each target's index among them.
   std::vector<int64_t> order(n);
-  for (int64_t i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return len[a] > len[b]; });
   d->sorted_of.resize(n);
   d->len_sorted.resize(n);
   d->tjb_sorted.resize(n);
+  d->lidx_sorted.resize(n);
+  if (maxlen <= (1 << 20)) {
+    // counting sort on the length: stable, so ties keep the caller's order
+    std::vector<int64_t> start((size_t) maxlen + 2, 0);
+    for (int64_t i = 0; i < n; ++i) start[(size_t) len[i]]++;
+    std::vector<int32_t> idx_of((size_t) maxlen + 1, -1);
+    std::vector<uint8_t> tjb_of((size_t) maxlen + 1, 0);
+    for (int32_t L = 0; L <= maxlen; ++L)
+      if (start[(size_t) L]) {
+        idx_of[(size_t) L] = (int32_t) d->Ls.size();
+        d->Ls.push_back(L);
+        tjb_of[(size_t) L] = (uint8_t) msvh_tjb(L);
+      }
+    int64_t acc = 0;
+    for (int32_t L = maxlen; L >= 0; --L) {
+      const int64_t k = start[(size_t) L];
+      start[(size_t) L] = acc;
+      acc += k;
+    }
+    for (int64_t i = 0; i < n; ++i) order[(size_t) start[(size_t) len[i]]++] = i;
+    for (int64_t sp = 0; sp < n; ++sp) {
+      const int32_t L = len[order[(size_t) sp]];
+      d->len_sorted[sp] = L;
+      d->tjb_sorted[sp] = tjb_of[(size_t) L];
+      d->lidx_sorted[sp] = idx_of[(size_t) L];
+    }
+  } else {
+    for (int64_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return len[a] > len[b]; });
+    for (int64_t sp = 0; sp < n; ++sp) {
+      d->len_sorted[sp] = len[order[sp]];
+      d->tjb_sorted[sp] = (uint8_t) msvh_tjb(d->len_sorted[sp]);
+    }
+    std::vector<int32_t> ls(d->len_sorted);
+    std::sort(ls.begin(), ls.end());
+    ls.erase(std::unique(ls.begin(), ls.end()), ls.end());
+    d->Ls = ls;
+    for (int64_t sp = 0; sp < n; ++sp)
+      d->lidx_sorted[sp] = (int32_t) (std::lower_bound(ls.begin(), ls.end(), d->len_sorted[sp]) - ls.begin());
+  }
   std::vector<int64_t> off_sorted(n);
   int64_t bytes = 0;
-  for (int64_t s = 0; s < n; ++s) {
-    const int64_t i = order[s];
-    d->sorted_of[i] = s;
-    d->len_sorted[s] = len[i];
-    d->tjb_sorted[s] = (uint8_t) msvh_tjb(len[i]);
-    off_sorted[s] = bytes;
-    bytes += ((int64_t) len[i] + 15) / 16 * 16;
+  for (int64_t sp = 0; sp < n; ++sp) {
+    d->sorted_of[order[sp]] = sp;
+    off_sorted[sp] = bytes;
+    bytes += ((int64_t) d->len_sorted[sp] + 15) / 16 * 16;
   }
   d->packed = bytes + 64;
-  std::vector<uint8_t> packed((size_t) bytes + 64, (uint8_t) d->R);
-  for (int64_t s = 0; s < n; ++s) {
-    const int64_t i = order[s];
-    const uint8_t *src = res + off[i];
-    uint8_t *dst = packed.data() + off_sorted[s];
-    for (int32_t k = 0; k < len[i]; ++k) dst[k] = rowof[src[k]];
-  }
-  // Distinct lengths.
-  std::vector<int32_t> ls(d->len_sorted);
-  std::sort(ls.begin(), ls.end());
-  ls.erase(std::unique(ls.begin(), ls.end()), ls.end());
-  d->Ls = ls;
-  d->lidx_sorted.resize(n);
-  for (int64_t s = 0; s < n; ++s)
-    d->lidx_sorted[s] = (int32_t) (std::lower_bound(ls.begin(), ls.end(), d->len_sorted[s]) - ls.begin());
+  // Remapped residues in sorted order, each target padded to 16 with the
+  // null row R, 64 more bytes of R at the end.
+  std::unique_ptr<uint8_t[]> packed(new uint8_t[(size_t) bytes + 64]);
+  std::memset(packed.get() + bytes, d->R, 64);
+  par(n, [&](int, int64_t lo, int64_t hi) {
+    for (int64_t sp = lo; sp < hi; ++sp) {
+      const int64_t i = order[sp];
+      const int32_t L = len[i];
+      const uint8_t *src = res + off[i];
+      uint8_t *dst = packed.get() + off_sorted[sp];
+      for (int32_t k = 0; k < L; ++k) dst[k] = rowof[src[k]];
+      std::memset(dst + L, d->R, (size_t) ((((int64_t) L + 15) & ~(int64_t) 15) - L));
+    }
+  });
   std::vector<int32_t> order32(n);
   for (int64_t i = 0; i < n; ++i) order32[i] = (int32_t) d->sorted_of[i];
 
-  HIPCHK(c, d->d_res.ensure(packed.size()));
-  HIPCHK(c, hipMemcpy(d->d_res.p, packed.data(), packed.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, d->d_res.ensure((size_t) bytes + 64));
+  HIPCHK(c, hipMemcpy(d->d_res.p, packed.get(), (size_t) bytes + 64, hipMemcpyHostToDevice));
   HIPCHK(c, d->d_off.ensure(std::max<int64_t>(n, 1) * 8));
   HIPCHK(c, hipMemcpy(d->d_off.p, off_sorted.data(), n * 8, hipMemcpyHostToDevice));
   HIPCHK(c, d->d_len.ensure(std::max<int64_t>(n, 1) * 4));

@@ -26,6 +26,14 @@
  * p7_oprofile_ReconfigLength() uses (p7_oprofile.c:897-962).  This path
  * makes one host-device round trip per target; msvg_search and
  * msvg_vit_search are the batch paths.
+ *
+ * p7_mi355x_PipelineBlock is the batch path inside HMMER: the block step of
+ * cudaHmmsearch (src/GPUFunction.cu:360-490: the GPU scores a block of
+ * sequences, then p7_PipelineShort, src/p7_pipeline.c:887-1000, runs the
+ * rest of p7_Pipeline on the host for each with its precomputed MSV score),
+ * with the ViterbiFilter of the block's MSV survivors scored on the GPU as
+ * well.  p7_PipelineShort calls p7_ViterbiFilter itself; inside the block
+ * step that call takes the precomputed score of its sequence.
  */
 #include "p7_config.h"
 
@@ -37,6 +45,7 @@
 #include "hmmer.h"
 
 #include "msv_gpu.h"
+#include "filters_mi355x.h"
 
 struct mi355x_state {
   msvg_ctx *ctx;
@@ -45,9 +54,17 @@ struct mi355x_state {
   size_t    keylen, keycap;
   uint8_t  *tmp;      /* the same for the profile of the current call         */
   size_t    tmpcap;
+  /* inside p7_mi355x_PipelineBlock: the ViterbiFilter scores of the block's
+   * MSV survivors, in block order                                            */
+  int             in_block;
+  const ESL_DSQ **vdsq;
+  int64_t        *vlen;
+  float          *vsc;
+  uint8_t        *vst;
+  int64_t         nv, vnext;
 };
 
-static __thread struct mi355x_state g_st = { NULL, -1, NULL, 0, 0, NULL, 0 };
+static __thread struct mi355x_state g_st = { NULL, -1, NULL, 0, 0, NULL, 0, 0, NULL, NULL, NULL, NULL, 0, 0 };
 
 static uint8_t *put(uint8_t *p, const void *src, size_t n) { memcpy(p, src, n); return p + n; }
 
@@ -187,9 +204,75 @@ int p7_ViterbiFilter(const ESL_DSQ *dsq, int L, const P7_OPROFILE *om, P7_OMX *o
   if (om->mode != p7_LOCAL && om->mode != p7_UNILOCAL)                                          /* :105 */
     ESL_EXCEPTION(eslEINVAL, "Fast filter only works for local alignment");
   ox->M = om->M;                                                                                /* :106 */
+  if (s->in_block) {               /* a survivor of the current block: its precomputed score */
+    int64_t k = s->vnext;
+    while (k < s->nv && s->vdsq[k] != dsq) k++;
+    if (k == s->nv) for (k = 0; k < s->nv && s->vdsq[k] != dsq; k++) ;
+    if (k < s->nv && s->vlen[k] == L) {
+      s->vnext = k + 1;
+      *ret_sc = s->vsc[k];
+      return s->vst[k] ? eslERANGE : eslOK;
+    }
+  }
   if ((status = prepare(s, om)) != eslOK) return status;
   status = msvg_ViterbiFilter(s->ctx, s->pid, dsq, L, ret_sc);
   if (status != MSVG_OK && status != MSVG_ERANGE)
     ESL_EXCEPTION(status, "p7_ViterbiFilter: %s", msvg_last_error(s->ctx));
+  return status;
+}
+
+int p7_mi355x_PipelineBlock(P7_PIPELINE *pli, P7_OPROFILE *om, P7_BG *bg, ESL_SQ **sq, int n, P7_TOPHITS *th)
+{
+  struct mi355x_state *s = &g_st;
+  uint8_t  *res = NULL, *st = NULL, *ps = NULL, *vst = NULL;
+  int64_t  *off = NULL, *tgt = NULL, *vlen = NULL;
+  int32_t  *len = NULL, did = -1;
+  float    *usc = NULL, *vsc = NULL;
+  const ESL_DSQ **vdsq = NULL;
+  int64_t   total = 0, nv = 0;
+  int       i, gpu = MSVG_OK, status = eslOK;
+
+  if (n <= 0) return eslOK;
+  if ((status = prepare(s, om)) != eslOK) return status;
+  for (i = 0; i < n; i++) total += sq[i]->n;
+  res = malloc(total > 0 ? (size_t) total : 1);
+  off = malloc(sizeof(int64_t) * n);   len = malloc(sizeof(int32_t) * n);
+  usc = malloc(sizeof(float) * n);     st = malloc(n);   ps = malloc(n);
+  tgt = malloc(sizeof(int64_t) * n);   vsc = malloc(sizeof(float) * n);   vst = malloc(n);
+  vdsq = malloc(sizeof(ESL_DSQ *) * n);  vlen = malloc(sizeof(int64_t) * n);
+  if (!res || !off || !len || !usc || !st || !ps || !tgt || !vsc || !vst || !vdsq || !vlen) { gpu = MSVG_EMEM; goto DONE; }
+  total = 0;
+  for (i = 0; i < n; i++) {        /* digital residues without the sentinels */
+    off[i] = total;
+    len[i] = (int32_t) sq[i]->n;
+    if (sq[i]->n > 0) memcpy(res + total, sq[i]->dsq + 1, (size_t) sq[i]->n);
+    total += sq[i]->n;
+  }
+  if ((gpu = msvg_db_add(s->ctx, res, off, len, n, &did)) != MSVG_OK) goto DONE;
+  if ((gpu = msvg_search(s->ctx, &s->pid, 1, did, pli->F1, MSVG_MODE_BATCHED, usc, st, ps)) != MSVG_OK) goto DONE;
+  for (i = 0; i < n; i++)          /* the MSV survivors: p7_PipelineShort's first test, p7_pipeline.c:917-919 */
+    if (ps[i] && sq[i]->n > 0) tgt[nv++] = i;
+  if (nv > 0 && (gpu = msvg_vit_search(s->ctx, s->pid, did, tgt, nv, vsc, vst)) != MSVG_OK) goto DONE;
+  for (i = 0; i < nv; i++) { vdsq[i] = sq[tgt[i]]->dsq; vlen[i] = sq[tgt[i]]->n; }
+  s->vdsq = vdsq; s->vlen = vlen; s->vsc = vsc; s->vst = vst; s->nv = nv; s->vnext = 0;
+  s->in_block = 1;
+  for (i = 0; i < n && status == eslOK; i++) {     /* GPUFunction.cu:474-488 */
+    p7_pli_NewSeq(pli, sq[i]);
+    /* p7_PipelineShort returns at its MSV test (p7_pipeline.c:917-919) for
+     * the sequences whose pass bit is 0, with nothing changed: the same
+     * F1 decision on the same usc */
+    if (!ps[i]) continue;
+    p7_bg_SetLength(bg, sq[i]->n);
+    p7_oprofile_ReconfigLength(om, sq[i]->n);
+    status = p7_PipelineShort(pli, om, bg, sq[i], th, usc[i]);
+    p7_pipeline_Reuse(pli);
+  }
+  s->in_block = 0;
+  s->nv = 0;
+ DONE:
+  if (did >= 0) msvg_db_remove(s->ctx, did);
+  free(res); free(off); free(len); free(usc); free(st); free(ps); free(tgt); free(vsc); free(vst);
+  free(vdsq); free(vlen);
+  if (gpu != MSVG_OK) ESL_EXCEPTION(gpu, "p7_mi355x_PipelineBlock: %s", msvg_last_error(s->ctx));
   return status;
 }

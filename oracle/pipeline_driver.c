@@ -17,10 +17,17 @@
  * src/hmmsearch.c:549-559; the stopwatch is left out so the output holds no
  * timing.  Own code; only reference functions are called.
  *
- *   pipeline_driver <hmmfile> <seqfile> <outfile> [F1 [nobias]]
+ *   oracle/_ref/pipeline_block    the same library as pipeline_mi355x, the
+ *                                 database searched as one block by
+ *                                 p7_mi355x_PipelineBlock (both filters
+ *                                 batched on the GPU, then p7_PipelineShort)
+ *
+ *   pipeline_driver <hmmfile> <seqfile> <outfile> [F1|- [nobias|bias [nolines]]]
  *
  * After each query's report it lists, for every target, p7_MSVFilter's and
- * p7_ViterbiFilter's status and the float bits of their scores.
+ * p7_ViterbiFilter's status and the float bits of their scores (not with
+ * nolines).  The seconds spent in the search of each query, sequence reading
+ * included, go to stderr.
  */
 #include "p7_config.h"
 
@@ -28,11 +35,23 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <time.h>
+
 #include "easel.h"
 #include "esl_alphabet.h"
 #include "esl_sq.h"
 #include "esl_sqio.h"
 #include "hmmer.h"
+#ifdef MI355X_BLOCK
+#include "filters_mi355x.h"
+#endif
+
+static double now(void)
+{
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double) t.tv_sec + 1e-9 * (double) t.tv_nsec;
+}
 
 static void die(const char *msg, const char *arg)
 {
@@ -50,11 +69,12 @@ int main(int argc, char **argv)
   P7_BG       *bg = NULL;
   FILE        *ofp;
   double       F1 = -1.0;
-  int          nobias = 0, nquery = 0, hstatus, sstatus;
+  int          nobias = 0, nolines = 0, nquery = 0, hstatus, sstatus;
 
-  if (argc < 4) die("usage: pipeline_driver <hmmfile> <seqfile> <outfile> [F1 [nobias]]", NULL);
-  if (argc > 4) F1 = atof(argv[4]);
+  if (argc < 4) die("usage: pipeline_driver <hmmfile> <seqfile> <outfile> [F1|- [nobias|bias [nolines]]]", NULL);
+  if (argc > 4 && strcmp(argv[4], "-") != 0) F1 = atof(argv[4]);
   if (argc > 5) nobias = (strcmp(argv[5], "nobias") == 0);
+  if (argc > 6) nolines = (strcmp(argv[6], "nolines") == 0);
   if (p7_hmmfile_OpenE(argv[1], NULL, &hfp, errbuf) != eslOK) die("cannot open HMM file", argv[1]);
   if ((ofp = fopen(argv[3], "w")) == NULL) die("cannot write", argv[3]);
 
@@ -86,17 +106,54 @@ int main(int argc, char **argv)
     ox = p7_omx_Create(om->M, 0, 0);
     sq = esl_sq_CreateDigital(abc);
 
+#ifdef MI355X_BLOCK
+    /* the whole database as one block: read, searched, then listed */
+    ESL_SQ **blk = NULL;
+    int      nblk = 0, capblk = 0, k = 0;
+    const double t0 = now();                 /* reading included, as in the serial loop */
+    for (;;) {
+      ESL_SQ *x = esl_sq_CreateDigital(abc);
+      if ((sstatus = esl_sqio_Read(sqfp, x)) != eslOK) { esl_sq_Destroy(x); break; }
+      if (nblk == capblk) {
+        capblk = 2 * capblk + 1024;
+        if ((blk = realloc(blk, sizeof(ESL_SQ *) * capblk)) == NULL) die("out of memory", NULL);
+      }
+      blk[nblk++] = x;
+    }
+    if (sstatus != eslEOF) die("sequence file parse failed:", argv[2]);
+    {
+      const double t1 = now();
+      if (p7_mi355x_PipelineBlock(pli, wom, bg, blk, nblk, th) != eslOK) die("block pipeline failed:", pli->errbuf);
+      fprintf(stderr, "search_seconds %s %.6f (block step %.6f)\n", hmm->name, now() - t0, now() - t1);
+    }
+    while (k < nblk) {
+      esl_sq_Copy(blk[k], sq);
+      esl_sq_Destroy(blk[k++]);
+      sstatus = eslOK;
+#else
+    const double t0 = now();
     while ((sstatus = esl_sqio_Read(sqfp, sq)) == eslOK) {
+#endif
       float usc = 0.0f, vsc = 0.0f;
       int   st, vst;
       union { float f; uint32_t u; } bits;
       char  line[256];
       int   n;
 
+#ifndef MI355X_BLOCK
       p7_pli_NewSeq(pli, sq);
       p7_bg_SetLength(bg, sq->n);
       p7_oprofile_ReconfigLength(wom, sq->n);
       p7_Pipeline(pli, wom, bg, sq, th);
+#else
+      p7_oprofile_ReconfigLength(wom, sq->n);
+#endif
+      nseq++;
+      if (nolines) {
+        esl_sq_Reuse(sq);
+        p7_pipeline_Reuse(pli);
+        continue;
+      }
 
       st = p7_MSVFilter(sq->dsq, (int) sq->n, wom, ox, &usc);
       vst = p7_ViterbiFilter(sq->dsq, (int) sq->n, wom, ox, &vsc);
@@ -112,11 +169,15 @@ int main(int argc, char **argv)
       }
       memcpy(lines + nlines, line, n);
       nlines += n;
-      nseq++;
       esl_sq_Reuse(sq);
       p7_pipeline_Reuse(pli);
     }
+#ifdef MI355X_BLOCK
+    free(blk);
+#else
     if (sstatus != eslEOF) die("sequence file parse failed:", argv[2]);
+    if (nolines) fprintf(stderr, "search_seconds %s %.6f\n", hmm->name, now() - t0);
+#endif
 
     p7_tophits_SortBySortkey(th);
     p7_tophits_Threshold(th, pli);

@@ -4,9 +4,13 @@ p7_Pipeline (src/p7_pipeline.c:646-...) calls p7_MSVFilter for every target
 (p7_pipeline.c:671) and p7_ViterbiFilter for the MSV survivors (:701), and
 derives each filter's pass from its score.  The serial search of
 oracle/pipeline_driver.c runs that pipeline over the reference's own library
-twice: oracle/_ref/pipeline_ref with the reference's SSE filters,
+three times: oracle/_ref/pipeline_ref with the reference's SSE filters,
 oracle/_ref/pipeline_mi355x with cudahmmsearch_amd/hmmer/filters_mi355x.c
-(libmsvgpu on the GPU) in their place, nothing else changed.  Their output
+(libmsvgpu on the GPU) in their place, nothing else changed, and
+oracle/_ref/pipeline_block, which hands the whole database to
+p7_mi355x_PipelineBlock (both filters batched on the GPU, then the rest of
+the pipeline per sequence through p7_PipelineShort, as cudaHmmsearch's
+block step does, src/GPUFunction.cu:470-490).  Their output
 files -- hit tables, domain alignments, tabular output, the pipeline's
 filter pass counts and a line per target with both filters' statuses and
 score bits -- must be byte-identical.  Inputs: the reference's tutorial
@@ -25,6 +29,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REFBIN = os.path.join(ROOT, "oracle", "_ref", "pipeline_ref")
 GPUBIN = os.path.join(ROOT, "oracle", "_ref", "pipeline_mi355x")
+BLOCKBIN = os.path.join(ROOT, "oracle", "_ref", "pipeline_block")
 INPUTS = os.path.join(ROOT, "tests", "golden", "hmmer")
 AMINO = "ACDEFGHIKLMNPQRSTVWY"
 # Robinson & Robinson background frequencies, as p7_AminoFrequencies (hmmer.c:163-186) lists them.
@@ -32,7 +37,7 @@ BGF = np.array([0.0787945, 0.0151600, 0.0535222, 0.0668298, 0.0397062, 0.0695071
                 0.0594422, 0.0963728, 0.0237718, 0.0414386, 0.0482904, 0.0395639, 0.0540978, 0.0683364,
                 0.0540687, 0.0673417, 0.0114135, 0.0304133])
 
-needs_bins = pytest.mark.skipif(not (os.path.exists(REFBIN) and os.path.exists(GPUBIN)),
+needs_bins = pytest.mark.skipif(not all(os.path.exists(b) for b in (REFBIN, GPUBIN, BLOCKBIN)),
                                 reason="pipeline binaries not built (make -C oracle ref dropin)")
 
 
@@ -101,9 +106,9 @@ def synthetic_fasta(path, n, seed, donors):
                 fh.write("\n")
 
 
-def _run_both(tmp, hmm, seqs, extra=()):
+def _run_all(tmp, hmm, seqs, extra=()):
     outs = []
-    for b in (REFBIN, GPUBIN):
+    for b in (REFBIN, GPUBIN, BLOCKBIN):
         out = os.path.join(tmp, os.path.basename(b) + ".out")
         r = subprocess.run([b, hmm, seqs, out, *extra], capture_output=True, text=True, timeout=600)
         assert r.returncode == 0, f"{os.path.basename(b)} failed: {r.stderr[-2000:]}"
@@ -126,8 +131,9 @@ def _check_same(ref, gpu):
 def test_tutorial_fn3_vs_7less(tmp_path):
     hmm = _unpack("fn3.hmm", tmp_path)
     seq = _unpack("7LESS_DROME", tmp_path)
-    ref, gpu = _run_both(str(tmp_path), hmm, seq)
+    ref, gpu, blk = _run_all(str(tmp_path), hmm, seq)
     _check_same(ref, gpu)
+    _check_same(ref, blk)
     # the shipped answer tutorial/fn3.out:110: the one target passes the MSV filter
     assert any(l.startswith("Passed MSV filter:") and l.split()[3] == "1" for l in ref.splitlines())
 
@@ -135,15 +141,17 @@ def test_tutorial_fn3_vs_7less(tmp_path):
 @pytest.mark.gpu
 @needs_bins
 def test_tutorial_globins(tmp_path):
-    ref, gpu = _run_both(str(tmp_path), _unpack("globins4.hmm", tmp_path), _unpack("globins45.fa", tmp_path))
+    ref, gpu, blk = _run_all(str(tmp_path), _unpack("globins4.hmm", tmp_path), _unpack("globins45.fa", tmp_path))
     _check_same(ref, gpu)
+    _check_same(ref, blk)
 
 
 @pytest.mark.gpu
 @needs_bins
 def test_tutorial_dna(tmp_path):
-    ref, gpu = _run_both(str(tmp_path), _unpack("MADE1.hmm", tmp_path), _unpack("dna_target.fa", tmp_path))
+    ref, gpu, blk = _run_all(str(tmp_path), _unpack("MADE1.hmm", tmp_path), _unpack("dna_target.fa", tmp_path))
     _check_same(ref, gpu)
+    _check_same(ref, blk)
 
 
 def _models(tmp, names):
@@ -172,8 +180,9 @@ def test_synthetic_targets(tmp_path, models, extra):
     tmp = str(tmp_path)
     seqs = os.path.join(tmp, "targets.fa")
     synthetic_fasta(seqs, 700, 2024, _donors(tmp))
-    ref, gpu = _run_both(tmp, _models(tmp, models), seqs, extra)
+    ref, gpu, blk = _run_all(tmp, _models(tmp, models), seqs, extra)
     _check_same(ref, gpu)
+    _check_same(ref, blk)
     assert ref.count("\nmsv ") == len(models) * 700
     passed = [int(l.split()[3]) for l in ref.splitlines() if l.startswith("Passed MSV filter:")]
     assert len(passed) == len(models) and all(p > 0 for p in passed)
@@ -188,7 +197,10 @@ def test_synthetic_targets(tmp_path, models, extra):
 def test_dropin_fails_loudly_without_gpu(tmp_path):
     """No silent CPU fallback: without a device the drop-in filters raise
     through Easel's exception handler and the search stops."""
-    r = subprocess.run([GPUBIN, _unpack("fn3.hmm", tmp_path), _unpack("HBB_HUMAN", tmp_path),
-                        os.path.join(str(tmp_path), "o.txt")], capture_output=True, text=True, timeout=120)
-    assert r.returncode != 0
-    assert "no MI355X device" in r.stderr
+    for b in (GPUBIN, BLOCKBIN):
+        if not os.path.exists(b):
+            continue
+        r = subprocess.run([b, _unpack("fn3.hmm", tmp_path), _unpack("HBB_HUMAN", tmp_path),
+                            os.path.join(str(tmp_path), "o.txt")], capture_output=True, text=True, timeout=120)
+        assert r.returncode != 0
+        assert "no MI355X device" in r.stderr

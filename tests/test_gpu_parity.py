@@ -424,9 +424,21 @@ def test_scan_dna_and_mixed_alphabets(ctx, golden):
         ru, rs, rp = ctx.search([pd_], did, mode=ca.MODE_PLAIN)
         ctx.remove_database(did)
         assert np.array_equal(u.view(np.uint32), ru[:, 0].view(np.uint32)) and np.array_equal(s, rs[:, 0])
+    # a batch (the grouped kernel, tables over the 18 DNA codes) with a
+    # degenerate-code query and an empty one
+    qs = [rng.integers(0, 4, int(L)).astype(np.uint8) for L in (3, 60, 400, 900, 2000)]
+    qs += [rng.integers(0, 18, 300).astype(np.uint8), np.zeros(0, np.uint8)]
+    pd2 = ctx.add_profile(_bp(dna))
+    u, s, ps = ctx.scan_many([pd_, pd2], qs)
+    did = ctx.add_database(ca.Database.from_list(qs))
+    ru, rs, rp = ctx.search([pd_, pd2], did, mode=ca.MODE_PLAIN)
+    ctx.remove_database(did)
+    assert np.array_equal(u.view(np.uint32), ru.T.view(np.uint32)) and np.array_equal(s, rs.T)
+    assert np.array_equal(ps, rp.T)
     with pytest.raises(ca.MSVError):
         ctx.scan([pd_, pa], np.array([1, 2], np.uint8))
     ctx.remove_profile(pd_)
+    ctx.remove_profile(pd2)
     ctx.remove_profile(pa)
 
 

@@ -258,6 +258,28 @@ def test_empty_and_zero_length(ctx):
     ctx.remove_profile(pid)
 
 
+def test_megabase_targets(ctx):
+    """Targets longer than 2^20 residues (the database build's comparison-sort
+    path; the two of equal length keep the caller's order) beside short and
+    empty ones: every result equals the oracle's, in both modes."""
+    rng = np.random.default_rng(17)
+    prof = ca.sample_profile(33, 60)
+    pid = ctx.add_profile(prof)
+    prof.mu = ctx.calibrate(pid, prof.lam)
+    big = 1_100_000
+    seqs = [rng.integers(0, 20, big).astype(np.uint8), rng.integers(0, 20, 300).astype(np.uint8),
+            np.zeros(0, np.uint8), rng.integers(0, 20, big).astype(np.uint8), rng.integers(0, 20, 45).astype(np.uint8)]
+    frag = ca.core_emit(prof.hmm, 5, 1)[0]
+    seqs[3][500_000:500_000 + frag.size] = frag                            # a planted homolog in one of them
+    db = ca.Database.from_list(seqs)
+    print("megabase targets: plain", flush=True)
+    _, did, *_ = _check_vs_oracle(ctx, prof, db, ca.MODE_PLAIN, pid=pid)
+    print("megabase targets: batched", flush=True)
+    _check_vs_oracle(ctx, prof, db, ca.MODE_BATCHED, pid=pid, did=did)
+    ctx.remove_database(did)
+    ctx.remove_profile(pid)
+
+
 def test_order_invariance(ctx):
     """Permuting the database permutes the results (sorting is internal)."""
     prof, db = _workload(21, 300, 1500, 300, 0.8, 10, 4000)

@@ -995,9 +995,13 @@ int msvg_db_add(msvg_ctx *c, const uint8_t *res, const int64_t *off, const int32
   const int nth = (int) std::max<int64_t>(1, std::min<int64_t>({(int64_t) 16, n / 16384 + 1,
                                                                  (int64_t) std::max(1u, std::thread::hardware_concurrency())}));
   auto par = [nth](int64_t count, const std::function<void(int, int64_t, int64_t)> &fn) {
-    if (nth == 1) { fn(0, 0, count); return; }
     std::vector<std::thread> th;
-    for (int t = 1; t < nth; ++t) th.emplace_back(fn, t, count * t / nth, count * (t + 1) / nth);
+    int t = 1;
+    try {
+      for (; t < nth; ++t) th.emplace_back(fn, t, count * t / nth, count * (t + 1) / nth);
+    } catch (...) {                    // a thread that cannot start: its ranges run here
+    }
+    for (int u = t; u < nth; ++u) fn(u, count * u / nth, count * (u + 1) / nth);
     fn(0, 0, count / nth);
     for (auto &x : th) x.join();
   };
@@ -1858,11 +1862,14 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
       }
     }
   };
-  if (nth == 1) {
-    work(0);
-  } else {
+  {
     std::vector<std::thread> th;
-    for (int t = 1; t < nth; ++t) th.emplace_back(work, t);
+    int t = 1;
+    try {
+      for (; t < nth; ++t) th.emplace_back(work, t);
+    } catch (...) {                    // a thread that cannot start: its queries are decoded here
+    }
+    for (int u = t; u < nth; ++u) work(u);
     work(0);
     for (auto &x : th) x.join();
   }

@@ -195,3 +195,34 @@ def test_comments_and_blank_lines_are_skipped(tmp_path):
     (n0, p0, v0), = ca.read_hmm(_write(tmp_path, "g.hmm", text))
     (n1, p1, v1), = ca.read_hmm(_write(tmp_path, "g.noisy", noisy))
     assert n0 == n1 and np.array_equal(p0.rbv, p1.rbv) and np.array_equal(v0.twv, v1.twv)
+
+
+TS_MODELS = ["20aa", "3box", "Caudal_act", "LuxC", "M1", "PSE", "Patched", "RRM_1", "SMC_N", "XYPPX", "ecori"]
+
+
+@pytest.mark.parametrize("name", TS_MODELS)
+def test_testsuite_models_equal_compiled_reference(tmp_path, name):
+    """The reference's testsuite models (HMMER3/e; amino and DNA; M = 1 to
+    1142): every MSV and ViterbiFilter field from read_hmm equals what the
+    compiled reference builds from the same file (p7_hmmfile_Read,
+    p7_ProfileConfig at L = 100, p7_oprofile_Convert; oracle/_ref)."""
+    from oracle import Reference, vprofile_from_fields
+    try:
+        R = Reference()
+    except FileNotFoundError:
+        pytest.skip("oracle/_ref is not built (make -C oracle ref)")
+    with gzip.open(os.path.join(INPUTS, f"testsuite_{name}.hmm.gz"), "rt") as fh:
+        path = _write(tmp_path, name + ".hmm", fh.read())
+    (n, prof, vit), = ca.read_hmm(path)
+    rp = R.from_file(path, 0)
+    g = rp.byte_profile()
+    assert prof.rbv.shape == g["rbv"].shape and np.array_equal(prof.rbv, g["rbv"]), name
+    assert (prof.tbm, prof.tec, prof.base, prof.bias) == (g["tbm"], g["tec"], g["base"], g["bias"])
+    assert _f32(prof.scale) == _f32(g["scale"]) and _f32(prof.mu) == _f32(g["mu"])
+    assert _f32(prof.lam) == _f32(g["lambda"])
+    v = vprofile_from_fields(rp.vit_fields())
+    assert np.array_equal(vit.rwv, v["rwv"]) and np.array_equal(vit.twv, v["twv"]), name
+    assert (vit.e_loop, vit.e_move, vit.base_w, vit.ddbound_w) == (v["e_loop"], v["e_move"], v["base_w"],
+                                                                   v["ddbound_w"])
+    assert _f32(vit.scale_w) == _f32(v["scale_w"]) and _f32(vit.nj) == _f32(v["nj"])
+    assert _f32(vit.vmu) == _f32(v["vmu"]) and _f32(vit.vlambda) == _f32(v["vlambda"])

@@ -76,6 +76,7 @@ struct FastArgs {
   uint16_t *scan_out;          // mode 3: result word of (workgroup b, target s) at scan_out[s * scan_stride + b];
   int64_t scan_stride;         // no queues are written
   int32_t scan_nprof;          // mode 3: profiles; the grid is a multiple of it
+  const uint8_t *scan_rowcode; // mode 3: LDS row r holds table row scan_rowcode[r] (r = 0..db.R)
 };
 
 // J-aware stage: the MSV recurrence with per-row E/J/B updates in the f16
@@ -113,6 +114,7 @@ struct ScanArgs {              // one workgroup per profile, its waves over the 
   uint32_t *out;
   const int32_t *wq_off;       // non-null: workgroup b scores only the queries wq_idx[wq_off[b] ..
   const int32_t *wq_idx;       // wq_off[b+1]), the result of item k at out[k]
+  const uint8_t *rowcode;      // LDS row r holds table row rowcode[r] (r = 0..R; R is the null row)
 };
 
 struct SlowArgs {

@@ -260,7 +260,15 @@ __global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
   // the targets (the (b / scan_nprof)-th of gridDim.x / scan_nprof splits)
   const int sp = (a.mode == 3) ? (int) (blockIdx.x % (unsigned) a.scan_nprof) : 0;
   const uint4 *ftab = (a.mode == 3) ? a.scan_profs[sp].ftab : a.prof.ftab;
-  for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = ftab[i];
+  if (a.mode == 3) {                   // the rows of the codes present, then the null row
+    const int row = QB * TWG;
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x) {
+      const int r = i / row;
+      lds[i] = ftab[(int) a.scan_rowcode[r] * row + (i - r * row)];
+    }
+  } else {
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = ftab[i];
+  }
   __syncthreads();
 
   const int g = lane / G;
@@ -584,13 +592,13 @@ __global__ void __launch_bounds__(256) msv_scan(ScanArgs a) {
   constexpr int QB = (Q + 3) / 4;
   const ScanProf *sp = a.profs + blockIdx.x;
   const uint4 *tab = sp->tab;
-  const int ntab = (a.R + 1) * QB * 64;
-  int i = threadIdx.x;
-  for (; i + 768 < ntab; i += 1024) {
-    const uint4 v0 = tab[i], v1 = tab[i + 256], v2 = tab[i + 512], v3 = tab[i + 768];
-    lds[i] = v0; lds[i + 256] = v1; lds[i + 512] = v2; lds[i + 768] = v3;
+  // the table rows of the codes the queries hold (rowcode), then the null row
+  constexpr int ROW = QB * 64;
+  for (int r = 0; r <= a.R; ++r) {
+    const uint4 *src = tab + (int) a.rowcode[r] * ROW;
+    uint4 *dst = lds + r * ROW;
+    for (int i = threadIdx.x; i < ROW; i += 256) dst[i] = src[i];
   }
-  for (; i < ntab; i += 256) lds[i] = tab[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;

@@ -1618,6 +1618,22 @@ int scan_lanes(int M) { return M <= 128 ? 8 : M <= 768 ? 16 : M <= 1024 ? 32 : 6
 // fall into fewer (larger) launches at a padding cost below 1/8.
 int scan_regs(int Q) { return Q <= 8 ? Q : Q <= 16 ? (Q + 1) & ~1 : std::min(32, (Q + 3) & ~3); }
 
+// The table rows a batch of queries uses: the codes present (rowcode[r],
+// r < R, ascending) and the null row R, which takes table row Kp; rowof maps
+// a code to its row.  Returns R.
+int scan_rows(const uint8_t *res, const int64_t *qoff, const int32_t *qlen, const std::vector<int32_t> &qs, int Kp,
+              uint8_t rowof[256], std::vector<uint8_t> &rowcode) {
+  bool seen[256] = {false};
+  for (int32_t q : qs)
+    for (int32_t k = 0; k < qlen[q]; ++k) seen[res[qoff[q] + k]] = true;
+  rowcode.clear();
+  for (int x = 0; x < Kp; ++x)
+    if (seen[x]) { rowof[x] = (uint8_t) rowcode.size(); rowcode.push_back((uint8_t) x); }
+  const int R = (int) rowcode.size();
+  rowcode.push_back((uint8_t) Kp);
+  return R;
+}
+
 // Grouped tables and classes of a scan set (every profile that has a
 // one-query class has one: at G = 64 the grouped table is no larger).
 int scan_prepare_grouped(msvg_ctx *c, ScanSet *set) {
@@ -1697,11 +1713,17 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
   std::stable_sort(order.begin(), order.end(), [qlen](int32_t x, int32_t y) { return qlen[x] > qlen[y]; });
   int64_t padded = 0;
   for (int32_t q : order) padded += ((int64_t) qlen[q] + 15) & ~(int64_t) 15;
-  // staging: codes padded to 16 with the null row Kp, then offsets, lengths, tjb_b
+  uint8_t rowof[256];
+  std::vector<uint8_t> rowcode;
+  const int R = scan_rows(res, qoff, qlen, order, Kp, rowof, rowcode);
+  // staging: row indices of the codes (only the rows the batch uses are
+  // staged in LDS) padded to 16 with the null row R, then offsets, lengths,
+  // tjb_b and the row list
   const size_t o_off = ((size_t) padded + 15) & ~(size_t) 15;
   const size_t o_len = o_off + (size_t) nl * sizeof(int64_t);
   const size_t o_tjb = o_len + (size_t) nl * sizeof(int32_t);
-  const size_t total = o_tjb + (size_t) nl;
+  const size_t o_rc = o_tjb + (size_t) nl;
+  const size_t total = o_rc + rowcode.size();
   if (c->h_one_cap < total + 64) {
     if (c->h_one_res) { (void) hipHostFree(c->h_one_res); c->h_one_res = nullptr; c->h_one_cap = 0; }
     const size_t cap = std::max<size_t>(total + 64, 4096);
@@ -1712,13 +1734,15 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
   int64_t *h_off = reinterpret_cast<int64_t *>(h + o_off);
   int32_t *h_len = reinterpret_cast<int32_t *>(h + o_len);
   uint8_t *h_tjb = h + o_tjb;
+  std::memcpy(h + o_rc, rowcode.data(), rowcode.size());
   std::vector<int> tjb_k((size_t) nl);
   int64_t pos = 0;
   for (int32_t k = 0; k < nl; ++k) {
     const int32_t q = order[(size_t) k];
     const int64_t Lq = qlen[q], Lpad = (Lq + 15) & ~(int64_t) 15;
-    std::memcpy(h + pos, res + qoff[q], (size_t) Lq);
-    std::memset(h + pos + Lq, Kp, (size_t) (Lpad - Lq));
+    const uint8_t *src = res + qoff[q];
+    for (int64_t i = 0; i < Lq; ++i) h[pos + i] = rowof[src[i]];
+    std::memset(h + pos + Lq, R, (size_t) (Lpad - Lq));
     h_off[k] = pos; h_len[k] = (int32_t) Lq;
     tjb_k[(size_t) k] = msvh_tjb((int) Lq);
     h_tjb[k] = (uint8_t) tjb_k[(size_t) k];
@@ -1744,7 +1768,7 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
   qdb.len = reinterpret_cast<const int32_t *>(d + o_len);
   qdb.tjb = d + o_tjb;
   qdb.n = nl;
-  qdb.R = Kp;
+  qdb.R = R;
   uint16_t *d_words = c->d_scanout.as<uint16_t>();
   HIPCHK(c, hipEventRecord(c->ev[0], st));
   const int ns = std::min<int>(kVitStreams, (int) set->gcls.size());
@@ -1768,6 +1792,7 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
     a.scan_profs = gc->d_profs.as<ProfDev>();
     a.scan_out = d_words + base;                            // words [sorted query][row]
     a.scan_stride = (int64_t) nmem;
+    a.scan_rowcode = d + o_rc;
     // few profiles (long models): all query groups at once; many: fewer
     // waves per workgroup so that several workgroups share each CU
     const int nprof = (int) gc->members.size();
@@ -1777,7 +1802,7 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
     const int nsplit = std::max(1, std::min((kScanCUs + nprof - 1) / nprof, (a.ntasks + waves - 1) / waves));
     a.scan_nprof = nprof;
     hipStream_t cs = ns > 1 ? c->vstream[ci % ns] : st;
-    HIPCHK(c, msvk_launch_fast(gc->G, gc->Q, &a, nprof * nsplit, 64 * waves, fast_lds_g(Kp, gc->Q, gc->G), cs));
+    HIPCHK(c, msvk_launch_fast(gc->G, gc->Q, &a, nprof * nsplit, 64 * waves, fast_lds_g(R, gc->Q, gc->G), cs));
     base += gc->members.size();
   }
   if (ns > 1) {
@@ -1930,12 +1955,13 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
     a.profs = reinterpret_cast<const ScanProf *>(dj + pt.o_prof);
     a.res = d;
     a.qoff = qdb.off; a.qlen = qdb.len; a.qtjb = qdb.tjb;
-    a.nq = nl; a.R = Kp; a.stride = 0;
+    a.nq = nl; a.R = R; a.stride = 0;
+    a.rowcode = d + o_rc;
     a.out = reinterpret_cast<uint32_t *>(dj + o_jout) + pt.o_out;
     a.wq_off = reinterpret_cast<const int32_t *>(dj + pt.o_woff);
     a.wq_idx = reinterpret_cast<const int32_t *>(dj + pt.o_widx);
     const int Q = set->cls[pt.ci]->Q;
-    HIPCHK(c, msvk_launch_scan(Q, &a, (int) pt.nwg, fast_lds(Kp, Q), st));
+    HIPCHK(c, msvk_launch_scan(Q, &a, (int) pt.nwg, fast_lds(R, Q), st));
   }
   HIPCHK(c, hipEventRecord(c->ev[3], st));
   std::vector<uint32_t> jw(nout);
@@ -2004,9 +2030,13 @@ int msvg_scan_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t nq, con
     if ((rc = scan_grouped(c, set, np, res, qoff, qlen, F1, usc, status, pass, qlive, slow)) != MSVG_OK) return rc;
     c->stats.ms_total = c->stats.ms_fast;
   } else if (nl > 0) {
-    // stage the live queries: codes padded to 16 bytes with the null row Kp,
-    // then offsets, lengths and tjb_b, in one pinned buffer
-    const size_t meta = (size_t) nl * (sizeof(int64_t) + sizeof(int32_t) + 1);
+    // stage the live queries: row indices of their codes (only the rows
+    // they use are staged in LDS) padded to 16 bytes with the null row R,
+    // then offsets, lengths, tjb_b and the row list, in one pinned buffer
+    uint8_t rowof[256];
+    std::vector<uint8_t> rowcode;
+    const int R = scan_rows(res, qoff, qlen, qlive, Kp, rowof, rowcode);
+    const size_t meta = (size_t) nl * (sizeof(int64_t) + sizeof(int32_t) + 1) + rowcode.size();
     const size_t need = (size_t) padded + meta + 64;
     if (c->h_one_cap < need) {
       if (c->h_one_res) { (void) hipHostFree(c->h_one_res); c->h_one_res = nullptr; c->h_one_cap = 0; }
@@ -2025,12 +2055,15 @@ int msvg_scan_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t nq, con
     int64_t *h_off = reinterpret_cast<int64_t *>(h + o_off);
     int32_t *h_len = reinterpret_cast<int32_t *>(h + o_off + (size_t) nl * sizeof(int64_t));
     uint8_t *h_tjb = h + o_off + (size_t) nl * (sizeof(int64_t) + sizeof(int32_t));
+    uint8_t *h_rc = h_tjb + nl;
+    std::memcpy(h_rc, rowcode.data(), rowcode.size());
     int64_t pos = 0;
     for (int32_t k = 0; k < nl; ++k) {
       const int32_t q = qlive[(size_t) k];
       const int64_t Lq = qlen[q], Lpad = (Lq + 15) & ~(int64_t) 15;
-      std::memcpy(h + pos, res + qoff[q], (size_t) Lq);
-      std::memset(h + pos + Lq, Kp, (size_t) (Lpad - Lq));
+      const uint8_t *src = res + qoff[q];
+      for (int64_t i = 0; i < Lq; ++i) h[pos + i] = rowof[src[i]];
+      std::memset(h + pos + Lq, R, (size_t) (Lpad - Lq));
       h_off[k] = pos;
       h_len[k] = (int32_t) Lq;
       h_tjb[k] = (uint8_t) msvh_tjb((int) Lq);
@@ -2065,11 +2098,12 @@ int msvg_scan_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t nq, con
       a.qoff = reinterpret_cast<const int64_t *>(d + o_off);
       a.qlen = reinterpret_cast<const int32_t *>(d + o_off + (size_t) nl * sizeof(int64_t));
       a.qtjb = d + o_off + (size_t) nl * (sizeof(int64_t) + sizeof(int32_t));
-      a.nq = nl; a.R = Kp;
+      a.nq = nl; a.R = R;
+      a.rowcode = d + (h_rc - h);
       a.stride = np;
       a.out = c->d_scanout.as<uint32_t>() + base;
       hipStream_t cs = ns > 1 ? c->vstream[ci % ns] : st;
-      HIPCHK(c, msvk_launch_scan(cl->Q, &a, (int) cl->members.size(), fast_lds(Kp, cl->Q), cs));
+      HIPCHK(c, msvk_launch_scan(cl->Q, &a, (int) cl->members.size(), fast_lds(R, cl->Q), cs));
       base += (int64_t) cl->members.size();
     }
     if (ns > 1) {

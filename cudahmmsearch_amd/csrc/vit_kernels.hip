@@ -66,16 +66,35 @@ __device__ __forceinline__ uint32_t wave_max2(uint32_t v) {
 }
 
 
-// One step of the lane scan of maps x -> max(c, x + T) (floor -32768): the
-// source lane's map (c, T) from DPP control CTRL (rows outside ROW_MASK, and
-// lanes whose source lies outside their row, read the identity (-32768, 0))
-// composed before this lane's.
+// The lane scan composes maps x -> max(c, x + T) (floor -32768), the source
+// lane's map (from DPP control CTRL; rows outside ROW_MASK, and lanes whose
+// source lies outside their row, read the identity (-32768, 0)) before this
+// lane's: c <- max(c, c_src + T), T <- T_src + T.  The T terms are sums of
+// tDD over slots, the same on every row, so they are scanned once per kernel
+// (t_next) and each row's step only carries c (c_step).  No floor is needed
+// there: c >= -32768 always, so max(c, max(-32768, c_src + T)) =
+// max(c, c_src + T).  T is kept negated in 16 bits, saturated at 65535
+// (negsat): a T <= -65535 acts as minus infinity either way, because
+// c_src + T <= 32767 - 65535 = -32768 <= c.
 template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ void scan_step(int &c, int &T) {
+__device__ __forceinline__ int t_next(int T) {
+  return T + __builtin_amdgcn_update_dpp(0, T, CTRL, ROW_MASK, 0xf, false);
+}
+__device__ __forceinline__ uint32_t negsat(int T) { return (uint32_t) min(-T, 65535); }
+__device__ __forceinline__ uint32_t negsat2(int Ta, int Tb) { return negsat(Ta) | (negsat(Tb) << 16); }
+// c <- max(c, c_src + T), T the HALF-th 16-bit field of tp, negated.
+template <int CTRL, int ROW_MASK, int HALF>
+__device__ __forceinline__ int c_step(int c, uint32_t tp) {
   const int co = __builtin_amdgcn_update_dpp(-32768, c, CTRL, ROW_MASK, 0xf, false);
-  const int To = __builtin_amdgcn_update_dpp(0, T, CTRL, ROW_MASK, 0xf, false);
-  c = max(c, fT(co, T));
-  T = To + T;
+  const int nt = HALF ? (int) (tp >> 16) : (int) (tp & 0xffffu);
+  return max(c, co - nt);
+}
+// (max over both halves of xEv, max over both halves of Dmv) as one packed
+// pair: two byte permutes and one v_pk_max_i16.
+__device__ __forceinline__ uint32_t halves_max(uint32_t xEv, uint32_t Dmv) {
+  const uint32_t lo = __builtin_amdgcn_perm(Dmv, xEv, 0x05040100u);   // (xEv.lo, Dmv.lo)
+  const uint32_t hi = __builtin_amdgcn_perm(Dmv, xEv, 0x07060302u);   // (xEv.hi, Dmv.hi)
+  return maxs(lo, hi);
 }
 
 }  // namespace
@@ -86,6 +105,8 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
   __shared__ uint32_t red[16], dlast[16], eM[16], eI[16], eD[16];
   __shared__ int scC[16], scT[16];
   constexpr int C = 64 * W;                            // lanes per target
+  __shared__ uint4 tsc[C];                             // the lane scan's T terms per lane
+  __shared__ uint32_t tse[W > 1 ? C : 1];              // and its cross-wave terms
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int w = (W == 1) ? 0 : wave;
@@ -106,6 +127,22 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
   int Tlo = 0, Thi = 0;                                // tDD summed over each slot
 #pragma unroll
   for (int j = 0; j < Q; ++j) { Tlo += lo16(tDD[j]); Thi += hi16(tDD[j]); }
+  // the lane scan's T terms, once: the lane's two slots, before each of the
+  // six steps, after the last, and the previous lane's (exclusive) total
+  const int Ts0 = Tlo + Thi;
+  const int Ts1 = t_next<0x111, 0xf>(Ts0);
+  const int Ts2 = t_next<0x112, 0xf>(Ts1);
+  const int Ts3 = t_next<0x114, 0xf>(Ts2);
+  const int Ts4 = t_next<0x118, 0xf>(Ts3);
+  const int Ts5 = t_next<0x142, 0xa>(Ts4);
+  const int Ts6 = t_next<0x143, 0xc>(Ts5);
+  const int Tex = __builtin_amdgcn_update_dpp(0, Ts6, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+  // kept in LDS, read on the rows that run the closure
+  if (W > 1 || threadIdx.x < 64) {
+    tsc[c] = make_uint4(negsat2(Tlo, Thi), negsat2(Ts0, Ts1), negsat2(Ts2, Ts3), negsat2(Ts4, Ts5));
+    if constexpr (W > 1) tse[c] = negsat2(Ts6, Tex);
+  }
+  __syncthreads();
 
   const uint32_t *et = a.etab;
   if (W == 1 && a.etab_in_lds) {
@@ -175,7 +212,7 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
           Dmv = maxs(Dmv, dn);                                       // :168
           if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;              // :162
         }
-        uint32_t r = wave_max2(pack16(max(lo16(xEv), hi16(xEv)), max(lo16(Dmv), hi16(Dmv))));
+        uint32_t r = wave_max2(halves_max(xEv, Dmv));
         uint32_t dedge = NEG2;
         if constexpr (W > 1) {
           const uint32_t dl = __builtin_amdgcn_readlane(dcv, 63);
@@ -199,34 +236,37 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
 
         Dr[0] = shift_in(dcv, dedge, lane);                          // :230-231 / :203-204
         if (Dmax + a.ddbound_w > xB) {                               // :199
+          const uint4 tt = tsc[c];
           // closure inside each slot
 #pragma unroll
           for (int j = 1; j < Q; ++j) Dr[j] = maxs(Dr[j], adds(Dr[j - 1], tDD[j - 1]));
           const uint32_t outl = adds(Dr[Q - 1], tDD[Q - 1]);
           const int clo = lo16(outl);
+          const uint32_t tsl = tt.x, ts01 = tt.y, ts23 = tt.z, ts45 = tt.w;
           // the lane's two slots as one map x -> max(c, x + T): low slot, then high
-          int cs = max(hi16(outl), fT(clo, Thi)), Ts = Tlo + Thi;
+          int cs = max(hi16(outl), clo - (int) (tsl >> 16));        // clo through the high slot
           // inclusive scan over the lanes (lane l: lanes 0..l composed, in
           // order), in DPP: row_shr 1/2/4/8 inside each row of 16 lanes,
           // then row_bcast 15 and 31 across rows; lanes without a source
           // take the identity map (-32768, 0), which leaves them unchanged
-          scan_step<0x111, 0xf>(cs, Ts);
-          scan_step<0x112, 0xf>(cs, Ts);
-          scan_step<0x114, 0xf>(cs, Ts);
-          scan_step<0x118, 0xf>(cs, Ts);
-          scan_step<0x142, 0xa>(cs, Ts);
-          scan_step<0x143, 0xc>(cs, Ts);
+          cs = c_step<0x111, 0xf, 0>(cs, ts01);
+          cs = c_step<0x112, 0xf, 1>(cs, ts01);
+          cs = c_step<0x114, 0xf, 0>(cs, ts23);
+          cs = c_step<0x118, 0xf, 1>(cs, ts23);
+          cs = c_step<0x142, 0xa, 0>(cs, ts45);
+          cs = c_step<0x143, 0xc, 1>(cs, ts45);
           // exclusive: the previous lane's prefix (wave_shr:1; lane 0 the identity)
           const int ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x138, 0xf, 0xf, false);
-          const int Te = __builtin_amdgcn_update_dpp(0, Ts, 0x138, 0xf, 0xf, false);
-          int carry = -32768;                                        // into this wave's first slot
+          int cin_lo = ce;             // one wave: nothing enters its first slot
           if constexpr (W > 1) {
-            if (lane == 63) { scC[w] = cs; scT[w] = Ts; }
+            int carry = -32768;                                      // into this wave's first slot
+            const uint32_t ts6e = tse[c];
+            if (lane == 63) { scC[w] = cs; scT[w] = -(int) (ts6e & 0xffffu); }
             __syncthreads();
             for (int v = 0; v < w; ++v) carry = max(scC[v], fT(carry, scT[v]));
+            cin_lo = max(ce, fT(carry, -(int) (ts6e >> 16)));
           }
-          const int cin_lo = max(ce, fT(carry, Te));
-          const int cin_hi = max(clo, fT(cin_lo, Tlo));
+          const int cin_hi = max(clo, cin_lo - (int) (tsl & 0xffffu));
           uint32_t y = pack16(cin_lo, cin_hi);
 #pragma unroll
           for (int j = 0; j < Q; ++j) { Dr[j] = maxs(Dr[j], y); y = adds(y, tDD[j]); }
@@ -374,6 +414,15 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
   int Tlo = 0, Thi = 0;                                // tDD summed over each slot
 #pragma unroll
   for (int j = 0; j < Q; ++j) { Tlo += lo16(tDD[j]); Thi += hi16(tDD[j]); }
+  // the lane scan's T terms, once: the lane's two slots and before each step
+  const int Ts0 = Tlo + Thi;
+  const int Ts1 = t_next<0x111, 0xf>(Ts0);
+  const int Ts2 = t_next<0x112, 0xf>(Ts1);
+  const int Ts3 = t_next<0x114, 0xf>(Ts2);
+  const int Ts4 = (G == 32) ? t_next<0x118, 0xf>(Ts3) : 0;
+  // kept in LDS, read on the rows that run the closure
+  __shared__ uint4 tsc[G];
+  if (threadIdx.x < G) tsc[c] = make_uint4(negsat2(Tlo, Thi), negsat2(Ts0, Ts1), negsat2(Ts2, Ts3), negsat(Ts4));
 
   const int ntab = (a.db.R + 1) * Q * G;
   for (int i = threadIdx.x; i < ntab; i += blockDim.x) etab_lds[i] = a.etab[i];
@@ -459,7 +508,7 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
           Dmv = maxs(Dmv, dn);                                       // :168
           if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;              // :162
         }
-        const uint32_t r = group_max2<G>(pack16(max(lo16(xEv), hi16(xEv)), max(lo16(Dmv), hi16(Dmv))));
+        const uint32_t r = group_max2<G>(halves_max(xEv, Dmv));
         int cond[NG];
         bool any = false, anylive = false, xbchg = false;
 #pragma unroll
@@ -492,6 +541,7 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
 
         Dr[0] = shift_grp<G>(dcv, lane);                             // :230-231 / :203-204
         if (any) {
+          const uint4 tt = tsc[c];
           const bool mine = of_group<NG>(cond, ge) != 0;
           // closure inside each slot, for the groups whose reference row runs it
           if (mine) {
@@ -500,27 +550,25 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
           }
           const uint32_t outl = adds(Dr[Q - 1], tDD[Q - 1]);
           const int clo = lo16(outl);
-          int cs = max(hi16(outl), fT(clo, Thi)), Ts = Tlo + Thi;
+          const uint32_t tsl = tt.x, ts01 = tt.y, ts23 = tt.z, ts4 = tt.w;
+          int cs = max(hi16(outl), clo - (int) (tsl >> 16));        // clo through the high slot
           // inclusive scan inside each group (rows of 16 lanes; for 32-lane
           // groups the second row also composes the first row's total)
-          scan_step<0x111, 0xf>(cs, Ts);
-          scan_step<0x112, 0xf>(cs, Ts);
-          scan_step<0x114, 0xf>(cs, Ts);
-          scan_step<0x118, 0xf>(cs, Ts);
-          if constexpr (G == 32) scan_step<0x142, 0xa>(cs, Ts);
+          cs = c_step<0x111, 0xf, 0>(cs, ts01);
+          cs = c_step<0x112, 0xf, 1>(cs, ts01);
+          cs = c_step<0x114, 0xf, 0>(cs, ts23);
+          cs = c_step<0x118, 0xf, 1>(cs, ts23);
+          if constexpr (G == 32) cs = c_step<0x142, 0xa, 0>(cs, ts4);
           // exclusive: the previous lane's prefix; a group's first lane the identity
-          int ce, Te;
+          int ce;
           if constexpr (G == 16) {
             ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x111, 0xf, 0xf, false);
-            Te = __builtin_amdgcn_update_dpp(0, Ts, 0x111, 0xf, 0xf, false);
           } else {
             ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x138, 0xf, 0xf, false);
-            Te = __builtin_amdgcn_update_dpp(0, Ts, 0x138, 0xf, 0xf, false);
-            if (c == 0) { ce = -32768; Te = 0; }
+            if (c == 0) ce = -32768;
           }
-          (void) Te;
           const int cin_lo = ce;                                     // nothing enters a group's first slot
-          const int cin_hi = max(clo, fT(cin_lo, Tlo));
+          const int cin_hi = max(clo, cin_lo - (int) (tsl & 0xffffu));
           if (mine) {
             uint32_t y = pack16(cin_lo, cin_hi);
 #pragma unroll

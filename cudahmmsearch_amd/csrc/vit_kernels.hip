@@ -49,17 +49,25 @@ __device__ __forceinline__ int fT(int x, int T) { return max(-32768, x + T); }
 // lane's high half, its high half its own low half; lane 0 takes the high
 // half of edge (the last slot of the previous wave, or minus infinity).
 __device__ __forceinline__ uint32_t shift_in(uint32_t last, uint32_t edge, int lane) {
-  uint32_t prev = __builtin_amdgcn_update_dpp(NEG2, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+  uint32_t prev = __builtin_amdgcn_mov_dpp(last, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
   prev = (lane == 0) ? edge : prev;
   return __builtin_amdgcn_alignbit(last, prev, 16);
 }
 
+// Packed int16 max over each row of 16 lanes, in every lane of the row: DPP
+// rotations, so every lane has a source (max is idempotent, the wrap-around
+// is harmless) and no fill value is needed.
+__device__ __forceinline__ uint32_t row_max2(uint32_t v) {
+  v = maxs(v, __builtin_amdgcn_mov_dpp(v, 0x121 /* row_ror:1 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_mov_dpp(v, 0x122 /* row_ror:2 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_mov_dpp(v, 0x124 /* row_ror:4 */, 0xf, 0xf, false));
+  v = maxs(v, __builtin_amdgcn_mov_dpp(v, 0x128 /* row_ror:8 */, 0xf, 0xf, false));
+  return v;
+}
+
 // Packed int16 max over the wave, returned in every lane (lane 63's value).
 __device__ __forceinline__ uint32_t wave_max2(uint32_t v) {
-  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x111 /* row_shr:1 */, 0xf, 0xf, false));
-  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x112 /* row_shr:2 */, 0xf, 0xf, false));
-  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x114 /* row_shr:4 */, 0xf, 0xf, false));
-  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x118 /* row_shr:8 */, 0xf, 0xf, false));
+  v = row_max2(v);
   v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x142 /* row_bcast:15 */, 0xa, 0xf, false));
   v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x143 /* row_bcast:31 */, 0xc, 0xf, false));
   return __builtin_amdgcn_readlane(v, 63);
@@ -89,6 +97,19 @@ __device__ __forceinline__ int c_step(int c, uint32_t tp) {
   const int nt = HALF ? (int) (tp >> 16) : (int) (tp & 0xffffu);
   return max(c, co - nt);
 }
+// The same for the steps inside rows of 16 lanes (row_shr by 1, 2, 4, 8):
+// a lane without a source reads 0, and its T term in tp is stored as 65535
+// (minus infinity; in_row_t), so 0 - 65535 <= c leaves it unchanged.
+template <int CTRL, int HALF>
+__device__ __forceinline__ int c_step_row(int c, uint32_t tp) {
+  const int co = __builtin_amdgcn_mov_dpp(c, CTRL, 0xf, 0xf, true);
+  const int nt = HALF ? (int) (tp >> 16) : (int) (tp & 0xffffu);
+  return max(c, co - nt);
+}
+// T of an in-row step with shift SH for this lane: minus infinity where the
+// lane has no source in that step.
+template <int SH>
+__device__ __forceinline__ int in_row_t(int T, int lane) { return ((lane & 15) >= SH) ? T : -65535; }
 // (max over both halves of xEv, max over both halves of Dmv) as one packed
 // pair: two byte permutes and one v_pk_max_i16.
 __device__ __forceinline__ uint32_t halves_max(uint32_t xEv, uint32_t Dmv) {
@@ -139,7 +160,8 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
   const int Tex = __builtin_amdgcn_update_dpp(0, Ts6, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
   // kept in LDS, read on the rows that run the closure
   if (W > 1 || threadIdx.x < 64) {
-    tsc[c] = make_uint4(negsat2(Tlo, Thi), negsat2(Ts0, Ts1), negsat2(Ts2, Ts3), negsat2(Ts4, Ts5));
+    tsc[c] = make_uint4(negsat2(Tlo, Thi), negsat2(in_row_t<1>(Ts0, lane), in_row_t<2>(Ts1, lane)),
+                        negsat2(in_row_t<4>(Ts2, lane), in_row_t<8>(Ts3, lane)), negsat2(Ts4, Ts5));
     if constexpr (W > 1) tse[c] = negsat2(Ts6, Tex);
   }
   __syncthreads();
@@ -249,10 +271,10 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
           // order), in DPP: row_shr 1/2/4/8 inside each row of 16 lanes,
           // then row_bcast 15 and 31 across rows; lanes without a source
           // take the identity map (-32768, 0), which leaves them unchanged
-          cs = c_step<0x111, 0xf, 0>(cs, ts01);
-          cs = c_step<0x112, 0xf, 1>(cs, ts01);
-          cs = c_step<0x114, 0xf, 0>(cs, ts23);
-          cs = c_step<0x118, 0xf, 1>(cs, ts23);
+          cs = c_step_row<0x111, 0>(cs, ts01);
+          cs = c_step_row<0x112, 1>(cs, ts01);
+          cs = c_step_row<0x114, 0>(cs, ts23);
+          cs = c_step_row<0x118, 1>(cs, ts23);
           cs = c_step<0x142, 0xa, 0>(cs, ts45);
           cs = c_step<0x143, 0xc, 1>(cs, ts45);
           // exclusive: the previous lane's prefix (wave_shr:1; lane 0 the identity)
@@ -352,7 +374,7 @@ __device__ __forceinline__ uint32_t shift_grp(uint32_t last, int lane) {
   if constexpr (G == 16) {
     prev = __builtin_amdgcn_update_dpp(NEG2, last, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
   } else {
-    prev = __builtin_amdgcn_update_dpp(NEG2, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    prev = __builtin_amdgcn_mov_dpp(last, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
     prev = ((lane & (G - 1)) == 0) ? NEG2 : prev;
   }
   return __builtin_amdgcn_alignbit(last, prev, 16);
@@ -361,10 +383,7 @@ __device__ __forceinline__ uint32_t shift_grp(uint32_t last, int lane) {
 // Packed int16 max over each G-lane group; the group's last lane holds it.
 template <int G>
 __device__ __forceinline__ uint32_t group_max2(uint32_t v) {
-  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x111 /* row_shr:1 */, 0xf, 0xf, false));
-  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x112 /* row_shr:2 */, 0xf, 0xf, false));
-  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x114 /* row_shr:4 */, 0xf, 0xf, false));
-  v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x118 /* row_shr:8 */, 0xf, 0xf, false));
+  v = row_max2(v);
   if constexpr (G == 32) v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x142 /* row_bcast:15 */, 0xa, 0xf, false));
   return v;
 }
@@ -422,7 +441,9 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
   const int Ts4 = (G == 32) ? t_next<0x118, 0xf>(Ts3) : 0;
   // kept in LDS, read on the rows that run the closure
   __shared__ uint4 tsc[G];
-  if (threadIdx.x < G) tsc[c] = make_uint4(negsat2(Tlo, Thi), negsat2(Ts0, Ts1), negsat2(Ts2, Ts3), negsat(Ts4));
+  if (threadIdx.x < G)
+    tsc[c] = make_uint4(negsat2(Tlo, Thi), negsat2(in_row_t<1>(Ts0, lane), in_row_t<2>(Ts1, lane)),
+                        negsat2(in_row_t<4>(Ts2, lane), in_row_t<8>(Ts3, lane)), negsat(Ts4));
 
   const int ntab = (a.db.R + 1) * Q * G;
   for (int i = threadIdx.x; i < ntab; i += blockDim.x) etab_lds[i] = a.etab[i];
@@ -554,10 +575,10 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
           int cs = max(hi16(outl), clo - (int) (tsl >> 16));        // clo through the high slot
           // inclusive scan inside each group (rows of 16 lanes; for 32-lane
           // groups the second row also composes the first row's total)
-          cs = c_step<0x111, 0xf, 0>(cs, ts01);
-          cs = c_step<0x112, 0xf, 1>(cs, ts01);
-          cs = c_step<0x114, 0xf, 0>(cs, ts23);
-          cs = c_step<0x118, 0xf, 1>(cs, ts23);
+          cs = c_step_row<0x111, 0>(cs, ts01);
+          cs = c_step_row<0x112, 1>(cs, ts01);
+          cs = c_step_row<0x114, 0>(cs, ts23);
+          cs = c_step_row<0x118, 1>(cs, ts23);
           if constexpr (G == 32) cs = c_step<0x142, 0xa, 0>(cs, ts4);
           // exclusive: the previous lane's prefix; a group's first lane the identity
           int ce;

@@ -834,8 +834,6 @@ int search_device(msvg_ctx *c, const int32_t *ids, int np, int dbid, double F1, 
 // ViterbiFilter tables (host-built, see VitArgs in msv_device.h).
 namespace {
 
-// Waves per target and registers per lane: one wave up to M = 1024, then the
-// fewest waves (up to 16) that keep Q <= 8.
 // ViterbiFilter kernel shape for a model of length M: lanes per target G,
 // waves per target W (G = 64 only: the fewest with Q <= 16 registers per
 // lane, W <= 8 for M <= 16384), registers per lane Q.  Short models

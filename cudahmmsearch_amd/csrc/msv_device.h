@@ -79,6 +79,48 @@ struct FastArgs {
   const uint8_t *scan_rowcode; // mode 3: LDS row r holds table row scan_rowcode[r] (r = 0..db.R)
 };
 
+// Batched J-free pass over many profiles in one persistent launch per lane
+// group size G (msv_batch<G>).  Each profile has its own queue of target
+// chunks (the database's guided chunk list for G, one counter per profile);
+// a workgroup stages one profile's table, its waves drain that queue, and
+// when it is empty the workgroup moves to the next profile with chunks left.
+//
+// Batched-pass tables (per profile x database x G), f16 emission pairs as in
+// the fast-stage table (value (bias_b - rbv[x][k]) * 2^-11, -1.0 for padding
+// and for the null row):
+//   G = 64:      the fast-stage layout above (entry (x, jb, lane)).
+//   G = 8, 16:   uint4 B[QB][32][16]: entry (jb, x, column), column = lane % 16
+//                (slots of lane % G); 32 residue rows per register block.
+//   G = 32:      uint4 B[QB][32][32]: entry (jb, x, lane % 32).
+// With 32 rows per block the byte address of (jb, x, column) is
+// jb * 32 * TW * 16 + x * TW * 16 + column * 16, so one v_perm_b32 builds the
+// row address from a residue byte (x * 256 for TW = 16; (2x + column / 16) *
+// 256 for TW = 32) and jb goes into the read's immediate offset.  Needs R + 1
+// <= 32 and, for G = 32, QB <= 4.
+struct BatProf {
+  const uint4 *tab;            // batched-pass table
+  ProfDev pd;                  // constants; pd.res receives the result words; pd.Q registers per lane
+  int32_t *counter;            // next chunk of this profile's queue (zeroed before the launch)
+  SlowItem *jslow;             // this profile's J-aware queue (null: deferred pairs go to the byte queue)
+  int32_t *jslow_count;
+  int32_t prof_idx;            // index into the search's profile array
+  int32_t tab_u4;              // table size in uint4 entries
+};
+
+struct BatArgs {
+  const BatProf *profs;
+  int32_t np;
+  DbDev db;
+  const int32_t *chunk_start;  // chunk c = tasks [chunk_start[c], chunk_start[c+1]) (task = 64/G targets)
+  int32_t nchunks;
+  const int32_t *first;        // profile each workgroup starts on
+  SlowItem *slow;
+  int32_t *slow_count;
+  int32_t slow_cap;
+  int32_t jslow_cap;
+  int32_t slot_u4;             // LDS uint4 index of the workgroup's broadcast word (past the largest table)
+};
+
 // J-aware stage: the MSV recurrence with per-row E/J/B updates in the f16
 // domain of the fast stage, one target per wave, for one profile's queue.
 struct JArgs {

@@ -38,6 +38,11 @@
 #include "../../include/msv_prep.h"
 
 extern "C" hipError_t msvk_prepare();
+extern "C" hipError_t msvk_prepare_batch();
+extern "C" hipError_t msvk_launch_batch(int G, const BatArgs *a, int grid, int block, size_t lds, hipStream_t st);
+extern "C" hipError_t msvk_batch_occupancy(int G, int block, size_t lds, int *per_cu);
+extern "C" hipError_t msvk_build_bat_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int G, int Q,
+                                           int bias, uint32_t *out, hipStream_t st);
 extern "C" hipError_t msvk_launch_fast(int G, int Q, const FastArgs *a, int grid, int block, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_seg(int Q, const SegArgs *a, int grid, int block, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st);
@@ -61,6 +66,7 @@ namespace {
 constexpr int kLdsBytes = 160 * 1024;
 constexpr int kQMax = 32;           // registers per lane in the fast kernels
 constexpr int kVitStreams = 4;      // streams of a multi-profile ViterbiFilter search (the default hardware queues)
+constexpr int kJStreams = 4;        // streams of a batched search's J-aware launches
 constexpr int kSegQMax = 28;        // registers per lane in one model segment (at most 3584 positions)
 
 // Segment width: the most registers whose table fits in LDS for R rows.
@@ -169,6 +175,7 @@ struct Db {
 
 struct Tables {                          // per (profile, db)
   std::map<int, std::unique_ptr<DevMem>> fast;   // G -> table
+  std::map<int, std::unique_ptr<DevMem>> bat;    // G -> batched-pass table (G = 8, 16, 32)
   std::vector<std::unique_ptr<DevMem>> seg;      // segment tables (M > fast limit)
   std::unique_ptr<DevMem> slow;
   std::unique_ptr<DevMem> vemit;                 // ViterbiFilter emission table
@@ -178,6 +185,7 @@ struct Tile { int lo, hi, lanes, waves; };
 
 struct Bufs {                             // device buffers one search writes
   DevMem d_res, d_prof, d_slow, d_counters, d_usc, d_status, d_pass, d_thr, d_usctab, d_seg, d_jslow;
+  DevMem d_bat;                          // batched launches: BatProf arrays and first-profile lists
   int64_t slow_cap = 0;
   int64_t jcap = 0;                      // per-profile J-aware queue capacity
 };
@@ -200,6 +208,8 @@ struct msvg_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;      // batched mode: J-aware launches beside the fast chain
   hipEvent_t fork = nullptr, join = nullptr;
+  hipStream_t jst[kJStreams] = {}; // batched mode: the J-aware launches after the batched pass
+  hipEvent_t jdone[kJStreams] = {};
   std::string err;
   int next_id = 1;
   std::map<int, std::unique_ptr<Profile>> profiles;
@@ -212,6 +222,7 @@ struct msvg_ctx {
   Bufs *last = nullptr;            // buffers holding the latest results
   int64_t slow_cap = 0;            // slow-queue capacity for the next search
   int last_np = 0, last_db = -1;
+  int last_launches = 0;           // J-free launches of the latest enqueued search
   msvg_stats stats{};
   hipEvent_t ev[8] = {};
   std::map<std::string, std::unique_ptr<Plan>> plans;
@@ -467,6 +478,41 @@ int ensure_fast_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **ou
   return MSVG_OK;
 }
 
+// Batched-pass tables (msv_device.h, BatProf): G = 64 uses the fast-stage
+// table; G = 8, 16, 32 the narrow [block][32 rows][TW lanes] layout.
+int bat_tw(int G) { return G < 16 ? 16 : G; }
+int bat_qmax(int G) { return G == 64 ? kQMax : G == 16 ? 24 : 16; }
+size_t bat_bytes(int G, int R, int Q) {
+  return G == 64 ? fast_lds(R, Q) : (size_t) ((Q + 3) / 4) * 32 * bat_tw(G) * 16;
+}
+// Does the batched kernel take this profile at lane-group size G?  (Its
+// table plus the workgroup's broadcast word fit LDS; narrow layouts need
+// at most 32 residue rows and bat_qmax registers.)
+bool bat_fits(int G, int M, int R) {
+  const int Q = qfor(M, 2 * G);
+  if (Q > bat_qmax(G)) return false;
+  if (G != 64 && R + 1 > 32) return false;
+  return bat_bytes(G, R, Q) + 16 <= (size_t) kLdsBytes;
+}
+
+int ensure_bat_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **out, hipStream_t st) {
+  if (G == 64) return ensure_fast_table(c, pid, dbid, 64, out, st);
+  Profile &p = *c->profiles[pid];
+  Db &d = *c->dbs[dbid];
+  Tables &t = tables_for(c, pid, dbid);
+  auto it = t.bat.find(G);
+  if (it == t.bat.end()) {
+    const int Q = qfor(p.M, 2 * G);
+    auto mem = std::make_unique<DevMem>();
+    HIPCHK(c, mem->ensure(bat_bytes(G, d.R, Q)));
+    HIPCHK(c, msvk_build_bat_table(p.d_rbv.as<uint8_t>(), p.M, d.d_rowcode.as<uint8_t>(), d.R, G, Q, p.bias,
+                                   mem->as<uint32_t>(), st));
+    it = t.bat.emplace(G, std::move(mem)).first;
+  }
+  *out = it->second->as<uint32_t>();
+  return MSVG_OK;
+}
+
 int ensure_seg_tables(msvg_ctx *c, int pid, int dbid, hipStream_t st) {
   Profile &p = *c->profiles[pid];
   Db &d = *c->dbs[dbid];
@@ -551,7 +597,11 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   int rc;
   std::vector<ProfDev> pds(np);
   std::vector<int> Gs(np), waves(np), segmented(np);
+  std::vector<int> Gb(np, 0);          // lane-group size of the profile's batched launch (0: none)
   std::vector<const uint4 *> jtab(np);
+  // Batched searches on persistent launches take every profile the batched
+  // kernel fits (msv_batch.hip); shared-schedule serving keeps msv_fast.
+  const bool batch = (mode == MSVG_MODE_BATCHED && sched == 0);
   int64_t cells = 0;
   for (int i = 0; i < np; ++i) {
     Profile &p = *c->profiles[ids[i]];
@@ -561,9 +611,19 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     const int Q = qfor(p.M, 2 * G);
     segmented[i] = (Q > kQMax || fast_lds(d.R, Q) > (size_t) kLdsBytes) ? 1 : 0;
     if (segmented[i]) G = 64;
+    if (!segmented[i] && batch) {
+      // the tile's G, or the next wider one the batched kernel takes
+      int g = G;
+      while (g < 64 && !bat_fits(g, p.M, d.R)) g *= 2;
+      if (bat_fits(g, p.M, d.R)) Gb[i] = G = g;
+    }
     Gs[i] = G; waves[i] = w;
     pd.ftab = nullptr;
-    if (!segmented[i]) {
+    if (Gb[i]) {
+      const uint32_t *bt;
+      if ((rc = ensure_bat_table(c, ids[i], dbid, G, &bt, st)) != MSVG_OK) return rc;
+      pd.ftab = reinterpret_cast<const uint4 *>(bt);
+    } else if (!segmented[i]) {
       const uint32_t *ft;
       if ((rc = ensure_fast_table(c, ids[i], dbid, G, &ft, st)) != MSVG_OK) return rc;
       pd.ftab = reinterpret_cast<const uint4 *>(ft);
@@ -627,9 +687,85 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   if (timed) HIPCHK(c, hipEventRecord(c->ev[0], st));
   HIPCHK(c, msvk_fill_zero(counters, 2 * np + 2, st));
   int32_t *jcounts = counters + np + 1;
+  int n_launch = 0;
+
+  if (batch) {
+    // One persistent launch per lane-group size G over the profiles it
+    // takes: per-profile chunk queues, workgroups spread over the profiles
+    // in proportion to M at the start, then moving to profiles with chunks
+    // left (msv_batch.hip).
+    struct BatLaunch { int G, grid, block; size_t lds, prof_at, first_at; int np; };
+    std::vector<BatLaunch> bl;
+    std::vector<BatProf> hprof;
+    std::vector<int32_t> hfirst;
+    for (int G : {64, 32, 16, 8}) {
+      std::vector<int> mem;
+      for (int i = 0; i < np; ++i)
+        if (Gb[i] == G) mem.push_back(i);
+      if (mem.empty()) continue;
+      if ((rc = build_chunks(c, d, G)) != MSVG_OK) return rc;
+      size_t lds = 0;
+      for (int i : mem) lds = std::max(lds, bat_bytes(G, d.R, pds[i].Q));
+      lds += 16;                                        // the workgroup's broadcast word
+      const int lcu = (int) std::max<size_t>(1, std::min<size_t>(8, kLdsBytes / lds));
+      const int block = 64 * std::min(16, std::max(4, (16 + lcu - 1) / lcu));
+      int occ = 0;
+      HIPCHK(c, msvk_batch_occupancy(G, block, lds, &occ));
+      if (occ < 1) return fail(c, MSVG_EDEVICE, "batched kernel cannot be resident");
+      BatLaunch L{G, c->ncu * occ, block, lds, hprof.size(), hfirst.size(), (int) mem.size()};
+      double wsum = 0;
+      for (int i : mem) wsum += c->profiles[ids[i]]->M;
+      double acc = 0;
+      size_t k = 0;
+      for (int bidx = 0; bidx < L.grid; ++bidx) {
+        const double at = (bidx + 0.5) / L.grid * wsum;
+        while (k + 1 < mem.size() && acc + c->profiles[ids[mem[k]]]->M <= at) acc += c->profiles[ids[mem[k++]]]->M;
+        hfirst.push_back((int32_t) k);
+      }
+      for (int i : mem) {
+        BatProf bp;
+        bp.tab = pds[i].ftab;
+        bp.pd = pds[i];
+        bp.pd.res = b.d_res.as<uint16_t>() + (size_t) i * n;
+        bp.counter = counters + i;
+        bp.jslow = jtab[i] ? b.d_jslow.as<SlowItem>() + (size_t) i * b.jcap : nullptr;
+        bp.jslow_count = jcounts + i;
+        bp.prof_idx = i;
+        bp.tab_u4 = (int32_t) (bat_bytes(G, d.R, pds[i].Q) / 16);
+        hprof.push_back(bp);
+      }
+      bl.push_back(L);
+    }
+    const size_t prof_bytes = (hprof.size() * sizeof(BatProf) + 255) & ~(size_t) 255;
+    HIPCHK(c, b.d_bat.ensure(prof_bytes + hfirst.size() * sizeof(int32_t)));
+    if (!capturing) {
+      HIPCHK(c, hipMemcpyAsync(b.d_bat.p, hprof.data(), hprof.size() * sizeof(BatProf), hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(b.d_bat.as<uint8_t>() + prof_bytes, hfirst.data(), hfirst.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipStreamSynchronize(st));             // the host vectors end with this block
+    }
+    for (const BatLaunch &L : bl) {
+      BatArgs a;
+      a.profs = b.d_bat.as<BatProf>() + L.prof_at;
+      a.np = L.np;
+      a.db = dd;
+      auto &ch = d.chunks[L.G];
+      a.chunk_start = ch.first->as<int32_t>();
+      a.nchunks = ch.second;
+      a.first = reinterpret_cast<const int32_t *>(b.d_bat.as<uint8_t>() + prof_bytes) + L.first_at;
+      a.slow = b.d_slow.as<SlowItem>();
+      a.slow_count = slow_count;
+      a.slow_cap = (int32_t) b.slow_cap;
+      a.jslow_cap = (int32_t) b.jcap;
+      a.slot_u4 = (int32_t) ((L.lds - 16) / 16);
+      HIPCHK(c, msvk_launch_batch(L.G, &a, L.grid, L.block, L.lds, st));
+      ++n_launch;
+    }
+  }
 
   for (int i = 0; i < np; ++i) {
     const Profile &p = *c->profiles[ids[i]];
+    if (Gb[i]) continue;                                // scored by its batched launch
     if (segmented[i]) {
       // Long model: column segments of kSegQ registers, carrying the last
       // column of each row from one segment to the next.
@@ -658,6 +794,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
         persistent_shape(c, lds, 0, grid, block);
         a.ntasks = (int32_t) n;
         HIPCHK(c, msvk_launch_seg(Q, &a, grid, block, lds, st));
+        ++n_launch;
       }
       if (jtab[i]) {
         JArgs ja;
@@ -706,6 +843,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       grid = c->ncu * (int) std::max<size_t>(1, std::min<size_t>(8, kLdsBytes / lds));
     }
     HIPCHK(c, msvk_launch_fast(G, pds[i].Q, &a, grid, block, lds, st));
+    ++n_launch;
     if (jtab[i]) {
       JArgs ja;
       ja.prof = pds[i];
@@ -724,7 +862,42 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     HIPCHK(c, hipEventRecord(c->join, side));
     HIPCHK(c, hipStreamWaitEvent(st, c->join, 0));
   }
+  if (batch) {
+    // J-aware launches of the batched profiles, once their queues are full:
+    // one per profile, round-robin over kJStreams streams forked from st (in
+    // line on st when no side stream is given).  Each lasts as long as its
+    // longest pair; workgroups without a pair leave before staging.
+    std::vector<int> jl;
+    for (int i = 0; i < np; ++i)
+      if (Gb[i] && jtab[i]) jl.push_back(i);
+    const int nst = (side && !jl.empty()) ? std::min<int>(kJStreams, (int) jl.size()) : 0;
+    if (nst) {
+      HIPCHK(c, hipEventRecord(c->fork, st));
+      for (int k = 0; k < nst; ++k) HIPCHK(c, hipStreamWaitEvent(c->jst[k], c->fork, 0));
+    }
+    for (size_t u = 0; u < jl.size(); ++u) {
+      const int i = jl[u];
+      JArgs ja;
+      ja.prof = pds[i];
+      ja.prof.res = b.d_res.as<uint16_t>() + (size_t) i * n;
+      ja.db = dd;
+      ja.tab = jtab[i];
+      ja.items = b.d_jslow.as<SlowItem>() + (size_t) i * b.jcap;
+      ja.count = jcounts + i;
+      ja.cap = (int32_t) b.jcap;
+      const int Q64 = qfor(c->profiles[ids[i]]->M, 128);
+      const size_t jlds = fast_lds(d.R, Q64);
+      const int per_cu = (int) std::max<size_t>(1, std::min<size_t>(4, kLdsBytes / jlds));
+      HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jlds, nst ? c->jst[u % nst] : st));
+    }
+    if (nst)
+      for (int k = 0; k < nst; ++k) {
+        HIPCHK(c, hipEventRecord(c->jdone[k], c->jst[k]));
+        HIPCHK(c, hipStreamWaitEvent(st, c->jdone[k], 0));
+      }
+  }
   if (timed) HIPCHK(c, hipEventRecord(c->ev[1], st));
+  c->last_launches = n_launch;
 
   SlowArgs sa;
   sa.profs = b.d_prof.as<ProfDev>();
@@ -814,12 +987,12 @@ int search_device(msvg_ctx *c, const int32_t *ids, int np, int dbid, double F1, 
       HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
       rc = after_search(c, plan->bufs, np, dbid, cells, false);
       c->stats.ms_total = t; c->stats.ms_fast = c->stats.ms_slow = c->stats.ms_finalize = 0;
-      c->stats.n_fast_launches = np;
+      c->stats.n_fast_launches = c->last_launches;
     } else {
       if ((rc = enqueue_search(c, c->eager, ids, np, dbid, F1, mode, true, false, &cells, nullptr, 0,
                                mode == MSVG_MODE_BATCHED ? c->side : nullptr)) != MSVG_OK) return rc;
       rc = after_search(c, c->eager, np, dbid, cells, true);
-      c->stats.n_fast_launches = np;
+      c->stats.n_fast_launches = c->last_launches;
     }
     if (rc != MSVG_ENORESULT) return rc;
     c->slow_cap = (int64_t) c->stats.n_slow + 4096;     // queue overflowed: grow and redo
@@ -913,9 +1086,14 @@ int msvg_init(int device, msvg_ctx **ret) {
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) return MSVG_EDEVICE;
   if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess) return MSVG_EDEVICE;
   if (hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) return MSVG_EDEVICE;
+  for (int k = 0; k < kJStreams; ++k) {
+    if (hipStreamCreateWithFlags(&c->jst[k], hipStreamNonBlocking) != hipSuccess) return MSVG_EDEVICE;
+    if (hipEventCreateWithFlags(&c->jdone[k], hipEventDisableTiming) != hipSuccess) return MSVG_EDEVICE;
+  }
   for (auto &e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return MSVG_EDEVICE;
   if (msvk_prepare() != hipSuccess) return MSVG_EDEVICE;
+  if (msvk_prepare_batch() != hipSuccess) return MSVG_EDEVICE;
   *ret = c.release();
   return MSVG_OK;
 }
@@ -932,6 +1110,10 @@ void msvg_free(msvg_ctx *c) {
   if (c->fork) (void) hipEventDestroy(c->fork);
   if (c->join) (void) hipEventDestroy(c->join);
   if (c->side) { (void) hipStreamSynchronize(c->side); (void) hipStreamDestroy(c->side); }
+  for (int k = 0; k < kJStreams; ++k) {
+    if (c->jst[k]) { (void) hipStreamSynchronize(c->jst[k]); (void) hipStreamDestroy(c->jst[k]); }
+    if (c->jdone[k]) (void) hipEventDestroy(c->jdone[k]);
+  }
   for (int k = 0; k < kVitStreams; ++k) {
     if (c->vstream[k]) { (void) hipStreamSynchronize(c->vstream[k]); (void) hipStreamDestroy(c->vstream[k]); }
     if (c->vdone[k]) (void) hipEventDestroy(c->vdone[k]);

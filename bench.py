@@ -1,17 +1,23 @@
 """MSV filter benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5|scan]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c1|c2|c4|c5|scan]
 
 A step is one complete MSV search (fast J-free stage, exact byte-MSV stage for
 the queued pairs, finalize to usc / status / F1 pass bit) of the workload's
 profiles over its resident target database; inputs are in HBM before the
 timed region starts.  Prints one JSON line.  GCUPS = sum over pairs of M * L
-per second / 1e9.  For c2 and c3 the line also carries the ViterbiFilter
+per second / 1e9.  The default workload is c3, the length-skewed 64-profile
+batch: value = batched GCUPS, beside the plain single-profile kernel on the
+same data (plain_gcups, batched_over_plain, bit identity), the roofline of
+the dominant batched launch (HIP events), the reference's SSE p7_MSVFilter
+on the host (N threads and one), and a shortened c5 serving phase (serving:
+p50/p99 at --rate req/s for --serving-duration s, capacity, p50/p99 at 70%
+of capacity).  For c2 and c3 the line also carries the ViterbiFilter
 stage on the MSV survivors (vit_stage: kernel time, GCUPS, roofline, the
 reference's SSE p7_ViterbiFilter on a sample with a bit-identity check).
 
-Workloads (synthetic data generated on the host from seeds; Easel-compatible
-generators in cudahmmsearch_amd):
+Workloads (synthetic data generated on the host from seeds by
+cudahmmsearch_amd.workloads; Easel-compatible generators):
   c1  the reference's CPU configuration: one sampled profile M=200 (seed 1)
       vs 20,000 targets, log-normal median 300, sigma 0.6, clipped 30..5000
       (seed 2), 1% replaced by p7_CoreEmit samples (seed 5); also times the
@@ -65,39 +71,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(ca, name):
-    if name == "c1":
-        profs = [ca.sample_profile(1, 200)]
-        db = ca.gen_database(2, 20_000, 300, 0.6, 30, 5000, nthreads=16)
-        hom = ca.core_emit(profs[0].hmm, 5, int(0.01 * db.n))
-        db, _ = ca.plant(db, hom, 5, 0.01)
-        desc = ("1 sampled profile M=200 (seed 1) vs 20,000 i.i.d. targets, log-normal median 300 sigma 0.6 "
-                "clip 30-5000, 1% planted p7_CoreEmit homologs")
-    elif name == "c2":
-        profs = [ca.sample_profile(3, 400)]
-        db = ca.gen_database(4, 2_000_000, 350, 0.7, 20, 10000, nthreads=16)
-        hom = ca.core_emit(profs[0].hmm, 5, int(0.005 * db.n))
-        db, _ = ca.plant(db, hom, 5, 0.005)
-        desc = ("1 sampled profile M=400 (seed 3) vs 2,000,000 i.i.d. targets, log-normal median 350 sigma 0.7 "
-                "clip 20-10000, 0.5% planted p7_CoreEmit homologs")
-    elif name == "c3":
-        rs = np.random.default_rng(1003)
-        Ms = rs.integers(50, 2501, 64)
-        profs = [ca.sample_profile(10 + i, int(Ms[i])) for i in range(64)]
-        db = ca.gen_database(6, 1_000_000, 300, 0.6, 20, 35000, tail_frac=0.02, tail_xm=1000, tail_alpha=1.5,
-                             tail_max=35000, nthreads=16)
-        desc = ("64 sampled profiles M~U[50,2500] (seeds 10-73) vs 1,000,000 targets, log-normal median 300 sigma 0.6 "
-                "+ 2% Pareto tail (x_m 1000, alpha 1.5, max 35000)")
-    elif name == "c4":
-        Ms = [6000, 6000, 8000, 8000, 10000, 10000, 12000, 12000]
-        profs = [ca.sample_profile(80 + i, M) for i, M in enumerate(Ms)]
-        db = ca.gen_database(7, 500_000, 1200, 0.6, 100, 35000, nthreads=16)
-        desc = "8 sampled profiles M in {6000,8000,10000,12000}x2 vs 500,000 targets, log-normal median 1200"
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    return profs, db, desc
-
-
 def time_mode(ctx, pids, did, mode, steps, warmup):
     for _ in range(warmup):
         ctx.search_device(pids, did, mode=mode)
@@ -109,11 +82,34 @@ def time_mode(ctx, pids, did, mode, steps, warmup):
     return t, ctx.stats()
 
 
-def cpu_baseline(profs, db, budget_s=12.0):
+def host_facts():
+    """CPU model and logical CPUs of this host (the box the GPU sits in)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+def cpu_baseline(profs, db, budget_s=12.0, one_budget_s=6.0):
     """The reference's SSE p7_MSVFilter (oracle/_ref, compiled from the
-    reference sources) on a bounded sample of the same targets."""
+    reference sources) on bounded samples of the same targets: N host
+    threads over a sample sized to budget_s, and one thread over a smaller
+    one (the reference's own per-core figure, msvfilter_bench's Mc/s).  The
+    profile is the workload's median-M one."""
     import oracle
-    nthreads = min(16, os.cpu_count() or 1)
+    import cudahmmsearch_amd as ca
+    host = host_facts()
+    nthreads = min(16, host["usable_cpus"] or 1)
     kind = "reference"
     try:
         R = oracle.Reference()
@@ -121,31 +117,38 @@ def cpu_baseline(profs, db, budget_s=12.0):
         R = None
         kind = "port"
     O = None if R is not None else oracle.Oracle()
-    p = profs[0]
+    p = sorted(profs, key=lambda q: q.M)[len(profs) // 2]
     if R is not None:
         mat, ins, t = p.hmm
         rp = R.from_probs(mat, t, 100, False)
-        run = lambda sub: rp.msv_batch(sub.res, sub.off, sub.lens, nthreads)  # noqa: E731
+        run = lambda sub, nt: rp.msv_batch(sub.res, sub.off, sub.lens, nt)  # noqa: E731
     else:
-        run = lambda sub: O.msv_batch(p.as_dict(), sub.res, sub.off, sub.lens, nthreads)  # noqa: E731
-    import cudahmmsearch_amd as ca
+        run = lambda sub, nt: O.msv_batch(p.as_dict(), sub.res, sub.off, sub.lens, nt)  # noqa: E731
     rs = np.random.default_rng(0)
-    n0 = min(db.n, 4000)
-    idx = rs.choice(db.n, n0, replace=False)
-    sub = ca.Database.from_list([db.seq(i) for i in idx])
-    t0 = time.perf_counter()
-    run(sub)
-    dt = time.perf_counter() - t0
-    n1 = int(min(db.n, max(n0, n0 * budget_s / max(dt, 1e-3))))
-    idx = rs.choice(db.n, n1, replace=False)
-    sub = ca.Database.from_list([db.seq(i) for i in idx])
-    t0 = time.perf_counter()
-    run(sub)
-    dt = time.perf_counter() - t0
-    cells = p.M * sub.residues
-    return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": nthreads, "kind": kind,
-            "sample": f"{n1} targets ({sub.residues} residues) drawn from the workload, profile M={p.M}, "
-                      f"p7_MSVFilter on {nthreads} threads, {dt:.1f} s"}
+
+    def timed(nt, budget):
+        n0 = min(db.n, 2000)
+        idx = rs.choice(db.n, n0, replace=False)
+        sub = ca.Database.from_list([db.seq(i) for i in idx])
+        t0 = time.perf_counter()
+        run(sub, nt)
+        dt = time.perf_counter() - t0
+        n1 = int(min(db.n, max(n0, n0 * budget / max(dt, 1e-3))))
+        idx = rs.choice(db.n, n1, replace=False)
+        sub = ca.Database.from_list([db.seq(i) for i in idx])
+        t0 = time.perf_counter()
+        run(sub, nt)
+        dt = time.perf_counter() - t0
+        return p.M * sub.residues / dt / 1e9, n1, sub.residues, dt
+
+    v, n1, res, dt = timed(nthreads, budget_s)
+    v1, m1, res1, dt1 = timed(1, one_budget_s)
+    return {"value": round(v, 3), "unit": "GCUPS", "cores": nthreads, "kind": kind,
+            "sample": f"{n1} targets ({res} residues) drawn from the workload, profile M={p.M}, "
+                      f"p7_MSVFilter on {nthreads} threads, {dt:.1f} s",
+            "one_thread": {"value": round(v1, 3), "unit": "GCUPS",
+                           "sample": f"{m1} targets ({res1} residues), same profile, 1 thread, {dt1:.1f} s"},
+            "host": host}
 
 
 def vit_stage(ca, ctx, profs, pids, did, db, reps=3, cpu=True):
@@ -376,13 +379,20 @@ def serving_phase(ca, srv, profs, rate, duration, rs, drain):
     return np.array(lat), len(tickets), unfinished, cells, time.perf_counter() - t0
 
 
-def serving(ca, args):
-    db = ca.gen_database(8, 10_000_000, 350, 0.7, 20, 10000, nthreads=16)
-    rs = np.random.default_rng(2005)
-    Ms = np.clip(np.round(np.exp(np.log(250.0) + 0.6 * rs.standard_normal(64))), 30, 4000).astype(int)
-    profs = [ca.sample_profile(1000 + i, int(M)) for i, M in enumerate(Ms)]
+def serving_run(ca, n_targets, rate, duration, slots, shared, stable_s):
+    """Serving on a resident database of n_targets (the c5 generator) with
+    the 64 c5 query profiles (calibrated first, as model files carry their
+    STATS lines): open-loop Poisson arrivals at rate for duration seconds,
+    then the closed-loop capacity (every slot busy), then p50/p99 at 70% of
+    that capacity over stable_s seconds, drained."""
+    from cudahmmsearch_amd import workloads
+    t0 = time.perf_counter()
+    db = workloads.serving_database(n_targets)
+    profs = workloads.serving_queries()
+    Ms = np.array([p.M for p in profs])
     log(f"serving database: {db.n} targets, {db.residues} residues; {len(profs)} query profiles, "
-        f"median M {int(np.median(Ms))}")
+        f"median M {int(np.median(Ms))} ({time.perf_counter() - t0:.1f} s)")
+    rs = np.random.default_rng(2006)
     ctx = ca.MSVContext(0)
     for p in profs:
         pid = ctx.add_profile(p)
@@ -391,18 +401,18 @@ def serving(ca, args):
     did = ctx.add_database(db)
     meanM = float(np.mean(Ms))
     # warm tables and thresholds for every query profile
-    srv = ca.Server(ctx, did, slots=args.slots, shared=args.shared)
+    srv = ca.Server(ctx, did, slots=slots, shared=shared)
     for t in [srv.submit(p) for p in profs]:
         srv.wait(t, want_hits=False)
     # 1. offered load
-    lat, sub, unf, cells, wall = serving_phase(ca, srv, profs, args.rate, args.duration, rs, drain=False)
+    lat, sub, unf, cells, wall = serving_phase(ca, srv, profs, rate, duration, rs, drain=False)
     st = srv.stats()
     srv.stop()
-    log(f"offered {args.rate}/s: submitted {sub}, finished {lat.size}, unfinished {unf}")
+    log(f"offered {rate}/s: submitted {sub}, finished {lat.size}, unfinished {unf}")
     # 2. service capacity: every slot busy
-    srv = ca.Server(ctx, did, slots=args.slots, shared=args.shared)
+    srv = ca.Server(ctx, did, slots=slots, shared=shared)
     t0 = time.perf_counter()
-    n_cap = 12 * args.slots
+    n_cap = 12 * slots
     tk = [srv.submit(profs[int(rs.integers(len(profs)))]) for _ in range(n_cap)]
     cap_cells = sum(srv.wait(t, want_hits=False)[0]["cells"] for t in tk)
     cap_s = time.perf_counter() - t0
@@ -410,32 +420,73 @@ def serving(ca, args):
     cap_qps = n_cap / cap_s
     log(f"capacity: {cap_qps:.1f} queries/s, {cap_cells / cap_s / 1e9:.0f} GCUPS")
     # 3. stable load: 70% of capacity, drained
-    srv = ca.Server(ctx, did, slots=args.slots, shared=args.shared)
+    srv = ca.Server(ctx, did, slots=slots, shared=shared)
     rate2 = 0.7 * cap_qps
-    lat2, sub2, _, cells2, wall2 = serving_phase(ca, srv, profs, rate2, max(10.0, args.duration / 3), rs, drain=True)
+    lat2, sub2, _, cells2, wall2 = serving_phase(ca, srv, profs, rate2, stable_s, rs, drain=True)
     srv.stop()
     ctx.close()
     pct = lambda a, q: round(float(np.percentile(a, q)), 2) if a.size else None  # noqa: E731
-    out = {
-        "metric": f"MSV serving p99 latency at {args.rate:g} req/s on 1 MI355X",
-        "value": pct(lat, 99), "unit": "ms", "n_gpus": 1, "steps": sub, "warmup": len(profs),
-        "ms_per_step": pct(lat, 50), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u8", "data": "synthetic",
-        "config": {"workload": f"c5: {db.n} resident targets ({db.residues} residues), 64 query profiles "
-                               f"M log-normal median 250 (mean {meanM:.0f}), Poisson {args.rate:g} req/s "
-                               f"for {args.duration:g} s, {args.slots} slots, "
-                               f"{'shared' if args.shared else 'persistent'} launches",
-                   "cells_per_query_mean": round(meanM * db.residues)},
-        "offered": {"rate_per_s": args.rate, "submitted": sub, "finished": int(lat.size), "unfinished": unf,
-                    "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_queue": st["max_pending"],
+    return {
+        "workload": f"c5: {db.n} resident targets ({db.residues} residues), 64 query profiles "
+                    f"M log-normal median 250 (mean {meanM:.0f}), Poisson {rate:g} req/s for {duration:g} s, "
+                    f"{slots} slots, {'shared' if shared else 'persistent'} launches",
+        "cells_per_query_mean": round(meanM * db.residues),
+        "offered": {"rate_per_s": rate, "duration_s": duration, "submitted": sub, "finished": int(lat.size),
+                    "unfinished": unf, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                    "max_queue": st["max_pending"],
                     "gcups_while_serving": round(cells / max(st["busy_s"], 1e-9) / 1e9, 1),
                     "note": "latencies of finished queries only; the offered load exceeds capacity, so the "
                             "queue grows for the whole run and the unfinished queries would wait longer"},
         "capacity": {"queries_per_s": round(cap_qps, 2), "gcups": round(cap_cells / cap_s / 1e9, 1)},
-        "stable": {"rate_per_s": round(rate2, 2), "submitted": sub2, "p50_ms": pct(lat2, 50),
-                   "p99_ms": pct(lat2, 99), "gcups": round(cells2 / wall2 / 1e9, 1)},
+        "stable": {"rate_per_s": round(rate2, 2), "duration_s": stable_s, "submitted": sub2,
+                   "p50_ms": pct(lat2, 50), "p99_ms": pct(lat2, 99), "gcups": round(cells2 / wall2 / 1e9, 1)},
+    }
+
+
+def serving(ca, args):
+    """The c5 workload as its own bench line (value = p99 at the offered rate)."""
+    r = serving_run(ca, 10_000_000, args.rate, args.duration, args.slots, args.shared, max(10.0, args.duration / 3))
+    out = {
+        "metric": f"MSV serving p99 latency at {args.rate:g} req/s on 1 MI355X",
+        "value": r["offered"]["p99_ms"], "unit": "ms", "n_gpus": 1, "steps": r["offered"]["submitted"],
+        "warmup": 64, "ms_per_step": r["offered"]["p50_ms"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": r["workload"], "cells_per_query_mean": r["cells_per_query_mean"]},
+        "offered": r["offered"], "capacity": r["capacity"], "stable": r["stable"],
     }
     print(json.dumps(out))
+
+
+def roofline(se, cells, npids):
+    """Roofline of the dominant kernel from an eager search's stats: the
+    batched launch (msv_batch, one per lane-group size) that ran longest,
+    or, when no profile went through the batched kernel (models beyond its
+    LDS tables), the whole J-free stage.  achieved = 0.75 packed
+    lane-instructions per cell x the launch's cells / its HIP-event time."""
+    nb = int(se.get("n_batch", 0))
+    if nb:
+        k = max(range(nb), key=lambda j: se["batch_ms"][j])
+        ms, lcells, G = se["batch_ms"][k], se["batch_cells"][k], se["batch_lanes"][k]
+        kernel = f"msv_batch<{G}> (one persistent launch over the {G}-lane profiles)"
+        tot_ms, tot_cells = sum(se["batch_ms"][:nb]), sum(se["batch_cells"][:nb])
+    else:
+        ms, lcells, kernel = se["ms_fast"] / max(1, se["n_fast_launches"]), cells / npids, "msv_fast / msv_seg"
+        tot_ms, tot_cells = se["ms_fast"], cells
+    ach = INSTR_PER_CELL * lcells / (ms / 1e3)
+    cps = lcells / (ms / 1e3)
+    roof = {"bound": "valu", "achieved": round(ach / 1e12, 3), "peak": round(PEAK_LANE_INSTR / 1e12, 3),
+            "unit": "T lane-instr/s", "frac": round(ach / PEAK_LANE_INSTR, 4), "traffic": None,
+            "kernel": kernel, "ms_per_launch": round(ms, 3), "cells_per_launch": int(lcells),
+            "cells_per_s": round(cps, 1),
+            "frac_3ops_packed16": round(3 * cps / (2 * PEAK_LANE_INSTR), 4),
+            "algorithmic": f"3 ops per DP cell = {INSTR_PER_CELL} packed lane-instructions per cell (fused: "
+                           f"v_pk_add_f16 with clamp per 2 cells + v_pk_maximum3_f16 per 4 cells); "
+                           f"frac_3ops_packed16 counts 3 ops per cell against 2 x the issue peak",
+            "all_batched_launches": {"launches": nb, "ms": round(tot_ms, 3), "cells": int(tot_cells),
+                                     "frac": round(INSTR_PER_CELL * tot_cells / (tot_ms / 1e3) / PEAK_LANE_INSTR, 4)
+                                     if tot_ms > 0 else None},
+            "j_stage_ms": round(se.get("ms_jrow", 0.0), 3)}
+    return roof
 
 
 def main():
@@ -443,15 +494,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run a 256 MiB D2D copy first (counter scaling)")
     ap.add_argument("--batched-only", action="store_true", help="time only the batched mode (profiling runs)")
-    ap.add_argument("--rate", type=float, default=500.0, help="c5: offered queries per second")
+    ap.add_argument("--rate", type=float, default=500.0, help="serving: offered queries per second")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
-    ap.add_argument("--slots", type=int, default=4, help="c5: queries on the GPU at once")
-    ap.add_argument("--shared", action="store_true", help="c5: one chunk per workgroup (queries share CUs)")
+    ap.add_argument("--slots", type=int, default=4, help="serving: queries on the GPU at once")
+    ap.add_argument("--shared", action="store_true", help="serving: one chunk per workgroup (queries share CUs)")
     ap.add_argument("--no-vit", action="store_true", help="skip the ViterbiFilter stage on the MSV survivors")
+    ap.add_argument("--no-serving", action="store_true", help="c3: skip the serving sub-object")
+    ap.add_argument("--serving-duration", type=float, default=10.0, help="c3: seconds of arrivals in the serving phase")
     ap.add_argument("--scan-profiles", type=int, default=10000, help="scan: profiles in the scanned database")
     ap.add_argument("--scan-queries", type=int, default=64, help="scan: query sequences")
     args = ap.parse_args()
@@ -462,6 +515,7 @@ def main():
         sys.exit(2)
 
     import cudahmmsearch_amd as ca
+    from cudahmmsearch_amd import workloads
     if args.workload == "c5":
         serving(ca, args)
         return
@@ -471,7 +525,7 @@ def main():
     if args.copy_probe:
         copy_probe()
     t0 = time.perf_counter()
-    profs, db, desc = make_workload(ca, args.workload)
+    profs, db, desc = workloads.make_workload(args.workload)
     log(f"workload {args.workload}: {len(profs)} profiles, {db.n} targets, {db.residues} residues "
         f"({time.perf_counter() - t0:.1f} s)")
     ctx = ca.MSVContext(0)
@@ -498,8 +552,9 @@ def main():
         p_b = np.zeros_like(n_plain)
         ctx.lib.msvg_results(ctx._c, u_b.ctypes.data, None, p_b.ctypes.data)
         identical = bool(np.array_equal(u_b.view(np.uint32), u_plain.view(np.uint32)) and np.array_equal(p_b, n_plain))
+        del u_b, p_b, u_plain, n_plain
     log(f"batched: {np.median(tb) * 1e3:.1f} ms/step; identical to plain: {identical}")
-    # one eager batched run for the per-stage breakdown (same kernels, no graph)
+    # one eager batched run for the per-stage and per-launch breakdown (same kernels, no graph)
     ctx.search_device(pids, did, mode=ca.MODE_BATCHED | ca.FLAG_EAGER)
     se = ctx.stats()
 
@@ -507,23 +562,16 @@ def main():
     ms_p = float(np.median(tp) * 1e3) if tp is not None else None
     gcups_b = cells / (ms_b / 1e3) / 1e9
     gcups_p = cells / (ms_p / 1e3) / 1e9 if ms_p else None
-    fast_ms_per_launch = se["ms_fast"] / max(1, se["n_fast_launches"])
-    cells_per_launch = cells / len(pids)
-    achieved = INSTR_PER_CELL * cells_per_launch / (fast_ms_per_launch / 1e3)
-    traffic = None
+    roof = roofline(se, cells, len(pids))
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh)["bytes_per_launch"]     # rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled
-    roof = {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": round(PEAK_LANE_INSTR / 1e12, 3),
-            "unit": "T lane-instr/s", "frac": round(achieved / PEAK_LANE_INSTR, 4), "traffic": traffic,
-            "kernel": "msv_fast (+ msv_jrow for the pairs whose J state may fire)",
-            "ms_per_launch": round(fast_ms_per_launch, 3),
-            "cells_per_s": round(cells_per_launch / (fast_ms_per_launch / 1e3), 1),
-            "algorithmic": f"3 ops per DP cell = {INSTR_PER_CELL} packed lane-instructions per cell, "
-                           f"{cells_per_launch:.4e} cells per launch"}
+            tr = json.load(fh)
+        if tr.get("kernel", "").split(" ")[0] == roof["kernel"].split(" ")[0]:
+            roof["traffic"] = tr["bytes_per_launch"]      # rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled
+            roof["traffic_source"] = os.path.relpath(tpath, ROOT)
     out = {
-        "metric": "MSV GCUPS on 1 MI355X, batched vs plain kernel (bit-exact)",
+        "metric": "MSV GCUPS on 1 MI355X, batched vs plain kernel (bit-exact); serving p50/p99 ms",
         "value": round(gcups_b, 1), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_b, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u8", "data": "synthetic",
@@ -535,9 +583,12 @@ def main():
         "bit_identical_plain_batched": identical,
         "stages_ms": {"fast": round(se["ms_fast"], 3), "slow": round(se["ms_slow"], 3),
                       "finalize": round(se["ms_finalize"], 3)},
+        "launches": {"j_free": int(se["n_fast_launches"]), "batched": int(se["n_batch"]),
+                     "batched_lanes": se["batch_lanes"][:se["n_batch"]],
+                     "batched_ms": [round(x, 3) for x in se["batch_ms"][:se["n_batch"]]]},
         "deferred_pairs": {"j_aware": int(se["n_jrow"]), "byte_msv": int(se["n_slow"])}, "roofline": roof,
     }
-    # LDS traffic of the fast stage: per target row each lane reads its Q
+    # LDS traffic of the J-free pass: per target row each lane reads its Q
     # registers' emissions (4 bytes each) and G lanes advance one target,
     # i.e. Q * 4 * G bytes per target row, Q = ceil(M / 2G).
     lds_bytes = 0
@@ -546,15 +597,16 @@ def main():
         if not seg:
             Q = -(-p.M // (2 * G))
             lds_bytes += Q * 4 * G * db.residues
-    if lds_bytes:
+    if lds_bytes and se["ms_fast"] > 0:
         peak_lds = ctx.probe_lds()
-        ach_lds = lds_bytes / len(pids) / (fast_ms_per_launch / 1e3)
+        ach_lds = lds_bytes / (se["ms_fast"] / 1e3)
         roof["lds"] = {"achieved": round(ach_lds / 1e12, 3), "probe_peak": round(peak_lds / 1e12, 3),
-                       "unit": "TB/s", "frac": round(ach_lds / peak_lds, 4),
-                       "bytes_per_launch": int(lds_bytes / len(pids)),
-                       "note": "emission reads from LDS against the device's measured ds_read_b128 bandwidth"}
+                       "unit": "TB/s", "frac": round(ach_lds / peak_lds, 4), "bytes_per_search": int(lds_bytes),
+                       "note": "emission reads from LDS over the whole J-free stage against the device's "
+                               "measured ds_read_b128 bandwidth"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(profs, db)
+        log(f"cpu baseline: {out['cpu_baseline']}")
     if args.workload == "c1":
         ctx.search_device(pids, did, mode=ca.MODE_BATCHED)
         ub = np.zeros((len(pids), db.n), np.float32)
@@ -569,6 +621,10 @@ def main():
         out["vit_stage"] = vit_stage(ca, ctx, profs, pids, did, db, cpu=not args.no_cpu)
         log(f"vit stage: {out['vit_stage']}")
     ctx.close()
+    del db
+    if args.workload == "c3" and not args.no_serving:
+        out["serving"] = serving_run(ca, 10_000_000, args.rate, args.serving_duration, args.slots, args.shared, 10.0)
+        log(f"serving: {out['serving']}")
     print(json.dumps(out))
 
 

@@ -57,7 +57,9 @@ class _Prof(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("ms_fast", C.c_double), ("ms_slow", C.c_double), ("ms_finalize", C.c_double),
                 ("ms_total", C.c_double), ("cells", C.c_int64), ("n_slow", C.c_int64), ("n_jrow", C.c_int64),
-                ("n_fast_launches", C.c_int32), ("dropin_path", C.c_int32)]
+                ("n_fast_launches", C.c_int32), ("dropin_path", C.c_int32), ("n_batch", C.c_int32),
+                ("batch_lanes", C.c_int32 * 4), ("batch_cells", C.c_int64 * 4), ("batch_ms", C.c_double * 4),
+                ("ms_jrow", C.c_double)]
 
 
 class _Hmm(C.Structure):
@@ -81,7 +83,7 @@ class _QRes(C.Structure):
 class _SInfo(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
                 ("pending", C.c_int64), ("running", C.c_int64), ("max_pending", C.c_int64),
-                ("busy_s", C.c_double), ("uptime_s", C.c_double)]
+                ("busy_s", C.c_double), ("uptime_s", C.c_double), ("cached_profiles", C.c_int64)]
 
 
 class _Rng(C.Structure):
@@ -615,7 +617,11 @@ class MSVContext:
     def stats(self):
         s = _Stats()
         self._chk(self.lib.msvg_last_stats(self._c, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+        out = {}
+        for k, _ in _Stats._fields_:
+            v = getattr(s, k)
+            out[k] = list(v) if isinstance(v, C.Array) else v
+        return out
 
     def msv_filter(self, pid, x):
         """p7_MSVFilter on one target (codes, no sentinels): (status, usc)."""

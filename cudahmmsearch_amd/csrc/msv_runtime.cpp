@@ -223,6 +223,10 @@ struct msvg_ctx {
   int64_t slow_cap = 0;            // slow-queue capacity for the next search
   int last_np = 0, last_db = -1;
   int last_launches = 0;           // J-free launches of the latest enqueued search
+  int last_nbatch = 0;             // batched launches of the latest enqueued search
+  int last_batch_G[4] = {}, last_batch_timed = 0;
+  int64_t last_batch_cells[4] = {};
+  hipEvent_t bev[10] = {};         // eager batched searches: around each batched launch, then the J stage
   msvg_stats stats{};
   hipEvent_t ev[8] = {};
   std::map<std::string, std::unique_ptr<Plan>> plans;
@@ -577,7 +581,7 @@ int check_ids(msvg_ctx *c, const int32_t *ids, int np, int dbid) {
 // the stream joins back before the byte stage.
 int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, double F1, int mode, bool timed,
                    bool capturing, int64_t *cells_out, hipStream_t st = nullptr, int sched = 0,
-                   hipStream_t side = nullptr) {
+                   hipStream_t side = nullptr, bool tables_only = false) {
   if (!st) st = c->stream;
   bool side_used = false;
   // J-aware launch of profile i on the side stream (or in line on st).
@@ -602,6 +606,8 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   // Batched searches on persistent launches take every profile the batched
   // kernel fits (msv_batch.hip); shared-schedule serving keeps msv_fast.
   const bool batch = (mode == MSVG_MODE_BATCHED && sched == 0);
+  c->last_nbatch = 0;
+  c->last_batch_timed = 0;
   int64_t cells = 0;
   for (int i = 0; i < np; ++i) {
     Profile &p = *c->profiles[ids[i]];
@@ -646,6 +652,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     cells += (int64_t) p.M * d.total_res;
   }
   *cells_out = cells;
+  if (tables_only) return MSVG_OK;          // the tables' builds are enqueued on st
 
   // Per-search buffers.
   HIPCHK(c, b.d_res.ensure((size_t) np * n * sizeof(uint16_t)));
@@ -744,7 +751,14 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
                                hipMemcpyHostToDevice, st));
       HIPCHK(c, hipStreamSynchronize(st));             // the host vectors end with this block
     }
+    c->last_nbatch = 0;
+    c->last_batch_timed = timed ? 1 : 0;
     for (const BatLaunch &L : bl) {
+      const int k = c->last_nbatch++;
+      c->last_batch_G[k] = L.G;
+      c->last_batch_cells[k] = 0;
+      for (int i = 0; i < np; ++i)
+        if (Gb[i] == L.G) c->last_batch_cells[k] += (int64_t) c->profiles[ids[i]]->M * d.total_res;
       BatArgs a;
       a.profs = b.d_bat.as<BatProf>() + L.prof_at;
       a.np = L.np;
@@ -758,7 +772,9 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       a.slow_cap = (int32_t) b.slow_cap;
       a.jslow_cap = (int32_t) b.jcap;
       a.slot_u4 = (int32_t) ((L.lds - 16) / 16);
+      if (timed) HIPCHK(c, hipEventRecord(c->bev[2 * k], st));
       HIPCHK(c, msvk_launch_batch(L.G, &a, L.grid, L.block, L.lds, st));
+      if (timed) HIPCHK(c, hipEventRecord(c->bev[2 * k + 1], st));
       ++n_launch;
     }
   }
@@ -871,6 +887,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     for (int i = 0; i < np; ++i)
       if (Gb[i] && jtab[i]) jl.push_back(i);
     const int nst = (side && !jl.empty()) ? std::min<int>(kJStreams, (int) jl.size()) : 0;
+    if (timed) HIPCHK(c, hipEventRecord(c->bev[8], st));
     if (nst) {
       HIPCHK(c, hipEventRecord(c->fork, st));
       for (int k = 0; k < nst; ++k) HIPCHK(c, hipStreamWaitEvent(c->jst[k], c->fork, 0));
@@ -895,6 +912,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
         HIPCHK(c, hipEventRecord(c->jdone[k], c->jst[k]));
         HIPCHK(c, hipStreamWaitEvent(st, c->jdone[k], 0));
       }
+    if (timed) HIPCHK(c, hipEventRecord(c->bev[9], st));
   }
   if (timed) HIPCHK(c, hipEventRecord(c->ev[1], st));
   c->last_launches = n_launch;
@@ -943,6 +961,23 @@ int after_search(msvg_ctx *c, Bufs &b, int np, int dbid, int64_t cells, bool tim
     HIPCHK(c, hipEventElapsedTime(&f, c->ev[2], c->ev[3]));
     HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[3]));
     c->stats.ms_fast = a; c->stats.ms_slow = b; c->stats.ms_finalize = f; c->stats.ms_total = t;
+  }
+  c->stats.n_batch = c->last_nbatch;
+  c->stats.ms_jrow = 0;
+  for (int k = 0; k < 4; ++k) {
+    c->stats.batch_lanes[k] = k < c->last_nbatch ? c->last_batch_G[k] : 0;
+    c->stats.batch_cells[k] = k < c->last_nbatch ? c->last_batch_cells[k] : 0;
+    c->stats.batch_ms[k] = 0;
+  }
+  if (timed && c->last_batch_timed && c->last_nbatch > 0) {
+    for (int k = 0; k < c->last_nbatch; ++k) {
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, c->bev[2 * k], c->bev[2 * k + 1]));
+      c->stats.batch_ms[k] = ms;
+    }
+    float mj = 0;
+    HIPCHK(c, hipEventElapsedTime(&mj, c->bev[8], c->bev[9]));
+    c->stats.ms_jrow = mj;
   }
   c->last_np = np;
   c->last_db = dbid;
@@ -1092,6 +1127,8 @@ int msvg_init(int device, msvg_ctx **ret) {
   }
   for (auto &e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return MSVG_EDEVICE;
+  for (auto &e : c->bev)
+    if (hipEventCreate(&e) != hipSuccess) return MSVG_EDEVICE;
   if (msvk_prepare() != hipSuccess) return MSVG_EDEVICE;
   if (msvk_prepare_batch() != hipSuccess) return MSVG_EDEVICE;
   *ret = c.release();
@@ -1107,6 +1144,7 @@ void msvg_free(msvg_ctx *c) {
   c->dbs.clear();
   c->profiles.clear();
   for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
+  for (auto &e : c->bev) if (e) (void) hipEventDestroy(e);
   if (c->fork) (void) hipEventDestroy(c->fork);
   if (c->join) (void) hipEventDestroy(c->join);
   if (c->side) { (void) hipStreamSynchronize(c->side); (void) hipStreamDestroy(c->side); }
@@ -2472,6 +2510,7 @@ namespace {
 
 struct ServerQuery {
   int64_t ticket = 0;
+  std::string key;                // profile-cache key (set when admitted)
   msvg_oprofile desc{};
   std::vector<uint8_t> rbv;
   double t_submit = 0, t_admit = -1, t_done = -1;
@@ -2506,7 +2545,13 @@ struct msvg_server {
   std::condition_variable cv;
   std::deque<ServerQuery *> pending;
   std::map<int64_t, std::unique_ptr<ServerQuery>> queries;
-  std::map<std::string, int> prof_cache;
+  // Query profiles added to the context, by content: reused by later
+  // queries of the same profile; at most kServerProfiles are kept, the least
+  // recently used one that no running query holds is removed first.
+  struct CachedProfile { int pid = -1; int refs = 0; uint64_t used = 0; };
+  std::map<std::string, CachedProfile> prof_cache;
+  uint64_t use_clock = 0;
+  int64_t cached = 0;             // prof_cache.size(), readable under mu
   std::thread th;
   std::atomic<bool> stopping{false};
   std::chrono::steady_clock::time_point t0;
@@ -2521,6 +2566,8 @@ struct msvg_server {
 };
 
 namespace {
+
+constexpr size_t kServerProfiles = 256;   // query profiles a server keeps on the device
 
 std::string profile_key(const ServerQuery &q) {
   std::string k(reinterpret_cast<const char *>(q.rbv.data()), q.rbv.size());
@@ -2539,21 +2586,58 @@ size_t query_lds(msvg_ctx *c, const Db &d, int M) {
   return fast_lds(d.R, Q);
 }
 
+// The query's profile leaves the cache's hold list (its search is over or
+// never started).
+void server_release(msvg_server *s, ServerQuery &q) {
+  auto it = s->prof_cache.find(q.key);
+  if (it != s->prof_cache.end() && it->second.refs > 0) it->second.refs--;
+}
+
+// Remove least recently used profiles beyond kServerProfiles that no running
+// query holds (their tables are idle; msvg_profile_remove frees them).
+void server_evict(msvg_server *s) {
+  while (s->prof_cache.size() > kServerProfiles) {
+    auto victim = s->prof_cache.end();
+    for (auto it = s->prof_cache.begin(); it != s->prof_cache.end(); ++it)
+      if (it->second.refs == 0 && (victim == s->prof_cache.end() || it->second.used < victim->second.used))
+        victim = it;
+    if (victim == s->prof_cache.end()) return;
+    (void) msvg_profile_remove(s->ctx, victim->second.pid);
+    s->prof_cache.erase(victim);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cached = (int64_t) s->prof_cache.size();
+  }
+}
+
 int server_start_query(msvg_server *s, Slot &sl, ServerQuery &q) {
   msvg_ctx *c = s->ctx;
   const Db &d = *c->dbs[s->db];
   int pid;
-  const std::string key = profile_key(q);
-  auto it = s->prof_cache.find(key);
+  q.key = profile_key(q);
+  auto it = s->prof_cache.find(q.key);
   if (it != s->prof_cache.end()) {
-    pid = it->second;
+    pid = it->second.pid;
   } else {
     q.desc.rbv = q.rbv.data();
     int32_t id;
     int rc = msvg_profile_add(c, &q.desc, &id);
     if (rc != MSVG_OK) return rc;
     pid = id;
-    s->prof_cache[key] = pid;
+    // Build the profile's tables on this slot's stream and wait for them
+    // before the id is cached: a later query of the same profile may run on
+    // another slot's stream, which must not read a table still being built.
+    int64_t cells0 = 0;
+    rc = enqueue_search(c, sl.bufs, &pid, 1, s->db, s->F1, MSVG_MODE_BATCHED, false, false, &cells0, sl.stream,
+                        s->sched, nullptr, true);
+    if (rc == MSVG_OK && hipStreamSynchronize(sl.stream) != hipSuccess) rc = fail(c, MSVG_EDEVICE, "table build");
+    if (rc != MSVG_OK) { (void) msvg_profile_remove(c, pid); return rc; }
+    it = s->prof_cache.emplace(q.key, msvg_server::CachedProfile{pid, 0, 0}).first;
+  }
+  it->second.refs++;
+  it->second.used = ++s->use_clock;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cached = (int64_t) s->prof_cache.size();
   }
   int64_t cells = 0;
   int rc = enqueue_search(c, sl.bufs, &pid, 1, s->db, s->F1, MSVG_MODE_BATCHED, false, false, &cells, sl.stream,
@@ -2602,6 +2686,8 @@ void server_loop(msvg_server *s) {
       if (hipEventQuery(sl.done) != hipSuccess) continue;
       ServerQuery *q = sl.q;
       const int rc = server_finish_query(s, sl, *q);
+      server_release(s, *q);
+      server_evict(s);
       std::lock_guard<std::mutex> lk(s->mu);
       q->status = rc;
       q->state = 2;
@@ -2650,6 +2736,7 @@ void server_loop(msvg_server *s) {
       free_slot->q = q;
       const int rc = server_start_query(s, *free_slot, *q);
       if (rc != MSVG_OK) {
+        server_release(s, *q);
         std::lock_guard<std::mutex> lk(s->mu);
         q->status = rc;
         q->state = 2;
@@ -2770,6 +2857,7 @@ int msvg_server_stats(msvg_server *s, msvg_server_info *out) {
   std::lock_guard<std::mutex> lk(s->mu);
   *out = s->st;
   out->uptime_s = s->now();
+  out->cached_profiles = s->cached;
   if (s->busy_since >= 0) out->busy_s += out->uptime_s - s->busy_since;
   return MSVG_OK;
 }
@@ -2792,7 +2880,7 @@ int msvg_server_stop(msvg_server *s) {
     (void) hipStreamDestroy(sl->stream);
     (void) hipHostFree(sl->h_count);
   }
-  for (auto &kv : s->prof_cache) (void) msvg_profile_remove(c, kv.second);
+  for (auto &kv : s->prof_cache) (void) msvg_profile_remove(c, kv.second.pid);
   delete s;
   return MSVG_OK;
 }

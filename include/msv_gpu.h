@@ -70,6 +70,14 @@ typedef struct {
   int64_t n_jrow;          /* pairs that ran the J-aware f16 stage */
   int32_t n_fast_launches;
   int32_t dropin_path;     /* after msvg_MSVFilter: 1 its one-launch kernel, 2 the search path */
+  /* Batched searches: the persistent launches of the batched pass (one per
+   * lane-group size, longest-M group first) and, for eager searches, each
+   * launch's duration and that of the J-aware launches after them. */
+  int32_t n_batch;
+  int32_t batch_lanes[4];  /* lanes per target of each batched launch */
+  int64_t batch_cells[4];  /* sum of M * L over the pairs each one scores */
+  double  batch_ms[4];     /* HIP events around each launch on the search stream (eager only, else 0) */
+  double  ms_jrow;         /* from the last batched launch's end to the J-aware launches' join (eager only) */
 } msvg_stats;
 
 int         msvg_init(int device, msvg_ctx **ret_ctx);
@@ -253,6 +261,8 @@ typedef struct {
   int64_t max_pending;
   double  busy_s;          /* time with at least one query on the GPU       */
   double  uptime_s;
+  int64_t cached_profiles; /* query profiles kept on the device for reuse (at most 256; the least
+                              recently used one that no running query holds is removed first) */
 } msvg_server_info;
 
 /* slots: queries admitted to the GPU at once (1..16); hbm_budget: device

@@ -120,11 +120,14 @@ __device__ __forceinline__ void task64(const BatArgs &a, const BatProf &bp, cons
 
   // Residues 16 at a time through the scalar cache, the next block fetched
   // while the current one runs; each row's emissions are read one row ahead.
-  // (A block past the target's end reads its first bytes again: in bounds.)
+  // Full blocks of 16 rows first, then the L % 16 rows of the last block
+  // (the padding rows after them would only add cells at the floor).  (A
+  // block past the target's end reads its first bytes again: in bounds.)
+  const int Lfull = L & ~15;
   uint4 c = (Lpad > 0) ? sload(reinterpret_cast<const uint4 *>(p)) : null4;
   uint32_t e[QB * 4];
   load_row<Q>(tl, byte_of(c, 0), e);
-  for (int i0 = 0; i0 < Lpad; i0 += 16) {
+  for (int i0 = 0; i0 < Lfull; i0 += 16) {
     const int i1 = i0 + 16;
     const int inext = __builtin_amdgcn_readfirstlane(i1 < Lpad ? i1 : 0);
     const uint4 cl = sload(reinterpret_cast<const uint4 *>(p + inext));
@@ -140,6 +143,26 @@ __device__ __forceinline__ void task64(const BatArgs &a, const BatProf &bp, cons
       asm volatile("" : "+v"(acc));
     });
     c = cn;
+  }
+  const int rem = L - Lfull;                           // wave-uniform
+  if (rem > 0) {
+    static_for<0, 16>([&](auto rrc) {
+      constexpr int rr = decltype(rrc)::value;
+      if (rr < rem) {
+        uint32_t en[QB * 4];
+        if constexpr (rr < 15) load_row<Q>(tl, byte_of(c, rr + 1), en);
+        row_step<64, Q, rr>(r, e, acc, lane);
+        if constexpr (rr < 15) {
+#pragma unroll
+          for (int k = 0; k < Q; ++k) e[k] = en[k];
+        }
+        row_fence<Q>(r);
+        asm volatile("" : "+v"(acc));
+      }
+    });
+    // odd Q: a last row of even index left its register 0 for a next row
+    if constexpr ((Q & 1) != 0)
+      if (rem & 1) acc = max2h(acc, r[0]);
   }
 
   const h2v ah = as_h2(acc);
@@ -184,10 +207,11 @@ __device__ __forceinline__ void task_narrow(const BatArgs &a, const BatProf &bp,
   const int Lpad = (L + 15) & ~15;
   const uint8_t *p = a.db.res + (valid ? a.db.off[s] : 0);
 
-  int Lmax = Lpad;
+  int Lmax = L;                                         // the wave's longest target
 #pragma unroll
   for (int o = 32; o >= G; o >>= 1) Lmax = max(Lmax, __shfl_xor(Lmax, o, 64));
   Lmax = __builtin_amdgcn_readfirstlane(Lmax);
+  const int Lfull = Lmax & ~15;
 
   const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
   const uint4 null4 = make_uint4(nb, nb, nb, nb);
@@ -203,7 +227,7 @@ __device__ __forceinline__ void task_narrow(const BatArgs &a, const BatProf &bp,
   uint4 cw = addr_bytes<TW>((0 < Lpad) ? cnext : null4, half);
   uint32_t e[QB * 4];
   load_at<Q, JB>(row_addr<0>(cw, col), e);
-  for (int i0 = 0; i0 < Lmax; i0 += 16) {
+  for (int i0 = 0; i0 < Lfull; i0 += 16) {
     const int i1 = i0 + 16;
     cnext = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
     const uint4 cnw = addr_bytes<TW>((i1 < Lpad) ? cnext : null4, half);
@@ -219,6 +243,25 @@ __device__ __forceinline__ void task_narrow(const BatArgs &a, const BatProf &bp,
       asm volatile("" : "+v"(acc));
     });
     cw = cnw;
+  }
+  const int rem = Lmax - Lfull;                        // wave-uniform; shorter targets read null rows
+  if (rem > 0) {
+    static_for<0, 16>([&](auto rrc) {
+      constexpr int rr = decltype(rrc)::value;
+      if (rr < rem) {
+        uint32_t en[QB * 4];
+        if constexpr (rr < 15) load_at<Q, JB>(row_addr<rr + 1>(cw, col), en);
+        row_step<G, Q, rr>(r, e, acc, lane);
+        if constexpr (rr < 15) {
+#pragma unroll
+          for (int k = 0; k < Q; ++k) e[k] = en[k];
+        }
+        row_fence<Q>(r);
+        asm volatile("" : "+v"(acc));
+      }
+    });
+    if constexpr ((Q & 1) != 0)
+      if (rem & 1) acc = max2h(acc, r[0]);
   }
 
   const h2v ah = as_h2(acc);

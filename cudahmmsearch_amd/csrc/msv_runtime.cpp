@@ -714,10 +714,14 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       size_t lds = 0;
       for (int i : mem) lds = std::max(lds, bat_bytes(G, d.R, pds[i].Q));
       lds += 16;                                        // the workgroup's broadcast word
-      const int lcu = (int) std::max<size_t>(1, std::min<size_t>(8, kLdsBytes / lds));
-      const int block = 64 * std::min(16, std::max(4, (16 + lcu - 1) / lcu));
-      int occ = 0;
-      HIPCHK(c, msvk_batch_occupancy(G, block, lds, &occ));
+      // Block size: the most resident waves per CU (LDS and VGPRs decide),
+      // at most 16 waves per workgroup; ties go to the smaller block.
+      int block = 0, occ = 0;
+      for (int w : {4, 6, 8, 12, 16}) {
+        int o = 0;
+        HIPCHK(c, msvk_batch_occupancy(G, 64 * w, lds, &o));
+        if (o * w > occ * (block / 64)) { block = 64 * w; occ = o; }
+      }
       if (occ < 1) return fail(c, MSVG_EDEVICE, "batched kernel cannot be resident");
       BatLaunch L{G, c->ncu * occ, block, lds, hprof.size(), hfirst.size(), (int) mem.size()};
       double wsum = 0;
@@ -879,10 +883,12 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     HIPCHK(c, hipStreamWaitEvent(st, c->join, 0));
   }
   if (batch) {
-    // J-aware launches of the batched profiles, once their queues are full:
-    // one per profile, round-robin over kJStreams streams forked from st (in
-    // line on st when no side stream is given).  Each lasts as long as its
-    // longest pair; workgroups without a pair leave before staging.
+    // J-aware launches of the batched profiles, once the batched launches
+    // have filled their queues: one per profile, round-robin over kJStreams
+    // streams forked from st (in line on st when no side stream is given).
+    // Each lasts as long as its longest pair; workgroups without a pair leave
+    // before staging.  (Forking each group's J launches right after its own
+    // batched launch, beside the later groups' launches, measured slower.)
     std::vector<int> jl;
     for (int i = 0; i < np; ++i)
       if (Gb[i] && jtab[i]) jl.push_back(i);
@@ -894,6 +900,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     }
     for (size_t u = 0; u < jl.size(); ++u) {
       const int i = jl[u];
+      hipStream_t js = nst ? c->jst[u % nst] : st;
       JArgs ja;
       ja.prof = pds[i];
       ja.prof.res = b.d_res.as<uint16_t>() + (size_t) i * n;
@@ -905,7 +912,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       const int Q64 = qfor(c->profiles[ids[i]]->M, 128);
       const size_t jlds = fast_lds(d.R, Q64);
       const int per_cu = (int) std::max<size_t>(1, std::min<size_t>(4, kLdsBytes / jlds));
-      HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jlds, nst ? c->jst[u % nst] : st));
+      HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jlds, js));
     }
     if (nst)
       for (int k = 0; k < nst; ++k) {

@@ -117,6 +117,7 @@ def load_library(path=LIB_PATH):
         "msvg_search": (C.c_int, [vp, vp, i32, i32, C.c_double, C.c_int, vp, vp, vp]),
         "msvg_search_device": (C.c_int, [vp, vp, i32, i32, C.c_double, C.c_int]),
         "msvg_results": (C.c_int, [vp, vp, vp, vp]),
+        "msvg_search_noalloc": (C.c_int, [vp, vp, i32, i32, C.c_double, vp, vp, vp]),
         "msvg_last_stats": (C.c_int, [vp, C.POINTER(_Stats)]),
         "msvg_MSVFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),

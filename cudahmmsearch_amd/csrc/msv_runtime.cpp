@@ -37,6 +37,13 @@
 #include "../../include/msv_prep.h"
 #include "../../include/msv_prep.h"
 
+// Scratch memory of the allocation-free search (msvg_search_noalloc): one
+// array per GPU, held until the process ends.  Host code never uses this
+// name as an address: msvg_init looks up each device's copy with
+// hipGetSymbolAddress and keeps that pointer in the context.
+constexpr size_t kScratchBytes = (size_t) 128 << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
 extern "C" hipError_t msvk_prepare();
 extern "C" hipError_t msvk_prepare_batch();
 extern "C" hipError_t msvk_launch_batch(int G, const BatArgs *a, int grid, int block, size_t lds, hipStream_t st);
@@ -93,15 +100,38 @@ bool alloc_allowed() {
   return g_alloc_budget.fetch_sub(1) > 0;
 }
 
+// A bump allocator over a fixed device range (the allocation-free search
+// carves its tables, queues and result words from the library's static
+// scratch array): nothing is freed, a call starts again from the bottom.
+struct Arena {
+  uint8_t *base = nullptr;
+  size_t cap = 0, used = 0;
+  void *take(size_t bytes) {
+    const size_t off = (used + 255) & ~(size_t) 255;
+    if (off > cap || bytes > cap - off) return nullptr;
+    used = off + bytes;
+    return base + off;
+  }
+};
+
 struct DevMem {
   void *p = nullptr;
   size_t n = 0;
+  Arena *arena = nullptr;          // non-null: memory comes from this arena (never freed here)
+  bool borrowed = false;           // p is the caller's memory
   DevMem() = default;
   DevMem(const DevMem &) = delete;
   DevMem &operator=(const DevMem &) = delete;
-  ~DevMem() { if (p) (void) hipFree(p); }
+  ~DevMem() { if (p && !arena && !borrowed) (void) hipFree(p); }
   hipError_t ensure(size_t bytes) {
     if (bytes <= n && p) return hipSuccess;
+    if (arena || borrowed) {       // no allocation: the range given must do
+      if (p || borrowed) return hipErrorOutOfMemory;
+      p = arena->take(bytes ? bytes : 16);
+      if (!p) return hipErrorOutOfMemory;
+      n = bytes;
+      return hipSuccess;
+    }
     if (p) { (void) hipFree(p); p = nullptr; n = 0; }
     if (!alloc_allowed()) return hipErrorOutOfMemory;
     hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
@@ -188,6 +218,14 @@ struct Bufs {                             // device buffers one search writes
   DevMem d_bat;                          // batched launches: BatProf arrays and first-profile lists
   int64_t slow_cap = 0;
   int64_t jcap = 0;                      // per-profile J-aware queue capacity
+  Arena *arena = nullptr;                // allocation-free search: tables, chunk lists and buffers from here
+  bool ext_out = false;                  // usc/status/pass are the caller's device arrays (any may be null)
+  Bufs() = default;
+  explicit Bufs(Arena *a) : arena(a) {
+    for (DevMem *m : {&d_res, &d_prof, &d_slow, &d_counters, &d_usc, &d_status, &d_pass, &d_thr, &d_usctab,
+                      &d_seg, &d_jslow, &d_bat})
+      m->arena = a;
+  }
 };
 
 struct Plan {                             // a captured batched search with its own buffers
@@ -222,6 +260,7 @@ struct msvg_ctx {
   Bufs *last = nullptr;            // buffers holding the latest results
   int64_t slow_cap = 0;            // slow-queue capacity for the next search
   int last_np = 0, last_db = -1;
+  uint8_t *scratch_dev = nullptr;  // this device's copy of the scratch array (hipGetSymbolAddress)
   int last_launches = 0;           // J-free launches of the latest enqueued search
   int last_nbatch = 0;             // batched launches of the latest enqueued search
   int last_batch_G[4] = {}, last_batch_timed = 0;
@@ -426,9 +465,7 @@ const std::pair<std::vector<uint16_t>, std::vector<float>> &final_tables(msvg_ct
   return c->finals.emplace(key, std::make_pair(std::move(thr), std::move(usc))).first->second;
 }
 
-int build_chunks(msvg_ctx *c, Db &d, int G) {
-  auto it = d.chunks.find(G);
-  if (it != d.chunks.end()) return MSVG_OK;
+std::vector<int32_t> guided_chunks(const msvg_ctx *c, const Db &d, int G) {
   const int NG = 64 / G;
   const int64_t ntasks = (d.n + NG - 1) / NG;
   // Guided chunks: each holds about 1/(8 waves) of the residues not yet
@@ -452,10 +489,36 @@ int build_chunks(msvg_ctx *c, Db &d, int G) {
     }
   }
   if (cs.back() != ntasks) cs.push_back((int32_t) ntasks);
+  return cs;
+}
+
+int build_chunks(msvg_ctx *c, Db &d, int G) {
+  auto it = d.chunks.find(G);
+  if (it != d.chunks.end()) return MSVG_OK;
+  const std::vector<int32_t> cs = guided_chunks(c, d, G);
   auto mem = std::make_unique<DevMem>();
   HIPCHK(c, mem->ensure(cs.size() * sizeof(int32_t)));
   HIPCHK(c, hipMemcpy(mem->p, cs.data(), cs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   d.chunks.emplace(G, std::make_pair(std::move(mem), (int) cs.size() - 1));
+  return MSVG_OK;
+}
+
+// The chunk list of lane-group size G over database d: cached on the
+// database, or (allocation-free search) copied into the arena per call.
+int chunk_list(msvg_ctx *c, Db &d, int G, Arena *ar, const int32_t **cs_out, int *nchunks) {
+  if (!ar) {
+    int rc = build_chunks(c, d, G);
+    if (rc != MSVG_OK) return rc;
+    *cs_out = d.chunks[G].first->as<int32_t>();
+    *nchunks = d.chunks[G].second;
+    return MSVG_OK;
+  }
+  const std::vector<int32_t> cs = guided_chunks(c, d, G);
+  void *m = ar->take(cs.size() * sizeof(int32_t));
+  if (!m) return fail(c, MSVG_EINVAL, "allocation-free search: scratch exhausted (chunk list)");
+  HIPCHK(c, hipMemcpy(m, cs.data(), cs.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  *cs_out = static_cast<const int32_t *>(m);
+  *nchunks = (int) cs.size() - 1;
   return MSVG_OK;
 }
 
@@ -465,9 +528,19 @@ Tables &tables_for(msvg_ctx *c, int pid, int dbid) {
   return *t;
 }
 
-int ensure_fast_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **out, hipStream_t st) {
+int ensure_fast_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **out, hipStream_t st,
+                      Arena *ar = nullptr) {
   Profile &p = *c->profiles[pid];
   Db &d = *c->dbs[dbid];
+  if (ar) {                                 // allocation-free search: built per call in the arena
+    const int Q = qfor(p.M, 2 * G);
+    void *m = ar->take(fast_lds(d.R, Q));
+    if (!m) return fail(c, MSVG_EINVAL, "allocation-free search: scratch exhausted (fast table)");
+    HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, d.d_rowcode.as<uint8_t>(), d.R, G, Q, 64, p.bias,
+                                    static_cast<uint32_t *>(m), st));
+    *out = static_cast<const uint32_t *>(m);
+    return MSVG_OK;
+  }
   Tables &t = tables_for(c, pid, dbid);
   auto it = t.fast.find(G);
   if (it == t.fast.end()) {
@@ -485,6 +558,8 @@ int ensure_fast_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **ou
 // Batched-pass tables (msv_device.h, BatProf): G = 64 uses the fast-stage
 // table; G = 8, 16, 32 the narrow [block][32 rows][TW lanes] layout.
 int bat_tw(int G) { return G < 16 ? 16 : G; }
+// Byte-MSV queue of the allocation-free search (fixed: overflow is an error).
+int64_t noalloc_slow_cap(int np, int64_t n) { return std::max<int64_t>(4096, (int64_t) np * n / 16); }
 int bat_qmax(int G) { return G == 64 ? kQMax : G == 16 ? 24 : 16; }
 size_t bat_bytes(int G, int R, int Q) {
   return G == 64 ? fast_lds(R, Q) : (size_t) ((Q + 3) / 4) * 32 * bat_tw(G) * 16;
@@ -499,10 +574,32 @@ bool bat_fits(int G, int M, int R) {
   return bat_bytes(G, R, Q) + 16 <= (size_t) kLdsBytes;
 }
 
-int ensure_bat_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **out, hipStream_t st) {
-  if (G == 64) return ensure_fast_table(c, pid, dbid, 64, out, st);
+// Lane-group size of a profile of length M in a batched search over d: the
+// tile's G, or the next wider one the batched kernel takes; 0 when none does
+// (column-segmented models, tables at the LDS limit).
+int batch_lanes(const msvg_ctx *c, int M, const Db &d) {
+  int G = 64, w = 0;
+  choose_tile(c, M, d, G, w);
+  const int Q = qfor(M, 2 * G);
+  if (Q > kQMax || fast_lds(d.R, Q) > (size_t) kLdsBytes) return 0;
+  while (G < 64 && !bat_fits(G, M, d.R)) G *= 2;
+  return bat_fits(G, M, d.R) ? G : 0;
+}
+
+int ensure_bat_table(msvg_ctx *c, int pid, int dbid, int G, const uint32_t **out, hipStream_t st,
+                     Arena *ar = nullptr) {
+  if (G == 64) return ensure_fast_table(c, pid, dbid, 64, out, st, ar);
   Profile &p = *c->profiles[pid];
   Db &d = *c->dbs[dbid];
+  if (ar) {
+    const int Q = qfor(p.M, 2 * G);
+    void *m = ar->take(bat_bytes(G, d.R, Q));
+    if (!m) return fail(c, MSVG_EINVAL, "allocation-free search: scratch exhausted (batched table)");
+    HIPCHK(c, msvk_build_bat_table(p.d_rbv.as<uint8_t>(), p.M, d.d_rowcode.as<uint8_t>(), d.R, G, Q, p.bias,
+                                   static_cast<uint32_t *>(m), st));
+    *out = static_cast<const uint32_t *>(m);
+    return MSVG_OK;
+  }
   Tables &t = tables_for(c, pid, dbid);
   auto it = t.bat.find(G);
   if (it == t.bat.end()) {
@@ -535,9 +632,18 @@ int ensure_seg_tables(msvg_ctx *c, int pid, int dbid, hipStream_t st) {
   return MSVG_OK;
 }
 
-int ensure_slow_table(msvg_ctx *c, int pid, int dbid, const uint32_t **out, hipStream_t st) {
+int ensure_slow_table(msvg_ctx *c, int pid, int dbid, const uint32_t **out, hipStream_t st, Arena *ar = nullptr) {
   Profile &p = *c->profiles[pid];
   Db &d = *c->dbs[dbid];
+  if (ar) {
+    const int Qs = qfor(p.M, 128);
+    void *m = ar->take((size_t) (d.R + 1) * Qs * 64 * 4);
+    if (!m) return fail(c, MSVG_EINVAL, "allocation-free search: scratch exhausted (byte-stage table)");
+    HIPCHK(c, msvk_build_slow_table(p.d_rbv.as<uint8_t>(), p.M, d.d_rowcode.as<uint8_t>(), d.R, Qs,
+                                    static_cast<uint32_t *>(m), st));
+    *out = static_cast<const uint32_t *>(m);
+    return MSVG_OK;
+  }
   Tables &t = tables_for(c, pid, dbid);
   if (!t.slow) {
     const int Qs = qfor(p.M, 128);
@@ -618,16 +724,17 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     segmented[i] = (Q > kQMax || fast_lds(d.R, Q) > (size_t) kLdsBytes) ? 1 : 0;
     if (segmented[i]) G = 64;
     if (!segmented[i] && batch) {
-      // the tile's G, or the next wider one the batched kernel takes
-      int g = G;
-      while (g < 64 && !bat_fits(g, p.M, d.R)) g *= 2;
-      if (bat_fits(g, p.M, d.R)) Gb[i] = G = g;
+      const int g = batch_lanes(c, p.M, d);
+      if (g) Gb[i] = G = g;
     }
     Gs[i] = G; waves[i] = w;
     pd.ftab = nullptr;
+    if (b.arena && !Gb[i])
+      return fail(c, MSVG_EINVAL, "allocation-free search: a profile's table does not fit the batched kernel "
+                                  "(column-segmented models are searched by msvg_search)");
     if (Gb[i]) {
       const uint32_t *bt;
-      if ((rc = ensure_bat_table(c, ids[i], dbid, G, &bt, st)) != MSVG_OK) return rc;
+      if ((rc = ensure_bat_table(c, ids[i], dbid, G, &bt, st, b.arena)) != MSVG_OK) return rc;
       pd.ftab = reinterpret_cast<const uint4 *>(bt);
     } else if (!segmented[i]) {
       const uint32_t *ft;
@@ -637,12 +744,13 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       return rc;
     }
     const uint32_t *stab;
-    if ((rc = ensure_slow_table(c, ids[i], dbid, &stab, st)) != MSVG_OK) return rc;
+    if ((rc = ensure_slow_table(c, ids[i], dbid, &stab, st, b.arena)) != MSVG_OK) return rc;
     const int Q64 = qfor(p.M, 128);
     jtab[i] = nullptr;
     if (Q64 <= kQMax && fast_lds(d.R, Q64) <= (size_t) kLdsBytes) {
       const uint32_t *jt;
-      if ((rc = ensure_fast_table(c, ids[i], dbid, 64, &jt, st)) != MSVG_OK) return rc;
+      if (b.arena && Gb[i] == 64) jt = reinterpret_cast<const uint32_t *>(pd.ftab);   // the same table
+      else if ((rc = ensure_fast_table(c, ids[i], dbid, 64, &jt, st, b.arena)) != MSVG_OK) return rc;
       jtab[i] = reinterpret_cast<const uint4 *>(jt);
     }
     pd.stab = stab;
@@ -659,15 +767,22 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   for (int i = 0; i < np; ++i) pds[i].res = b.d_res.as<uint16_t>() + (size_t) i * n;
   HIPCHK(c, b.d_prof.ensure(np * sizeof(ProfDev)));
   if (!capturing) HIPCHK(c, hipMemcpyAsync(b.d_prof.p, pds.data(), np * sizeof(ProfDev), hipMemcpyHostToDevice, st));
-  if (c->slow_cap < std::max<int64_t>(4096, (int64_t) np * n / 32)) c->slow_cap = std::max<int64_t>(4096, (int64_t) np * n / 32);
-  if (!capturing) b.slow_cap = c->slow_cap;
+  if (b.arena) {
+    b.slow_cap = noalloc_slow_cap(np, n);              // fixed: the scratch is sized for it
+  } else {
+    if (c->slow_cap < std::max<int64_t>(4096, (int64_t) np * n / 32))
+      c->slow_cap = std::max<int64_t>(4096, (int64_t) np * n / 32);
+    if (!capturing) b.slow_cap = c->slow_cap;
+  }
   HIPCHK(c, b.d_slow.ensure((size_t) b.slow_cap * sizeof(SlowItem)));
   HIPCHK(c, b.d_counters.ensure((size_t) (2 * np + 2) * sizeof(int32_t)));
   if (!capturing) b.jcap = std::max<int64_t>(1024, n / 16);
   HIPCHK(c, b.d_jslow.ensure((size_t) np * b.jcap * sizeof(SlowItem)));
-  HIPCHK(c, b.d_usc.ensure((size_t) np * n * sizeof(float)));
-  HIPCHK(c, b.d_status.ensure((size_t) np * n));
-  HIPCHK(c, b.d_pass.ensure((size_t) np * n));
+  if (!b.ext_out) {
+    HIPCHK(c, b.d_usc.ensure((size_t) np * n * sizeof(float)));
+    HIPCHK(c, b.d_status.ensure((size_t) np * n));
+    HIPCHK(c, b.d_pass.ensure((size_t) np * n));
+  }
 
   // Host-built finalize tables.
   std::vector<uint16_t> thr((size_t) np * d.Ls.size());
@@ -701,7 +816,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     // takes: per-profile chunk queues, workgroups spread over the profiles
     // in proportion to M at the start, then moving to profiles with chunks
     // left (msv_batch.hip).
-    struct BatLaunch { int G, grid, block; size_t lds, prof_at, first_at; int np; };
+    struct BatLaunch { int G, grid, block; size_t lds, prof_at, first_at; int np; const int32_t *cs; int nchunks; };
     std::vector<BatLaunch> bl;
     std::vector<BatProf> hprof;
     std::vector<int32_t> hfirst;
@@ -710,7 +825,9 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       for (int i = 0; i < np; ++i)
         if (Gb[i] == G) mem.push_back(i);
       if (mem.empty()) continue;
-      if ((rc = build_chunks(c, d, G)) != MSVG_OK) return rc;
+      const int32_t *cs;
+      int nch;
+      if ((rc = chunk_list(c, d, G, b.arena, &cs, &nch)) != MSVG_OK) return rc;
       size_t lds = 0;
       for (int i : mem) lds = std::max(lds, bat_bytes(G, d.R, pds[i].Q));
       lds += 16;                                        // the workgroup's broadcast word
@@ -723,7 +840,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
         if (o * w > occ * (block / 64)) { block = 64 * w; occ = o; }
       }
       if (occ < 1) return fail(c, MSVG_EDEVICE, "batched kernel cannot be resident");
-      BatLaunch L{G, c->ncu * occ, block, lds, hprof.size(), hfirst.size(), (int) mem.size()};
+      BatLaunch L{G, c->ncu * occ, block, lds, hprof.size(), hfirst.size(), (int) mem.size(), cs, nch};
       double wsum = 0;
       for (int i : mem) wsum += c->profiles[ids[i]]->M;
       double acc = 0;
@@ -767,9 +884,8 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       a.profs = b.d_bat.as<BatProf>() + L.prof_at;
       a.np = L.np;
       a.db = dd;
-      auto &ch = d.chunks[L.G];
-      a.chunk_start = ch.first->as<int32_t>();
-      a.nchunks = ch.second;
+      a.chunk_start = L.cs;
+      a.nchunks = L.nchunks;
       a.first = reinterpret_cast<const int32_t *>(b.d_bat.as<uint8_t>() + prof_bytes) + L.first_at;
       a.slow = b.d_slow.as<SlowItem>();
       a.slow_count = slow_count;
@@ -947,6 +1063,37 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
   HIPCHK(c, msvk_launch_finalize(&fa, np, n, st));
   if (timed) HIPCHK(c, hipEventRecord(c->ev[3], st));
   return MSVG_OK;
+}
+
+// Scratch bytes an allocation-free search of these profiles over database
+// dbid takes (the arena's 256-byte alignment included); 0 if a profile is
+// not taken by the batched kernel.
+size_t noalloc_bytes(msvg_ctx *c, const int32_t *ids, int np, int dbid) {
+  const Db &d = *c->dbs[dbid];
+  const int64_t n = d.n;
+  size_t tot = 0;
+  auto add = [&](size_t b) { tot += ((b + 255) & ~(size_t) 255) + 256; };
+  std::vector<int> Gs;
+  for (int i = 0; i < np; ++i) {
+    const int M = c->profiles[ids[i]]->M;
+    const int G = batch_lanes(c, M, d);
+    if (!G) return 0;
+    add(bat_bytes(G, d.R, qfor(M, 2 * G)));
+    const int Q64 = qfor(M, 128);
+    if (G != 64 && Q64 <= kQMax && fast_lds(d.R, Q64) <= (size_t) kLdsBytes) add(fast_lds(d.R, Q64));
+    add((size_t) (d.R + 1) * Q64 * 64 * 4);                          // byte-stage table
+    if (std::find(Gs.begin(), Gs.end(), G) == Gs.end()) Gs.push_back(G);
+  }
+  for (int G : Gs) add(guided_chunks(c, d, G).size() * sizeof(int32_t));
+  add((size_t) np * n * sizeof(uint16_t));                            // result words
+  add((size_t) np * sizeof(ProfDev));
+  add((size_t) noalloc_slow_cap(np, n) * sizeof(SlowItem));
+  add((size_t) (2 * np + 2) * sizeof(int32_t));
+  add((size_t) np * std::max<int64_t>(1024, n / 16) * sizeof(SlowItem));
+  add((size_t) np * d.Ls.size() * sizeof(uint16_t) + 16);
+  add((size_t) np * 511 * sizeof(float));
+  add(((np * sizeof(BatProf) + 255) & ~(size_t) 255) + Gs.size() * (size_t) c->ncu * 16 * sizeof(int32_t));
+  return tot;
 }
 
 int after_search(msvg_ctx *c, Bufs &b, int np, int dbid, int64_t cells, bool timed) {
@@ -1138,6 +1285,8 @@ int msvg_init(int device, msvg_ctx **ret) {
     if (hipEventCreate(&e) != hipSuccess) return MSVG_EDEVICE;
   if (msvk_prepare() != hipSuccess) return MSVG_EDEVICE;
   if (msvk_prepare_batch() != hipSuccess) return MSVG_EDEVICE;
+  if (hipGetSymbolAddress((void **) &c->scratch_dev, HIP_SYMBOL(scratch)) != hipSuccess || !c->scratch_dev)
+    return MSVG_EDEVICE;
   *ret = c.release();
   return MSVG_OK;
 }
@@ -1383,6 +1532,45 @@ int msvg_search_device(msvg_ctx *c, const int32_t *ids, int32_t np, int32_t dbid
   if ((mode & 0xf) != MSVG_MODE_PLAIN && (mode & 0xf) != MSVG_MODE_BATCHED) return fail(c, MSVG_EINVAL, "unknown mode");
   (void) hipSetDevice(c->device);
   return search_device(c, ids, np, dbid, F1, mode);
+}
+
+int msvg_search_noalloc(msvg_ctx *c, const int32_t *ids, int32_t np, int32_t dbid, double F1, float *d_usc,
+                        uint8_t *d_status, uint8_t *d_pass) {
+  int rc = check_ids(c, ids, np, dbid);
+  if (rc != MSVG_OK) return rc;
+  (void) hipSetDevice(c->device);
+  const size_t need = noalloc_bytes(c, ids, np, dbid);
+  if (need == 0)
+    return fail(c, MSVG_EINVAL, "allocation-free search: a profile's table does not fit the batched kernel "
+                                "(column-segmented models are searched by msvg_search)");
+  if (need > kScratchBytes)
+    return fail(c, MSVG_EINVAL, "allocation-free search: " + std::to_string(np) + " profiles x " +
+                                    std::to_string(c->dbs[dbid]->n) + " targets need " + std::to_string(need) +
+                                    " bytes of scratch; the limit is " + std::to_string(kScratchBytes) +
+                                    " (search fewer profiles or targets per call)");
+  Arena ar;
+  ar.base = c->scratch_dev;
+  ar.cap = kScratchBytes;
+  Bufs b(&ar);
+  b.ext_out = true;                            // results go straight to the caller's arrays
+  for (DevMem *m : {&b.d_usc, &b.d_status, &b.d_pass}) { m->arena = nullptr; m->borrowed = true; }
+  b.d_usc.p = d_usc;
+  b.d_status.p = d_status;
+  b.d_pass.p = d_pass;
+  int64_t cells = 0;
+  rc = enqueue_search(c, b, ids, np, dbid, F1, MSVG_MODE_BATCHED, false, false, &cells, c->stream, 0, c->side);
+  if (rc != MSVG_OK) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int32_t n_slow = 0;
+  HIPCHK(c, hipMemcpy(&n_slow, b.d_counters.as<int32_t>() + np, sizeof(int32_t), hipMemcpyDeviceToHost));
+  c->stats = msvg_stats{};
+  c->stats.cells = cells;
+  c->stats.n_slow = n_slow;
+  c->stats.n_fast_launches = c->last_launches;
+  if (n_slow > b.slow_cap)
+    return fail(c, MSVG_ENORESULT, "allocation-free search: the byte-MSV queue overflowed (" + std::to_string(n_slow) +
+                                       " pairs, room for " + std::to_string(b.slow_cap) + "); use msvg_search");
+  return MSVG_OK;
 }
 
 int msvg_results(msvg_ctx *c, float *usc, uint8_t *status, uint8_t *pass) {

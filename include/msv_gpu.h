@@ -119,6 +119,25 @@ int msvg_search(msvg_ctx *ctx, const int32_t *profile_ids, int32_t np, int32_t d
 int msvg_search_device(msvg_ctx *ctx, const int32_t *profile_ids, int32_t np, int32_t db_id,
                        double F1, int mode);
 int msvg_results(msvg_ctx *ctx, float *usc, uint8_t *status, uint8_t *pass);
+
+/* Allocation-free batched search.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, hosts that forbid hipMalloc).  The same search as
+ * msvg_search(..., MSVG_MODE_BATCHED, ...) and bit for bit the same
+ * results, written to the caller's device arrays d_usc, d_status, d_pass
+ * ([np][n] in the caller's target order; any may be NULL).  Every table,
+ * queue, counter and result word of the call lives in the library's static
+ * device scratch array of 128 MiB per GPU (held until the process ends):
+ * the call makes no hipMalloc, hipFree or other allocating call.  Limits:
+ * EINVAL if the call needs more than the 128 MiB (about 2.6 bytes per
+ * (profile, target) pair plus each profile's tables; the message gives the
+ * need) or a profile takes the column-segmented path (M above the batched
+ * kernel's LDS tables, about 4096); ENORESULT if more pairs than np*n/16
+ * need the exact byte MSV.  All calls on one GPU share the scratch: calls
+ * that overlap on one GPU (from several host threads or contexts) race.
+ * Returns when the results are in place. */
+int msvg_search_noalloc(msvg_ctx *ctx, const int32_t *profile_ids, int32_t np, int32_t db_id, double F1,
+                        float *d_usc, uint8_t *d_status, uint8_t *d_pass);
 int msvg_last_stats(msvg_ctx *ctx, msvg_stats *out);
 
 /* Drop-in for p7_MSVFilter: dsq is 1-based with sentinels at 0 and L+1

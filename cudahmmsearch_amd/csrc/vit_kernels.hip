@@ -120,6 +120,13 @@ __device__ __forceinline__ uint32_t halves_max(uint32_t xEv, uint32_t Dmv) {
 
 }  // namespace
 
+namespace {
+template <int G> __device__ __forceinline__ uint32_t shift_grp(uint32_t last, int lane);
+template <int G, int Q>
+__device__ __forceinline__ void close_d(uint32_t (&Dr)[Q], const uint32_t (&tDD)[Q], bool mine, const uint4 *tsc,
+                                        int c, int lane);
+}  // namespace
+
 template <int Q, int W>
 __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
   extern __shared__ uint32_t etab_lds[];               // one wave per target: the emission table
@@ -189,6 +196,9 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
     for (int j = 0; j < Q; ++j) Mr[j] = Ir[j] = Dr[j] = NEG2;        // :115
     int xB = (int16_t) (xN + move);                                  // :117
     int xJ = -32768, xC = -32768, status = 0;
+    int xacc = -32768;                  // one wave: this lane's largest row maximum (xC at the end)
+    // one wave: the row votes' thresholds (as in vit_group)
+    int tE = min(max(xJ, xN) - a.e_loop, 32766), tD = xB - a.ddbound_w;
     uint32_t edM = NEG2, edI = NEG2, edD = NEG2;
     // xB + tBM per register, the first term of every M cell: xB changes on
     // few rows (xJ is a running maximum), so it is recomputed only then.
@@ -233,6 +243,30 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
           const uint32_t dn = adds(sv, tMD[j]);                      // :167
           Dmv = maxs(Dmv, dn);                                       // :168
           if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;              // :162
+        }
+        if constexpr (W == 1) {
+          // This lane's share of the row's xE and Dmax (:176, :198), voted
+          // on instead of reduced: the exact special-state update runs only
+          // when some lane could overflow or move B.
+          const uint32_t hv = halves_max(xEv, Dmv);
+          const int xe = lo16(hv), dm = hi16(hv);
+          xacc = max(xacc, xe);
+          if (__builtin_amdgcn_ballot_w64(xe > tE)) {
+            const int xE = lo16(wave_max2(hv));
+            if (xE >= 32767) { status = 16; break; }                   // :177
+            xJ = (int16_t) max(xJ, xE + a.e_loop);                     // :180 (J loop 0)
+            const int xBn = (int16_t) max(xJ + move, xN + move);       // :181
+            if (xBn != xB) {
+#pragma unroll
+              for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xBn, xBn), tBM[j]);
+            }
+            xB = xBn;
+            tE = min(max(xJ, xN) - a.e_loop, 32766);
+            tD = xB - a.ddbound_w;
+          }
+          Dr[0] = shift_in(dcv, NEG2, lane);                           // :230-231 / :203-204
+          if (__builtin_amdgcn_ballot_w64(dm > tD)) close_d<64, Q>(Dr, tDD, true, tsc, c, lane);   // :199-227
+          continue;
         }
         uint32_t r = wave_max2(halves_max(xEv, Dmv));
         uint32_t dedge = NEG2;
@@ -299,6 +333,10 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
           if (w > 0) { edM = eM[w - 1]; edI = eI[w - 1]; edD = eD[w - 1]; }
         }
       }
+    }
+    if constexpr (W == 1) {
+      // xC = max over the rows of xE + e_move (:179, C loop 0)
+      xC = max(-32768, lo16(wave_max2(pack16(xacc, xacc))) + a.e_move);
     }
     if (threadIdx.x % C == 0) a.out[it] = (status << 16) | (int) (uint16_t) xC;
   }
@@ -378,6 +416,60 @@ __device__ __forceinline__ uint32_t shift_grp(uint32_t last, int lane) {
     prev = ((lane & (G - 1)) == 0) ? NEG2 : prev;
   }
   return __builtin_amdgcn_alignbit(last, prev, 16);
+}
+
+// Ballot bits of one G-lane group.
+template <int G>
+constexpr uint64_t GMASK = (G == 64) ? ~0ull : ((1ull << G) - 1);
+
+// The delete closure the reference's lazy-F passes compute (vitfilter.c:
+// 203-227), for the lanes whose group runs them (mine).  Dr[0] already
+// holds the D entering each slot's first position.  One pass along each
+// slot (the reference's first pass); then, if the value leaving some slot
+// raises the next slot's first D, the carries across slots follow from
+// the max-plus scan over the lanes (tt: the lane's T terms, see tsc) and a
+// second pass applies them.  With slots of only Q positions the carries
+// cross slot boundaries often, so repeating whole passes (the reference
+// stops at the first position no slot raises) would take many passes.
+template <int G, int Q>
+__device__ __forceinline__ void close_d(uint32_t (&Dr)[Q], const uint32_t (&tDD)[Q], bool mine, const uint4 *tsc,
+                                        int c, int lane) {
+  if (mine) {
+#pragma unroll
+    for (int j = 1; j < Q; ++j) Dr[j] = maxs(Dr[j], adds(Dr[j - 1], tDD[j - 1]));
+  }
+  const uint32_t outl = adds(Dr[Q - 1], tDD[Q - 1]);
+  const uint32_t d0 = maxs(Dr[0], shift_grp<G>(outl, lane));
+  if (__builtin_amdgcn_ballot_w64(mine && d0 != Dr[0]) == 0) return;
+  const uint4 tt = tsc[c];
+  const int clo = lo16(outl);
+  const uint32_t tsl = tt.x, ts01 = tt.y, ts23 = tt.z, ts45 = tt.w;
+  // the lane's two slots as one map x -> max(c, x + T): low slot, then high
+  int cs = max(hi16(outl), clo - (int) (tsl >> 16));          // clo through the high slot
+  // inclusive scan over the lanes of each group (lane l: its group's lanes
+  // up to l composed, in order): row_shr 1/2/4/8 inside rows of 16 lanes,
+  // then row_bcast 15 (and 31) across rows; lanes without a source take the
+  // identity map (-32768, 0), which leaves them unchanged
+  cs = c_step_row<0x111, 0>(cs, ts01);
+  cs = c_step_row<0x112, 1>(cs, ts01);
+  cs = c_step_row<0x114, 0>(cs, ts23);
+  cs = c_step_row<0x118, 1>(cs, ts23);
+  if constexpr (G >= 32) cs = c_step<0x142, 0xa, 0>(cs, ts45);
+  if constexpr (G == 64) cs = c_step<0x143, 0xc, 1>(cs, ts45);
+  // exclusive: the previous lane's prefix; a group's first lane the identity
+  int ce;
+  if constexpr (G == 16) {
+    ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x111, 0xf, 0xf, false);
+  } else {
+    ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x138, 0xf, 0xf, false);
+    if (G == 32 && (lane & 31) == 0) ce = -32768;
+  }
+  const int cin_hi = max(clo, ce - (int) (tsl & 0xffffu));
+  if (mine) {
+    uint32_t y = pack16(ce, cin_hi);
+#pragma unroll
+    for (int j = 0; j < Q; ++j) { Dr[j] = maxs(Dr[j], y); y = adds(y, tDD[j]); }
+  }
 }
 
 // Packed int16 max over each G-lane group; the group's last lane holds it.
@@ -464,31 +556,40 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
     const uint8_t *p = a.db.res + (valid ? a.db.off[s] : 0);
     const int mvl = valid ? (int) a.move[it] : 0;
 
-    int Lg[NG], mv[NG], xB[NG], xJ[NG], xC[NG], st[NG];
+    int mv[NG], xB[NG], xJ[NG], st[NG];
     int Lmax = 0;
 #pragma unroll
     for (int gg = 0; gg < NG; ++gg) {
-      Lg[gg] = __builtin_amdgcn_readlane(L, gg * G);
+      Lmax = max(Lmax, __builtin_amdgcn_readlane(L, gg * G));
       mv[gg] = __builtin_amdgcn_readlane(mvl, gg * G);
       xB[gg] = (int16_t) (xN + mv[gg]);                              // :117
-      xJ[gg] = -32768; xC[gg] = -32768; st[gg] = 0;
-      Lmax = max(Lmax, Lg[gg]);
+      xJ[gg] = -32768; st[gg] = 0;
     }
     uint32_t Mr[Q], Ir[Q], Dr[Q];
 #pragma unroll
     for (int j = 0; j < Q; ++j) Mr[j] = Ir[j] = Dr[j] = NEG2;        // :115
-    // xB + tBM per register (each lane with its group's xB), recomputed only
-    // on the rows where some group's xB changes
+    // Per group: xB + tBM per register (recomputed only when xB changes),
+    // and the two thresholds of the row votes (special_thresholds).
     uint32_t bm[Q];
-    {
+    int tE, tD;
+    auto refresh = [&]() {
+      int te[NG], td[NG];
+#pragma unroll
+      for (int gg = 0; gg < NG; ++gg) {
+        te[gg] = st[gg] ? 0x7fffffff : min(max(xJ[gg], xN) - a.e_loop, 32766);
+        td[gg] = xB[gg] - a.ddbound_w;
+      }
+      tE = of_group<NG>(te, ge);
+      tD = of_group<NG>(td, ge);
       const int xb = of_group<NG>(xB, ge);
 #pragma unroll
       for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xb, xb), tBM[j]);
-    }
+    };
+    refresh();
+    int xacc = -32768;                  // this lane's largest row maximum (xC at the end)
 
-    bool live = true;
     uint4 cnext = *reinterpret_cast<const uint4 *>(p);
-    for (int i0 = 0; i0 < Lmax && live; i0 += 16) {
+    for (int i0 = 0; i0 < Lmax; i0 += 16) {
       const uint4 ch = (i0 < Lpad) ? cnext : null4;
       const int i1 = i0 + 16;
       cnext = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
@@ -529,78 +630,45 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
           Dmv = maxs(Dmv, dn);                                       // :168
           if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;              // :162
         }
-        const uint32_t r = group_max2<G>(halves_max(xEv, Dmv));
-        int cond[NG];
-        bool any = false, anylive = false, xbchg = false;
+        // This lane's share of the row's xE and Dmax (:176, :198), voted on
+        // instead of reduced: the exact special-state update runs only when
+        // some lane of a group could overflow or move the group's B.
+        const uint32_t hv = halves_max(xEv, Dmv);
+        const int xe = lo16(hv), dm = hi16(hv);
+        const bool alive = i < L;                                    // the lane's target has this row
+        xacc = max(xacc, alive ? xe : -32768);
+        const uint64_t mE = __builtin_amdgcn_ballot_w64(alive && xe > tE);
+        if (mE) {
+          const uint32_t r = group_max2<G>(hv);
 #pragma unroll
-        for (int gg = 0; gg < NG; ++gg) {
-          const uint32_t rg = __builtin_amdgcn_readlane(r, gg * G + G - 1);
-          const int xE = lo16(rg), Dmax = hi16(rg);                  // :176, :198
-          cond[gg] = 0;
-          if (i < Lg[gg] && st[gg] == 0) {
-            if (xE >= 32767) {                                       // :177
-              st[gg] = 16;
-            } else {
-              xC[gg] = (int16_t) max(xC[gg], xE + a.e_move);         // :179 (C loop 0)
-              xJ[gg] = (int16_t) max(xJ[gg], xE + a.e_loop);         // :180 (J loop 0)
-              const int xBn = (int16_t) max(xJ[gg] + mv[gg], xN + mv[gg]);  // :181
-              xbchg = xbchg || (xBn != xB[gg]);
-              xB[gg] = xBn;
-              cond[gg] = (Dmax + a.ddbound_w > xB[gg]) ? 1 : 0;      // :199
-              anylive = anylive || (i + 1 < Lg[gg]);
-            }
+          for (int gg = 0; gg < NG; ++gg) {
+            if (((mE >> (gg * G)) & GMASK<G>) == 0) continue;
+            const int xE = lo16(__builtin_amdgcn_readlane(r, gg * G + G - 1));
+            if (xE >= 32767) { st[gg] = 16; continue; }                  // :177
+            xJ[gg] = (int16_t) max(xJ[gg], xE + a.e_loop);               // :180 (J loop 0)
+            xB[gg] = (int16_t) max(xJ[gg] + mv[gg], xN + mv[gg]);        // :181
           }
-          any = any || cond[gg];
+          refresh();
         }
-        live = anylive;
-        if (!live) break;
-        if (xbchg) {
-          const int xb = of_group<NG>(xB, ge);
-#pragma unroll
-          for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xb, xb), tBM[j]);
-        }
-
         Dr[0] = shift_grp<G>(dcv, lane);                             // :230-231 / :203-204
-        if (any) {
-          const uint4 tt = tsc[c];
+        const uint64_t mD = __builtin_amdgcn_ballot_w64(alive && dm > tD);   // :199, per group
+        if (mD) {
+          int cond[NG];
+#pragma unroll
+          for (int gg = 0; gg < NG; ++gg) cond[gg] = (((mD >> (gg * G)) & GMASK<G>) != 0 && st[gg] == 0) ? 1 : 0;
           const bool mine = of_group<NG>(cond, ge) != 0;
-          // closure inside each slot, for the groups whose reference row runs it
-          if (mine) {
-#pragma unroll
-            for (int j = 1; j < Q; ++j) Dr[j] = maxs(Dr[j], adds(Dr[j - 1], tDD[j - 1]));
-          }
-          const uint32_t outl = adds(Dr[Q - 1], tDD[Q - 1]);
-          const int clo = lo16(outl);
-          const uint32_t tsl = tt.x, ts01 = tt.y, ts23 = tt.z, ts4 = tt.w;
-          int cs = max(hi16(outl), clo - (int) (tsl >> 16));        // clo through the high slot
-          // inclusive scan inside each group (rows of 16 lanes; for 32-lane
-          // groups the second row also composes the first row's total)
-          cs = c_step_row<0x111, 0>(cs, ts01);
-          cs = c_step_row<0x112, 1>(cs, ts01);
-          cs = c_step_row<0x114, 0>(cs, ts23);
-          cs = c_step_row<0x118, 1>(cs, ts23);
-          if constexpr (G == 32) cs = c_step<0x142, 0xa, 0>(cs, ts4);
-          // exclusive: the previous lane's prefix; a group's first lane the identity
-          int ce;
-          if constexpr (G == 16) {
-            ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x111, 0xf, 0xf, false);
-          } else {
-            ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x138, 0xf, 0xf, false);
-            if (c == 0) ce = -32768;
-          }
-          const int cin_lo = ce;                                     // nothing enters a group's first slot
-          const int cin_hi = max(clo, cin_lo - (int) (tsl & 0xffffu));
-          if (mine) {
-            uint32_t y = pack16(cin_lo, cin_hi);
-#pragma unroll
-            for (int j = 0; j < Q; ++j) { Dr[j] = maxs(Dr[j], y); y = adds(y, tDD[j]); }
-          }
+          close_d<G, Q>(Dr, tDD, mine, tsc, c, lane);                 // :203-227
         }
       }
     }
+    // xC = max over the target's rows of xE + e_move (:179, C loop 0): the
+    // group's largest row maximum plus e_move, floored at xC's start value
+    const uint32_t xm = group_max2<G>(pack16(xacc, xacc));
 #pragma unroll
-    for (int gg = 0; gg < NG; ++gg)
-      if (lane == gg * G && valid) a.out[it] = (st[gg] << 16) | (int) (uint16_t) xC[gg];
+    for (int gg = 0; gg < NG; ++gg) {
+      const int xC = max(-32768, lo16(__builtin_amdgcn_readlane(xm, gg * G + G - 1)) + a.e_move);
+      if (lane == gg * G && valid) a.out[it] = (st[gg] << 16) | (int) (uint16_t) xC;
+    }
   }
 }
 

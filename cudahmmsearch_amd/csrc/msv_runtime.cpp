@@ -56,6 +56,7 @@ extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t 
 extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, hipStream_t st);
 extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int ncu, hipStream_t st);
+extern "C" int msvk_vit_slots(int G, int Q, int W, const VitArgs *a, int ncu, int *per);
 extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st);
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
                                             int Q, int W, int bias, uint32_t *out, hipStream_t st);
@@ -1723,17 +1724,8 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   // Longest targets first, so the last waves to start are the shortest.
   job.order.resize((size_t) n);
   for (int64_t i = 0; i < n; ++i) job.order[(size_t) i] = i;
-  std::stable_sort(job.order.begin(), job.order.end(), [&](int64_t a, int64_t b) {
-    return d.len_sorted[d.sorted_of[targets[a]]] > d.len_sorted[d.sorted_of[targets[b]]];
-  });
-  job.items.resize((size_t) n);
-  job.mv.resize((size_t) n);
-  for (int64_t q = 0; q < n; ++q) {
-    const int64_t s = d.sorted_of[targets[job.order[(size_t) q]]];
-    job.items[(size_t) q] = s;
-    job.mv[(size_t) q] = (int16_t) msvh_vit_move(v.scale_w, v.nj, d.len_sorted[s]);
-    job.cells += (int64_t) p.M * d.len_sorted[s];
-  }
+  auto len_of = [&](int64_t q) { return d.len_sorted[d.sorted_of[targets[q]]]; };
+  std::stable_sort(job.order.begin(), job.order.end(), [&](int64_t a, int64_t b) { return len_of(a) > len_of(b); });
   VitArgs &a = job.a;
   a.etab = etab;
   a.ttab = v.d_ttab.as<uint32_t>();
@@ -1742,6 +1734,44 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   a.n = (int32_t) n;
   a.e_loop = v.e_loop; a.e_move = v.e_move; a.base_w = v.base_w; a.ddbound_w = v.ddbound_w;
   a.etab_in_lds = (v.W == 1 && (size_t) (d.R + 1) * v.Q * 64 * 4 <= 65536) ? 1 : 0;
+  // The kernels stride over the items statically: item position q runs on
+  // slot (q / per) % S.  Dealt in sorted order, slot 0 would take the
+  // longest item of every round; instead each round's units (per targets,
+  // longest first) go to the slots with the fewest rows so far.  The last
+  // round stays in order, so a partial last unit keeps the last position.
+  int per = 1;
+  const int S = msvk_vit_slots(v.G, v.Q, v.W, &a, c->ncu, &per);
+  const int64_t nunits = (n + per - 1) / per;
+  if (S > 1 && nunits > S) {
+    const int64_t lpt_rounds = (nunits - 1) / S;
+    std::vector<int64_t> unit_at((size_t) nunits), load((size_t) S, 0);
+    std::vector<int> slots((size_t) S);
+    for (int64_t r = 0; r < lpt_rounds; ++r) {
+      for (int k = 0; k < S; ++k) slots[(size_t) k] = k;
+      std::stable_sort(slots.begin(), slots.end(), [&](int x, int y) { return load[(size_t) x] < load[(size_t) y]; });
+      for (int i = 0; i < S; ++i) {
+        const int64_t u = r * S + i;
+        const int k = slots[(size_t) i];
+        unit_at[(size_t) (r * S + k)] = u;
+        load[(size_t) k] += len_of(job.order[(size_t) (u * per)]);
+      }
+    }
+    for (int64_t u = lpt_rounds * S; u < nunits; ++u) unit_at[(size_t) u] = u;
+    std::vector<int64_t> dealt;
+    dealt.reserve((size_t) n);
+    for (int64_t pos = 0; pos < nunits; ++pos)
+      for (int64_t t = unit_at[(size_t) pos] * per; t < std::min<int64_t>(n, (unit_at[(size_t) pos] + 1) * per); ++t)
+        dealt.push_back(job.order[(size_t) t]);
+    job.order.swap(dealt);
+  }
+  job.items.resize((size_t) n);
+  job.mv.resize((size_t) n);
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t s = d.sorted_of[targets[job.order[(size_t) q]]];
+    job.items[(size_t) q] = s;
+    job.mv[(size_t) q] = (int16_t) msvh_vit_move(v.scale_w, v.nj, d.len_sorted[s]);
+    job.cells += (int64_t) p.M * d.len_sorted[s];
+  }
   return MSVG_OK;
 }
 

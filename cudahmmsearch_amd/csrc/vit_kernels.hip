@@ -355,16 +355,21 @@ static int resident_per_cu(K kernel, int block, size_t lds) {
 // fewer; multi-wave targets keep one workgroup per target up to 8 waves per
 // CU.
 template <int Q, int W>
+static int grid_vit(const VitArgs *a, int ncu) {
+  const int block = (W == 1) ? 256 : 64 * W;
+  const size_t lds = (W == 1 && a->etab_in_lds) ? (size_t) (a->db.R + 1) * Q * 64 * 4 : 0;
+  if (W == 1) {
+    const int cap = ncu * resident_per_cu(vit_kernel<Q, W>, block, lds);
+    return std::max(1, std::min((a->n + 3) / 4, cap));
+  }
+  return std::max(1, std::min(a->n, ncu * std::max(1, 8 / W)));
+}
+
+template <int Q, int W>
 static hipError_t launch_vit(const VitArgs *a, int ncu, hipStream_t st) {
   const int block = (W == 1) ? 256 : 64 * W;
   const size_t lds = (W == 1 && a->etab_in_lds) ? (size_t) (a->db.R + 1) * Q * 64 * 4 : 0;
-  int grid;
-  if (W == 1) {
-    const int cap = ncu * resident_per_cu(vit_kernel<Q, W>, block, lds);
-    grid = std::max(1, std::min((a->n + 3) / 4, cap));
-  } else {
-    grid = std::max(1, std::min(a->n, ncu * std::max(1, 8 / W)));
-  }
+  const int grid = grid_vit<Q, W>(a, ncu);
   hipLaunchKernelGGL((vit_kernel<Q, W>), dim3(grid), dim3(block), lds, st, *a);
   return hipGetLastError();
 }
@@ -496,7 +501,7 @@ __device__ __forceinline__ int of_group(const int (&v)[NG], const int (&ge)[NG])
 }  // namespace
 
 template <int G, int Q>
-__global__ void __launch_bounds__(256) vit_group(VitArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q <= 7 ? 4 : 1, 8))) vit_group(VitArgs a) {
   extern __shared__ uint32_t etab_lds[];
   constexpr int NG = 64 / G;
   const int lane = threadIdx.x & 63;
@@ -510,10 +515,12 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
     asm volatile("" : "+v"(ge[gg]));
   }
 
-  uint32_t tBM[Q], tMM[Q], tIM[Q], tDM[Q], tMD[Q], tMI[Q], tII[Q], tDD[Q];
+  // tBM stays in the table: it is read only on the rows where a group's xB
+  // changes (a few per target), and leaving it out of the registers keeps
+  // four waves per SIMD at Q = 7
+  uint32_t tMM[Q], tIM[Q], tDM[Q], tMD[Q], tMI[Q], tII[Q], tDD[Q];
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
-    tBM[j] = a.ttab[(0 * Q + j) * G + c];
     tMM[j] = a.ttab[(1 * Q + j) * G + c];
     tIM[j] = a.ttab[(2 * Q + j) * G + c];
     tDM[j] = a.ttab[(3 * Q + j) * G + c];
@@ -583,7 +590,7 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
       tD = of_group<NG>(td, ge);
       const int xb = of_group<NG>(xB, ge);
 #pragma unroll
-      for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xb, xb), tBM[j]);
+      for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xb, xb), a.ttab[j * G + c]);
     };
     refresh();
     int xacc = -32768;                  // this lane's largest row maximum (xC at the end)
@@ -673,12 +680,18 @@ __global__ void __launch_bounds__(256) vit_group(VitArgs a) {
 }
 
 template <int G, int Q>
-static hipError_t launch_group(const VitArgs *a, int ncu, hipStream_t st) {
+static int grid_group(const VitArgs *a, int ncu) {
   constexpr int NG = 64 / G;
   const size_t lds = (size_t) (a->db.R + 1) * Q * G * 4;
   const int waves = (a->n + NG - 1) / NG;
   const int cap = ncu * resident_per_cu(vit_group<G, Q>, 256, lds);
-  const int grid = std::max(1, std::min((waves + 3) / 4, cap));
+  return std::max(1, std::min((waves + 3) / 4, cap));
+}
+
+template <int G, int Q>
+static hipError_t launch_group(const VitArgs *a, int ncu, hipStream_t st) {
+  const size_t lds = (size_t) (a->db.R + 1) * Q * G * 4;
+  const int grid = grid_group<G, Q>(a, ncu);
   hipLaunchKernelGGL((vit_group<G, Q>), dim3(grid), dim3(256), lds, st, *a);
   return hipGetLastError();
 }
@@ -698,4 +711,37 @@ extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int 
     case 32: return dispatch_group<32>(Q, a, ncu, st, qs);
     default: return hipErrorInvalidValue;
   }
+}
+
+// The static schedule of a launch: item position q (in a.items order) goes
+// to slot (q / per) % slots, where per targets share a wave (64/G for the
+// grouped shapes, else 1) and slots is the number of waves (workgroups for
+// multi-wave targets) striding over the items.  The host orders the items
+// so that every slot gets about the same number of rows.
+template <int... Qs>
+static int slots_vit(int Q, int W, const VitArgs *a, int ncu, std::integer_sequence<int, Qs...>) {
+  int r = 0;
+  ((Q == Qs + 1 ? (r = (W == 1) ? grid_vit<Qs + 1, 1>(a, ncu) * 4
+                    : (W == 2) ? grid_vit<Qs + 1, 2>(a, ncu)
+                    : (W == 4) ? grid_vit<Qs + 1, 4>(a, ncu) : grid_vit<Qs + 1, 8>(a, ncu), 0) : 0), ...);
+  return r;
+}
+template <int G, int... Qs>
+static int slots_group(int Q, const VitArgs *a, int ncu, std::integer_sequence<int, Qs...>) {
+  int r = 0;
+  ((Q == Qs + 1 ? (r = grid_group<G, Qs + 1>(a, ncu) * 4, 0) : 0), ...);
+  return r;
+}
+
+extern "C" int msvk_vit_slots(int G, int Q, int W, const VitArgs *a, int ncu, int *per) {
+  if (a->n < 1) return 0;
+  if (G < 64) {
+    *per = 64 / G;
+    if (Q < 1 || Q > 8) return 0;
+    return G == 16 ? slots_group<16>(Q, a, ncu, std::make_integer_sequence<int, 8>{})
+                   : slots_group<32>(Q, a, ncu, std::make_integer_sequence<int, 8>{});
+  }
+  *per = 1;
+  if (Q < 1 || Q > 16) return 0;
+  return slots_vit(Q, W, a, ncu, std::make_integer_sequence<int, 16>{});
 }

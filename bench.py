@@ -183,7 +183,7 @@ def vit_stage(ca, ctx, profs, pids, did, db, reps=3, cpu=True):
         ach = VIT_INSTR_PER_CELL * tot_cells / (tot_ms / 1e3)
         out["roofline"] = {"bound": "valu", "achieved": round(ach / 1e12, 3),
                            "peak": round(PEAK_LANE_INSTR / 1e12, 3), "unit": "T lane-instr/s",
-                           "frac": round(ach / PEAK_LANE_INSTR, 4), "kernel": "vit_kernel",
+                           "frac": round(ach / PEAK_LANE_INSTR, 4), "kernel": "vit_group / vit_kernel (one launch per shape)",
                            "algorithmic": f"14 int16 ops per DP cell = {VIT_INSTR_PER_CELL} packed lane-instructions"}
     if cpu and first is not None:
         out["cpu_baseline"] = vit_cpu_baseline(ca, db, *first)

@@ -34,9 +34,20 @@
  * with the ViterbiFilter of the block's MSV survivors scored on the GPU as
  * well.  p7_PipelineShort calls p7_ViterbiFilter itself; inside the block
  * step that call takes the precomputed score of its sequence.
+ *
+ * p7_mi355x_ThreadLoop is hmmsearch's thread_loop with pipeline_thread
+ * (src/hmmsearch.c:1317-1425; cudaHmmsearch.cu:670-800 drives its GPU
+ * worker the same way): the calling thread reads the database in blocks
+ * while worker threads each take a block and run the block step on it, so
+ * one block's GPU filters, another's host tail (p7_PipelineShort on its
+ * survivors) and the reading of the next overlap.  A worker thread borrows
+ * a device context from a pool kept for the process (the contexts live on
+ * between queries), so a worker's set-up costs a profile upload, not a
+ * device initialisation.
  */
 #include "p7_config.h"
 
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -62,9 +73,45 @@ struct mi355x_state {
   float          *vsc;
   uint8_t        *vst;
   int64_t         nv, vnext;
+  struct mi355x_state *next;   /* in the pool of idle worker states          */
 };
 
-static __thread struct mi355x_state g_st = { NULL, -1, NULL, 0, 0, NULL, 0, 0, NULL, NULL, NULL, NULL, 0, 0 };
+/* The calling thread's state: its own (created on first use, kept for the
+ * thread's life) or, inside a p7_mi355x_ThreadLoop worker, one borrowed
+ * from the pool. */
+static __thread struct mi355x_state *g_cur = NULL;
+static pthread_mutex_t      g_pool_mu = PTHREAD_MUTEX_INITIALIZER;
+static struct mi355x_state *g_pool = NULL;
+
+static struct mi355x_state *new_state(void)
+{
+  struct mi355x_state *s = calloc(1, sizeof *s);
+  if (s) s->pid = -1;
+  return s;
+}
+
+static struct mi355x_state *cur_state(void)
+{
+  if (g_cur == NULL) g_cur = new_state();
+  return g_cur;
+}
+
+static struct mi355x_state *pool_take(void)
+{
+  struct mi355x_state *s;
+  pthread_mutex_lock(&g_pool_mu);
+  if ((s = g_pool) != NULL) g_pool = s->next;
+  pthread_mutex_unlock(&g_pool_mu);
+  return s ? s : new_state();
+}
+
+static void pool_put(struct mi355x_state *s)
+{
+  pthread_mutex_lock(&g_pool_mu);
+  s->next = g_pool;
+  g_pool  = s;
+  pthread_mutex_unlock(&g_pool_mu);
+}
 
 static uint8_t *put(uint8_t *p, const void *src, size_t n) { memcpy(p, src, n); return p + n; }
 
@@ -183,11 +230,12 @@ static int prepare(struct mi355x_state *s, const P7_OPROFILE *om)
 
 int p7_MSVFilter(const ESL_DSQ *dsq, int L, const P7_OPROFILE *om, P7_OMX *ox, float *ret_sc)
 {
-  struct mi355x_state *s = &g_st;
+  struct mi355x_state *s = cur_state();
   int status;
 
   if (p7O_NQB(om->M) > ox->allocQ16) ESL_EXCEPTION(eslEINVAL, "DP matrix allocated too small");  /* msvfilter.c:103 */
   ox->M = om->M;                                                                             /* :104 */
+  if (s == NULL) ESL_EXCEPTION(eslEMEM, "allocation failed");
   if ((status = prepare(s, om)) != eslOK) return status;
   status = msvg_MSVFilter(s->ctx, s->pid, dsq, L, ret_sc);
   if (status != MSVG_OK && status != MSVG_ERANGE)
@@ -197,13 +245,14 @@ int p7_MSVFilter(const ESL_DSQ *dsq, int L, const P7_OPROFILE *om, P7_OMX *ox, f
 
 int p7_ViterbiFilter(const ESL_DSQ *dsq, int L, const P7_OPROFILE *om, P7_OMX *ox, float *ret_sc)
 {
-  struct mi355x_state *s = &g_st;
+  struct mi355x_state *s = cur_state();
   int status;
 
   if (p7O_NQW(om->M) > ox->allocQ8) ESL_EXCEPTION(eslEINVAL, "DP matrix allocated too small");  /* vitfilter.c:104 */
   if (om->mode != p7_LOCAL && om->mode != p7_UNILOCAL)                                          /* :105 */
     ESL_EXCEPTION(eslEINVAL, "Fast filter only works for local alignment");
   ox->M = om->M;                                                                                /* :106 */
+  if (s == NULL) ESL_EXCEPTION(eslEMEM, "allocation failed");
   if (s->in_block) {               /* a survivor of the current block: its precomputed score */
     int64_t k = s->vnext;
     while (k < s->nv && s->vdsq[k] != dsq) k++;
@@ -223,7 +272,7 @@ int p7_ViterbiFilter(const ESL_DSQ *dsq, int L, const P7_OPROFILE *om, P7_OMX *o
 
 int p7_mi355x_PipelineBlock(P7_PIPELINE *pli, P7_OPROFILE *om, P7_BG *bg, ESL_SQ **sq, int n, P7_TOPHITS *th)
 {
-  struct mi355x_state *s = &g_st;
+  struct mi355x_state *s = cur_state();
   uint8_t  *res = NULL, *st = NULL, *ps = NULL, *vst = NULL;
   int64_t  *off = NULL, *tgt = NULL, *vlen = NULL;
   int32_t  *len = NULL, did = -1;
@@ -233,6 +282,7 @@ int p7_mi355x_PipelineBlock(P7_PIPELINE *pli, P7_OPROFILE *om, P7_BG *bg, ESL_SQ
   int       i, gpu = MSVG_OK, status = eslOK;
 
   if (n <= 0) return eslOK;
+  if (s == NULL) ESL_EXCEPTION(eslEMEM, "allocation failed");
   if ((status = prepare(s, om)) != eslOK) return status;
   for (i = 0; i < n; i++) total += sq[i]->n;
   res = malloc(total > 0 ? (size_t) total : 1);
@@ -249,7 +299,9 @@ int p7_mi355x_PipelineBlock(P7_PIPELINE *pli, P7_OPROFILE *om, P7_BG *bg, ESL_SQ
     total += sq[i]->n;
   }
   if ((gpu = msvg_db_add(s->ctx, res, off, len, n, &did)) != MSVG_OK) goto DONE;
-  if ((gpu = msvg_search(s->ctx, &s->pid, 1, did, pli->F1, MSVG_MODE_BATCHED, usc, st, ps)) != MSVG_OK) goto DONE;
+  /* eager: the block's database is searched once, so no graph is captured */
+  if ((gpu = msvg_search(s->ctx, &s->pid, 1, did, pli->F1, MSVG_MODE_BATCHED | MSVG_FLAG_EAGER, usc, st, ps)) != MSVG_OK)
+    goto DONE;
   for (i = 0; i < n; i++)          /* the MSV survivors: p7_PipelineShort's first test, p7_pipeline.c:917-919 */
     if (ps[i] && sq[i]->n > 0) tgt[nv++] = i;
   if (nv > 0 && (gpu = msvg_vit_search(s->ctx, s->pid, did, tgt, nv, vsc, vst)) != MSVG_OK) goto DONE;
@@ -275,4 +327,136 @@ int p7_mi355x_PipelineBlock(P7_PIPELINE *pli, P7_OPROFILE *om, P7_BG *bg, ESL_SQ
   free(vdsq); free(vlen);
   if (gpu != MSVG_OK) ESL_EXCEPTION(gpu, "p7_mi355x_PipelineBlock: %s", msvg_last_error(s->ctx));
   return status;
+}
+
+int p7_mi355x_PipelineSqBlock(P7_PIPELINE *pli, P7_OPROFILE *om, P7_BG *bg, ESL_SQ_BLOCK *block, P7_TOPHITS *th)
+{
+  ESL_SQ **sq;
+  int      i, status;
+
+  if (block->count <= 0) return eslOK;
+  if ((sq = malloc(sizeof(ESL_SQ *) * block->count)) == NULL) ESL_EXCEPTION(eslEMEM, "allocation failed");
+  for (i = 0; i < block->count; i++) sq[i] = block->list + i;
+  status = p7_mi355x_PipelineBlock(pli, om, bg, sq, block->count, th);
+  for (i = 0; i < block->count; i++) esl_sq_Reuse(block->list + i);   /* hmmsearch.c:1410 */
+  free(sq);
+  return status;
+}
+
+/* The work queue between the reader and the workers: blocks cycle from
+ * idle (the reader fills one) to full (a worker takes one) and back. */
+struct tl_queue {
+  pthread_mutex_t mu;
+  pthread_cond_t  filled;      /* a full block, or the end of the input    */
+  pthread_cond_t  emptied;     /* an idle block, or a failure              */
+  ESL_SQ_BLOCK  **full, **idle;
+  int             nfull, nidle, head, cap;
+  int             done;        /* the reader has queued its last block     */
+  int             status;      /* eslOK, or the first worker failure       */
+};
+
+struct tl_worker {
+  struct tl_queue  *q;
+  P7_MI355X_WORKER *w;
+  pthread_t         tid;
+};
+
+static void *tl_work(void *arg)
+{
+  struct tl_worker    *tw = arg;
+  struct tl_queue     *q = tw->q;
+  struct mi355x_state *s = pool_take();
+  ESL_SQ_BLOCK        *b;
+  int                  st, i;
+
+  g_cur = s;
+  for (;;) {
+    pthread_mutex_lock(&q->mu);
+    while (q->nfull == 0 && !q->done) pthread_cond_wait(&q->filled, &q->mu);
+    if (q->nfull == 0) { pthread_mutex_unlock(&q->mu); break; }
+    b = q->full[q->head];
+    q->head = (q->head + 1) % q->cap;
+    q->nfull--;
+    st = q->status;
+    pthread_mutex_unlock(&q->mu);
+
+    if (st == eslOK && s == NULL) st = eslEMEM;
+    if (st == eslOK) st = p7_mi355x_PipelineSqBlock(tw->w->pli, tw->w->om, tw->w->bg, b, tw->w->th);
+    else for (i = 0; i < b->count; i++) esl_sq_Reuse(b->list + i);      /* after a failure: drained */
+
+    pthread_mutex_lock(&q->mu);
+    q->idle[q->nidle++] = b;
+    if (st != eslOK && q->status == eslOK) q->status = st;
+    pthread_cond_signal(&q->emptied);
+    pthread_mutex_unlock(&q->mu);
+  }
+  g_cur = NULL;
+  if (s) pool_put(s);
+  return NULL;
+}
+
+int p7_mi355x_ThreadLoop(P7_MI355X_WORKER *info, int nworkers, ESL_SQFILE *dbfp, int block_seqs, int n_targetseqs)
+{
+  struct tl_queue   q;
+  struct tl_worker *tw = NULL;
+  int               nblk, started = 0, i, sstatus = eslOK;
+
+  if (nworkers < 1 || nworkers > 256 || block_seqs < 1) ESL_EXCEPTION(eslEINVAL, "p7_mi355x_ThreadLoop: bad arguments");
+  memset(&q, 0, sizeof q);
+  q.status = eslOK;
+  nblk = q.cap = 2 * nworkers + 1;        /* every worker busy, one more full each, and the one being read */
+  pthread_mutex_init(&q.mu, NULL);
+  pthread_cond_init(&q.filled, NULL);
+  pthread_cond_init(&q.emptied, NULL);
+  q.full = calloc(nblk, sizeof(ESL_SQ_BLOCK *));
+  q.idle = calloc(nblk, sizeof(ESL_SQ_BLOCK *));
+  tw     = calloc(nworkers, sizeof *tw);
+  if (!q.full || !q.idle || !tw) { sstatus = eslEMEM; goto DONE; }
+  for (i = 0; i < nblk; i++) {
+    if ((q.idle[i] = esl_sq_CreateDigitalBlock(block_seqs, info[0].om->abc)) == NULL) { sstatus = eslEMEM; goto DONE; }
+    q.nidle++;
+  }
+  for (i = 0; i < nworkers; i++, started++) {
+    tw[i].q = &q;
+    tw[i].w = &info[i];
+    if (pthread_create(&tw[i].tid, NULL, tl_work, &tw[i]) != 0) { sstatus = eslESYS; break; }
+  }
+
+  /* the reader: thread_loop, hmmsearch.c:1317-1369 */
+  while (sstatus == eslOK) {
+    ESL_SQ_BLOCK *b;
+    pthread_mutex_lock(&q.mu);
+    while (q.nidle == 0 && q.status == eslOK) pthread_cond_wait(&q.emptied, &q.mu);
+    if (q.status != eslOK) { pthread_mutex_unlock(&q.mu); break; }
+    b = q.idle[--q.nidle];
+    pthread_mutex_unlock(&q.mu);
+
+    if (n_targetseqs == 0) { b->count = 0; sstatus = eslEOF; }
+    else {
+      sstatus = esl_sqio_ReadBlock(dbfp, b, -1, n_targetseqs, FALSE);
+      if (sstatus == eslOK && n_targetseqs > 0) n_targetseqs -= b->count;
+    }
+
+    pthread_mutex_lock(&q.mu);
+    if (sstatus == eslOK) {
+      q.full[(q.head + q.nfull) % q.cap] = b;
+      q.nfull++;
+      pthread_cond_signal(&q.filled);
+    } else q.idle[q.nidle++] = b;
+    pthread_mutex_unlock(&q.mu);
+  }
+
+ DONE:
+  pthread_mutex_lock(&q.mu);
+  q.done = 1;
+  pthread_cond_broadcast(&q.filled);
+  pthread_mutex_unlock(&q.mu);
+  for (i = 0; i < started; i++) pthread_join(tw[i].tid, NULL);
+  if (q.idle) for (i = 0; i < q.nidle; i++) esl_sq_DestroyBlock(q.idle[i]);
+  free(q.full); free(q.idle); free(tw);
+  pthread_cond_destroy(&q.filled);
+  pthread_cond_destroy(&q.emptied);
+  pthread_mutex_destroy(&q.mu);
+  if (q.status != eslOK) return q.status;
+  return sstatus;
 }

@@ -18,11 +18,18 @@
  * timing.  Own code; only reference functions are called.
  *
  *   oracle/_ref/pipeline_block    the same library as pipeline_mi355x, the
- *                                 database searched as one block by
+ *                                 database searched by the block step,
  *                                 p7_mi355x_PipelineBlock (both filters
- *                                 batched on the GPU, then p7_PipelineShort)
+ *                                 batched on the GPU, then p7_PipelineShort):
+ *                                 with workers = 0 as one block in the main
+ *                                 thread; with workers > 0 by
+ *                                 p7_mi355x_ThreadLoop in blocks of
+ *                                 block_seqs sequences over that many worker
+ *                                 threads, their hit lists and pipelines
+ *                                 merged as hmmsearch merges its workers'
+ *                                 (src/hmmsearch.c:505-542)
  *
- *   pipeline_driver <hmmfile> <seqfile> <outfile> [F1|- [nobias|bias [nolines]]]
+ *   pipeline_driver <hmmfile> <seqfile> <outfile> [F1|- [nobias|bias [nolines|lines [workers [block_seqs]]]]]
  *
  * After each query's report it lists, for every target, p7_MSVFilter's and
  * p7_ViterbiFilter's status and the float bits of their scores (not with
@@ -70,11 +77,16 @@ int main(int argc, char **argv)
   FILE        *ofp;
   double       F1 = -1.0;
   int          nobias = 0, nolines = 0, nquery = 0, hstatus, sstatus;
+  int          workers = 0, block_seqs = 10000;
 
-  if (argc < 4) die("usage: pipeline_driver <hmmfile> <seqfile> <outfile> [F1|- [nobias|bias [nolines]]]", NULL);
+  if (argc < 4) die("usage: pipeline_driver <hmmfile> <seqfile> <outfile> "
+                    "[F1|- [nobias|bias [nolines|lines [workers [block_seqs]]]]]", NULL);
   if (argc > 4 && strcmp(argv[4], "-") != 0) F1 = atof(argv[4]);
   if (argc > 5) nobias = (strcmp(argv[5], "nobias") == 0);
   if (argc > 6) nolines = (strcmp(argv[6], "nolines") == 0);
+  if (argc > 7) workers = atoi(argv[7]);
+  if (argc > 8) block_seqs = atoi(argv[8]);
+  if (workers < 0 || block_seqs < 1) die("bad workers / block_seqs", NULL);
   if (p7_hmmfile_OpenE(argv[1], NULL, &hfp, errbuf) != eslOK) die("cannot open HMM file", argv[1]);
   if ((ofp = fopen(argv[3], "w")) == NULL) die("cannot write", argv[3]);
 
@@ -107,29 +119,74 @@ int main(int argc, char **argv)
     sq = esl_sq_CreateDigital(abc);
 
 #ifdef MI355X_BLOCK
-    /* the whole database as one block: read, searched, then listed */
     ESL_SQ **blk = NULL;
     int      nblk = 0, capblk = 0, k = 0;
-    const double t0 = now();                 /* reading included, as in the serial loop */
-    for (;;) {
-      ESL_SQ *x = esl_sq_CreateDigital(abc);
-      if ((sstatus = esl_sqio_Read(sqfp, x)) != eslOK) { esl_sq_Destroy(x); break; }
-      if (nblk == capblk) {
-        capblk = 2 * capblk + 1024;
-        if ((blk = realloc(blk, sizeof(ESL_SQ *) * capblk)) == NULL) die("out of memory", NULL);
+    if (workers > 0) {
+      /* hmmsearch's threaded search: per-worker objects (src/hmmsearch.c:
+       * 505-510), the read/search loop, then the merge (:538-542) */
+      P7_MI355X_WORKER *info = calloc(workers, sizeof *info);
+      const double      t0 = now();
+      int               w;
+      if (info == NULL) die("out of memory", NULL);
+      for (w = 0; w < workers; w++) {
+        info[w].om  = p7_oprofile_Clone(om);
+        info[w].bg  = p7_bg_Clone(bg);
+        info[w].th  = p7_tophits_Create();
+        info[w].pli = p7_pipeline_Create(NULL, om->M, 100, FALSE, p7_SEARCH_SEQS);
+        if (F1 > 0.0) info[w].pli->F1 = F1;
+        if (nobias) info[w].pli->do_biasfilter = FALSE;
+        p7_pli_NewModel(info[w].pli, info[w].om, info[w].bg);
       }
-      blk[nblk++] = x;
+      sstatus = p7_mi355x_ThreadLoop(info, workers, sqfp, block_seqs, -1);
+      if (sstatus == eslEFORMAT) die("sequence file parse failed:", esl_sqfile_GetErrorBuf(sqfp));
+      if (sstatus != eslEOF) die("threaded block search failed:", info[0].pli->errbuf);
+      for (w = 1; w < workers; w++) {
+        p7_tophits_Merge(info[0].th, info[w].th);
+        p7_pipeline_Merge(info[0].pli, info[w].pli);
+      }
+      fprintf(stderr, "search_seconds %s %.6f (%d workers, blocks of %d)\n", hmm->name, now() - t0, workers,
+              block_seqs);
+      p7_pipeline_Destroy(pli);
+      p7_tophits_Destroy(th);
+      pli = info[0].pli;
+      th  = info[0].th;
+      if (nolines) nseq = pli->nseqs;        /* no listing pass to count them */
+      for (w = 0; w < workers; w++) {
+        if (w > 0) { p7_pipeline_Destroy(info[w].pli); p7_tophits_Destroy(info[w].th); }
+        p7_oprofile_Destroy(info[w].om);
+        p7_bg_Destroy(info[w].bg);
+      }
+      free(info);
+      /* the listing: a second pass over the file */
+      if (!nolines && esl_sqfile_Position(sqfp, 0) != eslOK) die("cannot rewind", argv[2]);
+    } else {
+      /* the whole database as one block: read, searched, then listed */
+      const double t0 = now();               /* reading included, as in the serial loop */
+      for (;;) {
+        ESL_SQ *x = esl_sq_CreateDigital(abc);
+        if ((sstatus = esl_sqio_Read(sqfp, x)) != eslOK) { esl_sq_Destroy(x); break; }
+        if (nblk == capblk) {
+          capblk = 2 * capblk + 1024;
+          if ((blk = realloc(blk, sizeof(ESL_SQ *) * capblk)) == NULL) die("out of memory", NULL);
+        }
+        blk[nblk++] = x;
+      }
+      if (sstatus != eslEOF) die("sequence file parse failed:", argv[2]);
+      {
+        const double t1 = now();
+        if (p7_mi355x_PipelineBlock(pli, wom, bg, blk, nblk, th) != eslOK) die("block pipeline failed:", pli->errbuf);
+        fprintf(stderr, "search_seconds %s %.6f (block step %.6f)\n", hmm->name, now() - t0, now() - t1);
+      }
     }
-    if (sstatus != eslEOF) die("sequence file parse failed:", argv[2]);
-    {
-      const double t1 = now();
-      if (p7_mi355x_PipelineBlock(pli, wom, bg, blk, nblk, th) != eslOK) die("block pipeline failed:", pli->errbuf);
-      fprintf(stderr, "search_seconds %s %.6f (block step %.6f)\n", hmm->name, now() - t0, now() - t1);
-    }
-    while (k < nblk) {
-      esl_sq_Copy(blk[k], sq);
-      esl_sq_Destroy(blk[k++]);
-      sstatus = eslOK;
+    for (;;) {
+      if (workers > 0) {
+        if (nolines || (sstatus = esl_sqio_Read(sqfp, sq)) != eslOK) break;
+      } else {
+        if (k >= nblk) break;
+        esl_sq_Copy(blk[k], sq);
+        esl_sq_Destroy(blk[k++]);
+        sstatus = eslOK;
+      }
 #else
     const double t0 = now();
     while ((sstatus = esl_sqio_Read(sqfp, sq)) == eslOK) {
@@ -174,6 +231,7 @@ int main(int argc, char **argv)
     }
 #ifdef MI355X_BLOCK
     free(blk);
+    if (workers > 0 && !nolines && sstatus != eslEOF) die("sequence file parse failed:", argv[2]);
 #else
     if (sstatus != eslEOF) die("sequence file parse failed:", argv[2]);
     if (nolines) fprintf(stderr, "search_seconds %s %.6f\n", hmm->name, now() - t0);

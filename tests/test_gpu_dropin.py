@@ -10,7 +10,10 @@ oracle/_ref/pipeline_mi355x with cudahmmsearch_amd/hmmer/filters_mi355x.c
 oracle/_ref/pipeline_block, which hands the whole database to
 p7_mi355x_PipelineBlock (both filters batched on the GPU, then the rest of
 the pipeline per sequence through p7_PipelineShort, as cudaHmmsearch's
-block step does, src/GPUFunction.cu:470-490).  Their output
+block step does, src/GPUFunction.cu:470-490) -- once as one block in one
+thread, once through p7_mi355x_ThreadLoop with 3 worker threads over blocks
+of 16 sequences (hmmsearch's threaded loop, src/hmmsearch.c:1317-1425, its
+workers' hits and pipelines merged).  Their output
 files -- hit tables, domain alignments, tabular output, the pipeline's
 filter pass counts and a line per target with both filters' statuses and
 score bits -- must be byte-identical.  Inputs: the reference's tutorial
@@ -107,11 +110,16 @@ def synthetic_fasta(path, n, seed, donors):
 
 
 def _run_all(tmp, hmm, seqs, extra=()):
+    """Outputs of the SSE build, the per-call GPU build, the one-block build
+    and the threaded block build (3 workers, blocks of 16 sequences)."""
+    full = list(extra) + ["-", "bias"][len(extra):]
+    runs = [(REFBIN, list(extra)), (GPUBIN, list(extra)), (BLOCKBIN, list(extra)),
+            (BLOCKBIN, full + ["lines", "3", "16"])]
     outs = []
-    for b in (REFBIN, GPUBIN, BLOCKBIN):
-        out = os.path.join(tmp, os.path.basename(b) + ".out")
-        r = subprocess.run([b, hmm, seqs, out, *extra], capture_output=True, text=True, timeout=600)
-        assert r.returncode == 0, f"{os.path.basename(b)} failed: {r.stderr[-2000:]}"
+    for k, (b, args) in enumerate(runs):
+        out = os.path.join(tmp, f"{os.path.basename(b)}_{k}.out")
+        r = subprocess.run([b, hmm, seqs, out, *args], capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, f"{os.path.basename(b)} {args} failed: {r.stderr[-2000:]}"
         with open(out) as fh:
             outs.append(fh.read())
     return outs
@@ -131,9 +139,9 @@ def _check_same(ref, gpu):
 def test_tutorial_fn3_vs_7less(tmp_path):
     hmm = _unpack("fn3.hmm", tmp_path)
     seq = _unpack("7LESS_DROME", tmp_path)
-    ref, gpu, blk = _run_all(str(tmp_path), hmm, seq)
-    _check_same(ref, gpu)
-    _check_same(ref, blk)
+    ref, *others = _run_all(str(tmp_path), hmm, seq)
+    for o in others:
+        _check_same(ref, o)
     # the shipped answer tutorial/fn3.out:110: the one target passes the MSV filter
     assert any(l.startswith("Passed MSV filter:") and l.split()[3] == "1" for l in ref.splitlines())
 
@@ -141,17 +149,17 @@ def test_tutorial_fn3_vs_7less(tmp_path):
 @pytest.mark.gpu
 @needs_bins
 def test_tutorial_globins(tmp_path):
-    ref, gpu, blk = _run_all(str(tmp_path), _unpack("globins4.hmm", tmp_path), _unpack("globins45.fa", tmp_path))
-    _check_same(ref, gpu)
-    _check_same(ref, blk)
+    ref, *others = _run_all(str(tmp_path), _unpack("globins4.hmm", tmp_path), _unpack("globins45.fa", tmp_path))
+    for o in others:
+        _check_same(ref, o)
 
 
 @pytest.mark.gpu
 @needs_bins
 def test_tutorial_dna(tmp_path):
-    ref, gpu, blk = _run_all(str(tmp_path), _unpack("MADE1.hmm", tmp_path), _unpack("dna_target.fa", tmp_path))
-    _check_same(ref, gpu)
-    _check_same(ref, blk)
+    ref, *others = _run_all(str(tmp_path), _unpack("MADE1.hmm", tmp_path), _unpack("dna_target.fa", tmp_path))
+    for o in others:
+        _check_same(ref, o)
 
 
 def _models(tmp, names):
@@ -180,9 +188,9 @@ def test_synthetic_targets(tmp_path, models, extra):
     tmp = str(tmp_path)
     seqs = os.path.join(tmp, "targets.fa")
     synthetic_fasta(seqs, 700, 2024, _donors(tmp))
-    ref, gpu, blk = _run_all(tmp, _models(tmp, models), seqs, extra)
-    _check_same(ref, gpu)
-    _check_same(ref, blk)
+    ref, *others = _run_all(tmp, _models(tmp, models), seqs, extra)
+    for o in others:
+        _check_same(ref, o)
     assert ref.count("\nmsv ") == len(models) * 700
     passed = [int(l.split()[3]) for l in ref.splitlines() if l.startswith("Passed MSV filter:")]
     assert len(passed) == len(models) and all(p > 0 for p in passed)
@@ -197,10 +205,69 @@ def test_synthetic_targets(tmp_path, models, extra):
 def test_dropin_fails_loudly_without_gpu(tmp_path):
     """No silent CPU fallback: without a device the drop-in filters raise
     through Easel's exception handler and the search stops."""
-    for b in (GPUBIN, BLOCKBIN):
+    for b, args in ((GPUBIN, []), (BLOCKBIN, []), (BLOCKBIN, ["-", "bias", "lines", "2", "4"])):
         if not os.path.exists(b):
             continue
         r = subprocess.run([b, _unpack("fn3.hmm", tmp_path), _unpack("HBB_HUMAN", tmp_path),
-                            os.path.join(str(tmp_path), "o.txt")], capture_output=True, text=True, timeout=120)
+                            os.path.join(str(tmp_path), "o.txt"), *args], capture_output=True, text=True, timeout=120)
         assert r.returncode != 0
         assert "no MI355X device" in r.stderr
+
+
+def fast_fasta(path, n, seed, donors):
+    """n targets (log-normal lengths, median 350, sigma 0.7, clipped 20..10000)
+    of i.i.d. background residues, every 7th with a fragment of a real family
+    member planted; generated in bulk for large n."""
+    rs = np.random.default_rng(seed)
+    lens = np.clip(np.round(np.exp(np.log(350.0) + 0.7 * rs.standard_normal(n))), 20, 10000).astype(np.int64)
+    cdf = np.cumsum(BGF / BGF.sum())
+    letters = np.frombuffer(AMINO.encode(), np.uint8)
+    total = int(lens.sum())
+    res = np.empty(total, np.uint8)
+    for a in range(0, total, 1 << 24):
+        b = min(total, a + (1 << 24))
+        res[a:b] = letters[np.minimum(np.searchsorted(cdf, rs.random(b - a)), 19)]
+    off = np.concatenate([[0], np.cumsum(lens)])
+    with open(path, "wb") as fh:
+        for i in range(n):
+            s = res[off[i]:off[i + 1]].tobytes()
+            if i % 7 == 3:
+                d = donors[int(rs.integers(len(donors)))].encode()
+                w = int(rs.integers(40, min(300, len(d)) + 1))
+                a = int(rs.integers(0, len(d) - w + 1))
+                at = int(rs.integers(0, len(s) + 1))
+                s = s[:at] + d[a:a + w] + s[at:]
+            fh.write(b">t%d\n" % i)
+            fh.write(b"\n".join(s[k:k + 60] for k in range(0, len(s), 60)) + b"\n")
+
+
+def _seconds(stderr):
+    return [float(l.split()[2]) for l in stderr.splitlines() if l.startswith("search_seconds")]
+
+
+@pytest.mark.gpu
+@needs_bins
+def test_threaded_block_step_500k(tmp_path):
+    """500,000 targets, each model in a fresh process: the threaded block
+    step (4 workers, blocks of 10,000) gives the SSE build's output byte for
+    byte, as the one-block step does; the wall times of the three (FASTA
+    reading included) are printed."""
+    tmp = str(tmp_path)
+    seqs = os.path.join(tmp, "db500k.fa")
+    fast_fasta(seqs, 500_000, 2025, _donors(tmp))
+    for m in ("fn3.hmm", "globins4.hmm", "Pkinase.hmm"):
+        hmm = _unpack(m, tmp)
+        outs, secs = [], []
+        for k, (b, args) in enumerate([(REFBIN, []), (BLOCKBIN, []), (BLOCKBIN, ["4", "10000"])]):
+            out = os.path.join(tmp, f"{m}_{k}.out")
+            r = subprocess.run([b, hmm, seqs, out, "-", "bias", "nolines", *args], capture_output=True, text=True,
+                               timeout=300)
+            assert r.returncode == 0, f"{os.path.basename(b)} {args} failed: {r.stderr[-2000:]}"
+            secs.append(_seconds(r.stderr)[0])
+            with open(out) as fh:
+                outs.append(fh.read())
+        print(f"{m}: SSE {secs[0]:.3f} s, one block {secs[1]:.3f} s, 4 workers x blocks of 10,000 {secs[2]:.3f} s",
+              flush=True)
+        assert "targets 500000" in outs[0]
+        _check_same(outs[0], outs[1])
+        _check_same(outs[0], outs[2])

@@ -11,8 +11,10 @@ batch: value = batched GCUPS, beside the plain single-profile kernel on the
 same data (plain_gcups, batched_over_plain, bit identity), the roofline of
 the dominant batched launch (HIP events), the reference's SSE p7_MSVFilter
 on the host (N threads and one), and a shortened c5 serving phase (serving:
-p50/p99 at --rate req/s for --serving-duration s, capacity, p50/p99 at 70%
-of capacity).  For c2 and c3 the line also carries the ViterbiFilter
+persistent serving, one query per search, and batched serving, the queries
+waiting for a slot folded into one search; for each the capacity, p50/p99
+over a drained sweep at 50/75/90% of it, and p50/p99 at --rate req/s for
+--serving-duration s).  For c2 and c3 the line also carries the ViterbiFilter
 stage on the MSV survivors (vit_stage: kernel time, GCUPS, roofline, the
 reference's SSE p7_ViterbiFilter on a sample with a bit-identity check).
 
@@ -35,9 +37,10 @@ cudahmmsearch_amd.workloads; Easel-compatible generators):
       350, sigma 0.7, clipped 20..10000, seed 8); queries drawn from 64
       sampled profiles with M log-normal (median 250, sigma 0.6, clipped
       30..4000, seeds 1000..1063), calibrated beforehand (as HMM files carry
-      their STATS lines); Poisson arrivals at --rate req/s for --duration s.
-      Also measures the service capacity (closed loop, every slot busy) and
-      p50/p99 at 70% of it, where the queue is stable.
+      their STATS lines); persistent and batched serving, each: capacity
+      (192 queries queued at once), p50/p99 over Poisson arrivals at
+      50/75/90% of it (drained), and at --rate req/s for --duration s.
+      value = batched serving's capacity in queries/s.
   scan  hmmscan direction: --scan-profiles sampled profiles (M log-normal
       median 250, sigma 0.6, clipped 30..2560, seeds 100000..) against
       --scan-queries query sequences (log-normal median 350, sigma 0.7,
@@ -379,12 +382,57 @@ def serving_phase(ca, srv, profs, rate, duration, rs, drain):
     return np.array(lat), len(tickets), unfinished, cells, time.perf_counter() - t0
 
 
-def serving_run(ca, n_targets, rate, duration, slots, shared, stable_s):
+def serving_mode(ca, ctx, did, profs, mode, slots, rate, duration, sweep_s, rs):
+    """One serving mode on the resident database: warm (every query profile
+    once), the service capacity (3 x 64 queries queued at once), a drained
+    open-loop sweep at 50/75/90% of that capacity, then Poisson arrivals at
+    the offered rate for duration seconds (not drained: the server stops
+    with the unfinished queries)."""
+    kw = {"slots": slots, "shared": mode == "shared", "batched": mode == "batched"}
+    srv = ca.Server(ctx, did, **kw)
+    for t in [srv.submit(p) for p in profs]:
+        srv.wait(t, want_hits=False)
+    t0 = time.perf_counter()
+    tk = [srv.submit(profs[int(rs.integers(len(profs)))]) for _ in range(3 * len(profs))]
+    cap_cells = sum(srv.wait(t, want_hits=False)[0]["cells"] for t in tk)
+    cap_s = time.perf_counter() - t0
+    cap_qps = len(tk) / cap_s
+    log(f"serving {mode}: capacity {cap_qps:.1f} queries/s, {cap_cells / cap_s / 1e9:.0f} GCUPS")
+    pct = lambda a, q: round(float(np.percentile(a, q)), 2) if a.size else None  # noqa: E731
+    sweep = []
+    for f in (0.5, 0.75, 0.9):
+        b0 = srv.stats()["batches"]
+        a0 = srv.stats()["admitted"]
+        lat, sub, _, cells, wall = serving_phase(ca, srv, profs, f * cap_qps, sweep_s, rs, drain=True)
+        st = srv.stats()
+        sweep.append({"rate_per_s": round(f * cap_qps, 2), "submitted": sub, "p50_ms": pct(lat, 50),
+                      "p99_ms": pct(lat, 99), "gcups": round(cells / wall / 1e9, 1),
+                      "queries_per_search": round((st["admitted"] - a0) / max(1, st["batches"] - b0), 2)})
+        log(f"serving {mode}: {sweep[-1]}")
+    bounded = [x for x in sweep if x["p99_ms"] is not None and x["p99_ms"] <= 1000.0]
+    st0 = srv.stats()
+    lat, sub, unf, cells, wall = serving_phase(ca, srv, profs, rate, duration, rs, drain=False)
+    st = srv.stats()
+    srv.stop()
+    busy = st["busy_s"] - st0["busy_s"]
+    log(f"serving {mode}: offered {rate}/s: submitted {sub}, finished {lat.size}, unfinished {unf}")
+    return {
+        "capacity": {"queries_per_s": round(cap_qps, 2), "gcups": round(cap_cells / cap_s / 1e9, 1)},
+        "sweep": sweep,
+        "highest_bounded_rate_per_s": max(x["rate_per_s"] for x in bounded) if bounded else None,
+        "offered": {"rate_per_s": rate, "duration_s": duration, "submitted": sub, "finished": int(lat.size),
+                    "unfinished": unf, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                    "max_queue": st["max_pending"],
+                    "queries_per_search": round((st["admitted"] - st0["admitted"]) /
+                                                max(1, st["batches"] - st0["batches"]), 2),
+                    "gcups_while_serving": round(cells / max(busy, 1e-9) / 1e9, 1)},
+    }
+
+
+def serving_run(ca, n_targets, rate, duration, slots, modes, sweep_s):
     """Serving on a resident database of n_targets (the c5 generator) with
     the 64 c5 query profiles (calibrated first, as model files carry their
-    STATS lines): open-loop Poisson arrivals at rate for duration seconds,
-    then the closed-loop capacity (every slot busy), then p50/p99 at 70% of
-    that capacity over stable_s seconds, drained."""
+    STATS lines), in each of the given modes (serving_mode)."""
     from cudahmmsearch_amd import workloads
     t0 = time.perf_counter()
     db = workloads.serving_database(n_targets)
@@ -399,60 +447,37 @@ def serving_run(ca, n_targets, rate, duration, slots, shared, stable_s):
         p.mu = ctx.calibrate(pid, p.lam)
         ctx.remove_profile(pid)
     did = ctx.add_database(db)
-    meanM = float(np.mean(Ms))
-    # warm tables and thresholds for every query profile
-    srv = ca.Server(ctx, did, slots=slots, shared=shared)
-    for t in [srv.submit(p) for p in profs]:
-        srv.wait(t, want_hits=False)
-    # 1. offered load
-    lat, sub, unf, cells, wall = serving_phase(ca, srv, profs, rate, duration, rs, drain=False)
-    st = srv.stats()
-    srv.stop()
-    log(f"offered {rate}/s: submitted {sub}, finished {lat.size}, unfinished {unf}")
-    # 2. service capacity: every slot busy
-    srv = ca.Server(ctx, did, slots=slots, shared=shared)
-    t0 = time.perf_counter()
-    n_cap = 12 * slots
-    tk = [srv.submit(profs[int(rs.integers(len(profs)))]) for _ in range(n_cap)]
-    cap_cells = sum(srv.wait(t, want_hits=False)[0]["cells"] for t in tk)
-    cap_s = time.perf_counter() - t0
-    srv.stop()
-    cap_qps = n_cap / cap_s
-    log(f"capacity: {cap_qps:.1f} queries/s, {cap_cells / cap_s / 1e9:.0f} GCUPS")
-    # 3. stable load: 70% of capacity, drained
-    srv = ca.Server(ctx, did, slots=slots, shared=shared)
-    rate2 = 0.7 * cap_qps
-    lat2, sub2, _, cells2, wall2 = serving_phase(ca, srv, profs, rate2, stable_s, rs, drain=True)
-    srv.stop()
-    ctx.close()
-    pct = lambda a, q: round(float(np.percentile(a, q)), 2) if a.size else None  # noqa: E731
-    return {
+    out = {
         "workload": f"c5: {db.n} resident targets ({db.residues} residues), 64 query profiles "
-                    f"M log-normal median 250 (mean {meanM:.0f}), Poisson {rate:g} req/s for {duration:g} s, "
-                    f"{slots} slots, {'shared' if shared else 'persistent'} launches",
-        "cells_per_query_mean": round(meanM * db.residues),
-        "offered": {"rate_per_s": rate, "duration_s": duration, "submitted": sub, "finished": int(lat.size),
-                    "unfinished": unf, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                    "max_queue": st["max_pending"],
-                    "gcups_while_serving": round(cells / max(st["busy_s"], 1e-9) / 1e9, 1),
-                    "note": "latencies of finished queries only; the offered load exceeds capacity, so the "
-                            "queue grows for the whole run and the unfinished queries would wait longer"},
-        "capacity": {"queries_per_s": round(cap_qps, 2), "gcups": round(cap_cells / cap_s / 1e9, 1)},
-        "stable": {"rate_per_s": round(rate2, 2), "duration_s": stable_s, "submitted": sub2,
-                   "p50_ms": pct(lat2, 50), "p99_ms": pct(lat2, 99), "gcups": round(cells2 / wall2 / 1e9, 1)},
+                    f"M log-normal median 250 (mean {float(np.mean(Ms)):.0f}), {slots} slots",
+        "cells_per_query_mean": round(float(np.mean(Ms)) * db.residues),
+        "note": "capacity: 192 queries queued at once; sweep: Poisson arrivals at 50/75/90% of the mode's "
+                f"capacity for {sweep_s:g} s each, drained; highest_bounded_rate: the highest swept rate with "
+                f"p99 <= 1 s; offered: Poisson arrivals at {rate:g}/s for {duration:g} s, latencies of the "
+                "finished queries only (above capacity the queue grows for the whole run)",
     }
+    for m in modes:
+        out[m] = serving_mode(ca, ctx, did, profs, m, slots, rate, duration, sweep_s, rs)
+    if "persistent" in out and "batched" in out:
+        out["batched_over_persistent_capacity"] = round(out["batched"]["capacity"]["queries_per_s"] /
+                                                        out["persistent"]["capacity"]["queries_per_s"], 3)
+    ctx.close()
+    return out
 
 
 def serving(ca, args):
-    """The c5 workload as its own bench line (value = p99 at the offered rate)."""
-    r = serving_run(ca, 10_000_000, args.rate, args.duration, args.slots, args.shared, max(10.0, args.duration / 3))
+    """The c5 workload as its own bench line (value = the batched server's
+    capacity in queries per second)."""
+    r = serving_run(ca, 10_000_000, args.rate, args.duration, args.slots, ["persistent", "batched"],
+                    max(8.0, args.duration / 4))
+    b = r["batched"]
     out = {
-        "metric": f"MSV serving p99 latency at {args.rate:g} req/s on 1 MI355X",
-        "value": r["offered"]["p99_ms"], "unit": "ms", "n_gpus": 1, "steps": r["offered"]["submitted"],
-        "warmup": 64, "ms_per_step": r["offered"]["p50_ms"], "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "metric": "MSV serving capacity (queries/s) on 1 MI355X, batched serving",
+        "value": b["capacity"]["queries_per_s"], "unit": "queries/s", "n_gpus": 1, "steps": b["offered"]["submitted"],
+        "warmup": 64, "ms_per_step": round(1e3 / b["capacity"]["queries_per_s"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": r["workload"], "cells_per_query_mean": r["cells_per_query_mean"]},
-        "offered": r["offered"], "capacity": r["capacity"], "stable": r["stable"],
+        "serving": r,
     }
     print(json.dumps(out))
 
@@ -500,8 +525,7 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="time only the batched mode (profiling runs)")
     ap.add_argument("--rate", type=float, default=500.0, help="serving: offered queries per second")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
-    ap.add_argument("--slots", type=int, default=4, help="serving: queries on the GPU at once")
-    ap.add_argument("--shared", action="store_true", help="serving: one chunk per workgroup (queries share CUs)")
+    ap.add_argument("--slots", type=int, default=2, help="serving: searches on the GPU at once")
     ap.add_argument("--no-vit", action="store_true", help="skip the ViterbiFilter stage on the MSV survivors")
     ap.add_argument("--no-serving", action="store_true", help="c3: skip the serving sub-object")
     ap.add_argument("--serving-duration", type=float, default=10.0, help="c3: seconds of arrivals in the serving phase")
@@ -623,7 +647,8 @@ def main():
     ctx.close()
     del db
     if args.workload == "c3" and not args.no_serving:
-        out["serving"] = serving_run(ca, 10_000_000, args.rate, args.serving_duration, args.slots, args.shared, 10.0)
+        out["serving"] = serving_run(ca, 10_000_000, args.rate, args.serving_duration, args.slots,
+                                     ["persistent", "batched"], 6.0)
         log(f"serving: {out['serving']}")
     print(json.dumps(out))
 

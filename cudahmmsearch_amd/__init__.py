@@ -77,13 +77,15 @@ class _VProf(C.Structure):
 
 class _QRes(C.Structure):
     _fields_ = [("ticket", C.c_int64), ("status", C.c_int32), ("M", C.c_int32), ("n_pass", C.c_int64),
-                ("cells", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+                ("cells", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("batch", C.c_int32)]
 
 
 class _SInfo(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
                 ("pending", C.c_int64), ("running", C.c_int64), ("max_pending", C.c_int64),
-                ("busy_s", C.c_double), ("uptime_s", C.c_double), ("cached_profiles", C.c_int64)]
+                ("busy_s", C.c_double), ("uptime_s", C.c_double), ("cached_profiles", C.c_int64),
+                ("batches", C.c_int64)]
 
 
 class _Rng(C.Structure):
@@ -735,16 +737,20 @@ class Server:
     msvg_server_* in include/msv_gpu.h).  While it runs, make no other calls
     on its context."""
 
-    def __init__(self, ctx, did, slots=4, F1=0.02, hbm_budget=0, shared=False):
-        """shared=False: persistent launches (queries start as the running one
-        drains); shared=True: one chunk per workgroup (running queries share
-        the CUs from the start)."""
+    def __init__(self, ctx, did, slots=4, F1=0.02, hbm_budget=0, shared=False, batched=False):
+        """Default: persistent launches, one query per search (queries start
+        as the running one drains); shared=True: one chunk per workgroup
+        (running queries share the CUs from the start); batched=True: a free
+        slot takes every waiting query (up to 16) into one multi-profile
+        search."""
+        if shared and batched:
+            raise ValueError("shared and batched serving exclude each other")
         self.ctx, self.lib = ctx, ctx.lib
         self.n = self.lib.msvg_db_size(ctx._c, did)
         if self.n < 0:
             raise MSVError(eslEINVAL, f"no database {did}")
         self._s = C.c_void_p()
-        ctx._chk(self.lib.msvg_server_start(ctx._c, did, slots, F1, hbm_budget, 1 if shared else 0,
+        ctx._chk(self.lib.msvg_server_start(ctx._c, did, slots, F1, hbm_budget, 2 if batched else 1 if shared else 0,
                                             C.byref(self._s)))
 
     def submit(self, prof):

@@ -2743,6 +2743,7 @@ struct ServerQuery {
   int status = MSVG_OK;
   int slot = -1;
   int64_t cells = 0;
+  int batch = 0;                  // queries in the search that served this one
   size_t lds = 0, dev = 0;
   std::vector<int32_t> idx;
   std::vector<float> usc;
@@ -2752,9 +2753,9 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   Bufs bufs;
-  DevMem d_cidx, d_cusc, d_ccount;
-  int32_t *h_count = nullptr;     // pinned
-  ServerQuery *q = nullptr;
+  DevMem d_cidx, d_cusc, d_ccount;          // compacted pass lists, one row of n per query of the group
+  int32_t *h_count = nullptr;               // pinned: passing targets per query of the group
+  std::vector<ServerQuery *> group;         // the queries of the search on this slot (empty: free)
 };
 
 }  // namespace
@@ -2765,6 +2766,7 @@ struct msvg_server {
   double F1 = 0.02;
   int64_t hbm_budget = 0;
   int sched = 0;                  // 0 persistent launches, 2 one chunk per workgroup
+  int batch_max = 1;              // queries folded into one search (MSVG_SERVE_BATCHED: kServerBatch)
   std::vector<std::unique_ptr<Slot>> slots;
   std::mutex mu;
   std::condition_variable cv;
@@ -2793,6 +2795,7 @@ struct msvg_server {
 namespace {
 
 constexpr size_t kServerProfiles = 256;   // query profiles a server keeps on the device
+constexpr int kServerBatch = 16;          // queries one batched-serving search takes at most
 
 std::string profile_key(const ServerQuery &q) {
   std::string k(reinterpret_cast<const char *>(q.rbv.data()), q.rbv.size());
@@ -2834,23 +2837,19 @@ void server_evict(msvg_server *s) {
   }
 }
 
-int server_start_query(msvg_server *s, Slot &sl, ServerQuery &q) {
+// The query's profile id on the device: from the cache, or added, its
+// tables built on the slot's stream and waited for before the id is cached
+// (a later query of the same profile may run on another slot's stream,
+// which must not read a table still being built).  Takes a hold on it.
+int server_profile(msvg_server *s, Slot &sl, ServerQuery &q, int32_t *ret_pid) {
   msvg_ctx *c = s->ctx;
-  const Db &d = *c->dbs[s->db];
-  int pid;
   q.key = profile_key(q);
   auto it = s->prof_cache.find(q.key);
-  if (it != s->prof_cache.end()) {
-    pid = it->second.pid;
-  } else {
+  if (it == s->prof_cache.end()) {
     q.desc.rbv = q.rbv.data();
-    int32_t id;
-    int rc = msvg_profile_add(c, &q.desc, &id);
+    int32_t pid;
+    int rc = msvg_profile_add(c, &q.desc, &pid);
     if (rc != MSVG_OK) return rc;
-    pid = id;
-    // Build the profile's tables on this slot's stream and wait for them
-    // before the id is cached: a later query of the same profile may run on
-    // another slot's stream, which must not read a table still being built.
     int64_t cells0 = 0;
     rc = enqueue_search(c, sl.bufs, &pid, 1, s->db, s->F1, MSVG_MODE_BATCHED, false, false, &cells0, sl.stream,
                         s->sched, nullptr, true);
@@ -2864,38 +2863,87 @@ int server_start_query(msvg_server *s, Slot &sl, ServerQuery &q) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->cached = (int64_t) s->prof_cache.size();
   }
+  *ret_pid = it->second.pid;
+  return MSVG_OK;
+}
+
+// One search over the resident database for the slot's group of queries
+// (one profile each; batched mode, so each lane-group size's profiles share
+// one persistent launch), then each query's pass list compacted on the GPU.
+int server_start_group(msvg_server *s, Slot &sl) {
+  msvg_ctx *c = s->ctx;
+  const Db &d = *c->dbs[s->db];
+  const int k = (int) sl.group.size();
+  const int64_t n = std::max<int64_t>(d.n, 1);
+  std::vector<int32_t> pids((size_t) k);
+  for (int j = 0; j < k; ++j) {
+    const int rc = server_profile(s, sl, *sl.group[(size_t) j], &pids[(size_t) j]);
+    if (rc != MSVG_OK) return rc;
+  }
   int64_t cells = 0;
-  int rc = enqueue_search(c, sl.bufs, &pid, 1, s->db, s->F1, MSVG_MODE_BATCHED, false, false, &cells, sl.stream,
-                          s->sched);
+  int rc = enqueue_search(c, sl.bufs, pids.data(), k, s->db, s->F1, MSVG_MODE_BATCHED, false, false, &cells,
+                          sl.stream, s->sched);
   if (rc != MSVG_OK) return rc;
-  q.cells = cells;
-  HIPCHK(c, sl.d_ccount.ensure(sizeof(int32_t)));
-  HIPCHK(c, sl.d_cidx.ensure((size_t) std::max<int64_t>(d.n, 1) * sizeof(int32_t)));
-  HIPCHK(c, sl.d_cusc.ensure((size_t) std::max<int64_t>(d.n, 1) * sizeof(float)));
-  HIPCHK(c, msvk_fill_zero(sl.d_ccount.as<int32_t>(), 1, sl.stream));
-  HIPCHK(c, msvk_compact(sl.bufs.d_pass.as<uint8_t>(), sl.bufs.d_usc.as<float>(), d.n, sl.d_ccount.as<int32_t>(),
-                         sl.d_cidx.as<int32_t>(), sl.d_cusc.as<float>(), (int32_t) std::max<int64_t>(d.n, 1),
-                         sl.stream));
-  HIPCHK(c, hipMemcpyAsync(sl.h_count, sl.d_ccount.p, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
+  HIPCHK(c, sl.d_ccount.ensure((size_t) k * sizeof(int32_t)));
+  HIPCHK(c, sl.d_cidx.ensure((size_t) k * n * sizeof(int32_t)));
+  HIPCHK(c, sl.d_cusc.ensure((size_t) k * n * sizeof(float)));
+  HIPCHK(c, msvk_fill_zero(sl.d_ccount.as<int32_t>(), k, sl.stream));
+  for (int j = 0; j < k; ++j) {
+    ServerQuery &q = *sl.group[(size_t) j];
+    q.cells = (int64_t) q.desc.M * d.total_res;
+    q.batch = k;
+    HIPCHK(c, msvk_compact(sl.bufs.d_pass.as<uint8_t>() + (size_t) j * d.n, sl.bufs.d_usc.as<float>() + (size_t) j * d.n,
+                           d.n, sl.d_ccount.as<int32_t>() + j, sl.d_cidx.as<int32_t>() + (size_t) j * n,
+                           sl.d_cusc.as<float>() + (size_t) j * n, (int32_t) n, sl.stream));
+  }
+  HIPCHK(c, hipMemcpyAsync(sl.h_count, sl.d_ccount.p, (size_t) k * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           sl.stream));
   HIPCHK(c, hipEventRecord(sl.done, sl.stream));
   return MSVG_OK;
 }
 
-int server_finish_query(msvg_server *s, Slot &sl, ServerQuery &q) {
+int server_finish_group(msvg_server *s, Slot &sl) {
   msvg_ctx *c = s->ctx;
-  // Deferred-pair queues must not have overflowed (they are sized n/32 and
-  // n/64 per profile); a search that overflowed reports an error.
-  int32_t cn[4];
-  HIPCHK(c, hipMemcpy(cn, sl.bufs.d_counters.p, 4 * sizeof(int32_t), hipMemcpyDeviceToHost));
-  if (cn[1] > sl.bufs.slow_cap) return fail(c, MSVG_EMEM, "deferred-pair queue overflow");
-  const int32_t k = *sl.h_count;
-  q.idx.resize(k);
-  q.usc.resize(k);
-  if (k > 0) {
-    HIPCHK(c, hipMemcpy(q.idx.data(), sl.d_cidx.p, (size_t) k * sizeof(int32_t), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(q.usc.data(), sl.d_cusc.p, (size_t) k * sizeof(float), hipMemcpyDeviceToHost));
+  const int k = (int) sl.group.size();
+  const int64_t n = std::max<int64_t>(c->dbs[s->db]->n, 1);
+  // The byte-MSV queue (counters[k]) must not have overflowed; a search
+  // that overflowed reports an error.
+  int32_t n_slow = 0;
+  HIPCHK(c, hipMemcpy(&n_slow, sl.bufs.d_counters.as<int32_t>() + k, sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (n_slow > sl.bufs.slow_cap) return fail(c, MSVG_EMEM, "deferred-pair queue overflow");
+  for (int j = 0; j < k; ++j) {
+    ServerQuery &q = *sl.group[(size_t) j];
+    const int32_t m = sl.h_count[j];
+    q.idx.resize((size_t) m);
+    q.usc.resize((size_t) m);
+    if (m > 0) {
+      HIPCHK(c, hipMemcpy(q.idx.data(), sl.d_cidx.as<int32_t>() + (size_t) j * n, (size_t) m * sizeof(int32_t),
+                          hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(q.usc.data(), sl.d_cusc.as<float>() + (size_t) j * n, (size_t) m * sizeof(float),
+                          hipMemcpyDeviceToHost));
+    }
   }
   return MSVG_OK;             // the waiting caller orders the hits (msvg_server_wait)
+}
+
+// The slot's queries are over (rc: how): results released to the waiters.
+void server_retire(msvg_server *s, Slot &sl, int rc) {
+  for (ServerQuery *q : sl.group) server_release(s, *q);
+  server_evict(s);
+  std::lock_guard<std::mutex> lk(s->mu);
+  const double t = s->now();
+  for (ServerQuery *q : sl.group) {
+    q->status = rc;
+    q->state = 2;
+    q->t_done = t;
+    s->lds_in_use -= q->lds;
+    s->dev_in_use -= q->dev;
+    s->st.completed++;
+    s->st.running--;
+  }
+  sl.group.clear();
+  if (s->st.running == 0 && s->busy_since >= 0) { s->st.busy_s += t - s->busy_since; s->busy_since = -1; }
+  s->cv.notify_all();
 }
 
 void server_loop(msvg_server *s) {
@@ -2907,73 +2955,50 @@ void server_loop(msvg_server *s) {
     // Completions.
     for (auto &slp : s->slots) {
       Slot &sl = *slp;
-      if (!sl.q) continue;
-      if (hipEventQuery(sl.done) != hipSuccess) continue;
-      ServerQuery *q = sl.q;
-      const int rc = server_finish_query(s, sl, *q);
-      server_release(s, *q);
-      server_evict(s);
-      std::lock_guard<std::mutex> lk(s->mu);
-      q->status = rc;
-      q->state = 2;
-      q->t_done = s->now();
-      s->lds_in_use -= q->lds;
-      s->dev_in_use -= q->dev;
-      sl.q = nullptr;
-      s->st.completed++;
-      s->st.running--;
-      if (s->st.running == 0 && s->busy_since >= 0) { s->st.busy_s += q->t_done - s->busy_since; s->busy_since = -1; }
-      s->cv.notify_all();
+      if (sl.group.empty() || hipEventQuery(sl.done) != hipSuccess) continue;
+      server_retire(s, sl, server_finish_group(s, sl));
       progressed = true;
     }
     // Admissions: a free slot, the HBM budget and the LDS budget (all running
-    // queries' workgroups must be able to share a CU) decide.
+    // queries' workgroups must be able to share a CU) decide.  A free slot
+    // takes the first waiting query and, in batched serving, those behind it
+    // up to kServerBatch: the queries that arrived while the slots were busy
+    // go to the GPU together, in one search.
     for (;;) {
-      ServerQuery *q = nullptr;
       Slot *free_slot = nullptr;
       {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stopping || s->pending.empty()) break;
         for (auto &slp : s->slots)
-          if (!slp->q) { free_slot = slp.get(); break; }
+          if (slp->group.empty()) { free_slot = slp.get(); break; }
         if (!free_slot) break;
-        ServerQuery *cand = s->pending.front();
-        cand->lds = query_lds(c, d, cand->desc.M);
-        cand->dev = (size_t) d.n * 16 + (size_t) cand->desc.M * 29 * 8;
-        // Shared scheduling: the running queries' workgroups must fit a CU
-        // together.  Persistent launches queue behind each other anyway.
-        const bool lds_ok = s->sched != 2 || s->st.running == 0 || s->lds_in_use + cand->lds <= (size_t) kLdsBytes;
-        const bool hbm_ok = s->hbm_budget <= 0 || s->st.running == 0 ||
-                            (int64_t) (s->dev_in_use + cand->dev) <= s->hbm_budget;
-        if (!lds_ok || !hbm_ok) break;
-        s->pending.pop_front();
-        q = cand;
-        q->state = 1;
-        q->slot = (int) (free_slot - s->slots[0].get());
-        q->t_admit = s->now();
-        s->lds_in_use += q->lds;
-        s->dev_in_use += q->dev;
-        s->st.admitted++;
-        s->st.pending--;
-        s->st.running++;
-        if (s->busy_since < 0) s->busy_since = q->t_admit;
+        while (!s->pending.empty() && (int) free_slot->group.size() < s->batch_max) {
+          ServerQuery *cand = s->pending.front();
+          cand->lds = query_lds(c, d, cand->desc.M);
+          cand->dev = (size_t) d.n * 16 + (size_t) cand->desc.M * 29 * 8;
+          // Shared scheduling: the running queries' workgroups must fit a CU
+          // together.  Persistent launches queue behind each other anyway.
+          const bool first = s->st.running == 0;
+          const bool lds_ok = s->sched != 2 || first || s->lds_in_use + cand->lds <= (size_t) kLdsBytes;
+          const bool hbm_ok = s->hbm_budget <= 0 || first || (int64_t) (s->dev_in_use + cand->dev) <= s->hbm_budget;
+          if (!lds_ok || !hbm_ok) break;
+          s->pending.pop_front();
+          cand->state = 1;
+          cand->slot = (int) (free_slot - s->slots[0].get());
+          cand->t_admit = s->now();
+          s->lds_in_use += cand->lds;
+          s->dev_in_use += cand->dev;
+          s->st.admitted++;
+          s->st.pending--;
+          s->st.running++;
+          if (s->busy_since < 0) s->busy_since = cand->t_admit;
+          free_slot->group.push_back(cand);
+        }
+        if (free_slot->group.empty()) break;
+        s->st.batches++;
       }
-      free_slot->q = q;
-      const int rc = server_start_query(s, *free_slot, *q);
-      if (rc != MSVG_OK) {
-        server_release(s, *q);
-        std::lock_guard<std::mutex> lk(s->mu);
-        q->status = rc;
-        q->state = 2;
-        q->t_done = s->now();
-        s->lds_in_use -= q->lds;
-        s->dev_in_use -= q->dev;
-        free_slot->q = nullptr;
-        s->st.completed++;
-        s->st.running--;
-        if (s->st.running == 0 && s->busy_since >= 0) { s->st.busy_s += q->t_done - s->busy_since; s->busy_since = -1; }
-        s->cv.notify_all();
-      }
+      const int rc = server_start_group(s, *free_slot);
+      if (rc != MSVG_OK) server_retire(s, *free_slot, rc);
       progressed = true;
     }
     {
@@ -2991,7 +3016,8 @@ extern "C" {
 int msvg_server_start(msvg_ctx *c, int32_t db_id, int32_t slots, double F1, int64_t hbm_budget, int32_t flags,
                       msvg_server **ret) {
   if (!c || !ret) return MSVG_EINVAL;
-  if (flags != MSVG_SERVE_PERSISTENT && flags != MSVG_SERVE_SHARED) return fail(c, MSVG_EINVAL, "unknown flags");
+  if (flags != MSVG_SERVE_PERSISTENT && flags != MSVG_SERVE_SHARED && flags != MSVG_SERVE_BATCHED)
+    return fail(c, MSVG_EINVAL, "unknown flags");
   if (!c->dbs.count(db_id)) return fail(c, MSVG_EINVAL, "unknown database id");
   if (slots < 1 || slots > 16) return fail(c, MSVG_EINVAL, "slots must be 1..16");
   (void) hipSetDevice(c->device);
@@ -3001,11 +3027,12 @@ int msvg_server_start(msvg_ctx *c, int32_t db_id, int32_t slots, double F1, int6
   s->F1 = F1;
   s->hbm_budget = hbm_budget;
   s->sched = (flags == MSVG_SERVE_SHARED) ? 2 : 0;
+  s->batch_max = (flags == MSVG_SERVE_BATCHED) ? kServerBatch : 1;
   for (int i = 0; i < slots; ++i) {
     auto sl = std::make_unique<Slot>();
     HIPCHK(c, hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
-    HIPCHK(c, hipHostMalloc((void **) &sl->h_count, sizeof(int32_t), hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc((void **) &sl->h_count, (size_t) s->batch_max * sizeof(int32_t), hipHostMallocDefault));
     s->slots.push_back(std::move(sl));
   }
   s->t0 = std::chrono::steady_clock::now();
@@ -3063,6 +3090,7 @@ int msvg_server_wait(msvg_server *s, int64_t ticket, double timeout_s, msvg_quer
   out->t_submit = own->t_submit;
   out->t_admit = own->t_admit;
   out->t_done = own->t_done;
+  out->batch = own->batch;
   if (idx && cap > 0) {
     const int64_t k = (int64_t) own->idx.size();
     std::vector<int32_t> ord(k);

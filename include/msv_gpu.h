@@ -273,6 +273,7 @@ typedef struct {
   int64_t n_pass;          /* targets passing F1 (ERANGE included)          */
   int64_t cells;           /* M * residues of the database                  */
   double  t_submit, t_admit, t_done;   /* seconds since the server started */
+  int32_t batch;           /* queries in the search that served this one    */
 } msvg_query_result;
 
 typedef struct {
@@ -282,16 +283,24 @@ typedef struct {
   double  uptime_s;
   int64_t cached_profiles; /* query profiles kept on the device for reuse (at most 256; the least
                               recently used one that no running query holds is removed first) */
+  int64_t batches;         /* searches started (admitted / batches: queries per search) */
 } msvg_server_info;
 
 /* slots: queries admitted to the GPU at once (1..16); hbm_budget: device
  * bytes the server may hold for running queries (0: no limit); F1 as in
  * msvg_search.  flags: MSVG_SERVE_PERSISTENT (each query's fast stage is one
  * persistent launch; an admitted query's workgroups start as the running
- * ones retire) or MSVG_SERVE_SHARED (one chunk per workgroup, so running
- * queries' workgroups interleave on the CUs from the start). */
+ * ones retire), MSVG_SERVE_SHARED (one chunk per workgroup, so running
+ * queries' workgroups interleave on the CUs from the start) or
+ * MSVG_SERVE_BATCHED (a free slot takes every waiting query, up to 16, into
+ * one multi-profile search: msvg_search's batched mode, one persistent
+ * launch per lane-group size over all of them; each query's result equals
+ * the same query served alone).  hmmpgmd's worker runs one search command
+ * per query (src/hmmdwrkr.c:603-622); batched serving folds the commands
+ * that arrive together into one pass over the resident database. */
 #define MSVG_SERVE_PERSISTENT 0
 #define MSVG_SERVE_SHARED     1
+#define MSVG_SERVE_BATCHED    2
 int msvg_server_start(msvg_ctx *ctx, int32_t db_id, int32_t slots, double F1, int64_t hbm_budget,
                       int32_t flags, msvg_server **ret);
 /* Queue a query; the profile is copied.  Thread-safe. */

@@ -16,7 +16,9 @@ reference (oracle/_ref: HMMER's own p7_MSVFilter built from its sources).
   the c5 generator): the 64 query profiles submitted at once run
   overlapped (a query is admitted while another is on the GPU), every
   served pass list equals the reference's pass set, and every result
-  equals the same query served alone.
+  equals the same query served alone; with batched serving the 64 go to
+  the GPU in a few multi-profile searches and every result is again the
+  one served alone.
 
 The reference's own unit test sweeps model and target lengths the same way
 (impl_sse/msvfilter.c:1000-1037).
@@ -141,7 +143,19 @@ def test_c5_serving_vs_reference():
     tickets = [srv.submit(p) for p in profs]
     mixed = [srv.wait(t) for t in tickets]
     srv.stop()
+    # all 64 at once, batched serving: folded into shared searches
+    srv = ca.Server(ctx, did, slots=2, batched=True)
+    tickets = [srv.submit(p) for p in profs]
+    batched = [srv.wait(t) for t in tickets]
+    bst = srv.stats()
+    srv.stop()
     ctx.close()
+    print(f"c5 batched: 64 queries in {bst['batches']} searches", flush=True)
+    assert bst["batches"] < 16
+    for j, (info, idx, usc) in enumerate(batched):
+        assert info["status"] == 0
+        assert np.array_equal(idx, alone[j][0]) and np.array_equal(usc.view(np.uint32), alone[j][1].view(np.uint32)), \
+            f"query {j}: batched serving result differs from the query served alone"
     spans = sorted((m[0]["t_admit"], m[0]["t_done"]) for m in mixed)
     overlapped = sum(1 for k in range(1, len(spans)) if spans[k][0] < spans[k - 1][1])
     print(f"c5: {overlapped} of 63 queries admitted while the one admitted before them was still running",

@@ -440,7 +440,6 @@ def serving_run(ca, n_targets, rate, duration, slots, modes, sweep_s):
     Ms = np.array([p.M for p in profs])
     log(f"serving database: {db.n} targets, {db.residues} residues; {len(profs)} query profiles, "
         f"median M {int(np.median(Ms))} ({time.perf_counter() - t0:.1f} s)")
-    rs = np.random.default_rng(2006)
     ctx = ca.MSVContext(0)
     for p in profs:
         pid = ctx.add_profile(p)
@@ -456,11 +455,11 @@ def serving_run(ca, n_targets, rate, duration, slots, modes, sweep_s):
                 f"p99 <= 1 s; offered: Poisson arrivals at {rate:g}/s for {duration:g} s, latencies of the "
                 "finished queries only (above capacity the queue grows for the whole run)",
     }
-    for m in modes:
-        out[m] = serving_mode(ca, ctx, did, profs, m, slots, rate, duration, sweep_s, rs)
+    for m in modes:             # the same query sequence and arrival times in every mode
+        out[m] = serving_mode(ca, ctx, did, profs, m, slots, rate, duration, sweep_s, np.random.default_rng(2006))
     if "persistent" in out and "batched" in out:
-        out["batched_over_persistent_capacity"] = round(out["batched"]["capacity"]["queries_per_s"] /
-                                                        out["persistent"]["capacity"]["queries_per_s"], 3)
+        out["batched_over_persistent_capacity"] = round(out["batched"]["capacity"]["gcups"] /
+                                                        out["persistent"]["capacity"]["gcups"], 3)
     ctx.close()
     return out
 
@@ -485,8 +484,8 @@ def serving(ca, args):
 def roofline(se, cells, npids):
     """Roofline of the dominant kernel from an eager search's stats: the
     batched launch (msv_batch, one per lane-group size) that ran longest,
-    or, when no profile went through the batched kernel (models beyond its
-    LDS tables), the whole J-free stage.  achieved = 0.75 packed
+    or, when no profile went through the batched kernel (column-segmented
+    models, tables beyond LDS), the whole J-free stage over all its cells.  achieved = 0.75 packed
     lane-instructions per cell x the launch's cells / its HIP-event time."""
     nb = int(se.get("n_batch", 0))
     if nb:
@@ -494,8 +493,9 @@ def roofline(se, cells, npids):
         ms, lcells, G = se["batch_ms"][k], se["batch_cells"][k], se["batch_lanes"][k]
         kernel = f"msv_batch<{G}> (one persistent launch over the {G}-lane profiles)"
         tot_ms, tot_cells = sum(se["batch_ms"][:nb]), sum(se["batch_cells"][:nb])
-    else:
-        ms, lcells, kernel = se["ms_fast"] / max(1, se["n_fast_launches"]), cells / npids, "msv_fast / msv_seg"
+    else:                       # the whole J-free stage (its launches run one after another)
+        ms, lcells = se["ms_fast"], cells
+        kernel = f"msv_seg / msv_fast (the J-free stage's {int(se['n_fast_launches'])} launches together)"
         tot_ms, tot_cells = se["ms_fast"], cells
     ach = INSTR_PER_CELL * lcells / (ms / 1e3)
     cps = lcells / (ms / 1e3)

@@ -31,6 +31,9 @@
  *
  *   pipeline_driver <hmmfile> <seqfile> <outfile> [F1|- [nobias|bias [nolines|lines [workers [block_seqs]]]]]
  *
+ * In the SSE build, workers > 0 (with nolines) is hmmsearch --cpu workers:
+ * a reader thread and that many threads running p7_Pipeline, merged.
+ *
  * After each query's report it lists, for every target, p7_MSVFilter's and
  * p7_ViterbiFilter's status and the float bits of their scores (not with
  * nolines).  The seconds spent in the search of each query, sequence reading
@@ -51,6 +54,8 @@
 #include "hmmer.h"
 #ifdef MI355X_BLOCK
 #include "filters_mi355x.h"
+#else
+#include <pthread.h>
 #endif
 
 static double now(void)
@@ -65,6 +70,101 @@ static void die(const char *msg, const char *arg)
   fprintf(stderr, "pipeline_driver: %s %s\n", msg, arg ? arg : "");
   exit(1);
 }
+
+#ifndef MI355X_BLOCK
+/* hmmsearch's threaded search for the SSE build (hmmsearch --cpu N,
+ * src/hmmsearch.c:1317-1425): the main thread reads blocks of sequences,
+ * worker threads run p7_Pipeline on every sequence of a block with their own
+ * pipeline, profile clone, background and hit list (WORKER_INFO, :36-44);
+ * the caller merges them (:538-542).  The reference's own thread_loop
+ * needs esl_threads.c, which this image cannot build. */
+typedef struct { P7_PIPELINE *pli; P7_OPROFILE *om; P7_BG *bg; P7_TOPHITS *th; pthread_t tid; void *q; } sse_worker;
+typedef struct {
+  pthread_mutex_t mu;
+  pthread_cond_t  filled, emptied;
+  ESL_SQ_BLOCK  **full, **idle;
+  int             nfull, nidle, head, cap, done;
+} sse_queue;
+
+static void *sse_work(void *arg)
+{
+  sse_worker   *w = arg;
+  sse_queue    *q = w->q;
+  ESL_SQ_BLOCK *b;
+  int           i;
+  for (;;) {
+    pthread_mutex_lock(&q->mu);
+    while (q->nfull == 0 && !q->done) pthread_cond_wait(&q->filled, &q->mu);
+    if (q->nfull == 0) { pthread_mutex_unlock(&q->mu); break; }
+    b = q->full[q->head];
+    q->head = (q->head + 1) % q->cap;
+    q->nfull--;
+    pthread_mutex_unlock(&q->mu);
+    for (i = 0; i < b->count; i++) {          /* hmmsearch.c:1397-1410 */
+      ESL_SQ *x = b->list + i;
+      p7_pli_NewSeq(w->pli, x);
+      p7_bg_SetLength(w->bg, x->n);
+      p7_oprofile_ReconfigLength(w->om, x->n);
+      p7_Pipeline(w->pli, w->om, w->bg, x, w->th);
+      esl_sq_Reuse(x);
+      p7_pipeline_Reuse(w->pli);
+    }
+    pthread_mutex_lock(&q->mu);
+    q->idle[q->nidle++] = b;
+    pthread_cond_signal(&q->emptied);
+    pthread_mutex_unlock(&q->mu);
+  }
+  return NULL;
+}
+
+/* Returns the reader's last status: eslEOF after the whole file. */
+static int sse_threaded(sse_worker *w, int nw, ESL_SQFILE *sqfp, const ESL_ALPHABET *abc, int block_seqs)
+{
+  sse_queue q;
+  int       i, sstatus = eslOK;
+  memset(&q, 0, sizeof q);
+  q.cap = 2 * nw + 1;
+  pthread_mutex_init(&q.mu, NULL);
+  pthread_cond_init(&q.filled, NULL);
+  pthread_cond_init(&q.emptied, NULL);
+  q.full = calloc(q.cap, sizeof(ESL_SQ_BLOCK *));
+  q.idle = calloc(q.cap, sizeof(ESL_SQ_BLOCK *));
+  if (!q.full || !q.idle) die("out of memory", NULL);
+  for (i = 0; i < q.cap; i++)
+    if ((q.idle[q.nidle++] = esl_sq_CreateDigitalBlock(block_seqs, abc)) == NULL) die("out of memory", NULL);
+  for (i = 0; i < nw; i++) {
+    w[i].q = &q;
+    if (pthread_create(&w[i].tid, NULL, sse_work, &w[i]) != 0) die("cannot start a worker thread", NULL);
+  }
+  while (sstatus == eslOK) {
+    ESL_SQ_BLOCK *b;
+    pthread_mutex_lock(&q.mu);
+    while (q.nidle == 0) pthread_cond_wait(&q.emptied, &q.mu);
+    b = q.idle[--q.nidle];
+    pthread_mutex_unlock(&q.mu);
+    sstatus = esl_sqio_ReadBlock(sqfp, b, -1, -1, FALSE);
+    pthread_mutex_lock(&q.mu);
+    if (sstatus == eslOK) {
+      q.full[(q.head + q.nfull) % q.cap] = b;
+      q.nfull++;
+      pthread_cond_signal(&q.filled);
+    } else q.idle[q.nidle++] = b;
+    pthread_mutex_unlock(&q.mu);
+  }
+  pthread_mutex_lock(&q.mu);
+  q.done = 1;
+  pthread_cond_broadcast(&q.filled);
+  pthread_mutex_unlock(&q.mu);
+  for (i = 0; i < nw; i++) pthread_join(w[i].tid, NULL);
+  for (i = 0; i < q.nidle; i++) esl_sq_DestroyBlock(q.idle[i]);
+  free(q.full);
+  free(q.idle);
+  pthread_cond_destroy(&q.filled);
+  pthread_cond_destroy(&q.emptied);
+  pthread_mutex_destroy(&q.mu);
+  return sstatus;
+}
+#endif
 
 int main(int argc, char **argv)
 {
@@ -189,7 +289,40 @@ int main(int argc, char **argv)
       }
 #else
     const double t0 = now();
-    while ((sstatus = esl_sqio_Read(sqfp, sq)) == eslOK) {
+    if (workers > 0) {
+      /* hmmsearch --cpu N: per-worker objects (src/hmmsearch.c:505-510), the
+       * threaded loop, the merge (:538-542); output without the listing */
+      sse_worker *info = calloc(workers, sizeof *info);
+      int         w;
+      if (info == NULL) die("out of memory", NULL);
+      if (!nolines) die("workers > 0 needs nolines in this build", NULL);
+      for (w = 0; w < workers; w++) {
+        info[w].om  = p7_oprofile_Clone(om);
+        info[w].bg  = p7_bg_Clone(bg);
+        info[w].th  = p7_tophits_Create();
+        info[w].pli = p7_pipeline_Create(NULL, om->M, 100, FALSE, p7_SEARCH_SEQS);
+        if (F1 > 0.0) info[w].pli->F1 = F1;
+        if (nobias) info[w].pli->do_biasfilter = FALSE;
+        p7_pli_NewModel(info[w].pli, info[w].om, info[w].bg);
+      }
+      sstatus = sse_threaded(info, workers, sqfp, abc, block_seqs);
+      for (w = 1; w < workers; w++) {
+        p7_tophits_Merge(info[0].th, info[w].th);
+        p7_pipeline_Merge(info[0].pli, info[w].pli);
+      }
+      p7_pipeline_Destroy(pli);
+      p7_tophits_Destroy(th);
+      pli  = info[0].pli;
+      th   = info[0].th;
+      nseq = pli->nseqs;
+      for (w = 0; w < workers; w++) {
+        if (w > 0) { p7_pipeline_Destroy(info[w].pli); p7_tophits_Destroy(info[w].th); }
+        p7_oprofile_Destroy(info[w].om);
+        p7_bg_Destroy(info[w].bg);
+      }
+      free(info);
+    }
+    while (workers == 0 && (sstatus = esl_sqio_Read(sqfp, sq)) == eslOK) {
 #endif
       float usc = 0.0f, vsc = 0.0f;
       int   st, vst;

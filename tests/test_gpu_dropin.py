@@ -248,17 +248,22 @@ def _seconds(stderr):
 @pytest.mark.gpu
 @needs_bins
 def test_threaded_block_step_500k(tmp_path):
-    """500,000 targets, each model in a fresh process: the threaded block
-    step (4 workers, blocks of 10,000) gives the SSE build's output byte for
-    byte, as the one-block step does; the wall times of the three (FASTA
+    """500,000 targets, each model in a fresh process: the SSE pipeline
+    serial and as hmmsearch --cpu 4 and --cpu 16 (the driver's reader and
+    worker threads), the block step as one block and through
+    p7_mi355x_ThreadLoop (4 workers x blocks of 10,000, 8 x 5,000): every
+    output byte-identical to the serial SSE one; the wall times (FASTA
     reading included) are printed."""
     tmp = str(tmp_path)
     seqs = os.path.join(tmp, "db500k.fa")
     fast_fasta(seqs, 500_000, 2025, _donors(tmp))
+    runs = [("SSE serial", REFBIN, []), ("SSE 4 threads", REFBIN, ["4", "10000"]),
+            ("SSE 16 threads", REFBIN, ["16", "10000"]), ("block step, one block", BLOCKBIN, []),
+            ("block step 4 x 10,000", BLOCKBIN, ["4", "10000"]), ("block step 8 x 5,000", BLOCKBIN, ["8", "5000"])]
     for m in ("fn3.hmm", "globins4.hmm", "Pkinase.hmm"):
         hmm = _unpack(m, tmp)
         outs, secs = [], []
-        for k, (b, args) in enumerate([(REFBIN, []), (BLOCKBIN, []), (BLOCKBIN, ["4", "10000"])]):
+        for k, (_, b, args) in enumerate(runs):
             out = os.path.join(tmp, f"{m}_{k}.out")
             r = subprocess.run([b, hmm, seqs, out, "-", "bias", "nolines", *args], capture_output=True, text=True,
                                timeout=300)
@@ -266,8 +271,7 @@ def test_threaded_block_step_500k(tmp_path):
             secs.append(_seconds(r.stderr)[0])
             with open(out) as fh:
                 outs.append(fh.read())
-        print(f"{m}: SSE {secs[0]:.3f} s, one block {secs[1]:.3f} s, 4 workers x blocks of 10,000 {secs[2]:.3f} s",
-              flush=True)
+        print(f"{m}: " + ", ".join(f"{name} {t:.3f} s" for (name, _, _), t in zip(runs, secs)), flush=True)
         assert "targets 500000" in outs[0]
-        _check_same(outs[0], outs[1])
-        _check_same(outs[0], outs[2])
+        for o in outs[1:]:
+            _check_same(outs[0], o)

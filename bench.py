@@ -1,6 +1,6 @@
 """MSV filter benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c1|c2|c4|c5|scan]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c1|c2|c4|c5|scan|forms]
 
 A step is one complete MSV search (fast J-free stage, exact byte-MSV stage for
 the queued pairs, finalize to usc / status / F1 pass bit) of the workload's
@@ -45,6 +45,9 @@ cudahmmsearch_amd.workloads; Easel-compatible generators):
       median 250, sigma 0.6, clipped 30..2560, seeds 100000..) against
       --scan-queries query sequences (log-normal median 350, sigma 0.7,
       seed 6), one msvg_scan call per query
+  forms  the packed-form probe (msvg_probe_forms): the J-free recurrence in
+      the kernels' f16 clamp form and in an unsigned 16-bit form on the same
+      cells, value = the f16 form's speed-up
 """
 import argparse
 import json
@@ -340,6 +343,34 @@ def scan_workload(ca, args):
     ctx.close()
 
 
+def packed_forms(ca, args):
+    """The packed-form probe (msvg_probe_forms) as its own bench line: the
+    J-free recurrence of the 64-lane layout in the kernels' f16 form and in
+    an unsigned 16-bit form on the same cells (value = f16 cells/s over u16
+    cells/s)."""
+    ctx = ca.MSVContext(0)
+    r = [ctx.probe_forms(16384) for _ in range(3)]
+    ctx.close()
+    f16 = min(x["ms_f16"] for x in r)
+    u16 = min(x["ms_u16"] for x in r)
+    cells = r[0]["cells"]
+    out = {
+        "metric": "J-free cells per second, f16 clamp form over unsigned 16-bit form (probe on 1 MI355X)",
+        "value": round(u16 / f16, 3), "unit": "x", "n_gpus": 1, "steps": 3, "warmup": 1,
+        "ms_per_step": round(f16, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16 / u16", "data": "synthetic",
+        "config": {"workload": "forms: 64-lane groups, Q = 16 (2048 model positions), 16 waves per CU, 16,384 rows "
+                               "per wave, costs and residues from a fixed LCG"},
+        "f16": {"ms": round(f16, 3), "gcups": round(cells / f16 / 1e6, 1),
+                "lane_instr_per_cell": 0.75, "instructions": "v_pk_add_f16 clamp per 2 cells, v_pk_maximum3_f16 per 4"},
+        "u16": {"ms": round(u16, 3), "gcups": round(cells / u16 / 1e6, 1),
+                "lane_instr_per_cell": 1.5,
+                "instructions": "v_pk_add_u16 + v_pk_sub_u16 clamp + v_pk_max_u16 per 2 cells"},
+        "agree": all(x["agree"] for x in r), "cells_per_launch": cells,
+    }
+    print(json.dumps(out))
+
+
 def copy_probe(nbytes=256 << 20):
     """One device-to-device copy of a known size, for scaling FETCH_SIZE /
     WRITE_SIZE counters (the copy reads and writes nbytes each)."""
@@ -545,6 +576,9 @@ def main():
         return
     if args.workload == "scan":
         scan_workload(ca, args)
+        return
+    if args.workload == "forms":
+        packed_forms(ca, args)
         return
     if args.copy_probe:
         copy_probe()

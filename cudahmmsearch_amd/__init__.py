@@ -125,6 +125,8 @@ def load_library(path=LIB_PATH):
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
         "msvg_set_tail_guard": (C.c_int, [vp, C.c_int]),
         "msvg_probe_lds": (C.c_int, [vp, C.POINTER(C.c_double)]),
+        "msvg_probe_forms": (C.c_int, [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int32)]),
         "msvg_tile_of": (C.c_int, [vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "msvg_profile_set_viterbi": (C.c_int, [vp, i32, C.POINTER(_VProf)]),
         "msvg_vit_search": (C.c_int, [vp, i32, i32, vp, i64, vp, vp]),
@@ -719,6 +721,13 @@ class MSVContext:
         v = C.c_double()
         self._chk(self.lib.msvg_probe_lds(self._c, C.byref(v)))
         return v.value
+
+    def probe_forms(self, L=16384):
+        """The packed-form probe (msvg_probe_forms): {"ms_f16", "ms_u16",
+        "cells", "agree"} for one launch of each form."""
+        a, b, n, ok = C.c_double(), C.c_double(), C.c_int64(), C.c_int32()
+        self._chk(self.lib.msvg_probe_forms(self._c, L, C.byref(a), C.byref(b), C.byref(n), C.byref(ok)))
+        return {"ms_f16": a.value, "ms_u16": b.value, "cells": n.value, "agree": bool(ok.value)}
 
     def set_tile(self, m_lo, m_hi, lanes, waves=0):
         self._chk(self.lib.msvg_set_tile(self._c, m_lo, m_hi, lanes, waves))

@@ -920,3 +920,93 @@ extern "C" hipError_t msvk_lds_probe(void *out, int grid, int iters, hipStream_t
   hipLaunchKernelGGL(lds_probe, dim3(grid), dim3(1024), 0, st, static_cast<uint4 *>(out), iters);
   return hipGetLastError();
 }
+
+// Packed-form probe (measurement only): the J-free recurrence of the
+// batched kernel's 64-lane layout (one target per wave, Q = 16 registers =
+// 2048 model positions, emissions from an LDS table read one row ahead,
+// the one-slot shift) over L rows, in two encodings of the same cells:
+//   FORM 0, the kernels' form: W - B0 scaled by 2^-11 in f16;
+//     v_pk_add_f16 ... clamp per register, v_pk_maximum3_f16 per two;
+//   FORM 1, unsigned 16-bit units: v_pk_add_u16 of the bias, v_pk_sub_u16
+//     ... clamp of the cost (the floor at B0), v_pk_max_u16 per register.
+// Table entry (x, block, lane) as the fast-stage table (21 rows, 4 blocks):
+// FORM 0 holds (bias - cost) * 2^-11, FORM 1 the costs.  Residues: L bytes
+// per wave (L a multiple of 16), codes below 21.  out[wave] = the maximum
+// cell in units; both forms agree while it stays below 2048.
+template <int FORM>
+__global__ void __launch_bounds__(1024) forms_probe(const uint8_t *res, int L, const uint4 *tab, int tab_u4,
+                                                    uint32_t bias2, uint32_t *out) {
+  extern __shared__ uint4 lds[];
+  constexpr int Q = 16, QB = 4;
+  for (int i = threadIdx.x; i < tab_u4; i += blockDim.x) lds[i] = tab[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = (int) blockIdx.x * (int) (blockDim.x / 64) + (int) (threadIdx.x / 64);
+  const uint8_t *p = res + (size_t) wave * L;
+  const uint4 *tl = lds + lane;
+  uint32_t r[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) r[j] = 0u;
+  uint32_t acc = 0u;
+  uint4 c = *reinterpret_cast<const uint4 *>(p);
+  uint32_t e[QB * 4];
+  load_row<Q>(tl, (uint32_t) __builtin_amdgcn_readfirstlane((int) byte_of(c, 0)), e);
+  for (int i0 = 0; i0 < L; i0 += 16) {
+    const int i1 = (i0 + 16 < L) ? i0 + 16 : i0;
+    const uint4 cn = *reinterpret_cast<const uint4 *>(p + i1);
+    static_for<0, 16>([&](auto rrc) {
+      constexpr int rr = decltype(rrc)::value;
+      uint32_t en[QB * 4];
+      const uint32_t xn = (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0);
+      load_row<Q>(tl, (uint32_t) __builtin_amdgcn_readfirstlane((int) xn), en);
+      const uint32_t in0 = shift_slot<64>(r[Q - 1], lane);
+      if constexpr (FORM == 0) {
+#pragma unroll
+        for (int j = Q - 1; j > 0; --j) r[j] = cell(r[j - 1], e[j]);
+        r[0] = cell(in0, e[0]);
+#pragma unroll
+        for (int j = 0; j < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);
+      } else {
+        const u16x2 b = as_u2(bias2);
+#pragma unroll
+        for (int j = Q - 1; j > 0; --j)
+          r[j] = u2_u32(__builtin_elementwise_sub_sat(as_u2(r[j - 1]) + b, as_u2(e[j])));
+        r[0] = u2_u32(__builtin_elementwise_sub_sat(as_u2(in0) + b, as_u2(e[0])));
+#pragma unroll
+        for (int j = 0; j < Q; ++j) acc = u2_u32(__builtin_elementwise_max(as_u2(acc), as_u2(r[j])));
+      }
+#pragma unroll
+      for (int k = 0; k < Q; ++k) e[k] = en[k];
+      row_fence<Q>(r);
+      asm volatile("" : "+v"(acc));
+    });
+    c = cn;
+  }
+  int m;
+  if constexpr (FORM == 0) {
+    const h2v a = as_h2(acc);
+    m = (int) (wave_max_f32(fmaxf((float) a.x, (float) a.y)) * 2048.0f + 0.5f);
+  } else {
+    const u16x2 a = as_u2(acc);
+    m = (int) (wave_max_f32((float) (a.x > a.y ? a.x : a.y)) + 0.5f);
+  }
+  if (lane == 0) out[wave] = (uint32_t) m;
+}
+
+extern "C" hipError_t msvk_forms_probe(int form, const uint8_t *res, int L, const void *tab, int tab_u4,
+                                       uint32_t bias2, uint32_t *out, int grid, hipStream_t st) {
+  const size_t lds = (size_t) tab_u4 * 16;
+  hipError_t e;
+  if (form == 0) {
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&forms_probe<0>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(forms_probe<0>, dim3(grid), dim3(1024), lds, st, res, L, static_cast<const uint4 *>(tab),
+                       tab_u4, bias2, out);
+  } else {
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&forms_probe<1>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(forms_probe<1>, dim3(grid), dim3(1024), lds, st, res, L, static_cast<const uint4 *>(tab),
+                       tab_u4, bias2, out);
+  }
+  return hipGetLastError();
+}

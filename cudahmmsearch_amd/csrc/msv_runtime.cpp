@@ -66,6 +66,8 @@ extern "C" hipError_t msvk_build_slow_table(const uint8_t *rbv, int M, const uin
                                             uint32_t *out, hipStream_t st);
 extern "C" hipError_t msvk_fill_zero(int32_t *p, int n, hipStream_t st);
 extern "C" hipError_t msvk_lds_probe(void *out, int grid, int iters, hipStream_t st);
+extern "C" hipError_t msvk_forms_probe(int form, const uint8_t *res, int L, const void *tab, int tab_u4,
+                                       uint32_t bias2, uint32_t *out, int grid, hipStream_t st);
 extern "C" hipError_t msvk_launch_one(int Q, const OneArgs *a, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_scan(int Q, const ScanArgs *a, int grid, size_t lds, hipStream_t st);
 
@@ -1965,6 +1967,62 @@ int msvg_probe_lds(msvg_ctx *c, double *ret_bytes_per_s) {
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   *ret_bytes_per_s = (double) grid * 1024 * iters * 8 * 16 / (ms / 1e3);
+  return MSVG_OK;
+}
+
+int msvg_probe_forms(msvg_ctx *c, int32_t L, double *ms_f16, double *ms_u16, int64_t *cells, int32_t *agree) {
+  if (!c || !ms_f16 || !ms_u16 || !cells || !agree || L < 16 || L % 16 != 0 || L > (1 << 20)) return MSVG_EINVAL;
+  (void) hipSetDevice(c->device);
+  constexpr int kRows = 21, kQB = 4, kWaves = 16;
+  const int grid = c->ncu, nw = grid * kWaves;
+  const int tab_u4 = kRows * kQB * 64;
+  const int bias = 20;
+  // Costs per (row, register, slot, lane) and residues from a fixed LCG;
+  // every residue code is a table row (< kRows).
+  uint32_t x = 12345u;
+  auto next = [&x]() { x = x * 1664525u + 1013904223u; return x >> 8; };
+  std::vector<uint32_t> t16((size_t) tab_u4 * 4), tu((size_t) tab_u4 * 4);
+  for (size_t i = 0; i < t16.size(); ++i) {
+    const uint32_t c0 = next() % 256u, c1 = next() % 256u;
+    tu[i] = c0 | (c1 << 16);
+    const _Float16 h0 = (_Float16) ((float) (bias - (int) c0) / 2048.0f);
+    const _Float16 h1 = (_Float16) ((float) (bias - (int) c1) / 2048.0f);
+    t16[i] = (uint32_t) __builtin_bit_cast(uint16_t, h0) | ((uint32_t) __builtin_bit_cast(uint16_t, h1) << 16);
+  }
+  std::vector<uint8_t> res((size_t) nw * L);
+  for (auto &b : res) b = (uint8_t) (next() % (uint32_t) kRows);
+  DevMem d_res, d_t16, d_tu, d_out0, d_out1;
+  HIPCHK(c, d_res.ensure(res.size()));
+  HIPCHK(c, d_t16.ensure(t16.size() * 4));
+  HIPCHK(c, d_tu.ensure(tu.size() * 4));
+  HIPCHK(c, d_out0.ensure((size_t) nw * 4));
+  HIPCHK(c, d_out1.ensure((size_t) nw * 4));
+  HIPCHK(c, hipMemcpy(d_res.p, res.data(), res.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d_t16.p, t16.data(), t16.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d_tu.p, tu.data(), tu.size() * 4, hipMemcpyHostToDevice));
+  const uint32_t bias2 = (uint32_t) bias | ((uint32_t) bias << 16);
+  double ms[2];
+  for (int form = 0; form < 2; ++form) {
+    const void *tab = form ? d_tu.p : d_t16.p;
+    uint32_t *out = form ? d_out1.as<uint32_t>() : d_out0.as<uint32_t>();
+    HIPCHK(c, msvk_forms_probe(form, d_res.as<uint8_t>(), L, tab, tab_u4, bias2, out, grid, c->stream));   // warm-up
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(c, msvk_forms_probe(form, d_res.as<uint8_t>(), L, tab, tab_u4, bias2, out, grid, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+    ms[form] = t;
+  }
+  std::vector<uint32_t> o0((size_t) nw), o1((size_t) nw);
+  HIPCHK(c, hipMemcpy(o0.data(), d_out0.p, (size_t) nw * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(o1.data(), d_out1.p, (size_t) nw * 4, hipMemcpyDeviceToHost));
+  int32_t same = 1;
+  for (int w = 0; w < nw; ++w) same &= (o0[(size_t) w] == o1[(size_t) w] && o0[(size_t) w] < 2048u) ? 1 : 0;
+  *ms_f16 = ms[0];
+  *ms_u16 = ms[1];
+  *cells = (int64_t) nw * L * 2048;
+  *agree = same;
   return MSVG_OK;
 }
 

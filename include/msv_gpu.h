@@ -234,6 +234,16 @@ int msvg_calibrate_vmu(msvg_ctx *ctx, int32_t profile_id, double lambda, float *
  * ds_read_b128 stream on every CU; bench.py reports the MSV kernel's LDS
  * traffic against it. */
 int msvg_probe_lds(msvg_ctx *ctx, double *ret_bytes_per_s);
+/* Measurement: the J-free recurrence of the 64-lane layout (Q = 16, 2048
+ * model positions per target, one target per wave, 16 waves per CU) over L
+ * rows per wave (L a multiple of 16), once in the kernels' f16 form
+ * (v_pk_add_f16 clamp + v_pk_maximum3_f16) and once in an unsigned 16-bit
+ * form (v_pk_add_u16 + v_pk_sub_u16 clamp + v_pk_max_u16) on the same
+ * synthetic costs and residues: each launch's time, the cells of one
+ * launch, and whether the two forms' maxima agree on every target.  No
+ * counterpart in the reference (its SSE filter has one form,
+ * impl_sse/msvfilter.c:150-153). */
+int msvg_probe_forms(msvg_ctx *ctx, int32_t L, double *ms_f16, double *ms_u16, int64_t *cells, int32_t *agree);
 
 /* Test hook: after k more device allocations, fail the rest as out of
  * memory (k < 0: never).  The environment variable MSVG_TEST_FAIL_ALLOC sets

@@ -530,3 +530,12 @@ def test_scan_many_equals_search(ctx):
     assert u.shape == (0, len(pids))
     for pid in pids:
         ctx.remove_profile(pid)
+
+
+def test_packed_forms_probe_agree(ctx):
+    """The packed-form probe's two encodings of the J-free recurrence (the
+    kernels' f16 clamp form and the unsigned 16-bit form) give the same
+    maximum on every one of its targets."""
+    r = ctx.probe_forms(256)
+    assert r["agree"], r
+    assert r["cells"] > 0 and r["ms_f16"] > 0 and r["ms_u16"] > 0

@@ -338,6 +338,11 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
       // xC = max over the rows of xE + e_move (:179, C loop 0)
       xC = max(-32768, lo16(wave_max2(pack16(xacc, xacc))) + a.e_move);
     }
+    // Several waves: every wave has read this item's last exchange (red[]
+    // on a row that overflowed) before any wave's next item writes it.  The
+    // item, its length and the overflow break are the same for all waves
+    // of the workgroup, so all of them reach this barrier once per item.
+    if constexpr (W > 1) __syncthreads();
     if (threadIdx.x % C == 0) a.out[it] = (status << 16) | (int) (uint16_t) xC;
   }
 }

@@ -22,8 +22,11 @@
  * profile is recognised by the contents of its MSV and ViterbiFilter fields,
  * not by the pointer: between queries a freed P7_OPROFILE and its successor
  * can share an address.  The length-dependent fields (tjb_b, xw[N|C|J][MOVE])
- * are not read: the library derives them from L with the expressions
- * p7_oprofile_ReconfigLength() uses (p7_oprofile.c:897-962).  This path
+ * are derived from L by the library with the expressions
+ * p7_oprofile_ReconfigLength() uses (p7_oprofile.c:897-962); an om whose
+ * fields are not those of L (a length model elided or set for another L, as
+ * hmmsim --x-no-lengthmodel does) is refused with eslEINVAL rather than
+ * scored differently from the SSE filters.  This path
  * makes one host-device round trip per target; msvg_search and
  * msvg_vit_search are the batch paths.
  *
@@ -56,6 +59,7 @@
 #include "hmmer.h"
 
 #include "msv_gpu.h"
+#include "msv_prep.h"
 #include "filters_mi355x.h"
 
 struct mi355x_state {
@@ -235,6 +239,8 @@ int p7_MSVFilter(const ESL_DSQ *dsq, int L, const P7_OPROFILE *om, P7_OMX *ox, f
 
   if (p7O_NQB(om->M) > ox->allocQ16) ESL_EXCEPTION(eslEINVAL, "DP matrix allocated too small");  /* msvfilter.c:103 */
   ox->M = om->M;                                                                             /* :104 */
+  if (om->tjb_b != (uint8_t) msvh_tjb(L))                                                    /* :120 */
+    ESL_EXCEPTION(eslEINVAL, "p7_MSVFilter: om's length model is not that of L = %d (p7_oprofile_ReconfigLength)", L);
   if (s == NULL) ESL_EXCEPTION(eslEMEM, "allocation failed");
   if ((status = prepare(s, om)) != eslOK) return status;
   status = msvg_MSVFilter(s->ctx, s->pid, dsq, L, ret_sc);
@@ -252,6 +258,12 @@ int p7_ViterbiFilter(const ESL_DSQ *dsq, int L, const P7_OPROFILE *om, P7_OMX *o
   if (om->mode != p7_LOCAL && om->mode != p7_UNILOCAL)                                          /* :105 */
     ESL_EXCEPTION(eslEINVAL, "Fast filter only works for local alignment");
   ox->M = om->M;                                                                                /* :106 */
+  {                                          /* the N/C/J costs of vitfilter.c:116-117, :179-181, :241 */
+    const int move = msvh_vit_move(om->scale_w, om->nj, L);
+    if (om->xw[p7O_N][p7O_MOVE] != move || om->xw[p7O_C][p7O_MOVE] != move || om->xw[p7O_J][p7O_MOVE] != move ||
+        om->xw[p7O_N][p7O_LOOP] != 0 || om->xw[p7O_C][p7O_LOOP] != 0 || om->xw[p7O_J][p7O_LOOP] != 0)
+      ESL_EXCEPTION(eslEINVAL, "p7_ViterbiFilter: om's length model is not that of L = %d (p7_oprofile_ReconfigLength)", L);
+  }
   if (s == NULL) ESL_EXCEPTION(eslEMEM, "allocation failed");
   if (s->in_block) {               /* a survivor of the current block: its precomputed score */
     int64_t k = s->vnext;

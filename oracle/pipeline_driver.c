@@ -33,6 +33,9 @@
  *
  * In the SSE build, workers > 0 (with nolines) is hmmsearch --cpu workers:
  * a reader thread and that many threads running p7_Pipeline, merged.
+ * badlength-msv / badlength-vit in place of nolines: before the first
+ * target, call that filter with om configured for 3L + 100 (the GPU builds
+ * refuse it through Easel's exception handler).
  *
  * After each query's report it lists, for every target, p7_MSVFilter's and
  * p7_ViterbiFilter's status and the float bits of their scores (not with
@@ -178,12 +181,15 @@ int main(int argc, char **argv)
   double       F1 = -1.0;
   int          nobias = 0, nolines = 0, nquery = 0, hstatus, sstatus;
   int          workers = 0, block_seqs = 10000;
+  int          badlength = 0;  /* test of the filters' length-model check: 1 MSV, 2 ViterbiFilter */
 
   if (argc < 4) die("usage: pipeline_driver <hmmfile> <seqfile> <outfile> "
                     "[F1|- [nobias|bias [nolines|lines [workers [block_seqs]]]]]", NULL);
   if (argc > 4 && strcmp(argv[4], "-") != 0) F1 = atof(argv[4]);
   if (argc > 5) nobias = (strcmp(argv[5], "nobias") == 0);
   if (argc > 6) nolines = (strcmp(argv[6], "nolines") == 0);
+  if (argc > 6 && strncmp(argv[6], "badlength-", 10) == 0)
+    badlength = (strcmp(argv[6] + 10, "msv") == 0) ? 1 : (strcmp(argv[6] + 10, "vit") == 0) ? 2 : 0;
   if (argc > 7) workers = atoi(argv[7]);
   if (argc > 8) block_seqs = atoi(argv[8]);
   if (workers < 0 || block_seqs < 1) die("bad workers / block_seqs", NULL);
@@ -330,6 +336,13 @@ int main(int argc, char **argv)
       char  line[256];
       int   n;
 
+      if (badlength) {                         /* om configured for another length than the target's */
+        p7_oprofile_ReconfigLength(wom, 3 * sq->n + 100);
+        if (badlength == 1) st = p7_MSVFilter(sq->dsq, (int) sq->n, wom, ox, &usc);
+        else st = p7_ViterbiFilter(sq->dsq, (int) sq->n, wom, ox, &vsc);
+        fprintf(stderr, "pipeline_driver: filter status %d on a mismatched length model\n", st);
+        badlength = 0;
+      }
 #ifndef MI355X_BLOCK
       p7_pli_NewSeq(pli, sq);
       p7_bg_SetLength(bg, sq->n);

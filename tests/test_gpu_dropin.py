@@ -275,3 +275,23 @@ def test_threaded_block_step_500k(tmp_path):
         assert "targets 500000" in outs[0]
         for o in outs[1:]:
             _check_same(outs[0], o)
+
+
+@pytest.mark.skipif(not os.path.exists(GPUBIN), reason="needs the built drop-in")
+@pytest.mark.parametrize("which", ["msv", "vit"])
+def test_dropin_refuses_a_mismatched_length_model(tmp_path, which):
+    """The library derives tjb_b and the N/C/J moves from the target length;
+    an om configured for another length (hmmsim --x-no-lengthmodel elides
+    the length model, hmmsim.c:946-952) is refused with eslEINVAL through
+    Easel's exception handler, before any device work, instead of being
+    scored differently from the SSE filters.  The SSE build scores it."""
+    hmm, seq = _unpack("fn3.hmm", tmp_path), _unpack("HBB_HUMAN", tmp_path)
+    out = os.path.join(str(tmp_path), "o.txt")
+    r = subprocess.run([GPUBIN, hmm, seq, out, "-", "bias", f"badlength-{which}"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "length model is not that of L" in r.stderr, r.stderr[-500:]
+    if os.path.exists(REFBIN):
+        r = subprocess.run([REFBIN, hmm, seq, out, "-", "bias", f"badlength-{which}"], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0 and "filter status" in r.stderr

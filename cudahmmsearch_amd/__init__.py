@@ -123,6 +123,8 @@ def load_library(path=LIB_PATH):
         "msvg_last_stats": (C.c_int, [vp, C.POINTER(_Stats)]),
         "msvg_MSVFilter": (C.c_int, [vp, i32, vp, C.c_int, C.POINTER(C.c_float)]),
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
+        "msvg_set_vit_tile": (C.c_int, [vp, i32, i32, i32, i32]),
+        "msvg_set_vit_vote": (C.c_int, [vp, i32]),
         "msvg_set_tail_guard": (C.c_int, [vp, C.c_int]),
         "msvg_probe_lds": (C.c_int, [vp, C.POINTER(C.c_double)]),
         "msvg_probe_forms": (C.c_int, [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64),
@@ -734,6 +736,20 @@ class MSVContext:
 
     def clear_tiles(self):
         self._chk(self.lib.msvg_set_tile(self._c, 0, 0, 0, 0))
+
+    def set_vit_tile(self, m_lo, m_hi, lanes, waves=1):
+        """ViterbiFilter shape for models of length m_lo..m_hi, applied by the
+        next set_viterbi (msvg_set_vit_tile)."""
+        self._chk(self.lib.msvg_set_vit_tile(self._c, m_lo, m_hi, lanes, waves))
+
+    def clear_vit_tiles(self):
+        self._chk(self.lib.msvg_set_vit_tile(self._c, 0, 0, 0, 0))
+
+    def set_vit_vote(self, mode):
+        """-1: profiles set after this vote on the lazy-F test by their
+        estimated rate (default); 0: every row closes the delete paths; 1:
+        every row votes (msvg_set_vit_vote).  Results do not change."""
+        self._chk(self.lib.msvg_set_vit_vote(self._c, mode))
 
     def set_tail_guard(self, on):
         """Batched mode: widen the lane group where the longest target would

@@ -226,4 +226,7 @@ struct VitArgs {
   int32_t *out;
   int32_t e_loop, e_move, base_w, ddbound_w;
   int32_t etab_in_lds;         // one-wave shape: stage the emission table in LDS
+  const int32_t *uoff;         // slot k (a wave, or a workgroup of a multi-wave
+                               // shape) runs units [uoff[k], uoff[k+1]) of the
+                               // items; a unit is one wave's targets
 };

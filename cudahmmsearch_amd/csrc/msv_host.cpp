@@ -419,6 +419,52 @@ int msvh_vit_move(float scale_w, float nj, int L) {
   else return (int16_t) sc;
 }
 
+// How often p7_ViterbiFilter's lazy-F test (Dmax + ddbound_w > xB,
+// vitfilter.c:199) holds, per row, on one pseudo-random target of `rows`
+// residues (uniform over the K = 20 or 4 canonical codes): the recurrence of
+// vitfilter.c:131-227 in model order, in int arithmetic clamped to int16.
+// A planning estimate for the kernels (whether a row should vote on the test
+// or always close the delete paths -- the results are the same either way),
+// not a score.
+double msvh_vit_lazyf_rate(const int16_t *rwv, const int16_t *twv, int M, int Kp, int e_loop, int base_w,
+                           int ddbound_w, int move, int rows) {
+  if (M < 1 || rows < 1) return 0.0;
+  const int K = (Kp >= 26) ? 20 : 4;
+  auto cl = [](int v) { return v < -32768 ? -32768 : (v > 32767 ? 32767 : v); };
+  auto T = [&](int t, int k) { return (int) twv[(size_t) t * M + k - 1]; };   // k = 1..M
+  std::vector<int> Mv(M + 2, -32768), Iv(M + 2, -32768), Dv(M + 2, -32768), Mn(M + 2), In(M + 2), Dn(M + 2);
+  const int xN = base_w;
+  int xB = cl(xN + move), xJ = -32768, fired = 0;
+  uint32_t r = 12345u;
+  for (int i = 0; i < rows; ++i) {
+    r = r * 1103515245u + 12345u;
+    const int x = (int) ((r >> 16) % (uint32_t) K);
+    int xE = -32768, Dmax = -32768;
+    Mn[0] = In[0] = Dn[0] = -32768;
+    for (int k = 1; k <= M; ++k) {
+      int sv = cl(xB + T(0, k));
+      sv = std::max(sv, cl(Mv[k - 1] + T(1, k)));
+      sv = std::max(sv, cl(Iv[k - 1] + T(2, k)));
+      sv = std::max(sv, cl(Dv[k - 1] + T(3, k)));
+      sv = cl(sv + rwv[(size_t) x * M + k - 1]);
+      Mn[k] = sv;
+      xE = std::max(xE, sv);
+      In[k] = std::max(cl(Mv[k] + T(5, k)), cl(Iv[k] + T(6, k)));
+      Dn[k] = (k > 1) ? cl(Mn[k - 1] + T(4, k - 1)) : -32768;
+      Dmax = std::max(Dmax, Dn[k]);
+    }
+    if (xE >= 32767) break;
+    xJ = cl(std::max(xJ, xE + e_loop));
+    xB = cl(std::max(xJ + move, xN + move));
+    if (Dmax + ddbound_w > xB) {
+      ++fired;
+      for (int k = 2; k <= M; ++k) Dn[k] = std::max(Dn[k], cl(Dn[k - 1] + T(7, k - 1)));
+    }
+    Mv.swap(Mn); Iv.swap(In); Dv.swap(Dn);
+  }
+  return (double) fired / rows;
+}
+
 float msvh_vit_score(int xC, int move, int base_w, float scale_w) {
   if (xC <= -32768) return -INFINITY;                         // vitfilter.c:240, :247
   float sc = (float) xC + (float) move - (float) base_w;      // :242

@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <queue>
 #include <memory>
 #include <string>
 #include <tuple>
@@ -55,8 +56,9 @@ extern "C" hipError_t msvk_launch_seg(int Q, const SegArgs *a, int grid, int blo
 extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st);
 extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, hipStream_t st);
-extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int ncu, hipStream_t st);
-extern "C" int msvk_vit_slots(int G, int Q, int W, const VitArgs *a, int ncu, int *per);
+extern "C" int msvk_vit_slots(int Q, int W, const VitArgs *a, int ncu);
+extern "C" hipError_t msvk_launch_vit_band(int G, int Q, int vote, const VitArgs *a, int ncu, hipStream_t st);
+extern "C" int msvk_vit_band_slots(int G, int Q, int vote, const VitArgs *a, int ncu, int *per);
 extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st);
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
                                             int Q, int W, int bias, uint32_t *out, hipStream_t st);
@@ -150,6 +152,8 @@ struct VitProfile {               // ViterbiFilter fields (model order)
   float scale_w = 0.f, nj = 0.f, vmu = 0.f, vlambda = 0.f;
   int Q = 1, W = 1;                // register blocks per lane, waves per target
   int G = 64;                      // lanes per target: 16 or 32 (several targets per wave), or 64 * W
+                                   // (W = 1: vit_band; W > 1: vit_kernel)
+  int vote = 1;                    // vit_band: rows vote on the lazy-F test (else always close)
   DevMem d_ttab;                   // transition table, VitArgs layout
 };
 
@@ -258,6 +262,8 @@ struct msvg_ctx {
   std::map<std::pair<int, int>, std::unique_ptr<Tables>> tables;
   std::map<std::tuple<int, int, double>, std::pair<std::vector<uint16_t>, std::vector<float>>> finals;
   std::vector<Tile> tiles;
+  std::vector<Tile> vtiles;        // ViterbiFilter shapes by model length (msvg_set_vit_tile)
+  int vit_vote = -1;               // msvg_set_vit_vote: -1 by the profile's lazy-F rate, else forced
   bool tail_guard = true;
   Bufs eager;                      // buffers of plain and eager searches
   Bufs *last = nullptr;            // buffers holding the latest results
@@ -285,7 +291,7 @@ struct msvg_ctx {
   uint32_t *h_scanout = nullptr;            // pinned result words
   size_t h_scancap = 0;
   // ViterbiFilter searches
-  DevMem d_vitems, d_vmove, d_vout;
+  DevMem d_vitems, d_vmove, d_vout, d_vuoff;
   hipStream_t vstream[kVitStreams] = {};   // multi-profile searches: launches round-robin
   hipEvent_t vdone[kVitStreams] = {};
 };
@@ -1199,24 +1205,16 @@ int search_device(msvg_ctx *c, const int32_t *ids, int np, int dbid, double F1, 
 // ViterbiFilter tables (host-built, see VitArgs in msv_device.h).
 namespace {
 
-// ViterbiFilter kernel shape for a model of length M: lanes per target G,
-// waves per target W (G = 64 only: the fewest with Q <= 16 registers per
-// lane, W <= 8 for M <= 16384), registers per lane Q.  Short models
-// share a wave: 4 targets of 16 lanes or 2 of 32, so the fixed work of a
-// row (reductions, shifts, special states, the delete-closure scan) is
-// shared.  Cost per target row, in packed instructions, from the kernels'
-// instruction counts: (19 Q + F) / targets per wave, F = 70 (one target),
-// 78 (two), 94 (four).
+// ViterbiFilter kernel shape for a model of length M (measured on gfx950,
+// survivors of up to 2,000 residues): vit_band with G = 16 lanes per target
+// (four targets per wave) up to M = 512, G = 32 up to 1024, G = 64 up to
+// 2048 (Q = ceil(M / 2G) <= 16 registers per lane); beyond, vit_kernel with
+// W = 2, 4 or 8 waves per target (Q <= 16).
 void vit_shape(int M, int &Q, int &W, int &G) {
   W = 1;
-  while (128 * W * 16 < M) W *= 2;
-  Q = (M + 128 * W - 1) / (128 * W);
-  G = 64;
-  if (W > 1) return;
-  double best = 19.0 * Q + 70.0;
-  const int q32 = (M + 63) / 64, q16 = (M + 31) / 32;
-  if (q32 <= 8 && (19.0 * q32 + 78.0) / 2 < best) { best = (19.0 * q32 + 78.0) / 2; G = 32; Q = q32; }
-  if (q16 <= 8 && (19.0 * q16 + 94.0) / 4 < best) { best = (19.0 * q16 + 94.0) / 4; G = 16; Q = q16; }
+  G = (M <= 512) ? 16 : (M <= 1024 ? 32 : 64);
+  if (M > 2048) while (128 * W * 16 < M) W *= 2;
+  Q = (M + 2 * G * W - 1) / (2 * G * W);
 }
 
 // Lanes per target of a shape.
@@ -1624,6 +1622,23 @@ int msvg_set_tile(msvg_ctx *c, int32_t lo, int32_t hi, int32_t lanes, int32_t wa
   return MSVG_OK;
 }
 
+int msvg_set_vit_tile(msvg_ctx *c, int32_t lo, int32_t hi, int32_t lanes, int32_t waves) {
+  if (!c) return MSVG_EINVAL;
+  if (lanes == 0) { c->vtiles.clear(); return MSVG_OK; }
+  if (lanes != 16 && lanes != 32 && lanes != 64) return fail(c, MSVG_EINVAL, "lanes must be 16, 32 or 64");
+  if (lanes == 64 && waves != 1 && waves != 2 && waves != 4 && waves != 8)
+    return fail(c, MSVG_EINVAL, "waves must be 1, 2, 4 or 8");
+  c->vtiles.push_back(Tile{lo, hi, lanes, waves});
+  return MSVG_OK;
+}
+
+int msvg_set_vit_vote(msvg_ctx *c, int32_t mode) {
+  if (!c) return MSVG_EINVAL;
+  if (mode < -1 || mode > 1) return fail(c, MSVG_EINVAL, "mode must be -1, 0 or 1");
+  c->vit_vote = mode;
+  return MSVG_OK;
+}
+
 int msvg_profile_set_evparams(msvg_ctx *c, int32_t id, float mmu, float mlambda) {
   if (!c || !c->profiles.count(id)) return fail(c, MSVG_EINVAL, "unknown profile id");
   c->profiles[id]->mu = mmu;
@@ -1675,6 +1690,18 @@ int msvg_profile_set_viterbi(msvg_ctx *c, int32_t pid, const msvg_vprofile *vp) 
   v->e_loop = vp->e_loop; v->e_move = vp->e_move; v->base_w = vp->base_w; v->ddbound_w = vp->ddbound_w;
   v->scale_w = vp->scale_w; v->nj = vp->nj; v->vmu = vp->vmu; v->vlambda = vp->vlambda;
   vit_shape(p.M, v->Q, v->W, v->G);
+  // Rows vote on the lazy-F test unless it holds on most rows (then the
+  // delete-path closure runs on every row, without the vote's work).
+  v->vote = c->vit_vote >= 0 ? c->vit_vote
+          : (msvh_vit_lazyf_rate(v->rwv.data(), v->twv.data(), p.M, p.Kp, v->e_loop, v->base_w, v->ddbound_w,
+                                 msvh_vit_move(v->scale_w, v->nj, 350), 128) < 0.75 ? 1 : 0);
+  for (const Tile &t : c->vtiles) {
+    if (p.M < t.lo || p.M > t.hi) continue;
+    v->G = t.lanes;
+    v->W = t.lanes == 64 ? std::max(1, t.waves) : 1;
+    v->Q = (p.M + 2 * vit_lanes(*v) - 1) / (2 * vit_lanes(*v));
+    if (v->Q > 16) return fail(c, MSVG_EINVAL, "Viterbi tile: too many registers per lane");
+  }
   const size_t per = (size_t) v->Q * vit_lanes(*v);
   std::vector<uint32_t> h(8 * per);
   for (int t = 0; t < 8; ++t)
@@ -1700,6 +1727,8 @@ struct VitJob {
   std::vector<int64_t> order;        // item q -> caller position
   std::vector<int64_t> items;        // item q -> sorted target index
   std::vector<int16_t> mv;
+  std::vector<int32_t> uoff;         // slot k runs units [uoff[k], uoff[k+1])
+  int64_t ubase = 0;                 // where uoff sits in the device array
   int64_t cells = 0;
   VitArgs a{};
 };
@@ -1735,35 +1764,42 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   a.db.tjb = d.d_tjb.as<uint8_t>(); a.db.n = d.n; a.db.R = d.R;
   a.n = (int32_t) n;
   a.e_loop = v.e_loop; a.e_move = v.e_move; a.base_w = v.base_w; a.ddbound_w = v.ddbound_w;
-  a.etab_in_lds = (v.W == 1 && (size_t) (d.R + 1) * v.Q * 64 * 4 <= 65536) ? 1 : 0;
-  // The kernels stride over the items statically: item position q runs on
-  // slot (q / per) % S.  Dealt in sorted order, slot 0 would take the
-  // longest item of every round; instead each round's units (per targets,
-  // longest first) go to the slots with the fewest rows so far.  The last
-  // round stays in order, so a partial last unit keeps the last position.
+  // Units of per targets (consecutive in length order, so a unit's rows
+  // are its first target's length) go to the kernel's S slots (waves, or
+  // workgroups of multi-wave targets) greedily: each unit, longest first, to
+  // the slot with the fewest rows so far, any number of units per slot.
+  // Slot k runs units [uoff[k], uoff[k+1]) of the reordered items.  A partial
+  // last unit (n not a multiple of per) stays the last unit, in slot S - 1,
+  // so that the positions past n are its own.
   int per = 1;
-  const int S = msvk_vit_slots(v.G, v.Q, v.W, &a, c->ncu, &per);
+  const int S = (v.W == 1) ? msvk_vit_band_slots(v.G, v.Q, v.vote, &a, c->ncu, &per)
+                           : msvk_vit_slots(v.Q, v.W, &a, c->ncu);
+  if (S < 1) return fail(c, MSVG_EINVAL, "ViterbiFilter shape has no launch");
   const int64_t nunits = (n + per - 1) / per;
-  if (S > 1 && nunits > S) {
-    const int64_t lpt_rounds = (nunits - 1) / S;
-    std::vector<int64_t> unit_at((size_t) nunits), load((size_t) S, 0);
-    std::vector<int> slots((size_t) S);
-    for (int64_t r = 0; r < lpt_rounds; ++r) {
-      for (int k = 0; k < S; ++k) slots[(size_t) k] = k;
-      std::stable_sort(slots.begin(), slots.end(), [&](int x, int y) { return load[(size_t) x] < load[(size_t) y]; });
-      for (int i = 0; i < S; ++i) {
-        const int64_t u = r * S + i;
-        const int k = slots[(size_t) i];
-        unit_at[(size_t) (r * S + k)] = u;
-        load[(size_t) k] += len_of(job.order[(size_t) (u * per)]);
-      }
+  const bool partial = (n % per) != 0;
+  std::vector<std::vector<int64_t>> lists((size_t) S);
+  {
+    using Slot = std::pair<int64_t, int>;                       // (rows so far, slot)
+    std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+    for (int k = 0; k < S; ++k) heap.push(Slot{0, k});
+    for (int64_t u = 0; u < nunits - (partial ? 1 : 0); ++u) {
+      Slot sl = heap.top();
+      heap.pop();
+      lists[(size_t) sl.second].push_back(u);
+      sl.first += len_of(job.order[(size_t) (u * per)]);
+      heap.push(sl);
     }
-    for (int64_t u = lpt_rounds * S; u < nunits; ++u) unit_at[(size_t) u] = u;
+    if (partial) lists[(size_t) (S - 1)].push_back(nunits - 1);
+  }
+  job.uoff.assign((size_t) S + 1, 0);
+  {
     std::vector<int64_t> dealt;
     dealt.reserve((size_t) n);
-    for (int64_t pos = 0; pos < nunits; ++pos)
-      for (int64_t t = unit_at[(size_t) pos] * per; t < std::min<int64_t>(n, (unit_at[(size_t) pos] + 1) * per); ++t)
-        dealt.push_back(job.order[(size_t) t]);
+    for (int k = 0; k < S; ++k) {
+      job.uoff[(size_t) k + 1] = job.uoff[(size_t) k] + (int32_t) lists[(size_t) k].size();
+      for (const int64_t u : lists[(size_t) k])
+        for (int64_t t = u * per; t < std::min<int64_t>(n, (u + 1) * per); ++t) dealt.push_back(job.order[(size_t) t]);
+    }
     job.order.swap(dealt);
   }
   job.items.resize((size_t) n);
@@ -1783,16 +1819,25 @@ int vit_launch(msvg_ctx *c, VitJob &job, hipStream_t st) {
   job.a.items = c->d_vitems.as<int64_t>() + job.base;
   job.a.move = c->d_vmove.as<int16_t>() + job.base;
   job.a.out = c->d_vout.as<int32_t>() + job.base;
-  if (v.G < 64) HIPCHK(c, msvk_launch_vit_group(v.G, v.Q, &job.a, c->ncu, st));
+  job.a.uoff = c->d_vuoff.as<int32_t>() + job.ubase;
+  if (v.W == 1) HIPCHK(c, msvk_launch_vit_band(v.G, v.Q, v.vote, &job.a, c->ncu, st));
   else HIPCHK(c, msvk_launch_vit(v.Q, v.W, &job.a, c->ncu, st));
   return MSVG_OK;
 }
 
 // Device item arrays for all jobs, uploaded on st.
 int vit_upload(msvg_ctx *c, std::vector<VitJob> &jobs, hipStream_t st, std::vector<int64_t> &hitems,
-               std::vector<int16_t> &hmv) {
-  int64_t tot = 0;
-  for (VitJob &j : jobs) { j.base = tot; tot += j.n; }
+               std::vector<int16_t> &hmv, std::vector<int32_t> &huoff) {
+  int64_t tot = 0, utot = 0;
+  for (VitJob &j : jobs) {
+    j.base = tot; tot += j.n;
+    j.ubase = utot; utot += (int64_t) j.uoff.size();
+  }
+  HIPCHK(c, c->d_vuoff.ensure((size_t) std::max<int64_t>(utot, 1) * sizeof(int32_t)));
+  huoff.resize((size_t) utot);
+  for (const VitJob &j : jobs) std::copy(j.uoff.begin(), j.uoff.end(), huoff.begin() + j.ubase);
+  if (utot > 0)
+    HIPCHK(c, hipMemcpyAsync(c->d_vuoff.p, huoff.data(), (size_t) utot * sizeof(int32_t), hipMemcpyHostToDevice, st));
   HIPCHK(c, c->d_vitems.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int64_t)));
   HIPCHK(c, c->d_vmove.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int16_t)));
   HIPCHK(c, c->d_vout.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int32_t)));
@@ -1855,7 +1900,8 @@ int msvg_vit_search_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t d
   hipStream_t st = c->stream;
   std::vector<int64_t> hitems;
   std::vector<int16_t> hmv;
-  int rc = vit_upload(c, jobs, st, hitems, hmv);
+  std::vector<int32_t> huoff;
+  int rc = vit_upload(c, jobs, st, hitems, hmv, huoff);
   if (rc != MSVG_OK) return rc;
   // Each launch lasts at least as long as its longest target's row chain
   // (one wave, or one workgroup, per target): profiles go round-robin over

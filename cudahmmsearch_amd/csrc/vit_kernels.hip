@@ -1,16 +1,20 @@
-// vit_kernels.hip -- gfx950 kernel for HMMER3's ViterbiFilter, the filter
+// vit_kernels.hip -- gfx950 kernels for HMMER3's ViterbiFilter, the filter
 // stage after MSV in p7_Pipeline (impl_sse/vitfilter.c:84-250), bit-exact.
 //
 // The reference runs Farrar's striped int16 recurrence over 8 words per SSE
-// vector.  Here W wavefronts (one when M <= 1024) own a target and stripe its
-// model over 128*W int16 slots, two per VGPR: register j of lane c holds model
-// positions k = j + s*Q + 1 for its slots s = 2c, 2c+1.  Every cell does the
-// reference's operations in the reference's order with packed saturating
-// int16 arithmetic (v_pk_add_i16 clamp = _mm_adds_epi16, v_pk_max_i16 =
-// _mm_max_epi16), so values do not depend on the striping.  Per row:
+// vector.  Here a target's model is striped over the int16 slots of G lanes
+// (vit_band: G = 16 or 32 with several targets per wave, or 64) or of W
+// waves (vit_kernel: W = 2, 4 or 8, models beyond 2048), two slots per VGPR:
+// register j of lane c holds model positions k = j + s*Q + 1 for its slots
+// s = 2c, 2c+1.  Every cell does the reference's operations in the
+// reference's order with packed saturating int16 arithmetic (v_pk_add_i16
+// clamp = _mm_adds_epi16, v_pk_max_i16 = _mm_max_epi16), so values do not
+// depend on the striping.  Per row:
 //   - the diagonal predecessors of register 0 come from the last register one
-//     slot down: DPP wave_shr:1 + v_alignbit, and across waves through LDS;
-//   - xE and Dmax (vitfilter.c:176, :198) are one packed DPP max reduction;
+//     slot down: DPP wave_shr:1 (or row_shr:1) + v_alignbit, and across waves
+//     through LDS;
+//   - xE and Dmax (vitfilter.c:176, :198) are packed DPP max reductions, or
+//     per-lane shares voted on against thresholds (vit_band);
 //   - the special states run on the scalar unit;
 //   - when the reference would run its lazy-F delete passes (Dmax + ddbound_w
 //     > xB, :199) the kernel computes the same result, the closure
@@ -120,24 +124,17 @@ __device__ __forceinline__ uint32_t halves_max(uint32_t xEv, uint32_t Dmv) {
 
 }  // namespace
 
-namespace {
-template <int G> __device__ __forceinline__ uint32_t shift_grp(uint32_t last, int lane);
-template <int G, int Q>
-__device__ __forceinline__ void close_d(uint32_t (&Dr)[Q], const uint32_t (&tDD)[Q], bool mine, const uint4 *tsc,
-                                        int c, int lane);
-}  // namespace
-
 template <int Q, int W>
-__global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
-  extern __shared__ uint32_t etab_lds[];               // one wave per target: the emission table
+__global__ void __launch_bounds__(64 * W) vit_kernel(VitArgs a) {
+  static_assert(W >= 2, "one wave per target runs vit_band");
   __shared__ uint32_t red[16], dlast[16], eM[16], eI[16], eD[16];
   __shared__ int scC[16], scT[16];
   constexpr int C = 64 * W;                            // lanes per target
   __shared__ uint4 tsc[C];                             // the lane scan's T terms per lane
-  __shared__ uint32_t tse[W > 1 ? C : 1];              // and its cross-wave terms
+  __shared__ uint32_t tse[C];                          // and its cross-wave terms
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int w = (W == 1) ? 0 : wave;
+  const int w = wave;
   const int c = w * 64 + lane;
 
   uint32_t tBM[Q], tMM[Q], tIM[Q], tDM[Q], tMD[Q], tMI[Q], tII[Q], tDD[Q];
@@ -166,27 +163,16 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
   const int Ts6 = t_next<0x143, 0xc>(Ts5);
   const int Tex = __builtin_amdgcn_update_dpp(0, Ts6, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
   // kept in LDS, read on the rows that run the closure
-  if (W > 1 || threadIdx.x < 64) {
-    tsc[c] = make_uint4(negsat2(Tlo, Thi), negsat2(in_row_t<1>(Ts0, lane), in_row_t<2>(Ts1, lane)),
-                        negsat2(in_row_t<4>(Ts2, lane), in_row_t<8>(Ts3, lane)), negsat2(Ts4, Ts5));
-    if constexpr (W > 1) tse[c] = negsat2(Ts6, Tex);
-  }
+  tsc[c] = make_uint4(negsat2(Tlo, Thi), negsat2(in_row_t<1>(Ts0, lane), in_row_t<2>(Ts1, lane)),
+                      negsat2(in_row_t<4>(Ts2, lane), in_row_t<8>(Ts3, lane)), negsat2(Ts4, Ts5));
+  tse[c] = negsat2(Ts6, Tex);
   __syncthreads();
 
   const uint32_t *et = a.etab;
-  if (W == 1 && a.etab_in_lds) {
-    const int ntab = (a.db.R + 1) * Q * 64;
-    for (int i = threadIdx.x; i < ntab; i += blockDim.x) etab_lds[i] = a.etab[i];
-    __syncthreads();
-    et = etab_lds;
-  }
-
-  const int nw = (W == 1) ? (int) (blockDim.x >> 6) : 1;
-  const int first = (W == 1) ? (int) blockIdx.x * nw + wave : (int) blockIdx.x;
-  const int stride = (W == 1) ? (int) gridDim.x * nw : (int) gridDim.x;
+  const int slot = (int) blockIdx.x;
   const int xN = a.base_w;                             // vitfilter.c:116; N loop cost is 0
 
-  for (int it = first; it < a.n; it += stride) {
+  for (int it = a.uoff[slot], iend = a.uoff[slot + 1]; it < iend; ++it) {
     const int64_t s = a.items[it];
     const int L = a.db.len[s];
     const uint8_t *p = a.db.res + a.db.off[s];
@@ -196,9 +182,6 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
     for (int j = 0; j < Q; ++j) Mr[j] = Ir[j] = Dr[j] = NEG2;        // :115
     int xB = (int16_t) (xN + move);                                  // :117
     int xJ = -32768, xC = -32768, status = 0;
-    int xacc = -32768;                  // one wave: this lane's largest row maximum (xC at the end)
-    // one wave: the row votes' thresholds (as in vit_group)
-    int tE = min(max(xJ, xN) - a.e_loop, 32766), tD = xB - a.ddbound_w;
     uint32_t edM = NEG2, edI = NEG2, edD = NEG2;
     // xB + tBM per register, the first term of every M cell: xB changes on
     // few rows (xJ is a running maximum), so it is recomputed only then.
@@ -244,33 +227,9 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
           Dmv = maxs(Dmv, dn);                                       // :168
           if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;              // :162
         }
-        if constexpr (W == 1) {
-          // This lane's share of the row's xE and Dmax (:176, :198), voted
-          // on instead of reduced: the exact special-state update runs only
-          // when some lane could overflow or move B.
-          const uint32_t hv = halves_max(xEv, Dmv);
-          const int xe = lo16(hv), dm = hi16(hv);
-          xacc = max(xacc, xe);
-          if (__builtin_amdgcn_ballot_w64(xe > tE)) {
-            const int xE = lo16(wave_max2(hv));
-            if (xE >= 32767) { status = 16; break; }                   // :177
-            xJ = (int16_t) max(xJ, xE + a.e_loop);                     // :180 (J loop 0)
-            const int xBn = (int16_t) max(xJ + move, xN + move);       // :181
-            if (xBn != xB) {
-#pragma unroll
-              for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xBn, xBn), tBM[j]);
-            }
-            xB = xBn;
-            tE = min(max(xJ, xN) - a.e_loop, 32766);
-            tD = xB - a.ddbound_w;
-          }
-          Dr[0] = shift_in(dcv, NEG2, lane);                           // :230-231 / :203-204
-          if (__builtin_amdgcn_ballot_w64(dm > tD)) close_d<64, Q>(Dr, tDD, true, tsc, c, lane);   // :199-227
-          continue;
-        }
         uint32_t r = wave_max2(halves_max(xEv, Dmv));
         uint32_t dedge = NEG2;
-        if constexpr (W > 1) {
+        {
           const uint32_t dl = __builtin_amdgcn_readlane(dcv, 63);
           if (lane == 0) { red[w] = r; dlast[w] = dl; }
           __syncthreads();
@@ -313,8 +272,8 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
           cs = c_step<0x143, 0xc, 1>(cs, ts45);
           // exclusive: the previous lane's prefix (wave_shr:1; lane 0 the identity)
           const int ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x138, 0xf, 0xf, false);
-          int cin_lo = ce;             // one wave: nothing enters its first slot
-          if constexpr (W > 1) {
+          int cin_lo = ce;
+          {
             int carry = -32768;                                      // into this wave's first slot
             const uint32_t ts6e = tse[c];
             if (lane == 63) { scC[w] = cs; scT[w] = -(int) (ts6e & 0xffffu); }
@@ -327,22 +286,18 @@ __global__ void __launch_bounds__(W == 1 ? 256 : 64 * W) vit_kernel(VitArgs a) {
 #pragma unroll
           for (int j = 0; j < Q; ++j) { Dr[j] = maxs(Dr[j], y); y = adds(y, tDD[j]); }
         }
-        if constexpr (W > 1) {
+        {
           if (lane == 63) { eM[w] = Mr[Q - 1]; eI[w] = Ir[Q - 1]; eD[w] = Dr[Q - 1]; }
           __syncthreads();
           if (w > 0) { edM = eM[w - 1]; edI = eI[w - 1]; edD = eD[w - 1]; }
         }
       }
     }
-    if constexpr (W == 1) {
-      // xC = max over the rows of xE + e_move (:179, C loop 0)
-      xC = max(-32768, lo16(wave_max2(pack16(xacc, xacc))) + a.e_move);
-    }
     // Several waves: every wave has read this item's last exchange (red[]
     // on a row that overflowed) before any wave's next item writes it.  The
     // item, its length and the overflow break are the same for all waves
     // of the workgroup, so all of them reach this barrier once per item.
-    if constexpr (W > 1) __syncthreads();
+    __syncthreads();
     if (threadIdx.x % C == 0) a.out[it] = (status << 16) | (int) (uint16_t) xC;
   }
 }
@@ -355,27 +310,17 @@ static int resident_per_cu(K kernel, int block, size_t lds) {
   return occ;
 }
 
-// Grids: one wave per target (or per NG targets) with a static stride, as
-// many workgroups as are needed or as fit the device at once, whichever is
-// fewer; multi-wave targets keep one workgroup per target up to 8 waves per
-// CU.
+// Grid of a multi-wave shape: one workgroup per target at a time, up to 8
+// waves per CU.
 template <int Q, int W>
 static int grid_vit(const VitArgs *a, int ncu) {
-  const int block = (W == 1) ? 256 : 64 * W;
-  const size_t lds = (W == 1 && a->etab_in_lds) ? (size_t) (a->db.R + 1) * Q * 64 * 4 : 0;
-  if (W == 1) {
-    const int cap = ncu * resident_per_cu(vit_kernel<Q, W>, block, lds);
-    return std::max(1, std::min((a->n + 3) / 4, cap));
-  }
   return std::max(1, std::min(a->n, ncu * std::max(1, 8 / W)));
 }
 
 template <int Q, int W>
 static hipError_t launch_vit(const VitArgs *a, int ncu, hipStream_t st) {
-  const int block = (W == 1) ? 256 : 64 * W;
-  const size_t lds = (W == 1 && a->etab_in_lds) ? (size_t) (a->db.R + 1) * Q * 64 * 4 : 0;
   const int grid = grid_vit<Q, W>(a, ncu);
-  hipLaunchKernelGGL((vit_kernel<Q, W>), dim3(grid), dim3(block), lds, st, *a);
+  hipLaunchKernelGGL((vit_kernel<Q, W>), dim3(grid), dim3(64 * W), 0, st, *a);
   return hipGetLastError();
 }
 
@@ -386,107 +331,43 @@ static hipError_t dispatch_q(int Q, const VitArgs *a, int ncu, hipStream_t st, s
   return e;
 }
 
-// Q <= 16 registers per lane and W = 1, 2, 4 or 8 waves per target (M up to
+// Q <= 16 registers per lane and W = 2, 4 or 8 waves per target (M up to
 // 16384): the transitions of 16 registers still fit the 256 VGPRs a lane
 // has when a workgroup holds at most two waves per SIMD.
 extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, hipStream_t st) {
   if (Q < 1 || Q > 16 || a->n < 1) return hipErrorInvalidValue;
   auto qs = std::make_integer_sequence<int, 16>{};
   switch (W) {
-    case 1: return dispatch_q<1>(Q, a, ncu, st, qs);
     case 2: return dispatch_q<2>(Q, a, ncu, st, qs);
     case 4: return dispatch_q<4>(Q, a, ncu, st, qs);
     case 8: return dispatch_q<8>(Q, a, ncu, st, qs);
     default: return hipErrorInvalidValue;
   }
 }
-// ---------------------------------------------------------------------------
-// Short models (M <= 512): NG = 64/G targets per wave, G = 16 or 32 lanes
-// each (register j of lane c of a group holds k = j + s*Q + 1 for its slots
-// s = 2c, 2c+1), so the per-row fixed work -- the reductions, the shifts,
-// the special states, the delete-closure scan -- is shared by NG targets.
-// Cells are the same operations as vit_kernel in the same order.  The
-// special states of each group are wave-uniform values (one set per group);
-// a group whose target has ended (its rows are done, or it overflowed)
-// keeps computing padding rows but no longer updates its states, and its
-// closure is never applied.  The emission table (G lanes wide) is staged in
-// LDS; items are taken NG at a time by a static stride, longest first.
-// ---------------------------------------------------------------------------
+
+// The slots of a multi-wave launch: its workgroups, each running the targets
+// the host lists for it in a.uoff.
+template <int... Qs>
+static int slots_vit(int Q, int W, const VitArgs *a, int ncu, std::integer_sequence<int, Qs...>) {
+  int r = 0;
+  ((Q == Qs + 1 ? (r = (W == 2) ? grid_vit<Qs + 1, 2>(a, ncu)
+                    : (W == 4) ? grid_vit<Qs + 1, 4>(a, ncu) : grid_vit<Qs + 1, 8>(a, ncu), 0) : 0), ...);
+  return r;
+}
+
+extern "C" int msvk_vit_slots(int Q, int W, const VitArgs *a, int ncu) {
+  if (a->n < 1 || Q < 1 || Q > 16 || (W != 2 && W != 4 && W != 8)) return 0;
+  return slots_vit(Q, W, a, ncu, std::make_integer_sequence<int, 16>{});
+}
+
 namespace {
-
-// Slot s <- slot s-1 inside each G-lane group; the group's first slot takes
-// minus infinity.
-template <int G>
-__device__ __forceinline__ uint32_t shift_grp(uint32_t last, int lane) {
-  uint32_t prev;
-  if constexpr (G == 16) {
-    prev = __builtin_amdgcn_update_dpp(NEG2, last, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
-  } else {
-    prev = __builtin_amdgcn_mov_dpp(last, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
-    prev = ((lane & (G - 1)) == 0) ? NEG2 : prev;
-  }
-  return __builtin_amdgcn_alignbit(last, prev, 16);
-}
-
-// Ballot bits of one G-lane group.
-template <int G>
-constexpr uint64_t GMASK = (G == 64) ? ~0ull : ((1ull << G) - 1);
-
-// The delete closure the reference's lazy-F passes compute (vitfilter.c:
-// 203-227), for the lanes whose group runs them (mine).  Dr[0] already
-// holds the D entering each slot's first position.  One pass along each
-// slot (the reference's first pass); then, if the value leaving some slot
-// raises the next slot's first D, the carries across slots follow from
-// the max-plus scan over the lanes (tt: the lane's T terms, see tsc) and a
-// second pass applies them.  With slots of only Q positions the carries
-// cross slot boundaries often, so repeating whole passes (the reference
-// stops at the first position no slot raises) would take many passes.
-template <int G, int Q>
-__device__ __forceinline__ void close_d(uint32_t (&Dr)[Q], const uint32_t (&tDD)[Q], bool mine, const uint4 *tsc,
-                                        int c, int lane) {
-  if (mine) {
-#pragma unroll
-    for (int j = 1; j < Q; ++j) Dr[j] = maxs(Dr[j], adds(Dr[j - 1], tDD[j - 1]));
-  }
-  const uint32_t outl = adds(Dr[Q - 1], tDD[Q - 1]);
-  const uint32_t d0 = maxs(Dr[0], shift_grp<G>(outl, lane));
-  if (__builtin_amdgcn_ballot_w64(mine && d0 != Dr[0]) == 0) return;
-  const uint4 tt = tsc[c];
-  const int clo = lo16(outl);
-  const uint32_t tsl = tt.x, ts01 = tt.y, ts23 = tt.z, ts45 = tt.w;
-  // the lane's two slots as one map x -> max(c, x + T): low slot, then high
-  int cs = max(hi16(outl), clo - (int) (tsl >> 16));          // clo through the high slot
-  // inclusive scan over the lanes of each group (lane l: its group's lanes
-  // up to l composed, in order): row_shr 1/2/4/8 inside rows of 16 lanes,
-  // then row_bcast 15 (and 31) across rows; lanes without a source take the
-  // identity map (-32768, 0), which leaves them unchanged
-  cs = c_step_row<0x111, 0>(cs, ts01);
-  cs = c_step_row<0x112, 1>(cs, ts01);
-  cs = c_step_row<0x114, 0>(cs, ts23);
-  cs = c_step_row<0x118, 1>(cs, ts23);
-  if constexpr (G >= 32) cs = c_step<0x142, 0xa, 0>(cs, ts45);
-  if constexpr (G == 64) cs = c_step<0x143, 0xc, 1>(cs, ts45);
-  // exclusive: the previous lane's prefix; a group's first lane the identity
-  int ce;
-  if constexpr (G == 16) {
-    ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x111, 0xf, 0xf, false);
-  } else {
-    ce = __builtin_amdgcn_update_dpp(-32768, cs, 0x138, 0xf, 0xf, false);
-    if (G == 32 && (lane & 31) == 0) ce = -32768;
-  }
-  const int cin_hi = max(clo, ce - (int) (tsl & 0xffffu));
-  if (mine) {
-    uint32_t y = pack16(ce, cin_hi);
-#pragma unroll
-    for (int j = 0; j < Q; ++j) { Dr[j] = maxs(Dr[j], y); y = adds(y, tDD[j]); }
-  }
-}
 
 // Packed int16 max over each G-lane group; the group's last lane holds it.
 template <int G>
 __device__ __forceinline__ uint32_t group_max2(uint32_t v) {
   v = row_max2(v);
-  if constexpr (G == 32) v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x142 /* row_bcast:15 */, 0xa, 0xf, false));
+  if constexpr (G >= 32) v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x142 /* row_bcast:15 */, 0xa, 0xf, false));
+  if constexpr (G == 64) v = maxs(v, __builtin_amdgcn_update_dpp(NEG2, v, 0x143 /* row_bcast:31 */, 0xc, 0xf, false));
   return v;
 }
 
@@ -503,10 +384,128 @@ __device__ __forceinline__ int of_group(const int (&v)[NG], const int (&ge)[NG])
   return r;
 }
 
+// Ballot bits of one G-lane group.
+template <int G>
+constexpr uint64_t GMASK = (G == 64) ? ~0ull : ((1ull << G) - 1);
+
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// vit_band<G, Q, VOTE>: models up to 2048 positions, Q <= 16 registers per
+// lane, NG = 64/G targets per wave with G = 16, 32 or 64 lanes each (the
+// wave's targets share the per-row fixed work: the shifts, the votes, the
+// special states, the delete-closure scan).  Per row:
+//   - rows run four at a time, so a residue is one byte extract;
+//   - the next row's emissions are read from LDS while this row's special
+//     states and delete closure run;
+//   - the slot shifts keep their fill value (minus infinity at a group's
+//     first lane) in a register the shift never writes there, so a shift is
+//     one DPP move and one v_alignbit;
+//   - xE and Dmax are not reduced on every row: each lane keeps its share
+//     (the max over its registers) and the wave votes (ballot) against two
+//     thresholds per group, tE = min(max(xJ, xN) - e_loop, 32766) and
+//     tD = xB - ddbound_w.  Only a row where some share exceeds tE can
+//     overflow or raise xJ (and with it xB), so only then is xE reduced and
+//     the special states updated (vitfilter.c:176-181); xC, a running
+//     maximum nothing else reads, is each lane's largest share, reduced when
+//     the targets end;
+//   - when some group's lazy-F test holds (vitfilter.c:199) -- or on every
+//     row, without the Dmax shares and the vote, for profiles whose test
+//     holds on most rows (!VOTE) -- the whole wave runs the delete closure:
+//     the pass inside each slot, the max-plus scan over the slots with its T
+//     terms in registers, the carries applied.  The closure is applied to
+//     every group: for a group whose test fails it changes no M value,
+//     because every D->...->D->M path it adds is bounded by
+//     Dmax + ddbound_w + tBM <= xB + tBM, the B->M term of the same cell
+//     (the bound the reference's test rests on, vitfilter.c:183-197), and D
+//     itself enters later rows only through M;
+//   - a group whose target has ended (or overflowed) keeps computing padding
+//     rows but no longer votes or updates its states.
+// The emission table (G lanes wide) is staged in LDS; each wave runs the
+// units (NG targets) the host lists for it.
+// ---------------------------------------------------------------------------
+namespace {
+
+// Slot s <- slot s-1 inside each G-lane group.  keep holds minus infinity in
+// every group's first lane: G = 16 lanes are DPP rows, whose first lane the
+// row shift never writes, and lane 0 has no source in the wave shift; G = 32
+// selects it back for the second group.
+template <int G>
+__device__ __forceinline__ uint32_t shift_keep(uint32_t last, uint32_t &keep, int lane) {
+  if constexpr (G == 16) {
+    keep = __builtin_amdgcn_update_dpp(keep, last, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+  } else if constexpr (G == 64) {
+    keep = __builtin_amdgcn_update_dpp(keep, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+  } else {
+    const uint32_t prev = __builtin_amdgcn_update_dpp(keep, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    keep = (lane == 32) ? NEG2 : prev;
+  }
+  return __builtin_amdgcn_alignbit(last, keep, 16);
+}
+
+// One in-row scan step with its negated T term nt in a register: c <- max(c,
+// c_src - nt); a lane without a source reads 0 and holds nt = 65535.
+template <int CTRL>
+__device__ __forceinline__ int c_row(int c, int nt) {
+  return max(c, __builtin_amdgcn_mov_dpp(c, CTRL, 0xf, 0xf, true) - nt);
+}
+
+// The scan's negated T terms of one lane (see t_next, negsat, in_row_t).
+struct ScanT {
+  int lo, hi;            // the lane's low and high slot
+  int s1, s2, s4, s8;    // in-row steps
+  int s16;               // row_bcast:15 step (G >= 32)
+  int s32;               // row_bcast:31 step (G = 64)
+};
+
+// The delete closure over all lanes (vitfilter.c:203-227; see close_d).
 template <int G, int Q>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q <= 7 ? 4 : 1, 8))) vit_group(VitArgs a) {
+__device__ __forceinline__ void close_all(uint32_t (&Dr)[Q], const uint32_t (&tDD)[Q], const ScanT &t, int &kC,
+                                          int lane) {
+#pragma unroll
+  for (int j = 1; j < Q; ++j) Dr[j] = maxs(Dr[j], adds(Dr[j - 1], tDD[j - 1]));
+  const uint32_t outl = adds(Dr[Q - 1], tDD[Q - 1]);
+  const int clo = lo16(outl);
+  int cs = max(hi16(outl), clo - t.hi);              // the lane's two slots as one map
+  cs = c_row<0x111>(cs, t.s1);
+  cs = c_row<0x112>(cs, t.s2);
+  cs = c_row<0x114>(cs, t.s4);
+  cs = c_row<0x118>(cs, t.s8);
+  int ce;
+  if constexpr (G >= 32)
+    cs = max(cs, __builtin_amdgcn_update_dpp(-32768, cs, 0x142 /* row_bcast:15 */, 0xa, 0xf, false) - t.s16);
+  if constexpr (G == 64) {
+    cs = max(cs, __builtin_amdgcn_update_dpp(-32768, cs, 0x143 /* row_bcast:31 */, 0xc, 0xf, false) - t.s32);
+    // exclusive: the previous lane's prefix; lane 0 keeps -32768
+    kC = __builtin_amdgcn_update_dpp(kC, cs, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    ce = kC;
+  } else if constexpr (G == 32) {
+    // exclusive: the previous lane's prefix; a group's first lane -32768
+    const int prev = __builtin_amdgcn_update_dpp(kC, cs, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+    kC = (lane == 32) ? -32768 : prev;
+    ce = kC;
+  } else {
+    kC = __builtin_amdgcn_update_dpp(kC, cs, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+    ce = kC;
+  }
+  const int cin_hi = max(clo, ce - t.lo);
+  uint32_t y = __builtin_amdgcn_perm((uint32_t) cin_hi, (uint32_t) ce, 0x05040100u);   // (ce, cin_hi)
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    Dr[j] = maxs(Dr[j], y);
+    if (j < Q - 1) y = adds(y, tDD[j]);
+  }
+}
+
+}  // namespace
+
+// Workgroup size: 64-lane targets, whose tables are the largest, share one
+// LDS copy among eight waves.
+template <int G> constexpr int band_block() { return G == 64 ? 512 : 256; }
+
+template <int G, int Q, bool VOTE>
+__global__ void __launch_bounds__(band_block<G>()) __attribute__((amdgpu_waves_per_eu(Q <= 5 ? 4 : (Q <= 8 ? 3 : 2), 8)))
+vit_band(VitArgs a) {
   extern __shared__ uint32_t etab_lds[];
   constexpr int NG = 64 / G;
   const int lane = threadIdx.x & 63;
@@ -520,9 +519,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q <= 7
     asm volatile("" : "+v"(ge[gg]));
   }
 
-  // tBM stays in the table: it is read only on the rows where a group's xB
-  // changes (a few per target), and leaving it out of the registers keeps
-  // four waves per SIMD at Q = 7
   uint32_t tMM[Q], tIM[Q], tDM[Q], tMD[Q], tMI[Q], tII[Q], tDD[Q];
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
@@ -537,29 +533,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q <= 7
   int Tlo = 0, Thi = 0;                                // tDD summed over each slot
 #pragma unroll
   for (int j = 0; j < Q; ++j) { Tlo += lo16(tDD[j]); Thi += hi16(tDD[j]); }
-  // the lane scan's T terms, once: the lane's two slots and before each step
-  const int Ts0 = Tlo + Thi;
-  const int Ts1 = t_next<0x111, 0xf>(Ts0);
-  const int Ts2 = t_next<0x112, 0xf>(Ts1);
-  const int Ts3 = t_next<0x114, 0xf>(Ts2);
-  const int Ts4 = (G == 32) ? t_next<0x118, 0xf>(Ts3) : 0;
-  // kept in LDS, read on the rows that run the closure
-  __shared__ uint4 tsc[G];
-  if (threadIdx.x < G)
-    tsc[c] = make_uint4(negsat2(Tlo, Thi), negsat2(in_row_t<1>(Ts0, lane), in_row_t<2>(Ts1, lane)),
-                        negsat2(in_row_t<4>(Ts2, lane), in_row_t<8>(Ts3, lane)), negsat(Ts4));
+  ScanT st_;
+  {
+    const int Ts0 = Tlo + Thi;
+    const int Ts1 = t_next<0x111, 0xf>(Ts0);
+    const int Ts2 = t_next<0x112, 0xf>(Ts1);
+    const int Ts3 = t_next<0x114, 0xf>(Ts2);
+    const int Ts4 = (G >= 32) ? t_next<0x118, 0xf>(Ts3) : 0;
+    const int Ts5 = (G == 64) ? t_next<0x142, 0xa>(Ts4) : 0;
+    st_.lo = (int) negsat(Tlo);
+    st_.hi = (int) negsat(Thi);
+    st_.s1 = (int) negsat(in_row_t<1>(Ts0, lane));
+    st_.s2 = (int) negsat(in_row_t<2>(Ts1, lane));
+    st_.s4 = (int) negsat(in_row_t<4>(Ts2, lane));
+    st_.s8 = (int) negsat(in_row_t<8>(Ts3, lane));
+    st_.s16 = (int) negsat(Ts4);
+    st_.s32 = (int) negsat(Ts5);
+  }
 
-  const int ntab = (a.db.R + 1) * Q * G;
-  for (int i = threadIdx.x; i < ntab; i += blockDim.x) etab_lds[i] = a.etab[i];
+  const int nemit = (a.db.R + 1) * Q * G;
+  for (int i = threadIdx.x; i < nemit; i += blockDim.x) etab_lds[i] = a.etab[i];
   __syncthreads();
 
   const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
   const uint4 null4 = make_uint4(nb, nb, nb, nb);
   const int xN = a.base_w;                             // vitfilter.c:116; N loop cost is 0
-  const int nw = blockDim.x >> 6;
-  const int nwi = (a.n + NG - 1) / NG;
+  const int slot = blockIdx.x * (blockDim.x >> 6) + wave;
 
-  for (int wi = blockIdx.x * nw + wave; wi < nwi; wi += gridDim.x * nw) {
+  for (int wi = a.uoff[slot], wend = a.uoff[slot + 1]; wi < wend; ++wi) {
     const int it = wi * NG + g;
     const bool valid = it < a.n;
     const int64_t s = valid ? a.items[it] : 0;
@@ -569,10 +570,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q <= 7
     const int mvl = valid ? (int) a.move[it] : 0;
 
     int mv[NG], xB[NG], xJ[NG], st[NG];
-    int Lmax = 0;
+    int Lmax = 0, Lmin = 0x7fffffff;
 #pragma unroll
     for (int gg = 0; gg < NG; ++gg) {
-      Lmax = max(Lmax, __builtin_amdgcn_readlane(L, gg * G));
+      const int Lg = __builtin_amdgcn_readlane(L, gg * G);
+      Lmax = max(Lmax, Lg);
+      Lmin = min(Lmin, Lg);
       mv[gg] = __builtin_amdgcn_readlane(mvl, gg * G);
       xB[gg] = (int16_t) (xN + mv[gg]);                              // :117
       xJ[gg] = -32768; st[gg] = 0;
@@ -580,8 +583,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q <= 7
     uint32_t Mr[Q], Ir[Q], Dr[Q];
 #pragma unroll
     for (int j = 0; j < Q; ++j) Mr[j] = Ir[j] = Dr[j] = NEG2;        // :115
-    // Per group: xB + tBM per register (recomputed only when xB changes),
-    // and the two thresholds of the row votes (special_thresholds).
     uint32_t bm[Q];
     int tE, tD;
     auto refresh = [&]() {
@@ -598,84 +599,99 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q <= 7
       for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xb, xb), a.ttab[j * G + c]);
     };
     refresh();
-    int xacc = -32768;                  // this lane's largest row maximum (xC at the end)
+    uint32_t xacc = NEG2;               // low half: this lane's largest row maximum (xC at the end)
+    uint32_t kM = NEG2, kI = NEG2, kD = NEG2, kD0 = NEG2;   // shift fills (see shift_keep)
+    int kC = -32768;                    // the closure scan's exclusive fill
+    // The emissions of the next row are read from LDS while the current
+    // row's special states and delete closure run (held behind compiler
+    // fences so the reads stay there).
+    uint32_t e[Q];
+    auto load_e = [&](int x) {
+      const uint32_t *ep = etab_lds + x * (Q * G) + c;
+#pragma unroll
+      for (int j = 0; j < Q; ++j) e[j] = ep[j * G];
+    };
 
-    uint4 cnext = *reinterpret_cast<const uint4 *>(p);
-    for (int i0 = 0; i0 < Lmax; i0 += 16) {
-      const uint4 ch = (i0 < Lpad) ? cnext : null4;
-      const int i1 = i0 + 16;
-      cnext = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
-      uint32_t w0 = ch.x, w1 = ch.y, w2 = ch.z, w3 = ch.w;   // 16 residues, consumed from the bottom
-      const int nr = min(16, Lmax - i0);
-      for (int rr = 0; rr < nr; ++rr) {
-        const int i = i0 + rr;
-        const int x = (int) (w0 & 0xffu);
-        w0 = __builtin_amdgcn_alignbit(w1, w0, 8);
-        w1 = __builtin_amdgcn_alignbit(w2, w1, 8);
-        w2 = __builtin_amdgcn_alignbit(w3, w2, 8);
-        w3 >>= 8;
-        const uint32_t *ep = etab_lds + x * Q * G + c;
-        uint32_t e[Q];
+    // One row; all_alive: every group's target has this row (no masks).
+    auto row = [&](int xnext, bool all_alive, bool alive) {
+      const uint32_t mpv = shift_keep<G>(Mr[Q - 1], kM, lane);      // :139-141
+      const uint32_t ipv = shift_keep<G>(Ir[Q - 1], kI, lane);
+      const uint32_t dpv = shift_keep<G>(Dr[Q - 1], kD, lane);
+      uint32_t xEv = NEG2, Dmv = NEG2, dcv = NEG2;
+      // Registers in descending order, each updated in place (see vit_group).
 #pragma unroll
-        for (int j = 0; j < Q; ++j) e[j] = ep[j * G];
-        uint32_t mpv = shift_grp<G>(Mr[Q - 1], lane);               // :139-141
-        uint32_t ipv = shift_grp<G>(Ir[Q - 1], lane);
-        uint32_t dpv = shift_grp<G>(Dr[Q - 1], lane);
-        uint32_t xEv = NEG2, Dmv = NEG2, dcv = NEG2;
-        // Registers in descending order, each updated in place (no copies
-        // across the row loop): register j reads register j-1's old M, I, D
-        // (not yet updated) and its own old M, I; the D entering register
-        // j+1 is this register's new M + tMD, written after register j+2
-        // has read the old one.  The same operations per cell as :146-172.
-#pragma unroll
-        for (int j = Q - 1; j >= 0; --j) {
-          const uint32_t pm = j ? Mr[j - 1] : mpv, pi = j ? Ir[j - 1] : ipv, pd = j ? Dr[j - 1] : dpv;
-          uint32_t sv = bm[j];                                       // :146-150
-          sv = maxs(sv, adds(pm, tMM[j]));
-          sv = maxs(sv, adds(pi, tIM[j]));
-          sv = maxs(sv, adds(pd, tDM[j]));
-          sv = adds(sv, e[j]);
-          xEv = maxs(xEv, sv);                                       // :151
-          Ir[j] = maxs(adds(Mr[j], tMI[j]), adds(Ir[j], tII[j]));    // :171-172
-          Mr[j] = sv;                                                // :161
-          const uint32_t dn = adds(sv, tMD[j]);                      // :167
-          Dmv = maxs(Dmv, dn);                                       // :168
-          if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;              // :162
-        }
-        // This lane's share of the row's xE and Dmax (:176, :198), voted on
-        // instead of reduced: the exact special-state update runs only when
-        // some lane of a group could overflow or move the group's B.
-        const uint32_t hv = halves_max(xEv, Dmv);
-        const int xe = lo16(hv), dm = hi16(hv);
-        const bool alive = i < L;                                    // the lane's target has this row
-        xacc = max(xacc, alive ? xe : -32768);
-        const uint64_t mE = __builtin_amdgcn_ballot_w64(alive && xe > tE);
-        if (mE) {
-          const uint32_t r = group_max2<G>(hv);
-#pragma unroll
-          for (int gg = 0; gg < NG; ++gg) {
-            if (((mE >> (gg * G)) & GMASK<G>) == 0) continue;
-            const int xE = lo16(__builtin_amdgcn_readlane(r, gg * G + G - 1));
-            if (xE >= 32767) { st[gg] = 16; continue; }                  // :177
-            xJ[gg] = (int16_t) max(xJ[gg], xE + a.e_loop);               // :180 (J loop 0)
-            xB[gg] = (int16_t) max(xJ[gg] + mv[gg], xN + mv[gg]);        // :181
-          }
-          refresh();
-        }
-        Dr[0] = shift_grp<G>(dcv, lane);                             // :230-231 / :203-204
-        const uint64_t mD = __builtin_amdgcn_ballot_w64(alive && dm > tD);   // :199, per group
-        if (mD) {
-          int cond[NG];
-#pragma unroll
-          for (int gg = 0; gg < NG; ++gg) cond[gg] = (((mD >> (gg * G)) & GMASK<G>) != 0 && st[gg] == 0) ? 1 : 0;
-          const bool mine = of_group<NG>(cond, ge) != 0;
-          close_d<G, Q>(Dr, tDD, mine, tsc, c, lane);                 // :203-227
-        }
+      for (int j = Q - 1; j >= 0; --j) {
+        const uint32_t pm = j ? Mr[j - 1] : mpv, pi = j ? Ir[j - 1] : ipv, pd = j ? Dr[j - 1] : dpv;
+        uint32_t sv = bm[j];                                         // :146-150
+        sv = maxs(sv, adds(pm, tMM[j]));
+        sv = maxs(sv, adds(pi, tIM[j]));
+        sv = maxs(sv, adds(pd, tDM[j]));
+        sv = adds(sv, e[j]);
+        xEv = maxs(xEv, sv);                                         // :151
+        Ir[j] = maxs(adds(Mr[j], tMI[j]), adds(Ir[j], tII[j]));      // :171-172
+        Mr[j] = sv;                                                  // :161
+        const uint32_t dn = adds(sv, tMD[j]);                        // :167
+        if constexpr (VOTE) Dmv = maxs(Dmv, dn);                     // :168
+        if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;                // :162
       }
+      asm volatile("" ::: "memory");
+      load_e(xnext);
+      asm volatile("" ::: "memory");
+      const uint32_t hv = halves_max(xEv, Dmv);                      // (xE share, Dmax share)
+      // Votes against per-lane thresholds; a lane past its target's end
+      // takes thresholds no value reaches.
+      const int te = (all_alive || alive) ? tE : 0x7fffffff;
+      const int td = (all_alive || alive) ? tD : 0x7fffffff;
+      xacc = maxs(xacc, (all_alive || alive) ? hv : NEG2);
+      const uint64_t mE = __builtin_amdgcn_ballot_w64(lo16(hv) > te);
+      if (mE) {
+        const uint32_t r = group_max2<G>(hv);
+#pragma unroll
+        for (int gg = 0; gg < NG; ++gg) {
+          // every group's states, unchanged where its lanes did not vote
+          const bool hit = ((mE >> (gg * G)) & GMASK<G>) != 0;
+          const int xE = lo16(__builtin_amdgcn_readlane(r, gg * G + G - 1));
+          const bool over = hit && xE >= 32767;                             // :177
+          const bool upd = hit && !over;
+          st[gg] = over ? 16 : st[gg];
+          xJ[gg] = upd ? (int) (int16_t) max(xJ[gg], xE + a.e_loop) : xJ[gg];   // :180 (J loop 0)
+          xB[gg] = upd ? (int) (int16_t) max(xJ[gg] + mv[gg], xN + mv[gg]) : xB[gg];   // :181
+        }
+        refresh();
+      }
+      Dr[0] = shift_keep<G>(dcv, kD0, lane);                         // :230-231 / :203-204
+      // :199 for some group (VOTE), or on every row: profiles whose test
+      // holds on most rows skip the Dmax share and the vote
+      if (!VOTE || __builtin_amdgcn_ballot_w64(hi16(hv) > td))
+        close_all<G, Q>(Dr, tDD, st_, kC, lane);                     // :203-227
+    };
+
+    // Chunks of 16 residues, read two chunks ahead; the loaded value is
+    // replaced by the null row only where it is used.
+    const uint4 c0 = *reinterpret_cast<const uint4 *>(p);
+    const uint4 c1 = *reinterpret_cast<const uint4 *>(p + (16 < Lpad ? 16 : 0));
+    uint4 cur = (0 < Lpad) ? c0 : null4;
+    uint4 nxt = (16 < Lpad) ? c1 : null4;
+    load_e((int) (cur.x & 0xffu));
+    for (int i0 = 0; i0 < Lmax; i0 += 16) {
+      const int i2 = i0 + 32;
+      const uint4 raw = *reinterpret_cast<const uint4 *>(p + (i2 < Lpad ? i2 : 0));
+      uint32_t w0 = cur.x, w1 = cur.y, w2 = cur.z, w3 = cur.w;   // 16 residues, four per word
+      // Rows past Lmax up to the next multiple of four are computed on the
+      // residues the padded chunk holds; no group is alive on them.
+      const int nq = min(4, (Lmax - i0 + 3) >> 2);
+      for (int qd = 0; qd < nq; ++qd) {
+        const int ib = i0 + 4 * qd;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          row((int) (((r < 3 ? w0 >> (8 * r + 8) : w1)) & 0xffu), false, ib + r < L);
+        w0 = w1; w1 = w2; w2 = w3; w3 = nxt.x;
+      }
+      cur = nxt;
+      nxt = (i2 < Lpad) ? raw : null4;
     }
-    // xC = max over the target's rows of xE + e_move (:179, C loop 0): the
-    // group's largest row maximum plus e_move, floored at xC's start value
-    const uint32_t xm = group_max2<G>(pack16(xacc, xacc));
+    // xC = max over the target's rows of xE + e_move (:179, C loop 0)
+    const uint32_t xm = group_max2<G>(xacc);
 #pragma unroll
     for (int gg = 0; gg < NG; ++gg) {
       const int xC = max(-32768, lo16(__builtin_amdgcn_readlane(xm, gg * G + G - 1)) + a.e_move);
@@ -685,68 +701,55 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q <= 7
 }
 
 template <int G, int Q>
-static int grid_group(const VitArgs *a, int ncu) {
+static size_t band_lds(const VitArgs *a) { return (size_t) (a->db.R + 1) * Q * G * 4; }
+
+template <int G, int Q, bool VOTE>
+static int grid_band(const VitArgs *a, int ncu) {
   constexpr int NG = 64 / G;
-  const size_t lds = (size_t) (a->db.R + 1) * Q * G * 4;
   const int waves = (a->n + NG - 1) / NG;
-  const int cap = ncu * resident_per_cu(vit_group<G, Q>, 256, lds);
-  return std::max(1, std::min((waves + 3) / 4, cap));
+  constexpr int nw = band_block<G>() / 64;
+  const int cap = ncu * resident_per_cu(vit_band<G, Q, VOTE>, band_block<G>(), band_lds<G, Q>(a));
+  return std::max(1, std::min((waves + nw - 1) / nw, cap));
 }
 
-template <int G, int Q>
-static hipError_t launch_group(const VitArgs *a, int ncu, hipStream_t st) {
-  const size_t lds = (size_t) (a->db.R + 1) * Q * G * 4;
-  const int grid = grid_group<G, Q>(a, ncu);
-  hipLaunchKernelGGL((vit_group<G, Q>), dim3(grid), dim3(256), lds, st, *a);
+template <int G, int Q, bool VOTE>
+static hipError_t launch_band(const VitArgs *a, int ncu, hipStream_t st) {
+  const int grid = grid_band<G, Q, VOTE>(a, ncu);
+  const size_t lds = band_lds<G, Q>(a);
+  hipLaunchKernelGGL((vit_band<G, Q, VOTE>), dim3(grid), dim3(band_block<G>()), lds, st, *a);
   return hipGetLastError();
 }
 
-template <int G, int... Qs>
-static hipError_t dispatch_group(int Q, const VitArgs *a, int ncu, hipStream_t st, std::integer_sequence<int, Qs...>) {
+template <int G, bool VOTE, int... Qs>
+static hipError_t dispatch_band(int Q, const VitArgs *a, int ncu, hipStream_t st, std::integer_sequence<int, Qs...>) {
   hipError_t e = hipErrorInvalidValue;
-  ((Q == Qs + 1 ? (e = launch_group<G, Qs + 1>(a, ncu, st), 0) : 0), ...);
+  ((Q == Qs + 1 ? (e = launch_band<G, Qs + 1, VOTE>(a, ncu, st), 0) : 0), ...);
   return e;
 }
 
-extern "C" hipError_t msvk_launch_vit_group(int G, int Q, const VitArgs *a, int ncu, hipStream_t st) {
-  if (Q < 1 || Q > 8 || a->n < 1) return hipErrorInvalidValue;
-  auto qs = std::make_integer_sequence<int, 8>{};
-  switch (G) {
-    case 16: return dispatch_group<16>(Q, a, ncu, st, qs);
-    case 32: return dispatch_group<32>(Q, a, ncu, st, qs);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-// The static schedule of a launch: item position q (in a.items order) goes
-// to slot (q / per) % slots, where per targets share a wave (64/G for the
-// grouped shapes, else 1) and slots is the number of waves (workgroups for
-// multi-wave targets) striding over the items.  The host orders the items
-// so that every slot gets about the same number of rows.
-template <int... Qs>
-static int slots_vit(int Q, int W, const VitArgs *a, int ncu, std::integer_sequence<int, Qs...>) {
+template <int G, bool VOTE, int... Qs>
+static int slots_band(int Q, const VitArgs *a, int ncu, std::integer_sequence<int, Qs...>) {
   int r = 0;
-  ((Q == Qs + 1 ? (r = (W == 1) ? grid_vit<Qs + 1, 1>(a, ncu) * 4
-                    : (W == 2) ? grid_vit<Qs + 1, 2>(a, ncu)
-                    : (W == 4) ? grid_vit<Qs + 1, 4>(a, ncu) : grid_vit<Qs + 1, 8>(a, ncu), 0) : 0), ...);
-  return r;
-}
-template <int G, int... Qs>
-static int slots_group(int Q, const VitArgs *a, int ncu, std::integer_sequence<int, Qs...>) {
-  int r = 0;
-  ((Q == Qs + 1 ? (r = grid_group<G, Qs + 1>(a, ncu) * 4, 0) : 0), ...);
+  ((Q == Qs + 1 ? (r = grid_band<G, Qs + 1, VOTE>(a, ncu) * (band_block<G>() / 64), 0) : 0), ...);
   return r;
 }
 
-extern "C" int msvk_vit_slots(int G, int Q, int W, const VitArgs *a, int ncu, int *per) {
-  if (a->n < 1) return 0;
-  if (G < 64) {
-    *per = 64 / G;
-    if (Q < 1 || Q > 8) return 0;
-    return G == 16 ? slots_group<16>(Q, a, ncu, std::make_integer_sequence<int, 8>{})
-                   : slots_group<32>(Q, a, ncu, std::make_integer_sequence<int, 8>{});
-  }
-  *per = 1;
-  if (Q < 1 || Q > 16) return 0;
-  return slots_vit(Q, W, a, ncu, std::make_integer_sequence<int, 16>{});
+// Q <= 16 registers per lane, G = 16, 32 or 64 lanes per target; vote: the rows
+// vote on the lazy-F test (else every row closes the delete paths).
+extern "C" hipError_t msvk_launch_vit_band(int G, int Q, int vote, const VitArgs *a, int ncu, hipStream_t st) {
+  if (Q < 1 || Q > 16 || a->n < 1) return hipErrorInvalidValue;
+  auto qs = std::make_integer_sequence<int, 16>{};
+  if (G == 16) return vote ? dispatch_band<16, true>(Q, a, ncu, st, qs) : dispatch_band<16, false>(Q, a, ncu, st, qs);
+  if (G == 32) return vote ? dispatch_band<32, true>(Q, a, ncu, st, qs) : dispatch_band<32, false>(Q, a, ncu, st, qs);
+  if (G == 64) return vote ? dispatch_band<64, true>(Q, a, ncu, st, qs) : dispatch_band<64, false>(Q, a, ncu, st, qs);
+  return hipErrorInvalidValue;
+}
+
+extern "C" int msvk_vit_band_slots(int G, int Q, int vote, const VitArgs *a, int ncu, int *per) {
+  if (a->n < 1 || Q < 1 || Q > 16 || (G != 16 && G != 32 && G != 64)) return 0;
+  *per = 64 / G;
+  auto qs = std::make_integer_sequence<int, 16>{};
+  if (G == 16) return vote ? slots_band<16, true>(Q, a, ncu, qs) : slots_band<16, false>(Q, a, ncu, qs);
+  if (G == 32) return vote ? slots_band<32, true>(Q, a, ncu, qs) : slots_band<32, false>(Q, a, ncu, qs);
+  return vote ? slots_band<64, true>(Q, a, ncu, qs) : slots_band<64, false>(Q, a, ncu, qs);
 }

@@ -254,6 +254,16 @@ void msvg_test_alloc_budget(long k);
  * per workgroup for profiles of length M; 0 restores the built-in choice.
  * The autotuner in the Python package persists its findings through this. */
 int msvg_set_tile(msvg_ctx *ctx, int32_t M_lo, int32_t M_hi, int32_t lanes, int32_t waves);
+/* ViterbiFilter shape for models with M_lo <= M <= M_hi, taking effect at the
+ * next msvg_profile_set_viterbi: lanes per target 16 or 32 (several targets
+ * per wave) or 64 (waves 1, 2, 4 or 8 per target).  lanes = 0 clears. */
+int msvg_set_vit_tile(msvg_ctx *ctx, int32_t M_lo, int32_t M_hi, int32_t lanes, int32_t waves);
+/* How the ViterbiFilter rows of profiles set after this call treat the
+ * lazy-F test (vitfilter.c:199): -1 (default) by the profile's estimated
+ * rate (msvh_vit_lazyf_rate: below 0.75 the rows vote on the test, else
+ * every row closes the delete paths), 0 every row closes, 1 every row votes.
+ * Scores and statuses are the same in all three. */
+int msvg_set_vit_vote(msvg_ctx *ctx, int32_t mode);
 /* Tail guard (on by default): where a database's longest target would make
  * the tile's launch last longer than the whole database's work at the
  * device's rate (small databases holding a long target), the batched mode

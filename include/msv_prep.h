@@ -108,6 +108,12 @@ int    msvh_vit_move(float scale_w, float nj, int L);
 /* p7_ViterbiFilter's score from its final xC (vitfilter.c:240-248): -inf when
  * xC is -32768. */
 float  msvh_vit_score(int xC, int move, int base_w, float scale_w);
+/* Fraction of rows on which p7_ViterbiFilter's lazy-F test (vitfilter.c:199)
+ * holds, on one pseudo-random target of `rows` residues: a planning estimate
+ * (the kernels vote on the test per row when it is rare, and close the
+ * delete paths on every row when it is not; scores are the same). */
+double msvh_vit_lazyf_rate(const int16_t *rwv, const int16_t *twv, int M, int Kp, int e_loop, int base_w,
+                           int ddbound_w, int move, int rows);
 /* mu from N ViterbiFilter scores of i.i.d. targets of length L as
  * p7_ViterbiMu computes it (evalues.c:313-352); ERANGE scores count as
  * (32767 - base_w) / scale_w. */

@@ -3,9 +3,10 @@
 Bar: bit-identical p7_ViterbiFilter scores and statuses (integer DP; the
 float score is the reference's expression of the integer xC).  Checked
 against the golden vectors written from the compiled reference, against the
-reference itself (oracle/_ref) for model lengths that exercise every wave
-count of the kernel (1, 2, 4 and 8 waves per target, M up to 16384), and
-the drop-in and calibration entry points.  Targets of one wave have
+reference itself (oracle/_ref) for model lengths that exercise every shape
+(16-, 32- and 64-lane targets, 2, 4 and 8 waves per target, M up to 16384)
+and both ways the rows treat the lazy-F test, and the drop-in and
+calibration entry points.  Targets of one wave have
 different lengths (some end, some overflow, while the others go on).
 """
 import numpy as np
@@ -49,13 +50,13 @@ def test_golden_vectors(vit_golden):
         ctx.close()
 
 
-@pytest.mark.parametrize("M", [1, 7, 33, 128, 129, 200, 256, 257, 400, 480, 481, 512, 513, 1024, 1025, 2048, 2049,
-                               4096, 4097, 8192, 8193, 9000, 16384])
+@pytest.mark.parametrize("M", [1, 7, 33, 128, 129, 200, 256, 257, 400, 480, 481, 512, 513, 640, 1024, 1025, 1536,
+                               2048, 2049, 4096, 4097, 8192, 8193, 9000, 16384])
 def test_every_wave_count_vs_reference(M):
     """Model lengths at the edges of each (lanes, waves, registers) shape
     against the compiled reference's p7_ViterbiFilter: four 16-lane targets
-    per wave up to M = 256, two 32-lane targets up to 512, one wave per
-    target up to 2048 (up to 16 registers per lane), then 2, 4 and 8 waves
+    per wave up to M = 512, two 32-lane targets up to 1024, one wave per
+    target up to 2048 (1 to 16 registers per lane), then 2, 4 and 8 waves
     per target."""
     from oracle import Reference
     R = Reference()
@@ -80,6 +81,53 @@ def test_every_wave_count_vs_reference(M):
     sc, st = ctx.vit_search(pid, did, np.arange(db.n))
     rst, rsc = rp.vit_batch(db.res, db.off, db.lens, nthreads=8)
     assert _same(sc, st, rsc, rst)
+    ctx.close()
+
+
+def _skewed_targets(seed, n, n_long):
+    """n i.i.d. amino targets, log-normal lengths (median 350, sigma 0.7,
+    clipped 20-3000), then n_long of 3,000-9,000 residues and one empty."""
+    rng = np.random.default_rng(seed)
+    lens = np.clip(np.exp(rng.normal(np.log(350), 0.7, n)), 20, 3000).astype(int)
+    lens = np.concatenate([lens, rng.integers(3000, 9000, n_long), [0]])
+    return [rng.integers(0, 20, int(L)).astype(np.uint8) for L in lens]
+
+
+@pytest.mark.parametrize("vote", [-1, 0, 1])
+def test_skewed_batch_vs_reference_every_vote_mode(vote):
+    """More wave units than the launch has slots (several units per slot,
+    dealt greedily by rows), targets of very different lengths, an empty one:
+    one sampled profile per band shape (16, 32 and 64 lanes) and one
+    multi-wave profile, in one multi-profile call, against the compiled
+    reference's p7_ViterbiFilter, with the rows voting on the lazy-F test
+    (1), closing the delete paths on every row (0), or chosen by the
+    profile's estimated rate (-1)."""
+    from oracle import Reference
+    R = Reference()
+    seqs = _skewed_targets(11, 12000, 12)
+    db = ca.Database.from_list(seqs)
+    ctx = ca.MSVContext(0, tiles=None)
+    ctx.set_vit_vote(vote)
+    did = ctx.add_database(db)
+    refs, pids, lists = [], [], []
+    rng = np.random.default_rng(5)
+    for k, M in enumerate([400, 900, 1700, 2300]):
+        rp = R.sample(700 + k, M, calibrate=False)
+        f = rp.vit_fields()
+        vd = {"rwv": f["rwv_pad"][:, :M], "twv": f["twv_pad"][:, :M], "e_loop": f["e_loop"], "e_move": f["e_move"],
+              "base_w": f["base_w"], "ddbound_w": f["ddbound_w"], "scale_w": f["scale_w"], "nj": f["nj"]}
+        pid = ctx.add_profile(_placeholder_msv(M, 29))
+        ctx.set_viterbi(pid, ca.ViterbiProfile.from_dict(vd))
+        t = np.arange(db.n) if k == 0 else np.sort(rng.choice(db.n, 3000, replace=False))
+        if k > 0:
+            t = np.concatenate([t, np.arange(12000, db.n)])     # every long target and the empty one
+        refs.append(rp); pids.append(pid); lists.append(t.astype(np.int64))
+    print(f"vote={vote} targets={db.n} lists={[len(t) for t in lists]}", flush=True)
+    many = ctx.vit_search_many(pids, did, lists)
+    for rp, t, (sc, st) in zip(refs, lists, many):
+        sub = ca.Database.from_list([seqs[i] for i in t])
+        rst, rsc = rp.vit_batch(sub.res, sub.off, sub.lens, nthreads=8)
+        assert _same(sc, st, rsc, rst)
     ctx.close()
 
 

@@ -16,7 +16,8 @@ waiting for a slot folded into one search; for each the capacity, p50/p99
 over a drained sweep at 50/75/90% of it, and p50/p99 at --rate req/s for
 --serving-duration s).  For c2 and c3 the line also carries the ViterbiFilter
 stage on the MSV survivors (vit_stage: kernel time, GCUPS, roofline, the
-reference's SSE p7_ViterbiFilter on a sample with a bit-identity check).
+reference's SSE p7_ViterbiFilter on a sample with a bit-identity check); the
+c3 line also carries it on the c2 survivors (vit_stage_c2).
 
 Workloads (synthetic data generated on the host from seeds by
 cudahmmsearch_amd.workloads; Easel-compatible generators):
@@ -189,10 +190,29 @@ def vit_stage(ca, ctx, profs, pids, did, db, reps=3, cpu=True):
         ach = VIT_INSTR_PER_CELL * tot_cells / (tot_ms / 1e3)
         out["roofline"] = {"bound": "valu", "achieved": round(ach / 1e12, 3),
                            "peak": round(PEAK_LANE_INSTR / 1e12, 3), "unit": "T lane-instr/s",
-                           "frac": round(ach / PEAK_LANE_INSTR, 4), "kernel": "vit_group / vit_kernel (one launch per shape)",
+                           "frac": round(ach / PEAK_LANE_INSTR, 4),
+                           "kernel": "vit_band (M <= 2048) / vit_kernel (multi-wave, M > 2048), one launch per profile",
                            "algorithmic": f"14 int16 ops per DP cell = {VIT_INSTR_PER_CELL} packed lane-instructions"}
     if cpu and first is not None:
         out["cpu_baseline"] = vit_cpu_baseline(ca, db, *first)
+    return out
+
+
+def vit_stage_of(name, ca, workloads, cpu=True):
+    """vit_stage on another workload's MSV survivors (its own context and
+    database, built and searched in batched mode first)."""
+    profs, db, desc = workloads.make_workload(name)
+    ctx = ca.MSVContext(0)
+    pids = []
+    for p in profs:
+        pid = ctx.add_profile(p)
+        p.mu = ctx.calibrate(pid, p.lam)
+        pids.append(pid)
+    did = ctx.add_database(db)
+    ctx.search_device(pids, did, mode=ca.MODE_BATCHED)
+    out = {"workload": f"{name}: {desc}"}
+    out.update(vit_stage(ca, ctx, profs, pids, did, db, cpu=cpu))
+    ctx.close()
     return out
 
 
@@ -559,6 +579,7 @@ def main():
     ap.add_argument("--slots", type=int, default=2, help="serving: searches on the GPU at once")
     ap.add_argument("--no-vit", action="store_true", help="skip the ViterbiFilter stage on the MSV survivors")
     ap.add_argument("--no-serving", action="store_true", help="c3: skip the serving sub-object")
+    ap.add_argument("--no-vit-c2", action="store_true", help="c3: skip the ViterbiFilter stage on the c2 survivors")
     ap.add_argument("--serving-duration", type=float, default=10.0, help="c3: seconds of arrivals in the serving phase")
     ap.add_argument("--scan-profiles", type=int, default=10000, help="scan: profiles in the scanned database")
     ap.add_argument("--scan-queries", type=int, default=64, help="scan: query sequences")
@@ -680,6 +701,9 @@ def main():
         log(f"vit stage: {out['vit_stage']}")
     ctx.close()
     del db
+    if not args.no_vit and args.workload == "c3" and not args.no_vit_c2:
+        out["vit_stage_c2"] = vit_stage_of("c2", ca, workloads, cpu=not args.no_cpu)
+        log(f"vit stage (c2 survivors): {out['vit_stage_c2']}")
     if args.workload == "c3" and not args.no_serving:
         out["serving"] = serving_run(ca, 10_000_000, args.rate, args.serving_duration, args.slots,
                                      ["persistent", "batched"], 6.0)

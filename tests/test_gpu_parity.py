@@ -397,7 +397,8 @@ def test_tail_guard(ctx):
 
 def test_scan_direction_equals_search(ctx):
     """msvg_scan (many profiles, one query; one workgroup per profile) gives
-    each profile what msvg_search gives on a one-target database: profiles
+    each profile what msvg_search gives on a one-target database, and what
+    the oracle gives: profiles
     of every register class (M = 1 .. 3000, beyond LDS at 2,561 and up,
     which take the search path), against random queries, planted homologs
     (J fires, overflow), a one-residue and an empty query."""
@@ -413,6 +414,7 @@ def test_scan_direction_equals_search(ctx):
     queries = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in (350, 1200, 37)]
     queries += ca.core_emit(profs[9].hmm, 78, 2) + ca.core_emit(profs[3].hmm, 79, 1)
     queries += [np.array([5], np.uint8), np.zeros(0, np.uint8)]
+    O = Oracle()
     for q in queries:
         print(f"query L={q.size} profiles={len(pids)}", flush=True)
         u, s, ps = ctx.scan(pids, q)
@@ -421,6 +423,12 @@ def test_scan_direction_equals_search(ctx):
         ctx.remove_database(did)
         assert np.array_equal(u.view(np.uint32), ru[:, 0].view(np.uint32))
         assert np.array_equal(s, rs[:, 0]) and np.array_equal(ps, rp[:, 0])
+        # and directly against the oracle
+        one = ca.Database.from_list([q])
+        for j, p in enumerate(profs):
+            ost, ousc = O.msv_batch(p.as_dict(), one.res, one.off, one.lens)
+            assert s[j] == ost[0] and u[j].view(np.uint32) == ousc[0].view(np.uint32), (j, q.size)
+            assert ps[j] == _oracle_pass(O, ost, ousc, one.lens, p.mu, p.lam)[0], (j, q.size)
     st = ctx.stats()
     assert st["cells"] == 0 or st["cells"] > 0
     with pytest.raises(ca.MSVError):
@@ -493,7 +501,8 @@ def test_scan_guard_regime_fallback(ctx):
 def test_scan_many_equals_search(ctx):
     """msvg_scan_many (a batch of queries; each workgroup's waves share its
     profile's staged table) gives, for every (query, profile), what
-    msvg_search gives over a database of the same queries: ragged lengths,
+    msvg_search gives over a database of the same queries and what the oracle
+    gives: ragged lengths,
     empty queries, planted homologs, and a profile beyond LDS that takes the
     search path; a batch of one equals msvg_scan."""
     rng = np.random.default_rng(91)
@@ -522,6 +531,13 @@ def test_scan_many_equals_search(ctx):
         ctx.remove_database(did)
         assert np.array_equal(u.view(np.uint32), ru.T.view(np.uint32))
         assert np.array_equal(s, rs.T) and np.array_equal(ps, rp.T)
+        # and directly against the oracle
+        O = Oracle()
+        for j, p in enumerate(profs):
+            ost, ousc = O.msv_batch(p.as_dict(), db.res, db.off, db.lens)
+            assert np.array_equal(s[:, j].astype(np.int32), ost) and np.array_equal(u[:, j].view(np.uint32),
+                                                                                  ousc.view(np.uint32)), j
+            assert np.array_equal(ps[:, j], _oracle_pass(O, ost, ousc, db.lens, p.mu, p.lam)), j
         if nq == 1:
             u1, s1, p1 = ctx.scan(pids, qs[0])
             assert np.array_equal(u1.view(np.uint32), u[0].view(np.uint32)) and np.array_equal(p1, ps[0])

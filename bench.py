@@ -191,7 +191,7 @@ def vit_stage(ca, ctx, profs, pids, did, db, reps=3, cpu=True):
         out["roofline"] = {"bound": "valu", "achieved": round(ach / 1e12, 3),
                            "peak": round(PEAK_LANE_INSTR / 1e12, 3), "unit": "T lane-instr/s",
                            "frac": round(ach / PEAK_LANE_INSTR, 4),
-                           "kernel": "vit_band (M <= 2048) / vit_kernel (multi-wave, M > 2048), one launch per profile",
+                           "kernel": "vit_band (one launch per lanes/registers class of the profiles) / vit_kernel (multi-wave, one launch per profile)",
                            "algorithmic": f"14 int16 ops per DP cell = {VIT_INSTR_PER_CELL} packed lane-instructions"}
     if cpu and first is not None:
         out["cpu_baseline"] = vit_cpu_baseline(ca, db, *first)

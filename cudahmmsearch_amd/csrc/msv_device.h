@@ -216,6 +216,13 @@ struct SegArgs {
 //                null row is never read: rows run over the real residues).
 // Result per item: (status << 16) | (uint16) xC, status 0 or 16 (eslERANGE);
 // the host turns xC into the score with the reference's float expression.
+struct VitProfDev {            // one profile of a vit_band launch
+  const uint32_t *etab;        // emissions, E layout above
+  const uint32_t *ttab;        // transitions, T layout above
+  int32_t e_loop, e_move, base_w, ddbound_w;
+  int32_t end;                 // its items end at this position of the item arrays
+};
+
 struct VitArgs {
   const uint32_t *etab;
   const uint32_t *ttab;
@@ -227,6 +234,13 @@ struct VitArgs {
   int32_t e_loop, e_move, base_w, ddbound_w;
   int32_t etab_in_lds;         // one-wave shape: stage the emission table in LDS
   const int32_t *uoff;         // slot k (a wave, or a workgroup of a multi-wave
-                               // shape) runs units [uoff[k], uoff[k+1]) of the
-                               // items; a unit is one wave's targets
+                               // shape) runs units [uoff[k], uoff[k+1]); a unit
+                               // is one wave's targets
+  // vit_band only (several profiles per launch): workgroup b works on
+  // profile profs[wg_prof[b]]; unit u's targets are the items at positions
+  // ustart[u] + 0 .. 64/G - 1 below that profile's end.  etab, ttab, n and
+  // the scalars above are not read.
+  const VitProfDev *profs;
+  const int32_t *wg_prof;
+  const int32_t *ustart;
 };

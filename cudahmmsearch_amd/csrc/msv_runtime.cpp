@@ -57,8 +57,9 @@ extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t 
 extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, hipStream_t st);
 extern "C" int msvk_vit_slots(int Q, int W, const VitArgs *a, int ncu);
-extern "C" hipError_t msvk_launch_vit_band(int G, int Q, int vote, const VitArgs *a, int ncu, hipStream_t st);
-extern "C" int msvk_vit_band_slots(int G, int Q, int vote, const VitArgs *a, int ncu, int *per);
+extern "C" hipError_t msvk_launch_vit_band(int G, int Q, int vote, const VitArgs *a, int grid, hipStream_t st);
+extern "C" int msvk_vit_band_block(int G);
+extern "C" int msvk_vit_band_cap(int G, int Q, int vote, int R, int ncu);
 extern "C" hipError_t msvk_launch_finalize(const FinArgs *a, int np, int64_t n, hipStream_t st);
 extern "C" hipError_t msvk_build_fast_table(const uint8_t *rbv, int M, int Kp, const uint8_t *rowcode, int R, int G,
                                             int Q, int W, int bias, uint32_t *out, hipStream_t st);
@@ -291,7 +292,7 @@ struct msvg_ctx {
   uint32_t *h_scanout = nullptr;            // pinned result words
   size_t h_scancap = 0;
   // ViterbiFilter searches
-  DevMem d_vitems, d_vmove, d_vout, d_vuoff;
+  DevMem d_vitems, d_vmove, d_vout, d_vuoff, d_vprofs;
   hipStream_t vstream[kVitStreams] = {};   // multi-profile searches: launches round-robin
   hipEvent_t vdone[kVitStreams] = {};
 };
@@ -1205,15 +1206,21 @@ int search_device(msvg_ctx *c, const int32_t *ids, int np, int dbid, double F1, 
 // ViterbiFilter tables (host-built, see VitArgs in msv_device.h).
 namespace {
 
-// ViterbiFilter kernel shape for a model of length M (measured on gfx950,
-// survivors of up to 2,000 residues): vit_band with G = 16 lanes per target
-// (four targets per wave) up to M = 512, G = 32 up to 1024, G = 64 up to
-// 2048 (Q = ceil(M / 2G) <= 16 registers per lane); beyond, vit_kernel with
-// W = 2, 4 or 8 waves per target (Q <= 16).
-void vit_shape(int M, int &Q, int &W, int &G) {
+// Registers per lane of the one-wave vit_band shape beyond 16 (MM, IM and DM
+// in LDS): as many as keep its tables within 160 KiB of LDS for a database
+// holding all Kp codes (Kp + 1 emission rows and the three transition rows,
+// 64 lanes x 4 bytes each), at most 20 -- 19 for amino, 20 for DNA.
+int vit_band64_qmax(int Kp) { return std::min(20, (160 * 1024) / (256 * (Kp + 4))); }
+
+// ViterbiFilter kernel shape for a model of length M over an alphabet of Kp
+// codes (measured on gfx950, survivors of up to 2,000 residues): vit_band
+// with G = 16 lanes per target (four targets per wave) up to M = 512, G = 32
+// up to 1024, G = 64 up to 128 x vit_band64_qmax(Kp) (Q = ceil(M / 2G));
+// beyond, vit_kernel with W = 2, 4 or 8 waves per target (Q <= 16).
+void vit_shape(int M, int Kp, int &Q, int &W, int &G) {
   W = 1;
   G = (M <= 512) ? 16 : (M <= 1024 ? 32 : 64);
-  if (M > 2048) while (128 * W * 16 < M) W *= 2;
+  if (M > 128 * vit_band64_qmax(Kp)) while (128 * W * 16 < M) W *= 2;
   Q = (M + 2 * G * W - 1) / (2 * G * W);
 }
 
@@ -1689,7 +1696,7 @@ int msvg_profile_set_viterbi(msvg_ctx *c, int32_t pid, const msvg_vprofile *vp) 
   v->twv.assign(vp->twv, vp->twv + (size_t) 8 * p.M);
   v->e_loop = vp->e_loop; v->e_move = vp->e_move; v->base_w = vp->base_w; v->ddbound_w = vp->ddbound_w;
   v->scale_w = vp->scale_w; v->nj = vp->nj; v->vmu = vp->vmu; v->vlambda = vp->vlambda;
-  vit_shape(p.M, v->Q, v->W, v->G);
+  vit_shape(p.M, p.Kp, v->Q, v->W, v->G);
   // Rows vote on the lazy-F test unless it holds on most rows (then the
   // delete-path closure runs on every row, without the vote's work).
   v->vote = c->vit_vote >= 0 ? c->vit_vote
@@ -1700,7 +1707,8 @@ int msvg_profile_set_viterbi(msvg_ctx *c, int32_t pid, const msvg_vprofile *vp) 
     v->G = t.lanes;
     v->W = t.lanes == 64 ? std::max(1, t.waves) : 1;
     v->Q = (p.M + 2 * vit_lanes(*v) - 1) / (2 * vit_lanes(*v));
-    if (v->Q > 16) return fail(c, MSVG_EINVAL, "Viterbi tile: too many registers per lane");
+    if (v->Q > (v->G == 64 && v->W == 1 ? vit_band64_qmax(p.Kp) : 16))
+      return fail(c, MSVG_EINVAL, "Viterbi tile: too many registers per lane");
   }
   const size_t per = (size_t) v->Q * vit_lanes(*v);
   std::vector<uint32_t> h(8 * per);
@@ -1720,16 +1728,28 @@ namespace {
 
 // One profile's share of a ViterbiFilter search: its items (longest target
 // first), their N/C/J move costs, where they sit in the device item arrays,
-// and the launch.
+// and, for the multi-wave shapes, the launch.
 struct VitJob {
   int pid = 0;
   int64_t n = 0, base = 0;           // items [base, base + n) of the device arrays
   std::vector<int64_t> order;        // item q -> caller position
   std::vector<int64_t> items;        // item q -> sorted target index
   std::vector<int16_t> mv;
-  std::vector<int32_t> uoff;         // slot k runs units [uoff[k], uoff[k+1])
-  int64_t ubase = 0;                 // where uoff sits in the device array
+  std::vector<int32_t> uoff;         // multi-wave shapes: workgroup k runs items [uoff[k], uoff[k+1])
+  int64_t ubase = 0;                 // where uoff sits in the device index array
   int64_t cells = 0;
+  VitArgs a{};
+};
+
+// One vit_band launch: the jobs of one (G, Q, vote) class.  Workgroup b
+// works on job wg_prof[b]'s profile; its waves run the units the plan lists
+// for them (unit = 64/G consecutive items of one job).
+struct BandLaunch {
+  int G = 16, Q = 1, vote = 1, grid = 0;
+  std::vector<int> jobs;             // indices into the search's jobs
+  std::vector<int32_t> wg_prof, uoff, ustart;
+  int64_t ibase = 0;                 // wg_prof, uoff, ustart in the device index array, in that order
+  int64_t pbase = 0;                 // its VitProfDev entries in the device profile array
   VitArgs a{};
 };
 
@@ -1752,7 +1772,8 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   const uint32_t *etab;
   int rc = vit_emission_table(c, pid, dbid, &etab);
   if (rc != MSVG_OK) return rc;
-  // Longest targets first, so the last waves to start are the shortest.
+  // Longest targets first: units (one wave's targets) are consecutive items,
+  // so a unit's rows are its first target's length.
   job.order.resize((size_t) n);
   for (int64_t i = 0; i < n; ++i) job.order[(size_t) i] = i;
   auto len_of = [&](int64_t q) { return d.len_sorted[d.sorted_of[targets[q]]]; };
@@ -1764,41 +1785,30 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   a.db.tjb = d.d_tjb.as<uint8_t>(); a.db.n = d.n; a.db.R = d.R;
   a.n = (int32_t) n;
   a.e_loop = v.e_loop; a.e_move = v.e_move; a.base_w = v.base_w; a.ddbound_w = v.ddbound_w;
-  // Units of per targets (consecutive in length order, so a unit's rows
-  // are its first target's length) go to the kernel's S slots (waves, or
-  // workgroups of multi-wave targets) greedily: each unit, longest first, to
-  // the slot with the fewest rows so far, any number of units per slot.
-  // Slot k runs units [uoff[k], uoff[k+1]) of the reordered items.  A partial
-  // last unit (n not a multiple of per) stays the last unit, in slot S - 1,
-  // so that the positions past n are its own.
-  int per = 1;
-  const int S = (v.W == 1) ? msvk_vit_band_slots(v.G, v.Q, v.vote, &a, c->ncu, &per)
-                           : msvk_vit_slots(v.Q, v.W, &a, c->ncu);
-  if (S < 1) return fail(c, MSVG_EINVAL, "ViterbiFilter shape has no launch");
-  const int64_t nunits = (n + per - 1) / per;
-  const bool partial = (n % per) != 0;
-  std::vector<std::vector<int64_t>> lists((size_t) S);
-  {
+  if (v.W > 1) {
+    // Multi-wave shapes, one launch per profile: items (one per workgroup at
+    // a time) go greedily to the launch's workgroups, longest first, each to
+    // the workgroup with the fewest rows so far, any number per workgroup;
+    // workgroup k runs items [uoff[k], uoff[k+1]) of the reordered list.
+    const int S = msvk_vit_slots(v.Q, v.W, &a, c->ncu);
+    if (S < 1) return fail(c, MSVG_EINVAL, "ViterbiFilter shape has no launch");
+    std::vector<std::vector<int64_t>> lists((size_t) S);
     using Slot = std::pair<int64_t, int>;                       // (rows so far, slot)
     std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
     for (int k = 0; k < S; ++k) heap.push(Slot{0, k});
-    for (int64_t u = 0; u < nunits - (partial ? 1 : 0); ++u) {
+    for (int64_t u = 0; u < n; ++u) {
       Slot sl = heap.top();
       heap.pop();
       lists[(size_t) sl.second].push_back(u);
-      sl.first += len_of(job.order[(size_t) (u * per)]);
+      sl.first += len_of(job.order[(size_t) u]);
       heap.push(sl);
     }
-    if (partial) lists[(size_t) (S - 1)].push_back(nunits - 1);
-  }
-  job.uoff.assign((size_t) S + 1, 0);
-  {
+    job.uoff.assign((size_t) S + 1, 0);
     std::vector<int64_t> dealt;
     dealt.reserve((size_t) n);
     for (int k = 0; k < S; ++k) {
       job.uoff[(size_t) k + 1] = job.uoff[(size_t) k] + (int32_t) lists[(size_t) k].size();
-      for (const int64_t u : lists[(size_t) k])
-        for (int64_t t = u * per; t < std::min<int64_t>(n, (u + 1) * per); ++t) dealt.push_back(job.order[(size_t) t]);
+      for (const int64_t u : lists[(size_t) k]) dealt.push_back(job.order[(size_t) u]);
     }
     job.order.swap(dealt);
   }
@@ -1813,31 +1823,118 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   return MSVG_OK;
 }
 
-int vit_launch(msvg_ctx *c, VitJob &job, hipStream_t st) {
-  if (job.n == 0) return MSVG_OK;
-  const VitProfile &v = *c->profiles[job.pid]->vit;
-  job.a.items = c->d_vitems.as<int64_t>() + job.base;
-  job.a.move = c->d_vmove.as<int16_t>() + job.base;
-  job.a.out = c->d_vout.as<int32_t>() + job.base;
-  job.a.uoff = c->d_vuoff.as<int32_t>() + job.ubase;
-  if (v.W == 1) HIPCHK(c, msvk_launch_vit_band(v.G, v.Q, v.vote, &job.a, c->ncu, st));
-  else HIPCHK(c, msvk_launch_vit(v.Q, v.W, &job.a, c->ncu, st));
+// vit_band launches for the W = 1 jobs: jobs of one (G, Q, vote) class
+// share a launch, so that the row chains of their longest targets run beside
+// each other and beside the other jobs' work instead of ending a launch
+// each.  Workgroups (each works on one job) are given out one at a time:
+// first one to every job, then each to the job with the most rows per wave
+// while it has units left for the workgroup's waves, up to what the device
+// holds at once (a launch that leaves SIMDs with fewer waves runs each of its
+// waves faster but issues less: C2 measured 5.5 ms on 449 of 512 workgroups
+// against 5.1 ms on all).  Each job's units then go greedily to its own waves:
+// longest first, each to the wave with the fewest rows so far.  A class with
+// more jobs than the device holds workgroups is split.  Positions (job.base)
+// must be assigned.
+int vit_plan_band(msvg_ctx *c, const std::vector<VitJob> &jobs, const Db &d, std::vector<BandLaunch> &out) {
+  std::map<std::tuple<int, int, int>, std::vector<int>> classes;
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    if (jobs[i].n == 0) continue;
+    const VitProfile &v = *c->profiles[jobs[i].pid]->vit;
+    if (v.W == 1) classes[std::make_tuple(v.G, v.Q, v.vote)].push_back((int) i);
+  }
+  for (auto &kv : classes) {
+    const int G = std::get<0>(kv.first), Q = std::get<1>(kv.first), vote = std::get<2>(kv.first);
+    const int nw = msvk_vit_band_block(G) / 64, NG = 64 / G;
+    const int cap = msvk_vit_band_cap(G, Q, vote, d.R, c->ncu);
+    if (cap < 1) return fail(c, MSVG_EINVAL, "ViterbiFilter shape has no launch");
+    const std::vector<int> &all = kv.second;
+    for (size_t first = 0; first < all.size(); first += (size_t) cap) {
+      BandLaunch L;
+      L.G = G; L.Q = Q; L.vote = vote;
+      L.jobs.assign(all.begin() + (long) first, all.begin() + (long) std::min(all.size(), first + (size_t) cap));
+      const int nj = (int) L.jobs.size();
+      auto unit_rows = [&](const VitJob &j, int64_t u) { return (int64_t) d.len_sorted[j.items[(size_t) (u * NG)]]; };
+      std::vector<int64_t> nunit((size_t) nj), rows((size_t) nj, 0), wg((size_t) nj, 1);
+      int64_t spare = cap - nj;
+      for (int q = 0; q < nj; ++q) {
+        const VitJob &j = jobs[(size_t) L.jobs[(size_t) q]];
+        nunit[(size_t) q] = (j.n + NG - 1) / NG;
+        for (int64_t u = 0; u < nunit[(size_t) q]; ++u) rows[(size_t) q] += unit_rows(j, u);
+      }
+      auto load = [&](int q) { return (double) rows[(size_t) q] / (double) (wg[(size_t) q] * nw); };
+      auto gains = [&](int q) { return wg[(size_t) q] * nw < nunit[(size_t) q]; };
+      std::priority_queue<std::pair<double, int>> heap;
+      for (int q = 0; q < nj; ++q) if (gains(q)) heap.push({load(q), q});
+      while (spare > 0 && !heap.empty()) {
+        const int q = heap.top().second;
+        heap.pop();
+        ++wg[(size_t) q];
+        --spare;
+        if (gains(q)) heap.push({load(q), q});
+      }
+      int64_t nwg = 0;
+      for (int q = 0; q < nj; ++q) nwg += wg[(size_t) q];
+      L.grid = (int) nwg;
+      L.wg_prof.resize((size_t) nwg);
+      L.uoff.assign((size_t) nwg * nw + 1, 0);
+      int64_t b0 = 0;
+      for (int q = 0; q < nj; ++q) {
+        const VitJob &j = jobs[(size_t) L.jobs[(size_t) q]];
+        const int S = (int) (wg[(size_t) q] * nw);
+        std::vector<std::vector<int64_t>> lists((size_t) S);
+        using Slot = std::pair<int64_t, int>;                   // (rows so far, wave)
+        std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap2;
+        for (int k = 0; k < S; ++k) heap2.push(Slot{0, k});
+        for (int64_t u = 0; u < nunit[(size_t) q]; ++u) {
+          Slot sl = heap2.top();
+          heap2.pop();
+          lists[(size_t) sl.second].push_back(u);
+          sl.first += unit_rows(j, u);
+          heap2.push(sl);
+        }
+        for (int64_t b = b0; b < b0 + wg[(size_t) q]; ++b) L.wg_prof[(size_t) b] = q;
+        for (int k = 0; k < S; ++k) {
+          const int64_t slot = b0 * nw + k;
+          L.uoff[(size_t) slot + 1] = L.uoff[(size_t) slot] + (int32_t) lists[(size_t) k].size();
+          for (const int64_t u : lists[(size_t) k]) L.ustart.push_back((int32_t) (j.base + u * NG));
+        }
+        b0 += wg[(size_t) q];
+      }
+      out.push_back(std::move(L));
+    }
+  }
   return MSVG_OK;
 }
 
-// Device item arrays for all jobs, uploaded on st.
-int vit_upload(msvg_ctx *c, std::vector<VitJob> &jobs, hipStream_t st, std::vector<int64_t> &hitems,
-               std::vector<int16_t> &hmv, std::vector<int32_t> &huoff) {
-  int64_t tot = 0, utot = 0;
-  for (VitJob &j : jobs) {
-    j.base = tot; tot += j.n;
-    j.ubase = utot; utot += (int64_t) j.uoff.size();
+// Device arrays of a search, uploaded on st: items and move costs of all
+// jobs, the multi-wave jobs' lists, and the vit_band launches' profiles,
+// workgroup profiles, lists and unit starts.
+int vit_upload(msvg_ctx *c, std::vector<VitJob> &jobs, std::vector<BandLaunch> &bands, hipStream_t st,
+               std::vector<int64_t> &hitems, std::vector<int16_t> &hmv, std::vector<int32_t> &hidx,
+               std::vector<VitProfDev> &hprof) {
+  int64_t tot = 0;
+  for (const VitJob &j : jobs) tot += j.n;
+  hidx.clear();
+  for (VitJob &j : jobs) { j.ubase = (int64_t) hidx.size(); hidx.insert(hidx.end(), j.uoff.begin(), j.uoff.end()); }
+  hprof.clear();
+  for (BandLaunch &L : bands) {
+    L.ibase = (int64_t) hidx.size();
+    hidx.insert(hidx.end(), L.wg_prof.begin(), L.wg_prof.end());
+    hidx.insert(hidx.end(), L.uoff.begin(), L.uoff.end());
+    hidx.insert(hidx.end(), L.ustart.begin(), L.ustart.end());
+    L.pbase = (int64_t) hprof.size();
+    for (const int ji : L.jobs) {
+      const VitJob &j = jobs[(size_t) ji];
+      VitProfDev pd{};
+      pd.etab = j.a.etab; pd.ttab = j.a.ttab;
+      pd.e_loop = j.a.e_loop; pd.e_move = j.a.e_move; pd.base_w = j.a.base_w; pd.ddbound_w = j.a.ddbound_w;
+      pd.end = (int32_t) (j.base + j.n);
+      hprof.push_back(pd);
+    }
   }
-  HIPCHK(c, c->d_vuoff.ensure((size_t) std::max<int64_t>(utot, 1) * sizeof(int32_t)));
-  huoff.resize((size_t) utot);
-  for (const VitJob &j : jobs) std::copy(j.uoff.begin(), j.uoff.end(), huoff.begin() + j.ubase);
-  if (utot > 0)
-    HIPCHK(c, hipMemcpyAsync(c->d_vuoff.p, huoff.data(), (size_t) utot * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  if (tot > INT32_MAX) return fail(c, MSVG_EINVAL, "too many ViterbiFilter pairs in one call");
+  HIPCHK(c, c->d_vuoff.ensure(std::max<size_t>(hidx.size(), 1) * sizeof(int32_t)));
+  HIPCHK(c, c->d_vprofs.ensure(std::max<size_t>(hprof.size(), 1) * sizeof(VitProfDev)));
   HIPCHK(c, c->d_vitems.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int64_t)));
   HIPCHK(c, c->d_vmove.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int16_t)));
   HIPCHK(c, c->d_vout.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int32_t)));
@@ -1847,9 +1944,39 @@ int vit_upload(msvg_ctx *c, std::vector<VitJob> &jobs, hipStream_t st, std::vect
     std::copy(j.items.begin(), j.items.end(), hitems.begin() + j.base);
     std::copy(j.mv.begin(), j.mv.end(), hmv.begin() + j.base);
   }
+  if (!hidx.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_vuoff.p, hidx.data(), hidx.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  if (!hprof.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_vprofs.p, hprof.data(), hprof.size() * sizeof(VitProfDev), hipMemcpyHostToDevice, st));
   if (tot == 0) return MSVG_OK;
   HIPCHK(c, hipMemcpyAsync(c->d_vitems.p, hitems.data(), (size_t) tot * sizeof(int64_t), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_vmove.p, hmv.data(), (size_t) tot * sizeof(int16_t), hipMemcpyHostToDevice, st));
+  return MSVG_OK;
+}
+
+int vit_launch_band(msvg_ctx *c, BandLaunch &L, const VitJob &any, hipStream_t st) {
+  VitArgs &a = L.a;
+  a = VitArgs{};
+  a.db = any.a.db;
+  a.items = c->d_vitems.as<int64_t>();
+  a.move = c->d_vmove.as<int16_t>();
+  a.out = c->d_vout.as<int32_t>();
+  const int32_t *idx = c->d_vuoff.as<int32_t>() + L.ibase;
+  a.wg_prof = idx;
+  a.uoff = idx + L.wg_prof.size();
+  a.ustart = idx + L.wg_prof.size() + L.uoff.size();
+  a.profs = c->d_vprofs.as<VitProfDev>() + L.pbase;
+  HIPCHK(c, msvk_launch_vit_band(L.G, L.Q, L.vote, &a, L.grid, st));
+  return MSVG_OK;
+}
+
+int vit_launch_multiwave(msvg_ctx *c, VitJob &job, hipStream_t st) {
+  const VitProfile &v = *c->profiles[job.pid]->vit;
+  job.a.items = c->d_vitems.as<int64_t>() + job.base;
+  job.a.move = c->d_vmove.as<int16_t>() + job.base;
+  job.a.out = c->d_vout.as<int32_t>() + job.base;
+  job.a.uoff = c->d_vuoff.as<int32_t>() + job.ubase;
+  HIPCHK(c, msvk_launch_vit(v.Q, v.W, &job.a, c->ncu, st));
   return MSVG_OK;
 }
 
@@ -1897,16 +2024,30 @@ int msvg_vit_search_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t d
     if (rc != MSVG_OK) return rc;
   }
   if (total == 0) return MSVG_OK;
+  int64_t pos = 0;
+  for (VitJob &j : jobs) { j.base = pos; pos += j.n; }
+  std::vector<BandLaunch> bands;
+  int rc = vit_plan_band(c, jobs, *c->dbs[dbid], bands);
+  if (rc != MSVG_OK) return rc;
   hipStream_t st = c->stream;
   std::vector<int64_t> hitems;
   std::vector<int16_t> hmv;
-  std::vector<int32_t> huoff;
-  int rc = vit_upload(c, jobs, st, hitems, hmv, huoff);
-  if (rc != MSVG_OK) return rc;
-  // Each launch lasts at least as long as its longest target's row chain
-  // (one wave, or one workgroup, per target): profiles go round-robin over
-  // the context's ViterbiFilter streams so that those tails overlap.
-  const int nstreams = np > 1 ? kVitStreams : 1;
+  std::vector<int32_t> hidx;
+  std::vector<VitProfDev> hprof;
+  if ((rc = vit_upload(c, jobs, bands, st, hitems, hmv, hidx, hprof)) != MSVG_OK) return rc;
+  // The vit_band launches (one per shape class), then the multi-wave
+  // profiles' launches, round-robin over the context's ViterbiFilter streams
+  // so that the launches' tails (each lasts at least as long as its longest
+  // target's row chain) overlap.
+  const VitJob *any = nullptr;
+  std::vector<int> multi;
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    if (jobs[i].n == 0) continue;
+    any = &jobs[i];
+    if (c->profiles[jobs[i].pid]->vit->W > 1) multi.push_back((int) i);
+  }
+  const int nlaunch = (int) (bands.size() + multi.size());
+  const int nstreams = nlaunch > 1 ? kVitStreams : 1;
   if (nstreams > 1 && !c->vstream[0]) {
     for (int k = 0; k < kVitStreams; ++k) {
       HIPCHK(c, hipStreamCreateWithFlags(&c->vstream[k], hipStreamNonBlocking));
@@ -1917,15 +2058,18 @@ int msvg_vit_search_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t d
   if (nstreams > 1) {
     HIPCHK(c, hipEventRecord(c->fork, st));
     for (int k = 0; k < nstreams; ++k) HIPCHK(c, hipStreamWaitEvent(c->vstream[k], c->fork, 0));
-    for (int32_t i = 0; i < np; ++i)
-      if ((rc = vit_launch(c, jobs[(size_t) i], c->vstream[i % nstreams])) != MSVG_OK) return rc;
-    for (int k = 0; k < nstreams; ++k) {
-      HIPCHK(c, hipEventRecord(c->vdone[k], c->vstream[k]));
-      HIPCHK(c, hipStreamWaitEvent(st, c->vdone[k], 0));
+  }
+  int k = 0;
+  for (BandLaunch &L : bands)
+    if ((rc = vit_launch_band(c, L, *any, nstreams > 1 ? c->vstream[k++ % nstreams] : st)) != MSVG_OK) return rc;
+  for (const int i : multi)
+    if ((rc = vit_launch_multiwave(c, jobs[(size_t) i], nstreams > 1 ? c->vstream[k++ % nstreams] : st)) != MSVG_OK)
+      return rc;
+  if (nstreams > 1) {
+    for (int q = 0; q < nstreams; ++q) {
+      HIPCHK(c, hipEventRecord(c->vdone[q], c->vstream[q]));
+      HIPCHK(c, hipStreamWaitEvent(st, c->vdone[q], 0));
     }
-  } else {
-    for (int32_t i = 0; i < np; ++i)
-      if ((rc = vit_launch(c, jobs[(size_t) i], st)) != MSVG_OK) return rc;
   }
   HIPCHK(c, hipEventRecord(c->ev[1], st));
   std::vector<int32_t> out((size_t) total);
@@ -1941,7 +2085,7 @@ int msvg_vit_search_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t d
   c->stats.ms_fast = ms;
   c->stats.ms_total = ms;
   c->stats.cells = cells;
-  c->stats.n_fast_launches = np;
+  c->stats.n_fast_launches = nlaunch;
   return MSVG_OK;
 }
 

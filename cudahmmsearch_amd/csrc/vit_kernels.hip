@@ -391,8 +391,8 @@ constexpr uint64_t GMASK = (G == 64) ? ~0ull : ((1ull << G) - 1);
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// vit_band<G, Q, VOTE>: models up to 2048 positions, Q <= 16 registers per
-// lane, NG = 64/G targets per wave with G = 16, 32 or 64 lanes each (the
+// vit_band<G, Q, VOTE>: models up to 2560 positions, Q <= 16 registers per
+// lane (20 for G = 64), NG = 64/G targets per wave with G = 16, 32 or 64 lanes each (the
 // wave's targets share the per-row fixed work: the shifts, the votes, the
 // special states, the delete-closure scan).  Per row:
 //   - rows run four at a time, so a residue is one byte extract;
@@ -502,12 +502,17 @@ __device__ __forceinline__ void close_all(uint32_t (&Dr)[Q], const uint32_t (&tD
 // Workgroup size: 64-lane targets, whose tables are the largest, share one
 // LDS copy among eight waves.
 template <int G> constexpr int band_block() { return G == 64 ? 512 : 256; }
+// Models beyond 2048 positions (G = 64, Q = 17 to 20) read their MM, IM and
+// DM transitions from LDS each row, which keeps the lane within 256 VGPRs
+// with two waves per SIMD.
+template <int Q> constexpr bool band_lt() { return Q > 16; }
 
 template <int G, int Q, bool VOTE>
 __global__ void __launch_bounds__(band_block<G>()) __attribute__((amdgpu_waves_per_eu(Q <= 5 ? 4 : (Q <= 8 ? 3 : 2), 8)))
 vit_band(VitArgs a) {
   extern __shared__ uint32_t etab_lds[];
   constexpr int NG = 64 / G;
+  const VitProfDev pd = a.profs[a.wg_prof[blockIdx.x]];   // this workgroup's profile
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int c = lane & (G - 1);                        // lane within its group
@@ -519,16 +524,20 @@ vit_band(VitArgs a) {
     asm volatile("" : "+v"(ge[gg]));
   }
 
-  uint32_t tMM[Q], tIM[Q], tDM[Q], tMD[Q], tMI[Q], tII[Q], tDD[Q];
+  constexpr bool LT = band_lt<Q>();
+  constexpr int QR = LT ? 1 : Q;
+  uint32_t tMM[QR], tIM[QR], tDM[QR], tMD[Q], tMI[Q], tII[Q], tDD[Q];
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
-    tMM[j] = a.ttab[(1 * Q + j) * G + c];
-    tIM[j] = a.ttab[(2 * Q + j) * G + c];
-    tDM[j] = a.ttab[(3 * Q + j) * G + c];
-    tMD[j] = a.ttab[(4 * Q + j) * G + c];
-    tMI[j] = a.ttab[(5 * Q + j) * G + c];
-    tII[j] = a.ttab[(6 * Q + j) * G + c];
-    tDD[j] = a.ttab[(7 * Q + j) * G + c];
+    if constexpr (!LT) {
+      tMM[j] = pd.ttab[(1 * Q + j) * G + c];
+      tIM[j] = pd.ttab[(2 * Q + j) * G + c];
+      tDM[j] = pd.ttab[(3 * Q + j) * G + c];
+    }
+    tMD[j] = pd.ttab[(4 * Q + j) * G + c];
+    tMI[j] = pd.ttab[(5 * Q + j) * G + c];
+    tII[j] = pd.ttab[(6 * Q + j) * G + c];
+    tDD[j] = pd.ttab[(7 * Q + j) * G + c];
   }
   int Tlo = 0, Thi = 0;                                // tDD summed over each slot
 #pragma unroll
@@ -552,17 +561,20 @@ vit_band(VitArgs a) {
   }
 
   const int nemit = (a.db.R + 1) * Q * G;
-  for (int i = threadIdx.x; i < nemit; i += blockDim.x) etab_lds[i] = a.etab[i];
+  for (int i = threadIdx.x; i < nemit; i += blockDim.x) etab_lds[i] = pd.etab[i];
+  if constexpr (LT)
+    for (int i = threadIdx.x; i < 3 * Q * G; i += blockDim.x) etab_lds[nemit + i] = pd.ttab[Q * G + i];
   __syncthreads();
+  const uint32_t *tl = etab_lds + nemit + c;           // LT: this lane's MM, IM, DM
 
   const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
   const uint4 null4 = make_uint4(nb, nb, nb, nb);
-  const int xN = a.base_w;                             // vitfilter.c:116; N loop cost is 0
+  const int xN = pd.base_w;                             // vitfilter.c:116; N loop cost is 0
   const int slot = blockIdx.x * (blockDim.x >> 6) + wave;
 
-  for (int wi = a.uoff[slot], wend = a.uoff[slot + 1]; wi < wend; ++wi) {
-    const int it = wi * NG + g;
-    const bool valid = it < a.n;
+  for (int k = a.uoff[slot], kend = a.uoff[slot + 1]; k < kend; ++k) {
+    const int it = a.ustart[k] + g;
+    const bool valid = it < pd.end;
     const int64_t s = valid ? a.items[it] : 0;
     const int L = valid ? a.db.len[s] : 0;
     const int Lpad = (L + 15) & ~15;
@@ -589,14 +601,14 @@ vit_band(VitArgs a) {
       int te[NG], td[NG];
 #pragma unroll
       for (int gg = 0; gg < NG; ++gg) {
-        te[gg] = st[gg] ? 0x7fffffff : min(max(xJ[gg], xN) - a.e_loop, 32766);
-        td[gg] = xB[gg] - a.ddbound_w;
+        te[gg] = st[gg] ? 0x7fffffff : min(max(xJ[gg], xN) - pd.e_loop, 32766);
+        td[gg] = xB[gg] - pd.ddbound_w;
       }
       tE = of_group<NG>(te, ge);
       tD = of_group<NG>(td, ge);
       const int xb = of_group<NG>(xB, ge);
 #pragma unroll
-      for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xb, xb), a.ttab[j * G + c]);
+      for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xb, xb), pd.ttab[j * G + c]);
     };
     refresh();
     uint32_t xacc = NEG2;               // low half: this lane's largest row maximum (xC at the end)
@@ -622,10 +634,13 @@ vit_band(VitArgs a) {
 #pragma unroll
       for (int j = Q - 1; j >= 0; --j) {
         const uint32_t pm = j ? Mr[j - 1] : mpv, pi = j ? Ir[j - 1] : ipv, pd = j ? Dr[j - 1] : dpv;
+        const uint32_t mm = LT ? tl[(0 * Q + j) * G] : tMM[LT ? 0 : j];
+        const uint32_t im = LT ? tl[(1 * Q + j) * G] : tIM[LT ? 0 : j];
+        const uint32_t dm = LT ? tl[(2 * Q + j) * G] : tDM[LT ? 0 : j];
         uint32_t sv = bm[j];                                         // :146-150
-        sv = maxs(sv, adds(pm, tMM[j]));
-        sv = maxs(sv, adds(pi, tIM[j]));
-        sv = maxs(sv, adds(pd, tDM[j]));
+        sv = maxs(sv, adds(pm, mm));
+        sv = maxs(sv, adds(pi, im));
+        sv = maxs(sv, adds(pd, dm));
         sv = adds(sv, e[j]);
         xEv = maxs(xEv, sv);                                         // :151
         Ir[j] = maxs(adds(Mr[j], tMI[j]), adds(Ir[j], tII[j]));      // :171-172
@@ -654,7 +669,7 @@ vit_band(VitArgs a) {
           const bool over = hit && xE >= 32767;                             // :177
           const bool upd = hit && !over;
           st[gg] = over ? 16 : st[gg];
-          xJ[gg] = upd ? (int) (int16_t) max(xJ[gg], xE + a.e_loop) : xJ[gg];   // :180 (J loop 0)
+          xJ[gg] = upd ? (int) (int16_t) max(xJ[gg], xE + pd.e_loop) : xJ[gg];   // :180 (J loop 0)
           xB[gg] = upd ? (int) (int16_t) max(xJ[gg] + mv[gg], xN + mv[gg]) : xB[gg];   // :181
         }
         refresh();
@@ -694,62 +709,66 @@ vit_band(VitArgs a) {
     const uint32_t xm = group_max2<G>(xacc);
 #pragma unroll
     for (int gg = 0; gg < NG; ++gg) {
-      const int xC = max(-32768, lo16(__builtin_amdgcn_readlane(xm, gg * G + G - 1)) + a.e_move);
+      const int xC = max(-32768, lo16(__builtin_amdgcn_readlane(xm, gg * G + G - 1)) + pd.e_move);
       if (lane == gg * G && valid) a.out[it] = (st[gg] << 16) | (int) (uint16_t) xC;
     }
   }
 }
 
 template <int G, int Q>
-static size_t band_lds(const VitArgs *a) { return (size_t) (a->db.R + 1) * Q * G * 4; }
-
-template <int G, int Q, bool VOTE>
-static int grid_band(const VitArgs *a, int ncu) {
-  constexpr int NG = 64 / G;
-  const int waves = (a->n + NG - 1) / NG;
-  constexpr int nw = band_block<G>() / 64;
-  const int cap = ncu * resident_per_cu(vit_band<G, Q, VOTE>, band_block<G>(), band_lds<G, Q>(a));
-  return std::max(1, std::min((waves + nw - 1) / nw, cap));
+static size_t band_lds(int R) {
+  return ((size_t) (R + 1) * Q * G + (band_lt<Q>() ? 3 * Q * G : 0)) * 4;
 }
 
 template <int G, int Q, bool VOTE>
-static hipError_t launch_band(const VitArgs *a, int ncu, hipStream_t st) {
-  const int grid = grid_band<G, Q, VOTE>(a, ncu);
-  const size_t lds = band_lds<G, Q>(a);
+static int cap_band(int R, int ncu) {
+  return ncu * resident_per_cu(vit_band<G, Q, VOTE>, band_block<G>(), band_lds<G, Q>(R));
+}
+
+template <int G, int Q, bool VOTE>
+static hipError_t launch_band(const VitArgs *a, int grid, hipStream_t st) {
+  const size_t lds = band_lds<G, Q>(a->db.R);
   hipLaunchKernelGGL((vit_band<G, Q, VOTE>), dim3(grid), dim3(band_block<G>()), lds, st, *a);
   return hipGetLastError();
 }
 
 template <int G, bool VOTE, int... Qs>
-static hipError_t dispatch_band(int Q, const VitArgs *a, int ncu, hipStream_t st, std::integer_sequence<int, Qs...>) {
+static hipError_t dispatch_band(int Q, const VitArgs *a, int grid, hipStream_t st, std::integer_sequence<int, Qs...>) {
   hipError_t e = hipErrorInvalidValue;
-  ((Q == Qs + 1 ? (e = launch_band<G, Qs + 1, VOTE>(a, ncu, st), 0) : 0), ...);
+  ((Q == Qs + 1 ? (e = launch_band<G, Qs + 1, VOTE>(a, grid, st), 0) : 0), ...);
   return e;
 }
 
 template <int G, bool VOTE, int... Qs>
-static int slots_band(int Q, const VitArgs *a, int ncu, std::integer_sequence<int, Qs...>) {
+static int caps_band(int Q, int R, int ncu, std::integer_sequence<int, Qs...>) {
   int r = 0;
-  ((Q == Qs + 1 ? (r = grid_band<G, Qs + 1, VOTE>(a, ncu) * (band_block<G>() / 64), 0) : 0), ...);
+  ((Q == Qs + 1 ? (r = cap_band<G, Qs + 1, VOTE>(R, ncu), 0) : 0), ...);
   return r;
 }
 
-// Q <= 16 registers per lane, G = 16, 32 or 64 lanes per target; vote: the rows
-// vote on the lazy-F test (else every row closes the delete paths).
-extern "C" hipError_t msvk_launch_vit_band(int G, int Q, int vote, const VitArgs *a, int ncu, hipStream_t st) {
-  if (Q < 1 || Q > 16 || a->n < 1) return hipErrorInvalidValue;
+// G = 16, 32 or 64 lanes per target, Q <= 16 registers per lane (20 for
+// G = 64); vote: the rows vote on the lazy-F test (else every row closes the
+// delete paths).  grid workgroups of msvk_vit_band_block(G) threads.
+extern "C" hipError_t msvk_launch_vit_band(int G, int Q, int vote, const VitArgs *a, int grid, hipStream_t st) {
+  if (Q < 1 || Q > (G == 64 ? 20 : 16) || grid < 1) return hipErrorInvalidValue;
   auto qs = std::make_integer_sequence<int, 16>{};
-  if (G == 16) return vote ? dispatch_band<16, true>(Q, a, ncu, st, qs) : dispatch_band<16, false>(Q, a, ncu, st, qs);
-  if (G == 32) return vote ? dispatch_band<32, true>(Q, a, ncu, st, qs) : dispatch_band<32, false>(Q, a, ncu, st, qs);
-  if (G == 64) return vote ? dispatch_band<64, true>(Q, a, ncu, st, qs) : dispatch_band<64, false>(Q, a, ncu, st, qs);
+  auto qw = std::make_integer_sequence<int, 20>{};
+  if (G == 16) return vote ? dispatch_band<16, true>(Q, a, grid, st, qs) : dispatch_band<16, false>(Q, a, grid, st, qs);
+  if (G == 32) return vote ? dispatch_band<32, true>(Q, a, grid, st, qs) : dispatch_band<32, false>(Q, a, grid, st, qs);
+  if (G == 64) return vote ? dispatch_band<64, true>(Q, a, grid, st, qw) : dispatch_band<64, false>(Q, a, grid, st, qw);
   return hipErrorInvalidValue;
 }
 
-extern "C" int msvk_vit_band_slots(int G, int Q, int vote, const VitArgs *a, int ncu, int *per) {
-  if (a->n < 1 || Q < 1 || Q > 16 || (G != 16 && G != 32 && G != 64)) return 0;
-  *per = 64 / G;
+// Threads per workgroup of a vit_band shape.
+extern "C" int msvk_vit_band_block(int G) { return G == 64 ? band_block<64>() : (G == 32 ? band_block<32>() : band_block<16>()); }
+
+// Workgroups of a vit_band shape that the device holds at once for a
+// database of R + 1 rows (0 if the shape does not exist).
+extern "C" int msvk_vit_band_cap(int G, int Q, int vote, int R, int ncu) {
+  if (Q < 1 || Q > (G == 64 ? 20 : 16) || (G != 16 && G != 32 && G != 64)) return 0;
   auto qs = std::make_integer_sequence<int, 16>{};
-  if (G == 16) return vote ? slots_band<16, true>(Q, a, ncu, qs) : slots_band<16, false>(Q, a, ncu, qs);
-  if (G == 32) return vote ? slots_band<32, true>(Q, a, ncu, qs) : slots_band<32, false>(Q, a, ncu, qs);
-  return vote ? slots_band<64, true>(Q, a, ncu, qs) : slots_band<64, false>(Q, a, ncu, qs);
+  auto qw = std::make_integer_sequence<int, 20>{};
+  if (G == 16) return vote ? caps_band<16, true>(Q, R, ncu, qs) : caps_band<16, false>(Q, R, ncu, qs);
+  if (G == 32) return vote ? caps_band<32, true>(Q, R, ncu, qs) : caps_band<32, false>(Q, R, ncu, qs);
+  return vote ? caps_band<64, true>(Q, R, ncu, qw) : caps_band<64, false>(Q, R, ncu, qw);
 }

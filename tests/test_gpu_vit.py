@@ -51,13 +51,14 @@ def test_golden_vectors(vit_golden):
 
 
 @pytest.mark.parametrize("M", [1, 7, 33, 128, 129, 200, 256, 257, 400, 480, 481, 512, 513, 640, 1024, 1025, 1536,
-                               2048, 2049, 4096, 4097, 8192, 8193, 9000, 16384])
+                               2048, 2049, 2304, 2432, 2433, 4096, 4097, 8192, 8193, 9000, 16384])
 def test_every_wave_count_vs_reference(M):
     """Model lengths at the edges of each (lanes, waves, registers) shape
     against the compiled reference's p7_ViterbiFilter: four 16-lane targets
     per wave up to M = 512, two 32-lane targets up to 1024, one wave per
-    target up to 2048 (1 to 16 registers per lane), then 2, 4 and 8 waves
-    per target."""
+    target up to 2432 (1 to 16 registers per lane, 19 for one amino wave,
+    whose MM, IM and DM transitions come from LDS beyond 16), then 2, 4 and
+    8 waves per target."""
     from oracle import Reference
     R = Reference()
     rp = R.sample(500 + M, M, calibrate=False)

@@ -1753,7 +1753,9 @@ struct BandLaunch {
   VitArgs a{};
 };
 
-int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, int64_t n, VitJob &job) {
+// The checks and device tables of one job (on the calling thread: they may
+// build and upload the profile's emission table for the database).
+int vit_check(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, int64_t n, VitJob &job) {
   if (n < 0 || (n > 0 && !targets)) return fail(c, MSVG_EINVAL, "null argument");
   auto pit = c->profiles.find(pid);
   if (pit == c->profiles.end()) return fail(c, MSVG_EINVAL, "unknown profile id");
@@ -1772,12 +1774,6 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   const uint32_t *etab;
   int rc = vit_emission_table(c, pid, dbid, &etab);
   if (rc != MSVG_OK) return rc;
-  // Longest targets first: units (one wave's targets) are consecutive items,
-  // so a unit's rows are its first target's length.
-  job.order.resize((size_t) n);
-  for (int64_t i = 0; i < n; ++i) job.order[(size_t) i] = i;
-  auto len_of = [&](int64_t q) { return d.len_sorted[d.sorted_of[targets[q]]]; };
-  std::stable_sort(job.order.begin(), job.order.end(), [&](int64_t a, int64_t b) { return len_of(a) > len_of(b); });
   VitArgs &a = job.a;
   a.etab = etab;
   a.ttab = v.d_ttab.as<uint32_t>();
@@ -1785,28 +1781,42 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   a.db.tjb = d.d_tjb.as<uint8_t>(); a.db.n = d.n; a.db.R = d.R;
   a.n = (int32_t) n;
   a.e_loop = v.e_loop; a.e_move = v.e_move; a.base_w = v.base_w; a.ddbound_w = v.ddbound_w;
+  return MSVG_OK;
+}
+
+// The host side of one checked job (no device calls, so jobs run on host
+// threads side by side): items longest target first -- units (one wave's
+// targets) are consecutive items, so a unit's rows are its first target's
+// length --, their move costs (one per distinct length) and cells, and for
+// the multi-wave shapes the lists of the job's own launch.
+void vit_fill(const VitProfile &v, int M, const Db &d, int slots, const int64_t *targets, VitJob &job) {
+  const int64_t n = job.n;
+  if (n == 0) return;
+  std::vector<std::pair<int32_t, int64_t>> key((size_t) n);       // (-length, caller position)
+  for (int64_t q = 0; q < n; ++q) key[(size_t) q] = {-d.len_sorted[(size_t) d.sorted_of[(size_t) targets[q]]], q};
+  std::sort(key.begin(), key.end());
+  job.order.resize((size_t) n);
+  for (int64_t q = 0; q < n; ++q) job.order[(size_t) q] = key[(size_t) q].second;
   if (v.W > 1) {
     // Multi-wave shapes, one launch per profile: items (one per workgroup at
     // a time) go greedily to the launch's workgroups, longest first, each to
     // the workgroup with the fewest rows so far, any number per workgroup;
     // workgroup k runs items [uoff[k], uoff[k+1]) of the reordered list.
-    const int S = msvk_vit_slots(v.Q, v.W, &a, c->ncu);
-    if (S < 1) return fail(c, MSVG_EINVAL, "ViterbiFilter shape has no launch");
-    std::vector<std::vector<int64_t>> lists((size_t) S);
+    std::vector<std::vector<int64_t>> lists((size_t) slots);
     using Slot = std::pair<int64_t, int>;                       // (rows so far, slot)
     std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
-    for (int k = 0; k < S; ++k) heap.push(Slot{0, k});
+    for (int k = 0; k < slots; ++k) heap.push(Slot{0, k});
     for (int64_t u = 0; u < n; ++u) {
       Slot sl = heap.top();
       heap.pop();
       lists[(size_t) sl.second].push_back(u);
-      sl.first += len_of(job.order[(size_t) u]);
+      sl.first += -key[(size_t) u].first;
       heap.push(sl);
     }
-    job.uoff.assign((size_t) S + 1, 0);
+    job.uoff.assign((size_t) slots + 1, 0);
     std::vector<int64_t> dealt;
     dealt.reserve((size_t) n);
-    for (int k = 0; k < S; ++k) {
+    for (int k = 0; k < slots; ++k) {
       job.uoff[(size_t) k + 1] = job.uoff[(size_t) k] + (int32_t) lists[(size_t) k].size();
       for (const int64_t u : lists[(size_t) k]) dealt.push_back(job.order[(size_t) u]);
     }
@@ -1814,13 +1824,15 @@ int vit_prepare(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, 
   }
   job.items.resize((size_t) n);
   job.mv.resize((size_t) n);
+  std::vector<int32_t> mv_of((size_t) d.Ls.size(), INT32_MIN);   // move cost per distinct length
   for (int64_t q = 0; q < n; ++q) {
-    const int64_t s = d.sorted_of[targets[job.order[(size_t) q]]];
+    const int64_t s = d.sorted_of[(size_t) targets[job.order[(size_t) q]]];
     job.items[(size_t) q] = s;
-    job.mv[(size_t) q] = (int16_t) msvh_vit_move(v.scale_w, v.nj, d.len_sorted[s]);
-    job.cells += (int64_t) p.M * d.len_sorted[s];
+    int32_t &m = mv_of[(size_t) d.lidx_sorted[(size_t) s]];
+    if (m == INT32_MIN) m = msvh_vit_move(v.scale_w, v.nj, d.len_sorted[(size_t) s]);
+    job.mv[(size_t) q] = (int16_t) m;
+    job.cells += (int64_t) M * d.len_sorted[(size_t) s];
   }
-  return MSVG_OK;
 }
 
 // vit_band launches for the W = 1 jobs: jobs of one (G, Q, vote) class
@@ -2020,10 +2032,38 @@ int msvg_vit_search_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t d
   (void) hipSetDevice(c->device);
   std::vector<VitJob> jobs((size_t) np);
   for (int32_t i = 0; i < np; ++i) {
-    const int rc = vit_prepare(c, pids[i], dbid, targets + offsets[i], offsets[i + 1] - offsets[i], jobs[(size_t) i]);
+    const int rc = vit_check(c, pids[i], dbid, targets + offsets[i], offsets[i + 1] - offsets[i], jobs[(size_t) i]);
     if (rc != MSVG_OK) return rc;
   }
   if (total == 0) return MSVG_OK;
+  {
+    // The jobs' host preparation on up to 16 threads, jobs claimed one at a time.
+    const Db &d = *c->dbs[dbid];
+    std::vector<int> slots((size_t) np, 0);
+    for (int32_t i = 0; i < np; ++i) {
+      const VitProfile &v = *c->profiles[pids[i]]->vit;
+      if (jobs[(size_t) i].n > 0 && v.W > 1) {
+        slots[(size_t) i] = msvk_vit_slots(v.Q, v.W, &jobs[(size_t) i].a, c->ncu);
+        if (slots[(size_t) i] < 1) return fail(c, MSVG_EINVAL, "ViterbiFilter shape has no launch");
+      }
+    }
+    std::atomic<int> next{0};
+    auto work = [&]() {
+      for (int i; (i = next.fetch_add(1)) < np;) {
+        const Profile &p = *c->profiles[pids[i]];
+        vit_fill(*p.vit, p.M, d, slots[(size_t) i], targets + offsets[i], jobs[(size_t) i]);
+      }
+    };
+    const int nth = std::max(1, std::min({16, (int) np, (int) std::max(1u, std::thread::hardware_concurrency()),
+                                          (int) (total / 20000) + 1}));
+    std::vector<std::thread> th;
+    try {
+      for (int t = 1; t < nth; ++t) th.emplace_back(work);
+    } catch (...) {                    // a thread that cannot start: the calling thread takes its jobs
+    }
+    work();
+    for (auto &x : th) x.join();
+  }
   int64_t pos = 0;
   for (VitJob &j : jobs) { j.base = pos; pos += j.n; }
   std::vector<BandLaunch> bands;

@@ -74,6 +74,10 @@ static void score_model(int alphatype, int M, int K, int Kp, const float *mat, c
   rc = msvh_vit_convert_abc(alphatype, M, mat, t, rwv.data(), twv.data(), vs, &sw, &nj);
   CHECK(rc == 0, "msvh_vit_convert_abc rc %d (M %d)", rc, M);
   orc_vprofile vp{M, Kp, rwv.data(), twv.data(), vs[0], vs[1], vs[2], vs[3], sw, nj};
+  // the ViterbiFilter planner's lazy-F estimate over the same fields
+  const double lr = msvh_vit_lazyf_rate(rwv.data(), twv.data(), M, Kp, vs[0], vs[2], vs[3],
+                                        msvh_vit_move(sw, nj, 350), 64);
+  CHECK(lr >= 0.0 && lr <= 1.0, "lazy-F rate %g (M %d)", lr, M);
 
   const auto tg = make_targets(K, Kp, seed);
   std::vector<uint8_t> res;

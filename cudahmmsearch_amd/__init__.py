@@ -543,20 +543,26 @@ class MSVContext:
     """One GPU, one context (not shared between host threads)."""
 
     def __init__(self, device=0, tiles="default"):
-        """tiles: "default" loads the persisted autotuner table if present,
-        None uses the built-in cost model, or a list of tile dicts."""
+        """tiles: "default" loads the persisted autotuner tables if present
+        (the batched MSV pass's and the ViterbiFilter shapes), None uses the
+        built-in rules, or a list of MSV tile dicts."""
         self.lib = load_library()
         self._c = C.c_void_p()
         rc = self.lib.msvg_init(device, C.byref(self._c))
         if rc != 0:
             raise MSVError(rc, f"msvg_init failed on device {device} (is a GPU visible?)")
         self._keep = {}
+        vit_tiles = None
         if tiles == "default":
             from . import autotune
             tiles = autotune.load()
+            vit_tiles = autotune.load(key="vit_tiles")
         if tiles:
             for t in tiles:
                 self.set_tile(t["lo"], t["hi"], t["lanes"], t["waves"])
+        if vit_tiles:
+            for t in vit_tiles:
+                self.set_vit_tile(t["lo"], t["hi"], t["lanes"], t["waves"])
 
     def close(self):
         if self._c:

@@ -63,8 +63,9 @@ def test_bins_follow_the_measured_crossovers():
     assert bins[0]["lo"] == 1 and bins[-1]["hi"] == 3000
     for a, b in zip(bins, bins[1:]):
         assert b["lo"] == a["hi"] + 1
-    for b, edge in zip(bins[:-1], (120, 700, 1025)):
+    for b, edge in zip(bins[:-1], (120, 700)):
         assert abs(b["hi"] + 1 - edge) <= max(3, edge // 32), (b["hi"], edge)
+    assert bins[2]["hi"] == 1024                # 32 lanes win up to where they stop fitting
     for b in bins:
         assert autotune.feasible(b["lo"], b["lanes"]) or b["lo"] == 1
 
@@ -80,6 +81,14 @@ def test_a_near_tie_keeps_the_previous_bin():
     assert [b["lanes"] for b in bins] == [8]
 
 
+def test_a_bin_ends_where_its_choice_stops_fitting():
+    # probes up to M = 4096, where no lane count fits: the 64-lane bin ends
+    # at its own limit (M = 3584 over 20 residue codes), not at the last probe
+    db = ca.gen_database(5, 200, 300, 0.5, 20, 2000)
+    bins = autotune.tune(FakeCtx(cost), db, m_lo=16, m_hi=4096, probes=18, log=lambda *_: None)
+    assert bins[-1]["lanes"] == 64 and bins[-1]["hi"] == 3584
+
+
 def test_feasible_matches_the_batched_kernel_limits():
     # registers per lane: 16 at 8 and 32 lanes, 24 at 16, 32 at 64
     assert autotune.feasible(256, 8) and not autotune.feasible(257, 8)
@@ -89,3 +98,73 @@ def test_feasible_matches_the_batched_kernel_limits():
     assert autotune.feasible(3584, 64) and not autotune.feasible(3585, 64)
     # narrow layouts hold at most 32 residue rows
     assert not autotune.feasible(100, 16, rows=33)
+
+
+# ViterbiFilter shapes: 16 lanes best below M = 300, 32 lanes to 800, one
+# wave to 3000 (but it stops fitting above M = 2432, 19 registers), two waves
+# beyond (up to M = 4096, 16 registers), where four waves take over; eight
+# waves always slower.
+VCROSS = {(16, 1): (0, 300), (32, 1): (300, 800), (64, 1): (800, 3000), (64, 2): (3000, 1 << 20),
+          (64, 4): (1 << 20, 1 << 21), (64, 8): (1 << 20, 1 << 21)}
+
+
+def vcost(M, shape):
+    lo, hi = VCROSS[shape]
+    d = 0.0 if lo <= M < hi else min(abs(M - lo), abs(M - hi)) / max(M, 1) + 0.05
+    return M * 1e-3 * (1.0 + d)
+
+
+class FakeVitCtx(FakeCtx):
+    def __init__(self, cost_fn):
+        super().__init__(cost_fn)
+        self.vtiles, self.shape, self.targets = [], None, None
+
+    def clear_vit_tiles(self):
+        self.vtiles = []
+
+    def set_vit_tile(self, lo, hi, lanes, waves=1):
+        self.vtiles.append((lo, hi, lanes, waves))
+
+    def set_viterbi(self, pid, vit):
+        assert vit.M == self.M
+        self.shape = None
+        for lo, hi, lanes, waves in self.vtiles:
+            if lo <= self.M <= hi:
+                self.shape = (lanes, waves)
+        assert self.shape is not None and autotune.vit_feasible(self.M, self.shape)
+
+    def vit_search(self, pid, did, targets):
+        self.targets = np.asarray(targets)
+        self.calls += 1
+
+    def stats(self):
+        return {"ms_fast": self.cost(self.M, self.shape)}
+
+
+def test_vit_bins_follow_the_measured_crossovers():
+    db = ca.gen_database(5, 300, 300, 0.5, 20, 4000)
+    ctx = FakeVitCtx(vcost)
+    bins = autotune.vit_tune(ctx, db, m_lo=16, m_hi=5000, probes=16, n_targets=100, max_len=1500,
+                             log=lambda *_: None)
+    assert [(b["lanes"], b["waves"]) for b in bins] == [(16, 1), (32, 1), (64, 1), (64, 2), (64, 4)]
+    assert bins[0]["lo"] == 1 and bins[-1]["hi"] == 5000
+    for a, b in zip(bins, bins[1:]):
+        assert b["lo"] == a["hi"] + 1
+    for b, edge in zip(bins[:2], (300, 800)):
+        assert abs(b["hi"] + 1 - edge) <= max(3, edge // 32), (b["hi"], edge)
+    # edges where the left shape stops fitting (and still wins) are exact
+    assert bins[2]["hi"] == 2432 and bins[3]["hi"] == 4096
+    # timed on a sample of the short targets, and the tiles cleared afterwards
+    assert ctx.targets.size == 100 and np.all(np.asarray(db.lens)[ctx.targets] <= 1500)
+    assert ctx.vtiles == []
+
+
+def test_vit_feasible_matches_the_kernel_limits():
+    # 16 registers per lane at 16 and 32 lanes and for 2, 4, 8 waves per target
+    assert autotune.vit_feasible(512, (16, 1)) and not autotune.vit_feasible(513, (16, 1))
+    assert autotune.vit_feasible(1024, (32, 1)) and not autotune.vit_feasible(1025, (32, 1))
+    assert autotune.vit_feasible(4096, (64, 2)) and not autotune.vit_feasible(4097, (64, 2))
+    assert autotune.vit_feasible(16384, (64, 8)) and not autotune.vit_feasible(16385, (64, 8))
+    # one wave: tables of Kp + 1 emission and 3 transition rows within 160 KiB
+    assert autotune.vit_feasible(2432, (64, 1)) and not autotune.vit_feasible(2433, (64, 1))
+    assert autotune.vit_feasible(2560, (64, 1), Kp=18)

@@ -29,17 +29,6 @@
 
 namespace {
 
-// Wave-uniform load through the scalar cache (read-only data only).  The
-// host pass of the compiler only parses device code; it gets a plain load.
-template <class T>
-__device__ __forceinline__ T sload(const T *p) {
-#if __HIP_DEVICE_COMPILE__
-  return *(const __attribute__((address_space(4))) T *) p;
-#else
-  return *p;
-#endif
-}
-
 // LDS byte address -> pointer.  msv_batch declares no static LDS, so its
 // dynamic LDS (the table) starts at address 0 and a table byte offset is
 // its LDS address; building the pointer from the integer keeps the register

@@ -53,6 +53,17 @@ __device__ __forceinline__ uint32_t shift_slot(uint32_t last, int lane) {
   return __builtin_amdgcn_alignbit(last, prev, 16);
 }
 
+// Wave-uniform load through the scalar cache (read-only data only).  The
+// host pass of the compiler only parses device code; it gets a plain load.
+template <class T>
+__device__ __forceinline__ T sload(const T *p) {
+#if __HIP_DEVICE_COMPILE__
+  return *(const __attribute__((address_space(4))) T *) p;
+#else
+  return *p;
+#endif
+}
+
 __device__ __forceinline__ uint32_t byte_of(const uint4 &c, int rr) {
   const uint32_t w = (rr < 4) ? c.x : (rr < 8) ? c.y : (rr < 12) ? c.z : c.w;
   return (w >> (8 * (rr & 3))) & 0xffu;

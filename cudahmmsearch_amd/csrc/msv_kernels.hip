@@ -167,91 +167,156 @@ __global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
 
 // ---------------------------------------------------------------------------
 // Stage 1b: the same J-free pass over one column segment of a long model.
-// One target per wave (G = 64), static round-robin over sorted targets.
+// One target per wave (G = 64), static round-robin over the sorted targets.
+// Row i's first position takes as its diagonal predecessor the value that
+// left the previous segment's last position in row i-1 (the carry: one u16
+// per residue, indexed like the residues, overwritten in place by this
+// segment's own carry); the running maximum of the earlier segments arrives
+// per target in partial[].
+//
+// The carries move by DPP, not through scalar registers.  One register x
+// holds two queues: lanes 0..15 the incoming carries of a block of 16 rows
+// (lane 0 = the current row's), lanes 48..63 the outgoing ones (lane 63 =
+// the latest).  Each row one row_shl:1 advances both queues (the row's
+// wave shift takes lane 0 of x where it leaves a hole, and lane 63, which
+// has no source, takes the value that left the segment's last position in
+// the row before); the block's outgoing carries are stored once per block.
+// Residues come through the scalar cache, so a row's table address is
+// scalar arithmetic plus one v_add; only a target's real rows are computed
+// (its last block runs L mod 16 rows).
 // ---------------------------------------------------------------------------
+
+// One row of a segment: the descending register update with the carry
+// entering lane 0's low half, and the row's cells into the running maximum
+// (odd Q: an even row leaves register 0 to the odd row after it).
+template <int Q, int RR>
+__device__ __forceinline__ void seg_row(uint32_t (&r)[Q], const uint32_t *e, uint32_t &acc, uint32_t &x) {
+  const uint32_t last = r[Q - 1];
+  const uint32_t xs = __builtin_amdgcn_update_dpp(last, x, 0x101 /* row_shl:1 */, 0xf, 0xf, false);
+  // wave_shr:1; lane 0 has no source and keeps x, whose high half there is
+  // this row's carry (alignbit moves it into the low half of sh)
+  const uint32_t prev = __builtin_amdgcn_update_dpp(x, last, 0x138, 0xf, 0xf, false);
+  x = xs;
+  const uint32_t sh = __builtin_amdgcn_alignbit(last, prev, 16);
+  const uint32_t r0_prev = r[0];
+#pragma unroll
+  for (int j = Q - 1; j >= 1; --j) r[j] = cell(r[j - 1], e[j]);
+  r[0] = cell(sh, e[0]);
+  if constexpr ((Q & 1) != 0 && (RR & 1) != 0) acc = max3h(acc, r0_prev, r[Q - 1]);
+  if constexpr ((Q & 1) != 0 && (RR & 1) == 0) {
+#pragma unroll
+    for (int j = 1; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);
+  } else {
+#pragma unroll
+    for (int j = 0; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);
+  }
+}
+
+template <int Q>
+__device__ __forceinline__ void seg_task(const SegArgs &a, const uint4 *tl, int s, int lane) {
+  constexpr int QB = (Q + 3) / 4;
+  const int L = sload(a.db.len + s);
+  const int Lpad = (L + 15) & ~15;
+  const int Lfull = L & ~15;
+  const int64_t base = sload(a.db.off + s);
+  const uint8_t *p = a.db.res + base;
+  uint16_t *cy = a.carry + base;
+  const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
+  const uint4 null4 = make_uint4(nb, nb, nb, nb);
+
+  uint32_t r[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) r[j] = 0u;
+  uint32_t acc = 0u;
+  uint32_t clast = 0u;               // carry of the row before the block, high half
+
+  // The incoming carries of rows i0 .. i0+15 (entering from rows i0-1 ..
+  // i0+14) in lanes 0..15, high halves; read before this block's own
+  // carries overwrite rows i0 .. i0+15.
+  auto carries_in = [&](int i0) -> uint32_t {
+    uint32_t cin = 0u;
+    if (!a.first && lane < 16) cin = (uint32_t) cy[i0 + lane] << 16;
+    const uint32_t x = __builtin_amdgcn_update_dpp(clast, cin, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+    clast = (uint32_t) __builtin_amdgcn_readlane((int) cin, 15);
+    return x;
+  };
+
+  uint4 c = (Lpad > 0) ? sload(reinterpret_cast<const uint4 *>(p)) : null4;
+  uint32_t e[QB * 4];
+  load_row<Q>(tl, byte_of(c, 0), e);
+  for (int i0 = 0; i0 < Lfull; i0 += 16) {
+    const int i1 = i0 + 16;
+    const int inext = __builtin_amdgcn_readfirstlane(i1 < Lpad ? i1 : 0);
+    const uint4 cl = sload(reinterpret_cast<const uint4 *>(p + inext));
+    const uint4 cn = (i1 < Lpad) ? cl : null4;
+    uint32_t x = carries_in(i0);
+    static_for<0, 16>([&](auto rrc) {
+      constexpr int rr = decltype(rrc)::value;
+      uint32_t en[QB * 4];
+      load_row<Q>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
+      seg_row<Q, rr>(r, e, acc, x);
+#pragma unroll
+      for (int k = 0; k < Q; ++k) e[k] = en[k];
+      row_fence<Q>(r);
+      asm volatile("" : "+v"(acc), "+v"(x));
+    });
+    // the last row's outgoing carry into lane 63: lane 48 + k = row k
+    x = __builtin_amdgcn_update_dpp(r[Q - 1], x, 0x101, 0xf, 0xf, false);
+    if (!a.last && lane >= 48) cy[i0 + lane - 48] = (uint16_t) (x >> 16);
+    c = cn;
+  }
+  const int rem = L - Lfull;                           // wave-uniform
+  if (rem > 0) {
+    uint32_t x = carries_in(Lfull);
+    static_for<0, 16>([&](auto rrc) {
+      constexpr int rr = decltype(rrc)::value;
+      if (rr < rem) {
+        uint32_t en[QB * 4];
+        if constexpr (rr < 15) load_row<Q>(tl, byte_of(c, rr + 1), en);
+        seg_row<Q, rr>(r, e, acc, x);
+        if constexpr (rr < 15) {
+#pragma unroll
+          for (int k = 0; k < Q; ++k) e[k] = en[k];
+        }
+        row_fence<Q>(r);
+        asm volatile("" : "+v"(acc), "+v"(x));
+      }
+    });
+    if constexpr ((Q & 1) != 0)
+      if (rem & 1) acc = max2h(acc, r[0]);
+    x = __builtin_amdgcn_update_dpp(r[Q - 1], x, 0x101, 0xf, 0xf, false);
+    // lane 64 - rem + k = row k
+    if (!a.last && lane >= 64 - rem) cy[Lfull + lane - (64 - rem)] = (uint16_t) (x >> 16);
+  }
+
+  const h2v ah = as_h2(acc);
+  const float m = wave_max_f32(fmaxf((float) ah.x, (float) ah.y));
+  if (lane == 0) {
+    int Rp = (int) (m * 2048.0f + 0.5f);
+    if (!a.first) Rp = max(Rp, (int) a.partial[s]);
+    if (!a.last) {
+      a.partial[s] = (uint16_t) Rp;
+    } else {
+      const uint32_t w = decide(Rp, sload(a.db.tjb + s), a.prof);
+      a.prof.res[s] = (uint16_t) w;
+      if ((w >> 8) == RES_SLOW)
+        defer(w, a.prof_idx, s, a.slow, a.slow_count, a.slow_cap, a.jslow, a.jslow_count, a.jslow_cap);
+    }
+  }
+}
+
 template <int Q>
 __global__ void __launch_bounds__(1024) msv_seg(SegArgs a) {
   extern __shared__ uint4 lds[];
   constexpr int QB = (Q + 3) / 4;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane((int) (threadIdx.x >> 6));
   const int nwaves = blockDim.x >> 6;
   const int ntab = (a.db.R + 1) * QB * 64;
   for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = a.tab[i];
   __syncthreads();
   const uint4 *tl = lds + lane;
-
-  for (int64_t s = (int64_t) blockIdx.x * nwaves + wave; s < a.db.n; s += (int64_t) gridDim.x * nwaves) {
-    const int L = a.db.len[s];
-    const int Lpad = (L + 15) & ~15;
-    const int64_t base = a.db.off[s];
-    const uint8_t *p = a.db.res + base;
-    uint16_t *cy = a.carry + base;
-    uint32_t r[Q];
-#pragma unroll
-    for (int j = 0; j < Q; ++j) r[j] = 0u;
-    uint32_t acc = 0u;
-    uint32_t cin_last = 0u;          // carry entering row i0 (value of row i0-1)
-
-    // Rows are pipelined by hand: the table entries of row i+1 are read while
-    // row i is computed, and no other reordering across rows is allowed
-    // (the compiler would otherwise hoist all sixteen rows' reads).
-    uint4 c = *reinterpret_cast<const uint4 *>(p);
-    uint32_t e[QB * 4];
-    load_row<Q>(tl, byte_of(c, 0), e);
-    for (int i0 = 0; i0 < Lpad; i0 += 16) {
-      const int i1 = i0 + 16;
-      const uint4 cn = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
-      uint32_t cin = 0u;
-      if (!a.first && lane < 16) cin = cy[i0 + lane];
-      uint32_t cout = 0u;
-      static_for<0, 16>([&](auto rrc) {
-        constexpr int rr = decltype(rrc)::value;
-        uint32_t en[QB * 4];
-        load_row<Q>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
-        const uint32_t last = r[Q - 1];
-        uint32_t prev = __builtin_amdgcn_update_dpp(0u, last, 0x138, 0xf, 0xf, true);
-        // carry from the previous segment (all zero in the first segment,
-        // which is what the lane shift supplies there anyway)
-        const uint32_t ci = (rr == 0) ? cin_last : (uint32_t) __builtin_amdgcn_readlane((int) cin, rr == 0 ? 0 : rr - 1);
-        prev = write_lane<0>(prev, ci << 16);
-        const uint32_t sh = __builtin_amdgcn_alignbit(last, prev, 16);
-#pragma unroll
-        for (int j = Q - 1; j >= 1; --j) r[j] = cell(r[j - 1], e[j]);
-        r[0] = cell(sh, e[0]);
-        int j = 0;
-#pragma unroll
-        for (; j + 1 < Q; j += 2) acc = max3h(acc, r[j], r[j + 1]);
-        if constexpr ((Q & 1) != 0) acc = max2h(acc, r[Q - 1]);
-        const uint32_t co = ((uint32_t) __builtin_amdgcn_readlane((int) r[Q - 1], 63)) >> 16;
-        cout = write_lane<rr>(cout, co);
-#pragma unroll
-        for (int k = 0; k < Q; ++k) e[k] = en[k];
-        row_fence<Q>(r);
-        asm volatile("" : "+v"(acc));
-      });
-      cin_last = (uint32_t) __builtin_amdgcn_readlane((int) cin, 15);
-      if (!a.last && lane < 16) cy[i0 + lane] = (uint16_t) cout;
-      c = cn;
-    }
-
-    const h2v ah = as_h2(acc);
-    float m = fmaxf((float) ah.x, (float) ah.y);
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if (lane == 0) {
-      int Rp = (int) (m * 2048.0f + 0.5f);
-      if (!a.first) Rp = max(Rp, (int) a.partial[s]);
-      if (!a.last) {
-        a.partial[s] = (uint16_t) Rp;
-      } else {
-        const uint32_t w = decide(Rp, a.db.tjb[s], a.prof);
-        a.prof.res[s] = (uint16_t) w;
-        if ((w >> 8) == RES_SLOW)
-          defer(w, a.prof_idx, s, a.slow, a.slow_count, a.slow_cap, a.jslow, a.jslow_count, a.jslow_cap);
-      }
-    }
-  }
+  for (int s = blockIdx.x * nwaves + wave; s < a.db.n; s += gridDim.x * nwaves) seg_task<Q>(a, tl, s, lane);
 }
 
 // ---------------------------------------------------------------------------

@@ -143,6 +143,28 @@ def test_long_profiles_vs_oracle(ctx, M):
     ctx.remove_profile(pid)
 
 
+@pytest.mark.parametrize("M", [5000, 7500])
+def test_segmented_every_row_count(ctx, M):
+    """Column segments (5000: 3584 + 1416 positions, an even last segment;
+    7500: 3584 + 3584 + 332, an odd one) against targets of every length
+    0..49 (blocks of 16 rows with 0..15 rows left over, the carry between
+    blocks), lengths at multiples of 16 up to 4096, and core-emitted
+    homologs of the whole model whose best diagonals cross the segment
+    edges, in both modes, against the oracle."""
+    prof = ca.sample_profile(900 + M, M)
+    rng = np.random.default_rng(M)
+    seqs = [rng.integers(0, 20, L).astype(np.uint8) for L in range(50)]
+    seqs += [rng.integers(0, 20, L).astype(np.uint8) for L in (64, 256, 1024, 4096)]
+    seqs += ca.core_emit(prof.hmm, M + 1, 12)
+    db = ca.Database.from_list(seqs)
+    print(f"segmented M={M} n={db.n}", flush=True)
+    for mode in (ca.MODE_PLAIN, ca.MODE_BATCHED):
+        pid, did, *_ = _check_vs_oracle(ctx, prof, db, mode)
+        assert ctx.tile_of(M, did)[2] == 1
+        ctx.remove_database(did)
+        ctx.remove_profile(pid)
+
+
 def test_batched_multi_profile(ctx):
     """Eight profiles of uneven M against length-skewed targets: batched equals
     plain equals the oracle on a sample."""

@@ -909,6 +909,23 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     }
   }
 
+  // Column-segmented profiles: one profile's segments run in order (each
+  // reads the carries the one before wrote), different profiles need not.
+  // With a side stream (batched searches) they go round-robin over up to
+  // kJStreams streams forked from st, each stream with its own carry and
+  // partial-maximum buffer, so one profile's launch tail overlaps the next
+  // profile's work; otherwise in line on st.
+  int nseg = 0;
+  for (int i = 0; i < np; ++i) nseg += (!Gb[i] && segmented[i]) ? 1 : 0;
+  const int nss = (side && nseg > 1) ? std::min(kJStreams, nseg) : 0;
+  const size_t seg_bytes = ((size_t) d.packed * sizeof(uint16_t) + (size_t) d.n * sizeof(uint16_t) + 64 + 255) &
+                           ~(size_t) 255;
+  if (nseg) HIPCHK(c, b.d_seg.ensure((size_t) std::max(1, nss) * seg_bytes));
+  if (nss) {
+    HIPCHK(c, hipEventRecord(c->fork, st));
+    for (int k = 0; k < nss; ++k) HIPCHK(c, hipStreamWaitEvent(c->jst[k], c->fork, 0));
+  }
+  int useg = 0;
   for (int i = 0; i < np; ++i) {
     const Profile &p = *c->profiles[ids[i]];
     if (Gb[i]) continue;                                // scored by its batched launch
@@ -916,7 +933,9 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       // Long model: column segments of kSegQ registers, carrying the last
       // column of each row from one segment to the next.
       Tables &t = tables_for(c, ids[i], dbid);
-      HIPCHK(c, b.d_seg.ensure((size_t) d.packed * sizeof(uint16_t) + (size_t) d.n * sizeof(uint16_t) + 64));
+      const hipStream_t ss = nss ? c->jst[useg % nss] : st;
+      uint8_t *segbuf = b.d_seg.as<uint8_t>() + (size_t) (nss ? useg % nss : 0) * seg_bytes;
+      ++useg;
       const int span = 128 * seg_q(d.R);
       int si = 0;
       for (int k0 = 0; k0 < p.M; k0 += span, ++si) {
@@ -927,8 +946,8 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
         a.prof.Q = Q; a.prof.QB = (Q + 3) / 4;
         a.db = dd;
         a.tab = t.seg[si]->as<uint4>();
-        a.carry = b.d_seg.as<uint16_t>();
-        a.partial = reinterpret_cast<uint16_t *>(b.d_seg.as<uint8_t>() + (size_t) d.packed * sizeof(uint16_t));
+        a.carry = reinterpret_cast<uint16_t *>(segbuf);
+        a.partial = reinterpret_cast<uint16_t *>(segbuf + (size_t) d.packed * sizeof(uint16_t));
         a.first = (k0 == 0); a.last = (k0 + span >= p.M);
         a.prof_idx = i; a.slow = b.d_slow.as<SlowItem>(); a.slow_count = slow_count; a.slow_cap = (int32_t) b.slow_cap;
         a.jslow = jtab[i] ? b.d_jslow.as<SlowItem>() + (size_t) i * b.jcap : nullptr;
@@ -939,7 +958,7 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
         int grid, block;
         persistent_shape(c, lds, 0, grid, block);
         a.ntasks = (int32_t) n;
-        HIPCHK(c, msvk_launch_seg(Q, &a, grid, block, lds, st));
+        HIPCHK(c, msvk_launch_seg(Q, &a, grid, block, lds, ss));
         ++n_launch;
       }
       if (jtab[i]) {
@@ -953,7 +972,8 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
         const int Q64 = qfor(p.M, 128);
         const size_t jl = fast_lds(d.R, Q64);
         const int per_cu = (int) std::max<size_t>(1, std::min<size_t>(4, kLdsBytes / jl));
-        if ((rc = launch_j(ja, Q64, jl, c->ncu * per_cu)) != MSVG_OK) return rc;
+        if (nss) HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jl, ss));   // after its last segment
+        else if ((rc = launch_j(ja, Q64, jl, c->ncu * per_cu)) != MSVG_OK) return rc;
       }
       continue;
     }
@@ -1004,6 +1024,11 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
       if ((rc = launch_j(ja, Q64, jl, c->ncu * per_cu)) != MSVG_OK) return rc;
     }
   }
+  if (nss)
+    for (int k = 0; k < nss; ++k) {
+      HIPCHK(c, hipEventRecord(c->jdone[k], c->jst[k]));
+      HIPCHK(c, hipStreamWaitEvent(st, c->jdone[k], 0));
+    }
   if (side_used) {
     HIPCHK(c, hipEventRecord(c->join, side));
     HIPCHK(c, hipStreamWaitEvent(st, c->join, 0));

@@ -222,7 +222,6 @@ __device__ __forceinline__ void seg_task(const SegArgs &a, const uint4 *tl, int 
   const uint8_t *p = a.db.res + base;
   uint16_t *cy = a.carry + base;
   const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
-  const uint4 null4 = make_uint4(nb, nb, nb, nb);
 
   uint32_t r[Q];
 #pragma unroll
@@ -230,26 +229,50 @@ __device__ __forceinline__ void seg_task(const SegArgs &a, const uint4 *tl, int 
   uint32_t acc = 0u;
   uint32_t clast = 0u;               // carry of the row before the block, high half
 
+  // Loads for a block are issued one block ahead, so their latency is
+  // covered by the block before.  Residues: dword (lane & 3) of the 16-byte
+  // block (the null row past the target's end), made wave-uniform with
+  // readlane when the block starts.  Incoming carries: rows i .. i+15 in
+  // lanes 0..15, read before this segment's carries of those rows are
+  // written (a block writes only its own rows).
+  auto load_res = [&](int i) -> uint32_t {
+    return (i < Lpad) ? reinterpret_cast<const uint32_t *>(p + i)[lane & 3] : nb;
+  };
+  auto res_of = [](uint32_t v) -> uint4 {
+    return make_uint4((uint32_t) __builtin_amdgcn_readlane((int) v, 0), (uint32_t) __builtin_amdgcn_readlane((int) v, 1),
+                      (uint32_t) __builtin_amdgcn_readlane((int) v, 2), (uint32_t) __builtin_amdgcn_readlane((int) v, 3));
+  };
+  auto load_cin = [&](int i) -> uint32_t {
+    return (!a.first && lane < 16 && i < Lpad) ? (uint32_t) cy[i + lane] : 0u;
+  };
   // The incoming carries of rows i0 .. i0+15 (entering from rows i0-1 ..
-  // i0+14) in lanes 0..15, high halves; read before this block's own
-  // carries overwrite rows i0 .. i0+15.
-  auto carries_in = [&](int i0) -> uint32_t {
-    uint32_t cin = 0u;
-    if (!a.first && lane < 16) cin = (uint32_t) cy[i0 + lane] << 16;
+  // i0+14) in lanes 0..15, high halves.
+  auto carries_of = [&](uint32_t cin16) -> uint32_t {
+    const uint32_t cin = cin16 << 16;
     const uint32_t x = __builtin_amdgcn_update_dpp(clast, cin, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
     clast = (uint32_t) __builtin_amdgcn_readlane((int) cin, 15);
     return x;
   };
 
-  uint4 c = (Lpad > 0) ? sload(reinterpret_cast<const uint4 *>(p)) : null4;
+  // A block's outgoing carries (lanes 48..63 of xo) are stored when the
+  // next block starts, after its wait for the loads issued a block before,
+  // so that wait does not include a store just issued.
+  auto store_out = [&](uint32_t xo, int io) {
+    if (!a.last && lane >= 48) cy[io + lane - 48] = (uint16_t) (xo >> 16);
+  };
+
+  uint4 c = res_of(load_res(0));
+  uint32_t rv = load_res(16);
+  uint32_t cv = load_cin(0);
+  uint32_t xo = 0u;
   uint32_t e[QB * 4];
   load_row<Q>(tl, byte_of(c, 0), e);
   for (int i0 = 0; i0 < Lfull; i0 += 16) {
-    const int i1 = i0 + 16;
-    const int inext = __builtin_amdgcn_readfirstlane(i1 < Lpad ? i1 : 0);
-    const uint4 cl = sload(reinterpret_cast<const uint4 *>(p + inext));
-    const uint4 cn = (i1 < Lpad) ? cl : null4;
-    uint32_t x = carries_in(i0);
+    const uint4 cn = res_of(rv);                      // block i0 + 16
+    uint32_t x = carries_of(cv);
+    if (i0 > 0) store_out(xo, i0 - 16);
+    rv = load_res(i0 + 32);
+    cv = load_cin(i0 + 16);
     static_for<0, 16>([&](auto rrc) {
       constexpr int rr = decltype(rrc)::value;
       uint32_t en[QB * 4];
@@ -261,13 +284,13 @@ __device__ __forceinline__ void seg_task(const SegArgs &a, const uint4 *tl, int 
       asm volatile("" : "+v"(acc), "+v"(x));
     });
     // the last row's outgoing carry into lane 63: lane 48 + k = row k
-    x = __builtin_amdgcn_update_dpp(r[Q - 1], x, 0x101, 0xf, 0xf, false);
-    if (!a.last && lane >= 48) cy[i0 + lane - 48] = (uint16_t) (x >> 16);
+    xo = __builtin_amdgcn_update_dpp(r[Q - 1], x, 0x101, 0xf, 0xf, false);
     c = cn;
   }
+  if (Lfull > 0) store_out(xo, Lfull - 16);
   const int rem = L - Lfull;                           // wave-uniform
   if (rem > 0) {
-    uint32_t x = carries_in(Lfull);
+    uint32_t x = carries_of(cv);
     static_for<0, 16>([&](auto rrc) {
       constexpr int rr = decltype(rrc)::value;
       if (rr < rem) {

@@ -161,6 +161,18 @@ __device__ __forceinline__ void defer(uint32_t w, int prof_idx, int64_t s, SlowI
   if (k < slow_cap) slow[k] = SlowItem{prof_idx, (int32_t) s};
 }
 
+// Wave maximum of non-negative integers by the same DPP steps (the zeros
+// that bound_ctrl shifts in never win); every lane gets it.
+__device__ __forceinline__ int wave_max_nonneg(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 __device__ __forceinline__ float wave_max_f32(float v) {
   // Row-wise DPP reduction, then across rows; lane 63 ends with the maximum.
   v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, true)));

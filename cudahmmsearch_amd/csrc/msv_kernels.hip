@@ -598,18 +598,26 @@ __global__ void __launch_bounds__(64 * SLOW_WAVES) msv_slow(SlowArgs a) {
         const uint32_t prev = __builtin_amdgcn_update_dpp(0u, last, 0x138, 0xf, 0xf, true);
         uint32_t mpv = __builtin_amdgcn_alignbit(last, prev, 16);  // :146, zero shifted in
         u16x2 xE = {0, 0};
-        for (int j = 0; j < Qs; ++j) {
-          u16x2 sv = __builtin_elementwise_max(as_u2(mpv), xBv);             // :150
-          sv = __builtin_elementwise_min(sv + biasv, c255);                    // :151
-          sv = __builtin_elementwise_sub_sat(sv, as_u2(cp[j * 64]));           // :152
-          xE = __builtin_elementwise_max(xE, sv);                              // :153
-          mpv = mydp[j * 64 + lane];
-          mydp[j * 64 + lane] = u2_u32(sv);
-        }
-        int m = max((int) xE.x, (int) xE.y);
+        // Sixteen registers at a time: their costs and old DP values are read
+        // together, so a row waits for one load latency per sixteen registers.
+        for (int j0 = 0; j0 < Qs; j0 += 16) {
+          uint32_t cst[16], old[16];
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-        m = __builtin_amdgcn_readfirstlane(m);
+          for (int u = 0; u < 16; ++u)
+            if (j0 + u < Qs) { cst[u] = cp[(j0 + u) * 64]; old[u] = mydp[(j0 + u) * 64 + lane]; }
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            if (j0 + u < Qs) {
+              u16x2 sv = __builtin_elementwise_max(as_u2(mpv), xBv);           // :150
+              sv = __builtin_elementwise_min(sv + biasv, c255);                  // :151
+              sv = __builtin_elementwise_sub_sat(sv, as_u2(cst[u]));             // :152
+              xE = __builtin_elementwise_max(xE, sv);                            // :153
+              mpv = old[u];
+              mydp[(j0 + u) * 64 + lane] = u2_u32(sv);
+            }
+          }
+        }
+        const int m = wave_max_nonneg(max((int) xE.x, (int) xE.y));
         if (m + pd.bias >= 255) { code = RES_ERANGE; break; }                 // :160-185
         const int e = max(0, m - pd.tec);                                      // :187
         xJ = max(xJ, e);                                                       // :188

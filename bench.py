@@ -14,8 +14,8 @@ on the host (N threads and one), and a shortened c5 serving phase (serving:
 persistent serving, one query per search, and batched serving, the queries
 waiting for a slot folded into one search; for each the capacity, p50/p99
 over a drained sweep at 50/75/90% of it, and p50/p99 at --rate req/s for
---serving-duration s).  For c2 and c3 the line also carries the ViterbiFilter
-stage on the MSV survivors (vit_stage: kernel time, GCUPS, roofline, the
+--serving-duration s).  For c2, c3 and c4 the line also carries the
+ViterbiFilter stage on the MSV survivors (vit_stage: kernel time, GCUPS, roofline, the
 reference's SSE p7_ViterbiFilter on a sample with a bit-identity check); the
 c3 line also carries it on the c2 survivors (vit_stage_c2).
 
@@ -191,7 +191,7 @@ def vit_stage(ca, ctx, profs, pids, did, db, reps=3, cpu=True):
         out["roofline"] = {"bound": "valu", "achieved": round(ach / 1e12, 3),
                            "peak": round(PEAK_LANE_INSTR / 1e12, 3), "unit": "T lane-instr/s",
                            "frac": round(ach / PEAK_LANE_INSTR, 4),
-                           "kernel": "vit_band (one launch per lanes/registers class of the profiles) / vit_kernel (multi-wave, one launch per profile)",
+                           "kernel": "vit_band (one launch per lanes/registers class of the profiles) / vit_seg (column segments of the longer models, one launch per segment)",
                            "algorithmic": f"14 int16 ops per DP cell = {VIT_INSTR_PER_CELL} packed lane-instructions"}
     if cpu and first is not None:
         out["cpu_baseline"] = vit_cpu_baseline(ca, db, *first)
@@ -694,7 +694,7 @@ def main():
         out["dropin"]["gcups"] = round(profs[0].M * sum(int(db.lens[i]) for i in range(out["dropin"]["calls"]))
                                        * out["dropin"]["calls_per_s"] / out["dropin"]["calls"] / 1e9, 3)
         log(f"drop-in: {out['dropin']}")
-    if not args.no_vit and args.workload in ("c2", "c3"):
+    if not args.no_vit and args.workload in ("c2", "c3", "c4"):
         # the batched search above left its results (and the pass bits) on the device
         ctx.search_device(pids, did, mode=ca.MODE_BATCHED)
         out["vit_stage"] = vit_stage(ca, ctx, profs, pids, did, db, cpu=not args.no_cpu)

@@ -125,6 +125,7 @@ def load_library(path=LIB_PATH):
         "msvg_set_tile": (C.c_int, [vp, i32, i32, i32, i32]),
         "msvg_set_vit_tile": (C.c_int, [vp, i32, i32, i32, i32]),
         "msvg_set_vit_vote": (C.c_int, [vp, i32]),
+        "msvg_set_vit_segments": (C.c_int, [vp, i32]),
         "msvg_set_tail_guard": (C.c_int, [vp, C.c_int]),
         "msvg_probe_lds": (C.c_int, [vp, C.POINTER(C.c_double)]),
         "msvg_probe_forms": (C.c_int, [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64),
@@ -756,6 +757,12 @@ class MSVContext:
         estimated rate (default); 0: every row closes the delete paths; 1:
         every row votes (msvg_set_vit_vote).  Results do not change."""
         self._chk(self.lib.msvg_set_vit_vote(self._c, mode))
+
+    def set_vit_segments(self, on):
+        """True: models beyond one wave's ViterbiFilter tables run as column
+        segments (default); False: on the multi-wave kernel
+        (msvg_set_vit_segments).  Results do not change."""
+        self._chk(self.lib.msvg_set_vit_segments(self._c, 1 if on else 0))
 
     def set_tail_guard(self, on):
         """Batched mode: widen the lane group where the longest target would

@@ -232,7 +232,6 @@ struct VitArgs {
   int32_t n;
   int32_t *out;
   int32_t e_loop, e_move, base_w, ddbound_w;
-  int32_t etab_in_lds;         // one-wave shape: stage the emission table in LDS
   const int32_t *uoff;         // slot k (a wave, or a workgroup of a multi-wave
                                // shape) runs units [uoff[k], uoff[k+1]); a unit
                                // is one wave's targets
@@ -243,4 +242,28 @@ struct VitArgs {
   const VitProfDev *profs;
   const int32_t *wg_prof;
   const int32_t *ustart;
+};
+
+// vit_seg: one column segment of a long model (its positions k0 + 1 ..
+// k0 + 128 Q, tables in the 64-lane layout of vit_band), one target per wave.
+// Per item row the segment reads what the segment before it left (the M, I
+// and D values at its last position, and the D value entering the next
+// position from it) and writes its own in their place; the items' largest
+// row maximum and whether any row could raise J (or overflow) carry over in
+// partial[].  The last segment writes the result word, or VIT_DEFER for an
+// item some row of which could change B: such items are scored again by the
+// exact multi-wave kernel.
+#define VIT_DEFER (1 << 24)
+struct VitSegArgs {
+  DbDev db;
+  const uint32_t *etab, *ttab;
+  int32_t e_loop, e_move, base_w, ddbound_w;
+  const int64_t *items;        // item -> sorted target index
+  const int16_t *move;         // item -> xw[N|C|J][MOVE] for its length
+  const int64_t *coff;         // item -> its first carry row
+  int16_t *cM, *cI, *cD, *cE;  // carries per item row
+  int32_t *partial;            // item -> largest row maximum (low 16 bits) | could-fire (bit 16)
+  int32_t *out;                // item -> result word (last segment)
+  const int32_t *uoff;         // wave slot k runs items [uoff[k], uoff[k+1])
+  int32_t first, last;
 };

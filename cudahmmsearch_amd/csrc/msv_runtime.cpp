@@ -53,6 +53,8 @@ extern "C" hipError_t msvk_build_bat_table(const uint8_t *rbv, int M, const uint
                                            int bias, uint32_t *out, hipStream_t st);
 extern "C" hipError_t msvk_launch_fast(int G, int Q, const FastArgs *a, int grid, int block, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_seg(int Q, const SegArgs *a, int grid, int block, size_t lds, hipStream_t st);
+extern "C" hipError_t msvk_launch_vit_seg(int Q, const VitSegArgs *a, int grid, hipStream_t st);
+extern "C" int msvk_vit_seg_cap(int Q, int R, int ncu);
 extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st);
 extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, hipStream_t st);
@@ -156,6 +158,11 @@ struct VitProfile {               // ViterbiFilter fields (model order)
                                    // (W = 1: vit_band; W > 1: vit_kernel)
   int vote = 1;                    // vit_band: rows vote on the lazy-F test (else always close)
   DevMem d_ttab;                   // transition table, VitArgs layout
+  // Multi-wave shapes (W > 1) also have column segments for vit_seg:
+  // positions per segment (128 x at most 16 registers) and each segment's
+  // transition table in the 64-lane layout.
+  int seg_span = 0;
+  std::vector<std::unique_ptr<DevMem>> d_seg_ttab;
 };
 
 struct Profile {
@@ -217,6 +224,7 @@ struct Tables {                          // per (profile, db)
   std::vector<std::unique_ptr<DevMem>> seg;      // segment tables (M > fast limit)
   std::unique_ptr<DevMem> slow;
   std::unique_ptr<DevMem> vemit;                 // ViterbiFilter emission table
+  std::vector<std::unique_ptr<DevMem>> vseg_emit; // ViterbiFilter column segments' emission tables
 };
 
 struct Tile { int lo, hi, lanes, waves; };
@@ -265,6 +273,7 @@ struct msvg_ctx {
   std::vector<Tile> tiles;
   std::vector<Tile> vtiles;        // ViterbiFilter shapes by model length (msvg_set_vit_tile)
   int vit_vote = -1;               // msvg_set_vit_vote: -1 by the profile's lazy-F rate, else forced
+  int vit_segments = 1;            // msvg_set_vit_segments: multi-wave shapes run as column segments
   bool tail_guard = true;
   Bufs eager;                      // buffers of plain and eager searches
   Bufs *last = nullptr;            // buffers holding the latest results
@@ -293,6 +302,8 @@ struct msvg_ctx {
   size_t h_scancap = 0;
   // ViterbiFilter searches
   DevMem d_vitems, d_vmove, d_vout, d_vuoff, d_vprofs;
+  DevMem d_vcoff, d_vpartial, d_vcarry;   // vit_seg: carry offsets, partial maxima, carries
+  int64_t vcrows = 0;                       // rows of each of the four carry arrays in d_vcarry
   hipStream_t vstream[kVitStreams] = {};   // multi-profile searches: launches round-robin
   hipEvent_t vdone[kVitStreams] = {};
 };
@@ -1286,6 +1297,31 @@ int vit_emission_table(msvg_ctx *c, int pid, int dbid, const uint32_t **out) {
   return MSVG_OK;
 }
 
+// The column segments' emission tables (vit_seg; 64-lane layout) over the
+// database's rows.
+int vit_seg_emission_tables(msvg_ctx *c, int pid, int dbid) {
+  Profile &p = *c->profiles[pid];
+  VitProfile &v = *p.vit;
+  Db &d = *c->dbs[dbid];
+  Tables &t = tables_for(c, pid, dbid);
+  if (!t.vseg_emit.empty()) return MSVG_OK;
+  for (int k0 = 0; k0 < p.M; k0 += v.seg_span) {
+    const int Ms = std::min(v.seg_span, p.M - k0), Qs = (Ms + 127) / 128;
+    const size_t per = (size_t) Qs * 64;
+    std::vector<uint32_t> h((size_t) (d.R + 1) * per);
+    for (int r = 0; r <= d.R; ++r) {
+      const int x = (r < d.R) ? d.rowcode[r] : -1;
+      vit_fill(h, (size_t) r * per, Qs, 64, Ms,
+               [&](int k) { return x < 0 ? -32768 : (int) v.rwv[(size_t) x * p.M + k0 + k - 1]; });
+    }
+    auto mem = std::make_unique<DevMem>();
+    HIPCHK(c, mem->ensure(h.size() * 4));
+    HIPCHK(c, hipMemcpy(mem->p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    t.vseg_emit.push_back(std::move(mem));
+  }
+  return MSVG_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1671,6 +1707,13 @@ int msvg_set_vit_vote(msvg_ctx *c, int32_t mode) {
   return MSVG_OK;
 }
 
+int msvg_set_vit_segments(msvg_ctx *c, int32_t on) {
+  if (!c) return MSVG_EINVAL;
+  if (on != 0 && on != 1) return fail(c, MSVG_EINVAL, "on must be 0 or 1");
+  c->vit_segments = on;
+  return MSVG_OK;
+}
+
 int msvg_profile_set_evparams(msvg_ctx *c, int32_t id, float mmu, float mlambda) {
   if (!c || !c->profiles.count(id)) return fail(c, MSVG_EINVAL, "unknown profile id");
   c->profiles[id]->mu = mmu;
@@ -1741,9 +1784,28 @@ int msvg_profile_set_viterbi(msvg_ctx *c, int32_t pid, const msvg_vprofile *vp) 
     vit_fill(h, t * per, v->Q, vit_lanes(*v), p.M, [&](int k) { return (int) v->twv[(size_t) t * p.M + k - 1]; });
   HIPCHK(c, v->d_ttab.ensure(h.size() * 4));
   HIPCHK(c, hipMemcpy(v->d_ttab.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  if (v->W > 1) {
+    // Column segments for vit_seg: as few as 15 registers per lane (1920
+    // positions) allow, of equal width (a multiple of 128 positions; the
+    // last one takes the rest), each with its transitions at their model
+    // positions.
+    const int nseg = (p.M + 1919) / 1920;
+    v->seg_span = 128 * (((p.M + nseg - 1) / nseg + 127) / 128);
+    for (int k0 = 0; k0 < p.M; k0 += v->seg_span) {
+      const int Ms = std::min(v->seg_span, p.M - k0), Qs = (Ms + 127) / 128;
+      std::vector<uint32_t> hs((size_t) 8 * Qs * 64);
+      for (int t = 0; t < 8; ++t)
+        vit_fill(hs, (size_t) t * Qs * 64, Qs, 64, Ms,
+                 [&](int k) { return (int) v->twv[(size_t) t * p.M + k0 + k - 1]; });
+      auto mem = std::make_unique<DevMem>();
+      HIPCHK(c, mem->ensure(hs.size() * 4));
+      HIPCHK(c, hipMemcpy(mem->p, hs.data(), hs.size() * 4, hipMemcpyHostToDevice));
+      v->d_seg_ttab.push_back(std::move(mem));
+    }
+  }
   p.vit = std::move(v);
   for (auto &kv : c->tables)                       // emission tables of the old fields
-    if (kv.first.first == pid) kv.second->vemit.reset();
+    if (kv.first.first == pid) { kv.second->vemit.reset(); kv.second->vseg_emit.clear(); }
   return MSVG_OK;
 }
 
@@ -1761,9 +1823,17 @@ struct VitJob {
   std::vector<int64_t> items;        // item q -> sorted target index
   std::vector<int16_t> mv;
   std::vector<int32_t> uoff;         // multi-wave shapes: workgroup k runs items [uoff[k], uoff[k+1])
+                                     // (column segments: wave slot k)
   int64_t ubase = 0;                 // where uoff sits in the device index array
   int64_t cells = 0;
   VitArgs a{};
+  // A multi-wave shape run as column segments (vit_seg): the launches'
+  // workgroups, each item's first carry row (from the job's own start) and
+  // the job's carry rows, and where they start in the device carry arrays.
+  bool seg = false;
+  int seg_grid = 0;
+  std::vector<int64_t> coff;
+  int64_t crows = 0, cbase = 0;
 };
 
 // One vit_band launch: the jobs of one (G, Q, vote) class.  Workgroup b
@@ -1799,6 +1869,8 @@ int vit_check(msvg_ctx *c, int32_t pid, int32_t dbid, const int64_t *targets, in
   const uint32_t *etab;
   int rc = vit_emission_table(c, pid, dbid, &etab);
   if (rc != MSVG_OK) return rc;
+  job.seg = v.W > 1 && c->vit_segments && v.seg_span > 0;
+  if (job.seg && (rc = vit_seg_emission_tables(c, pid, dbid)) != MSVG_OK) return rc;
   VitArgs &a = job.a;
   a.etab = etab;
   a.ttab = v.d_ttab.as<uint32_t>();
@@ -1857,6 +1929,15 @@ void vit_fill(const VitProfile &v, int M, const Db &d, int slots, const int64_t 
     if (m == INT32_MIN) m = msvh_vit_move(v.scale_w, v.nj, d.len_sorted[(size_t) s]);
     job.mv[(size_t) q] = (int16_t) m;
     job.cells += (int64_t) M * d.len_sorted[(size_t) s];
+  }
+  if (job.seg) {
+    job.coff.resize((size_t) n);
+    int64_t r = 0;
+    for (int64_t q = 0; q < n; ++q) {
+      job.coff[(size_t) q] = r;
+      r += (d.len_sorted[(size_t) job.items[(size_t) q]] + 15) & ~15;
+    }
+    job.crows = r;
   }
 }
 
@@ -1970,6 +2051,22 @@ int vit_upload(msvg_ctx *c, std::vector<VitJob> &jobs, std::vector<BandLaunch> &
     }
   }
   if (tot > INT32_MAX) return fail(c, MSVG_EINVAL, "too many ViterbiFilter pairs in one call");
+  // Column-segmented jobs: carry rows, each item's first carry row, partial maxima.
+  int64_t crows = 0;
+  for (VitJob &j : jobs)
+    if (j.seg) { j.cbase = crows; crows += j.crows; }
+  if (crows > 0) {
+    std::vector<int64_t> hcoff((size_t) tot, 0);
+    for (const VitJob &j : jobs)
+      if (j.seg)
+        for (int64_t q = 0; q < j.n; ++q) hcoff[(size_t) (j.base + q)] = j.cbase + j.coff[(size_t) q];
+    HIPCHK(c, c->d_vcoff.ensure((size_t) tot * sizeof(int64_t)));
+    HIPCHK(c, c->d_vpartial.ensure((size_t) tot * sizeof(int32_t)));
+    HIPCHK(c, c->d_vcarry.ensure((size_t) crows * 4 * sizeof(int16_t)));
+    HIPCHK(c, hipMemcpyAsync(c->d_vcoff.p, hcoff.data(), (size_t) tot * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipStreamSynchronize(st));               // hcoff ends with this block
+  }
+  c->vcrows = crows;
   HIPCHK(c, c->d_vuoff.ensure(std::max<size_t>(hidx.size(), 1) * sizeof(int32_t)));
   HIPCHK(c, c->d_vprofs.ensure(std::max<size_t>(hprof.size(), 1) * sizeof(VitProfDev)));
   HIPCHK(c, c->d_vitems.ensure((size_t) std::max<int64_t>(tot, 1) * sizeof(int64_t)));
@@ -2017,12 +2114,52 @@ int vit_launch_multiwave(msvg_ctx *c, VitJob &job, hipStream_t st) {
   return MSVG_OK;
 }
 
+// A column-segmented job: its segments in order on st (each reads the
+// carries the one before wrote).
+int vit_launch_segments(msvg_ctx *c, VitJob &job, int dbid, hipStream_t st) {
+  const Profile &p = *c->profiles[job.pid];
+  const VitProfile &v = *p.vit;
+  const Tables &t = tables_for(c, job.pid, dbid);
+  VitSegArgs a{};
+  a.db = job.a.db;
+  a.e_loop = v.e_loop; a.e_move = v.e_move; a.base_w = v.base_w; a.ddbound_w = v.ddbound_w;
+  a.items = c->d_vitems.as<int64_t>() + job.base;
+  a.move = c->d_vmove.as<int16_t>() + job.base;
+  a.coff = c->d_vcoff.as<int64_t>() + job.base;
+  int16_t *cb = c->d_vcarry.as<int16_t>();
+  a.cM = cb; a.cI = cb + c->vcrows; a.cD = cb + 2 * c->vcrows; a.cE = cb + 3 * c->vcrows;
+  a.partial = c->d_vpartial.as<int32_t>() + job.base;
+  a.out = c->d_vout.as<int32_t>() + job.base;
+  a.uoff = c->d_vuoff.as<int32_t>() + job.ubase;
+  size_t si = 0;
+  for (int k0 = 0; k0 < p.M; k0 += v.seg_span, ++si) {
+    if (si >= t.vseg_emit.size() || si >= v.d_seg_ttab.size()) return fail(c, MSVG_EINVAL, "segment tables missing");
+    const int Qs = (std::min(v.seg_span, p.M - k0) + 127) / 128;
+    a.etab = t.vseg_emit[si]->as<uint32_t>();
+    a.ttab = v.d_seg_ttab[si]->as<uint32_t>();
+    a.first = (k0 == 0);
+    a.last = (k0 + v.seg_span >= p.M);
+    HIPCHK(c, msvk_launch_vit_seg(Qs, &a, job.seg_grid, st));
+  }
+  return MSVG_OK;
+}
+
+// Workgroups of a column-segmented job's launches: what the device holds of
+// its widest segment, and no more than its items fill (eight per workgroup).
+int vit_seg_grid(msvg_ctx *c, const VitProfile &v, int M, int R, int64_t n) {
+  int cap = INT32_MAX;
+  for (int k0 = 0; k0 < M; k0 += v.seg_span) cap = std::min(cap, msvk_vit_seg_cap((std::min(v.seg_span, M - k0) + 127) / 128, R, c->ncu));
+  if (cap < 1) return 0;
+  return (int) std::max<int64_t>(1, std::min<int64_t>(cap, (n + 7) / 8));
+}
+
 // Scores of job j from the raw results (host expression of vitfilter.c:240-248).
 void vit_finish(msvg_ctx *c, const VitJob &j, const int32_t *out, float *sc, uint8_t *status) {
   const VitProfile &v = *c->profiles[j.pid]->vit;
   for (int64_t q = 0; q < j.n; ++q) {
     const int64_t i = j.order[(size_t) q];
     const int32_t r = out[j.base + q];
+    if (r == VIT_DEFER) continue;                              // scored again by the exact kernel
     if ((r >> 16) == MSVG_ERANGE) {
       status[i] = MSVG_ERANGE;
       sc[i] = INFINITY;                                        // vitfilter.c:177
@@ -2068,7 +2205,12 @@ int msvg_vit_search_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t d
     for (int32_t i = 0; i < np; ++i) {
       const VitProfile &v = *c->profiles[pids[i]]->vit;
       if (jobs[(size_t) i].n > 0 && v.W > 1) {
-        slots[(size_t) i] = msvk_vit_slots(v.Q, v.W, &jobs[(size_t) i].a, c->ncu);
+        if (jobs[(size_t) i].seg) {
+          jobs[(size_t) i].seg_grid = vit_seg_grid(c, v, c->profiles[pids[i]]->M, d.R, jobs[(size_t) i].n);
+          slots[(size_t) i] = 8 * jobs[(size_t) i].seg_grid;
+        } else {
+          slots[(size_t) i] = msvk_vit_slots(v.Q, v.W, &jobs[(size_t) i].a, c->ncu);
+        }
         if (slots[(size_t) i] < 1) return fail(c, MSVG_EINVAL, "ViterbiFilter shape has no launch");
       }
     }
@@ -2127,9 +2269,12 @@ int msvg_vit_search_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t d
   int k = 0;
   for (BandLaunch &L : bands)
     if ((rc = vit_launch_band(c, L, *any, nstreams > 1 ? c->vstream[k++ % nstreams] : st)) != MSVG_OK) return rc;
-  for (const int i : multi)
-    if ((rc = vit_launch_multiwave(c, jobs[(size_t) i], nstreams > 1 ? c->vstream[k++ % nstreams] : st)) != MSVG_OK)
-      return rc;
+  for (const int i : multi) {
+    hipStream_t ms = nstreams > 1 ? c->vstream[k++ % nstreams] : st;
+    rc = jobs[(size_t) i].seg ? vit_launch_segments(c, jobs[(size_t) i], dbid, ms)
+                              : vit_launch_multiwave(c, jobs[(size_t) i], ms);
+    if (rc != MSVG_OK) return rc;
+  }
   if (nstreams > 1) {
     for (int q = 0; q < nstreams; ++q) {
       HIPCHK(c, hipEventRecord(c->vdone[q], c->vstream[q]));
@@ -2147,9 +2292,38 @@ int msvg_vit_search_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t d
     vit_finish(c, jobs[(size_t) i], out.data(), sc + offsets[i], status + offsets[i]);
     cells += jobs[(size_t) i].cells;
   }
+  // Items of column-segmented jobs some row of which could have changed B
+  // (VIT_DEFER): scored again by the exact multi-wave kernel.
+  std::vector<int32_t> dpid;
+  std::vector<int64_t> doffs{0}, dtarg, dpos;
+  for (int32_t i = 0; i < np; ++i) {
+    const VitJob &j = jobs[(size_t) i];
+    if (!j.seg) continue;
+    for (int64_t q = 0; q < j.n; ++q)
+      if (out[(size_t) (j.base + q)] == VIT_DEFER) {
+        const int64_t at = offsets[i] + j.order[(size_t) q];
+        dtarg.push_back(targets[at]);
+        dpos.push_back(at);
+      }
+    if ((int64_t) dtarg.size() > doffs.back()) { dpid.push_back(pids[i]); doffs.push_back((int64_t) dtarg.size()); }
+  }
+  if (!dtarg.empty()) {
+    std::vector<float> dsc(dtarg.size());
+    std::vector<uint8_t> dst(dtarg.size());
+    const int keep = c->vit_segments;
+    c->vit_segments = 0;
+    rc = msvg_vit_search_many(c, (int32_t) dpid.size(), dpid.data(), dbid, doffs.data(), dtarg.data(), dsc.data(),
+                              dst.data());
+    c->vit_segments = keep;
+    if (rc != MSVG_OK) return rc;
+    for (size_t u = 0; u < dpos.size(); ++u) { sc[dpos[u]] = dsc[u]; status[dpos[u]] = dst[u]; }
+    ms += (float) c->stats.ms_fast;
+  }
+  c->stats = msvg_stats{};
   c->stats.ms_fast = ms;
   c->stats.ms_total = ms;
   c->stats.cells = cells;
+  c->stats.n_slow = (int64_t) dtarg.size();
   c->stats.n_fast_launches = nlaunch;
   return MSVG_OK;
 }

@@ -772,3 +772,265 @@ extern "C" int msvk_vit_band_cap(int G, int Q, int vote, int R, int ncu) {
   if (G == 32) return vote ? caps_band<32, true>(Q, R, ncu, qs) : caps_band<32, false>(Q, R, ncu, qs);
   return vote ? caps_band<64, true>(Q, R, ncu, qw) : caps_band<64, false>(Q, R, ncu, qw);
 }
+
+// ---------------------------------------------------------------------------
+// vit_seg<Q>: models beyond one wave's tables, cut into column segments of
+// 128 Q positions; one launch per segment over a profile's items, one
+// target per wave (vit_band<64, Q>'s row, eight waves sharing the segment's
+// tables in LDS, the delete paths closed on every row).  Row i of a segment
+// takes, where the segment's first position needs them,
+//   - the M, I and D values of row i-1 at the previous segment's last
+//     position (the diagonal into position k0 + 1), and
+//   - the D value entering position k0 + 1 in row i (max of M + tMD and
+//     D + tDD at the previous segment's last position, D after its closure),
+// and leaves its own for the next segment.  Each of the four moves through
+// one register holding two queues (the msv_seg scheme): lanes 0..15 the
+// incoming values of a block of 16 rows (lane 0 the current row's), lanes
+// 48..63 the outgoing ones; one row_shl:1 per row advances both, the row's
+// shift takes lane 0 where it leaves a hole, lane 63 takes the new value.
+// B stays xN + move on every row (the J-free case): a row where some lane's
+// xE share exceeds min(xN - e_loop, 32766) could raise J above xN (and with
+// it B) or overflow, so such an item is marked and, after the last segment,
+// scored again by the exact kernel (VIT_DEFER).  Otherwise J never exceeds
+// xN, B never changes, and xC = max over rows of xE + e_move (C loop 0), the
+// row maxima being maxima over the segments' shares: the result is exact.
+// ---------------------------------------------------------------------------
+template <int Q>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(Q <= 5 ? 4 : (Q <= 8 ? 3 : 2), 8)))
+vit_seg(VitSegArgs a) {
+  extern __shared__ uint32_t etab_lds[];
+  constexpr int G = 64;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int c = lane;
+
+  constexpr bool LT = band_lt<Q>();
+  constexpr int QR = LT ? 1 : Q;
+  uint32_t tMM[QR], tIM[QR], tDM[QR], tMD[Q], tMI[Q], tII[Q], tDD[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    if constexpr (!LT) {
+      tMM[j] = a.ttab[(1 * Q + j) * G + c];
+      tIM[j] = a.ttab[(2 * Q + j) * G + c];
+      tDM[j] = a.ttab[(3 * Q + j) * G + c];
+    }
+    tMD[j] = a.ttab[(4 * Q + j) * G + c];
+    tMI[j] = a.ttab[(5 * Q + j) * G + c];
+    tII[j] = a.ttab[(6 * Q + j) * G + c];
+    tDD[j] = a.ttab[(7 * Q + j) * G + c];
+  }
+  int Tlo = 0, Thi = 0;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) { Tlo += lo16(tDD[j]); Thi += hi16(tDD[j]); }
+  ScanT st_;
+  {
+    const int Ts0 = Tlo + Thi;
+    const int Ts1 = t_next<0x111, 0xf>(Ts0);
+    const int Ts2 = t_next<0x112, 0xf>(Ts1);
+    const int Ts3 = t_next<0x114, 0xf>(Ts2);
+    const int Ts4 = t_next<0x118, 0xf>(Ts3);
+    const int Ts5 = t_next<0x142, 0xa>(Ts4);
+    st_.lo = (int) negsat(Tlo);
+    st_.hi = (int) negsat(Thi);
+    st_.s1 = (int) negsat(in_row_t<1>(Ts0, lane));
+    st_.s2 = (int) negsat(in_row_t<2>(Ts1, lane));
+    st_.s4 = (int) negsat(in_row_t<4>(Ts2, lane));
+    st_.s8 = (int) negsat(in_row_t<8>(Ts3, lane));
+    st_.s16 = (int) negsat(Ts4);
+    st_.s32 = (int) negsat(Ts5);
+  }
+
+  const int nemit = (a.db.R + 1) * Q * G;
+  for (int i = threadIdx.x; i < nemit; i += blockDim.x) etab_lds[i] = a.etab[i];
+  if constexpr (LT)
+    for (int i = threadIdx.x; i < 3 * Q * G; i += blockDim.x) etab_lds[nemit + i] = a.ttab[Q * G + i];
+  __syncthreads();
+  const uint32_t *tl = etab_lds + nemit + c;           // LT: this lane's MM, IM, DM
+
+  const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
+  const uint4 null4 = make_uint4(nb, nb, nb, nb);
+  const int xN = a.base_w;                              // vitfilter.c:116; N loop cost is 0
+  const int tE = min(xN - a.e_loop, 32766);             // xJ = -32768 < xN: max(xJ, xN) = xN
+  constexpr uint32_t NEGH = 0x80000000u;                // -32768 in the high half
+  const int slot = blockIdx.x * (blockDim.x >> 6) + wave;
+
+  for (int it = a.uoff[slot], iend = a.uoff[slot + 1]; it < iend; ++it) {
+    const int64_t s = a.items[it];
+    const int L = a.db.len[s];
+    const int Lpad = (L + 15) & ~15;
+    const uint8_t *p = a.db.res + a.db.off[s];
+    const int64_t co = a.coff[it];
+    int16_t *cM = a.cM + co, *cI = a.cI + co, *cD = a.cD + co, *cE = a.cE + co;
+    const int xB = (int16_t) (xN + (int) a.move[it]);                 // :117, constant here
+    uint32_t bm[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) bm[j] = adds(pack16(xB, xB), a.ttab[j * G + c]);
+    uint32_t Mr[Q], Ir[Q], Dr[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) Mr[j] = Ir[j] = Dr[j] = NEG2;        // :115
+    uint32_t xacc = NEG2;                // low half: this lane's largest xE share
+    int kC = -32768;                     // the closure scan's exclusive fill
+    int fired = 0;
+    uint32_t qM = NEGH, qI = NEGH, qD = NEGH, qE = NEGH;     // carry queues
+    uint32_t clM = NEGH, clI = NEGH, clD = NEGH;             // carries of the row before the block
+    uint32_t dent = NEG2;                // the D value leaving the segment in the row before
+    uint32_t e[Q];
+    auto load_e = [&](int x) {
+      const uint32_t *ep = etab_lds + x * (Q * G) + c;
+#pragma unroll
+      for (int j = 0; j < Q; ++j) e[j] = ep[j * G];
+    };
+    // a queue step: the row's shifted register, lane 0 from the queue
+    auto qshift = [](uint32_t last, uint32_t &q, uint32_t out_prev) -> uint32_t {
+      const uint32_t xs = __builtin_amdgcn_update_dpp(out_prev, q, 0x101 /* row_shl:1 */, 0xf, 0xf, false);
+      const uint32_t prev = __builtin_amdgcn_update_dpp(q, last, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+      q = xs;
+      return __builtin_amdgcn_alignbit(last, prev, 16);
+    };
+
+    auto row = [&](int xnext, bool alive) {
+      const uint32_t lM = Mr[Q - 1], lI = Ir[Q - 1], lD = Dr[Q - 1];
+      const uint32_t mpv = qshift(lM, qM, lM);                      // :139-141
+      const uint32_t ipv = qshift(lI, qI, lI);
+      const uint32_t dpv = qshift(lD, qD, lD);
+      uint32_t xEv = NEG2, dcv = NEG2;
+#pragma unroll
+      for (int j = Q - 1; j >= 0; --j) {
+        const uint32_t pm = j ? Mr[j - 1] : mpv, pi = j ? Ir[j - 1] : ipv, pd = j ? Dr[j - 1] : dpv;
+        const uint32_t mm = LT ? tl[(0 * Q + j) * G] : tMM[LT ? 0 : j];
+        const uint32_t im = LT ? tl[(1 * Q + j) * G] : tIM[LT ? 0 : j];
+        const uint32_t dm = LT ? tl[(2 * Q + j) * G] : tDM[LT ? 0 : j];
+        uint32_t sv = bm[j];                                         // :146-150
+        sv = maxs(sv, adds(pm, mm));
+        sv = maxs(sv, adds(pi, im));
+        sv = maxs(sv, adds(pd, dm));
+        sv = adds(sv, e[j]);
+        xEv = maxs(xEv, sv);                                         // :151
+        Ir[j] = maxs(adds(Mr[j], tMI[j]), adds(Ir[j], tII[j]));      // :171-172
+        Mr[j] = sv;                                                  // :161
+        const uint32_t dn = adds(sv, tMD[j]);                        // :167
+        if (j < Q - 1) Dr[j + 1] = dn; else dcv = dn;                // :162
+      }
+      asm volatile("" ::: "memory");
+      load_e(xnext);
+      asm volatile("" ::: "memory");
+      const uint32_t hv = halves_max(xEv, NEG2);                     // low half: xE share
+      xacc = maxs(xacc, alive ? hv : NEG2);
+      if (__builtin_amdgcn_ballot_w64(lo16(hv) > (alive ? tE : 0x7fffffff))) fired = 1;
+      Dr[0] = qshift(dcv, qE, dent);                                 // :230-231 / :203-204
+      close_all<G, Q>(Dr, tDD, st_, kC, lane);                       // :203-227
+      dent = maxs(dcv, adds(Dr[Q - 1], tDD[Q - 1]));
+    };
+
+    const uint4 c0 = *reinterpret_cast<const uint4 *>(p);
+    const uint4 c1 = *reinterpret_cast<const uint4 *>(p + (16 < Lpad ? 16 : 0));
+    uint4 cur = (0 < Lpad) ? c0 : null4;
+    uint4 nxt = (16 < Lpad) ? c1 : null4;
+    load_e((int) (cur.x & 0xffu));
+    for (int i0 = 0; i0 < L; i0 += 16) {
+      const int i2 = i0 + 32;
+      const uint4 raw = *reinterpret_cast<const uint4 *>(p + (i2 < Lpad ? i2 : 0));
+      // Incoming carries: M, I, D of rows i0-1 .. i0+14 (lane 0 the row
+      // before the block) and the D entries of rows i0 .. i0+15, read before
+      // this block's own carries overwrite rows i0 .. i0+15.
+      {
+        uint32_t inM = NEGH, inI = NEGH, inD = NEGH, inE = NEGH;
+        if (!a.first && lane < 16) {
+          inM = (uint32_t) (uint16_t) cM[i0 + lane] << 16;
+          inI = (uint32_t) (uint16_t) cI[i0 + lane] << 16;
+          inD = (uint32_t) (uint16_t) cD[i0 + lane] << 16;
+          inE = (uint32_t) (uint16_t) cE[i0 + lane] << 16;
+        }
+        qM = __builtin_amdgcn_update_dpp(clM, inM, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+        qI = __builtin_amdgcn_update_dpp(clI, inI, 0x111, 0xf, 0xf, false);
+        qD = __builtin_amdgcn_update_dpp(clD, inD, 0x111, 0xf, 0xf, false);
+        qE = inE;
+        clM = (uint32_t) __builtin_amdgcn_readlane((int) inM, 15);
+        clI = (uint32_t) __builtin_amdgcn_readlane((int) inI, 15);
+        clD = (uint32_t) __builtin_amdgcn_readlane((int) inD, 15);
+      }
+      uint32_t w0 = cur.x, w1 = cur.y, w2 = cur.z, w3 = cur.w;     // 16 residues, four per word
+      // Rows past L up to the next multiple of four run on the padded
+      // chunk's residues; they neither vote nor count.
+      const int nq = min(4, (L - i0 + 3) >> 2);
+      for (int qd = 0; qd < nq; ++qd) {
+        const int ib = i0 + 4 * qd;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) row((int) (((r < 3 ? w0 >> (8 * r + 8) : w1)) & 0xffu), ib + r < L);
+        w0 = w1; w1 = w2; w2 = w3; w3 = nxt.x;
+      }
+      // Outgoing carries of the block's rows: the last row's values into
+      // lane 63, then lane 64 - nr + k holds row k.
+      if (!a.last) {
+        const uint32_t oM = __builtin_amdgcn_update_dpp(Mr[Q - 1], qM, 0x101, 0xf, 0xf, false);
+        const uint32_t oI = __builtin_amdgcn_update_dpp(Ir[Q - 1], qI, 0x101, 0xf, 0xf, false);
+        const uint32_t oD = __builtin_amdgcn_update_dpp(Dr[Q - 1], qD, 0x101, 0xf, 0xf, false);
+        const uint32_t oE = __builtin_amdgcn_update_dpp(dent, qE, 0x101, 0xf, 0xf, false);
+        const int nr = 4 * nq;
+        if (lane >= 64 - nr) {
+          const int pos = i0 + lane - (64 - nr);
+          cM[pos] = (int16_t) (oM >> 16);
+          cI[pos] = (int16_t) (oI >> 16);
+          cD[pos] = (int16_t) (oD >> 16);
+          cE[pos] = (int16_t) (oE >> 16);
+        }
+      }
+      cur = nxt;
+      nxt = (i2 < Lpad) ? raw : null4;
+    }
+    const uint32_t xm = group_max2<G>(xacc);
+    int xE = lo16(__builtin_amdgcn_readlane(xm, G - 1));
+    if (!a.first) {
+      const int pv = a.partial[it];
+      xE = max(xE, (int) (int16_t) (pv & 0xffff));
+      fired |= (pv >> 16) & 1;
+    }
+    if (lane == 0) {
+      if (!a.last) a.partial[it] = (fired << 16) | (int) (uint16_t) xE;
+      else a.out[it] = fired ? VIT_DEFER : (int) (uint16_t) max(-32768, xE + a.e_move);   // :179, C loop 0
+    }
+  }
+}
+
+template <int Q>
+static size_t seg_lds(int R) { return band_lds<64, Q>(R); }
+
+template <int Q>
+static int cap_seg(int R, int ncu) {
+  return ncu * resident_per_cu(vit_seg<Q>, 512, seg_lds<Q>(R));
+}
+
+template <int Q>
+static hipError_t launch_seg(const VitSegArgs *a, int grid, hipStream_t st) {
+  const size_t lds = seg_lds<Q>(a->db.R);
+  hipLaunchKernelGGL(vit_seg<Q>, dim3(grid), dim3(512), lds, st, *a);
+  return hipGetLastError();
+}
+
+template <int... Qs>
+static hipError_t dispatch_seg(int Q, const VitSegArgs *a, int grid, hipStream_t st, std::integer_sequence<int, Qs...>) {
+  hipError_t e = hipErrorInvalidValue;
+  ((Q == Qs + 1 ? (e = launch_seg<Qs + 1>(a, grid, st), 0) : 0), ...);
+  return e;
+}
+
+template <int... Qs>
+static int caps_seg(int Q, int R, int ncu, std::integer_sequence<int, Qs...>) {
+  int r = 0;
+  ((Q == Qs + 1 ? (r = cap_seg<Qs + 1>(R, ncu), 0) : 0), ...);
+  return r;
+}
+
+// One column segment of Q <= 15 registers per lane (beyond, the carries'
+// registers push the lane past 256 VGPRs): grid workgroups of 512 threads
+// (eight waves, wave slot b * 8 + w).
+extern "C" hipError_t msvk_launch_vit_seg(int Q, const VitSegArgs *a, int grid, hipStream_t st) {
+  if (Q < 1 || Q > 15 || grid < 1) return hipErrorInvalidValue;
+  return dispatch_seg(Q, a, grid, st, std::make_integer_sequence<int, 15>{});
+}
+
+// Workgroups of vit_seg<Q> the device holds at once for R + 1 table rows.
+extern "C" int msvk_vit_seg_cap(int Q, int R, int ncu) {
+  if (Q < 1 || Q > 15) return 0;
+  return caps_seg(Q, R, ncu, std::make_integer_sequence<int, 15>{});
+}

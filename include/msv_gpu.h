@@ -264,6 +264,16 @@ int msvg_set_vit_tile(msvg_ctx *ctx, int32_t M_lo, int32_t M_hi, int32_t lanes, 
  * every row closes the delete paths), 0 every row closes, 1 every row votes.
  * Scores and statuses are the same in all three. */
 int msvg_set_vit_vote(msvg_ctx *ctx, int32_t mode);
+
+/* Models too long for one wave's ViterbiFilter tables (the multi-wave
+ * shapes) run as column segments of at most 1920 positions, one wave per
+ * target, carrying each row's boundary values from one segment's launch to
+ * the next (on = 1, the default), or on the multi-wave kernel (0), taking
+ * effect at the next search.  A pair some row of which could raise J (or
+ * overflow) is scored again by the multi-wave kernel, so the results never
+ * depend on the setting; after a ViterbiFilter search n_slow counts those
+ * pairs.  No counterpart in the reference. */
+int msvg_set_vit_segments(msvg_ctx *ctx, int32_t on);
 /* Tail guard (on by default): where a database's longest target would make
  * the tile's launch last longer than the whole database's work at the
  * device's rate (small databases holding a long target), the batched mode

@@ -4,9 +4,9 @@ Bar: bit-identical p7_ViterbiFilter scores and statuses (integer DP; the
 float score is the reference's expression of the integer xC).  Checked
 against the golden vectors written from the compiled reference, against the
 reference itself (oracle/_ref) for model lengths that exercise every shape
-(16-, 32- and 64-lane targets, 2, 4 and 8 waves per target, M up to 16384)
-and both ways the rows treat the lazy-F test, and the drop-in and
-calibration entry points.  Targets of one wave have
+(16-, 32- and 64-lane targets, column segments and 2, 4 and 8 waves per
+target, M up to 16384) and both ways the rows treat the lazy-F test, and the
+drop-in and calibration entry points.  Targets of one wave have
 different lengths (some end, some overflow, while the others go on).
 """
 import numpy as np
@@ -57,8 +57,9 @@ def test_every_wave_count_vs_reference(M):
     against the compiled reference's p7_ViterbiFilter: four 16-lane targets
     per wave up to M = 512, two 32-lane targets up to 1024, one wave per
     target up to 2432 (1 to 16 registers per lane, 19 for one amino wave,
-    whose MM, IM and DM transitions come from LDS beyond 16), then 2, 4 and
-    8 waves per target."""
+    whose MM, IM and DM transitions come from LDS beyond 16), then column
+    segments of up to 1920 positions (2 to 9 of them here), with the
+    core-emitted homologs' pairs scored again by the multi-wave kernel."""
     from oracle import Reference
     R = Reference()
     rp = R.sample(500 + M, M, calibrate=False)
@@ -75,14 +76,22 @@ def test_every_wave_count_vs_reference(M):
         seqs.append(emit[o:o + L])
         o += L
     seqs.append(rng.integers(0, 29, 300).astype(np.uint8))
+    vp = ca.ViterbiProfile.from_dict(vd)
+    for w in (8, 30):                                  # local alignments that raise J
+        k0 = int(rng.integers(0, max(1, M - w)))
+        seqs.append(_best_residues(vp, k0, min(M, k0 + w), rng))
     db = ca.Database.from_list(seqs)
     print(f"M={M} targets={db.n}", flush=True)
-    ctx, pid = _ctx_with(vd, 29)
-    did = ctx.add_database(db)
-    sc, st = ctx.vit_search(pid, did, np.arange(db.n))
     rst, rsc = rp.vit_batch(db.res, db.off, db.lens, nthreads=8)
-    assert _same(sc, st, rsc, rst)
-    ctx.close()
+    # beyond one wave: column segments (default) and the multi-wave kernel
+    for seg in ((True, False) if M > 2432 else (True,)):
+        ctx, pid = _ctx_with(vd, 29)
+        ctx.set_vit_segments(seg)
+        ctx.set_viterbi(pid, vp)
+        did = ctx.add_database(db)
+        sc, st = ctx.vit_search(pid, did, np.arange(db.n))
+        assert _same(sc, st, rsc, rst), seg
+        ctx.close()
 
 
 def _skewed_targets(seed, n, n_long):
@@ -130,6 +139,53 @@ def test_skewed_batch_vs_reference_every_vote_mode(vote):
         rst, rsc = rp.vit_batch(sub.res, sub.off, sub.lens, nthreads=8)
         assert _same(sc, st, rsc, rst)
     ctx.close()
+
+
+def _best_residues(vp, k0, k1, rng, pad=30):
+    """Random residues around the best-scoring residue of every position
+    k0..k1-1 of the model: a local alignment that scores well above the
+    E->J loop cost (J fires), and past the 16-bit range when long."""
+    best = np.argmax(vp.rwv[:20, k0:k1], axis=0).astype(np.uint8)
+    return np.concatenate([rng.integers(0, 20, pad).astype(np.uint8), best, rng.integers(0, 20, pad).astype(np.uint8)])
+
+
+@pytest.mark.parametrize("M", [2433, 3841, 6000, 12000])
+def test_column_segments_equal_multiwave(M):
+    """Models beyond one wave's tables run as column segments (vit_seg): the
+    same scores and statuses, bit for bit, as the multi-wave kernel on
+    targets of every length 0..40, longer random ones, repeats,
+    core-emitted samples, and runs of each position's best residue (short
+    ones placed across a segment edge raise J, long ones overflow: those
+    pairs are scored again by the multi-wave kernel, counted in n_slow)."""
+    prof = ca.sample_profile(3000 + M, M)
+    vp = prof.viterbi()
+    rng = np.random.default_rng(M)
+    seqs = [rng.integers(0, 20, L).astype(np.uint8) for L in range(41)]
+    seqs += [rng.integers(0, 20, int(L)).astype(np.uint8) for L in rng.integers(41, 3000, 60)]
+    seqs += ca.core_emit(prof.hmm, M + 7, 3)
+    nseg = -(-M // 1920)                                # the runtime's segment width rule
+    edge = 128 * -(-(-(-M // nseg)) // 128)
+    hot = [_best_residues(vp, max(0, edge - w // 2), min(M, edge + w // 2), rng) for w in (6, 10, 20, 40)]
+    hot += [_best_residues(vp, 0, M, rng), _best_residues(vp, M - 200, M, rng)]
+    seqs += hot
+    db = ca.Database.from_list(seqs)
+    t = np.concatenate([np.arange(db.n), rng.integers(0, db.n, 20)]).astype(np.int64)
+    print(f"segments M={M} targets={db.n} pairs={t.size}", flush=True)
+    out = []
+    for on in (True, False):
+        ctx = ca.MSVContext(0, tiles=None)
+        ctx.set_vit_segments(on)
+        pid = ctx.add_profile(prof)
+        ctx.set_viterbi(pid, prof.viterbi())
+        did = ctx.add_database(db)
+        out.append(ctx.vit_search(pid, did, t) + (ctx.stats()["n_slow"],))
+        ctx.close()
+    (sc1, st1, nd1), (sc0, st0, nd0) = out
+    assert _same(sc1, st1, sc0, st0)
+    # the long best-residue runs at least (short ones may stay below the
+    # B->M entry cost), not the random targets
+    assert nd0 == 0 and 2 <= nd1 < t.size // 4
+    assert np.all(st1[db.n - 2:db.n] == 16)              # the long runs overflow
 
 
 def test_many_profiles_equal_separate_searches():

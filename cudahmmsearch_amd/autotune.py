@@ -168,21 +168,30 @@ def tune(ctx, db, m_lo=16, m_hi=4096, probes=24, tol=0.02, seed=900, log=print):
 
 
 # ViterbiFilter shapes (lanes per target, waves per target): vit_band with 16
-# or 32 lanes (several targets per wave) or a whole wave, and vit_kernel with
-# 2, 4 or 8 waves.
+# or 32 lanes (several targets per wave) or a whole wave; beyond one wave's
+# tables, the column segments (vit_seg), whose exact fallback runs on the
+# multi-wave kernel with the fewest waves (2, 4 or 8) that fit.
 VIT_SHAPES = ((16, 1), (32, 1), (64, 1), (64, 2), (64, 4), (64, 8))
+
+
+def _one_wave_qmax(Kp):
+    return min(20, LDS_BYTES // (256 * (Kp + 4)))
 
 
 def vit_feasible(M, shape, Kp=29):
     """Does the ViterbiFilter take a model of length M in this shape?  At
     most 16 registers per lane, except one wave per target: as many as keep
     its tables (Kp + 1 emission rows and 3 transition rows of 64 lanes x 4
-    bytes per register) within 160 KiB of LDS, at most 20."""
+    bytes per register) within 160 KiB of LDS, at most 20.  Several waves
+    (column segments, the fallback on that many waves) only beyond one
+    wave's limit and only with the fewest waves that fit: the waves change
+    nothing but the rare fallback, so the tuner has one such candidate."""
     lanes, waves = shape
     if waves == 1:
         Q = (M + 2 * lanes - 1) // (2 * lanes)
-        return Q <= (16 if lanes < 64 else min(20, LDS_BYTES // (256 * (Kp + 4))))
-    return (M + 128 * waves - 1) // (128 * waves) <= 16
+        return Q <= (16 if lanes < 64 else _one_wave_qmax(Kp))
+    fits = lambda w: (M + 128 * w - 1) // (128 * w) <= 16    # noqa: E731
+    return M > 128 * _one_wave_qmax(Kp) and fits(waves) and (waves == 2 or not fits(waves // 2))
 
 
 def time_vit_shape(ctx, prof, did, targets, shape, reps=3):

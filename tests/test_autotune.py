@@ -165,6 +165,10 @@ def test_vit_feasible_matches_the_kernel_limits():
     assert autotune.vit_feasible(1024, (32, 1)) and not autotune.vit_feasible(1025, (32, 1))
     assert autotune.vit_feasible(4096, (64, 2)) and not autotune.vit_feasible(4097, (64, 2))
     assert autotune.vit_feasible(16384, (64, 8)) and not autotune.vit_feasible(16385, (64, 8))
+    # several waves only beyond one wave's limit, and only the fewest that fit
+    assert not autotune.vit_feasible(2432, (64, 2)) and autotune.vit_feasible(2433, (64, 2))
+    assert not autotune.vit_feasible(3000, (64, 4)) and autotune.vit_feasible(4097, (64, 4))
+    assert not autotune.vit_feasible(8192, (64, 8))
     # one wave: tables of Kp + 1 emission and 3 transition rows within 160 KiB
     assert autotune.vit_feasible(2432, (64, 1)) and not autotune.vit_feasible(2433, (64, 1))
     assert autotune.vit_feasible(2560, (64, 1), Kp=18)

@@ -1785,11 +1785,11 @@ int msvg_profile_set_viterbi(msvg_ctx *c, int32_t pid, const msvg_vprofile *vp) 
   HIPCHK(c, v->d_ttab.ensure(h.size() * 4));
   HIPCHK(c, hipMemcpy(v->d_ttab.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
   if (v->W > 1) {
-    // Column segments for vit_seg: as few as 15 registers per lane (1920
+    // Column segments for vit_seg: as few as 14 registers per lane (1792
     // positions) allow, of equal width (a multiple of 128 positions; the
     // last one takes the rest), each with its transitions at their model
     // positions.
-    const int nseg = (p.M + 1919) / 1920;
+    const int nseg = (p.M + 1791) / 1792;
     v->seg_span = 128 * (((p.M + nseg - 1) / nseg + 127) / 128);
     for (int k0 = 0; k0 < p.M; k0 += v->seg_span) {
       const int Ms = std::min(v->seg_span, p.M - k0), Qs = (Ms + 127) / 128;

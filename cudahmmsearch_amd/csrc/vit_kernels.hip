@@ -920,33 +920,53 @@ vit_seg(VitSegArgs a) {
       dent = maxs(dcv, adds(Dr[Q - 1], tDD[Q - 1]));
     };
 
+    // Carry loads are issued a block ahead (rows i .. i+15 in lanes 0..15,
+    // high halves), and a block's outgoing carries are stored when the next
+    // block starts, so no block waits for a load or store it has just
+    // issued.  A block writes only its own rows, which the loads of the
+    // next block do not touch.
+    uint32_t pM = NEGH, pI = NEGH, pD = NEGH, pE = NEGH;
+    auto load_carries = [&](int i) {
+      pM = pI = pD = pE = NEGH;
+      if (!a.first && lane < 16 && i < Lpad) {
+        pM = (uint32_t) (uint16_t) cM[i + lane] << 16;
+        pI = (uint32_t) (uint16_t) cI[i + lane] << 16;
+        pD = (uint32_t) (uint16_t) cD[i + lane] << 16;
+        pE = (uint32_t) (uint16_t) cE[i + lane] << 16;
+      }
+    };
+    uint32_t sM = 0, sI = 0, sD = 0, sE = 0;             // outgoing carries not yet stored
+    int snr = 0, si = 0;                                  // their rows, from row si
+    auto store_carries = [&]() {
+      if (!a.last && snr > 0 && lane >= 64 - snr) {
+        const int pos = si + lane - (64 - snr);            // lane 64 - snr + k holds row si + k
+        cM[pos] = (int16_t) (sM >> 16);
+        cI[pos] = (int16_t) (sI >> 16);
+        cD[pos] = (int16_t) (sD >> 16);
+        cE[pos] = (int16_t) (sE >> 16);
+      }
+    };
+
     const uint4 c0 = *reinterpret_cast<const uint4 *>(p);
     const uint4 c1 = *reinterpret_cast<const uint4 *>(p + (16 < Lpad ? 16 : 0));
     uint4 cur = (0 < Lpad) ? c0 : null4;
     uint4 nxt = (16 < Lpad) ? c1 : null4;
     load_e((int) (cur.x & 0xffu));
+    load_carries(0);
     for (int i0 = 0; i0 < L; i0 += 16) {
       const int i2 = i0 + 32;
       const uint4 raw = *reinterpret_cast<const uint4 *>(p + (i2 < Lpad ? i2 : 0));
       // Incoming carries: M, I, D of rows i0-1 .. i0+14 (lane 0 the row
-      // before the block) and the D entries of rows i0 .. i0+15, read before
-      // this block's own carries overwrite rows i0 .. i0+15.
-      {
-        uint32_t inM = NEGH, inI = NEGH, inD = NEGH, inE = NEGH;
-        if (!a.first && lane < 16) {
-          inM = (uint32_t) (uint16_t) cM[i0 + lane] << 16;
-          inI = (uint32_t) (uint16_t) cI[i0 + lane] << 16;
-          inD = (uint32_t) (uint16_t) cD[i0 + lane] << 16;
-          inE = (uint32_t) (uint16_t) cE[i0 + lane] << 16;
-        }
-        qM = __builtin_amdgcn_update_dpp(clM, inM, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
-        qI = __builtin_amdgcn_update_dpp(clI, inI, 0x111, 0xf, 0xf, false);
-        qD = __builtin_amdgcn_update_dpp(clD, inD, 0x111, 0xf, 0xf, false);
-        qE = inE;
-        clM = (uint32_t) __builtin_amdgcn_readlane((int) inM, 15);
-        clI = (uint32_t) __builtin_amdgcn_readlane((int) inI, 15);
-        clD = (uint32_t) __builtin_amdgcn_readlane((int) inD, 15);
-      }
+      // before the block) and the D entries of rows i0 .. i0+15.
+      qM = __builtin_amdgcn_update_dpp(clM, pM, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+      qI = __builtin_amdgcn_update_dpp(clI, pI, 0x111, 0xf, 0xf, false);
+      qD = __builtin_amdgcn_update_dpp(clD, pD, 0x111, 0xf, 0xf, false);
+      qE = pE;
+      clM = (uint32_t) __builtin_amdgcn_readlane((int) pM, 15);
+      clI = (uint32_t) __builtin_amdgcn_readlane((int) pI, 15);
+      clD = (uint32_t) __builtin_amdgcn_readlane((int) pD, 15);
+      store_carries();
+      load_carries(i0 + 16);
       uint32_t w0 = cur.x, w1 = cur.y, w2 = cur.z, w3 = cur.w;     // 16 residues, four per word
       // Rows past L up to the next multiple of four run on the padded
       // chunk's residues; they neither vote nor count.
@@ -960,22 +980,17 @@ vit_seg(VitSegArgs a) {
       // Outgoing carries of the block's rows: the last row's values into
       // lane 63, then lane 64 - nr + k holds row k.
       if (!a.last) {
-        const uint32_t oM = __builtin_amdgcn_update_dpp(Mr[Q - 1], qM, 0x101, 0xf, 0xf, false);
-        const uint32_t oI = __builtin_amdgcn_update_dpp(Ir[Q - 1], qI, 0x101, 0xf, 0xf, false);
-        const uint32_t oD = __builtin_amdgcn_update_dpp(Dr[Q - 1], qD, 0x101, 0xf, 0xf, false);
-        const uint32_t oE = __builtin_amdgcn_update_dpp(dent, qE, 0x101, 0xf, 0xf, false);
-        const int nr = 4 * nq;
-        if (lane >= 64 - nr) {
-          const int pos = i0 + lane - (64 - nr);
-          cM[pos] = (int16_t) (oM >> 16);
-          cI[pos] = (int16_t) (oI >> 16);
-          cD[pos] = (int16_t) (oD >> 16);
-          cE[pos] = (int16_t) (oE >> 16);
-        }
+        sM = __builtin_amdgcn_update_dpp(Mr[Q - 1], qM, 0x101, 0xf, 0xf, false);
+        sI = __builtin_amdgcn_update_dpp(Ir[Q - 1], qI, 0x101, 0xf, 0xf, false);
+        sD = __builtin_amdgcn_update_dpp(Dr[Q - 1], qD, 0x101, 0xf, 0xf, false);
+        sE = __builtin_amdgcn_update_dpp(dent, qE, 0x101, 0xf, 0xf, false);
+        snr = 4 * nq;
+        si = i0;
       }
       cur = nxt;
       nxt = (i2 < Lpad) ? raw : null4;
     }
+    store_carries();
     const uint32_t xm = group_max2<G>(xacc);
     int xE = lo16(__builtin_amdgcn_readlane(xm, G - 1));
     if (!a.first) {
@@ -1019,16 +1034,16 @@ static int caps_seg(int Q, int R, int ncu, std::integer_sequence<int, Qs...>) {
   return r;
 }
 
-// One column segment of Q <= 15 registers per lane (beyond, the carries'
+// One column segment of Q <= 14 registers per lane (beyond, the carries'
 // registers push the lane past 256 VGPRs): grid workgroups of 512 threads
 // (eight waves, wave slot b * 8 + w).
 extern "C" hipError_t msvk_launch_vit_seg(int Q, const VitSegArgs *a, int grid, hipStream_t st) {
-  if (Q < 1 || Q > 15 || grid < 1) return hipErrorInvalidValue;
-  return dispatch_seg(Q, a, grid, st, std::make_integer_sequence<int, 15>{});
+  if (Q < 1 || Q > 14 || grid < 1) return hipErrorInvalidValue;
+  return dispatch_seg(Q, a, grid, st, std::make_integer_sequence<int, 14>{});
 }
 
 // Workgroups of vit_seg<Q> the device holds at once for R + 1 table rows.
 extern "C" int msvk_vit_seg_cap(int Q, int R, int ncu) {
-  if (Q < 1 || Q > 15) return 0;
-  return caps_seg(Q, R, ncu, std::make_integer_sequence<int, 15>{});
+  if (Q < 1 || Q > 14) return 0;
+  return caps_seg(Q, R, ncu, std::make_integer_sequence<int, 14>{});
 }

@@ -266,7 +266,7 @@ int msvg_set_vit_tile(msvg_ctx *ctx, int32_t M_lo, int32_t M_hi, int32_t lanes, 
 int msvg_set_vit_vote(msvg_ctx *ctx, int32_t mode);
 
 /* Models too long for one wave's ViterbiFilter tables (the multi-wave
- * shapes) run as column segments of at most 1920 positions, one wave per
+ * shapes) run as column segments of at most 1792 positions, one wave per
  * target, carrying each row's boundary values from one segment's launch to
  * the next (on = 1, the default), or on the multi-wave kernel (0), taking
  * effect at the next search.  A pair some row of which could raise J (or

@@ -58,7 +58,7 @@ def test_every_wave_count_vs_reference(M):
     per wave up to M = 512, two 32-lane targets up to 1024, one wave per
     target up to 2432 (1 to 16 registers per lane, 19 for one amino wave,
     whose MM, IM and DM transitions come from LDS beyond 16), then column
-    segments of up to 1920 positions (2 to 9 of them here), with the
+    segments of up to 1792 positions (2 to 10 of them here), with the
     core-emitted homologs' pairs scored again by the multi-wave kernel."""
     from oracle import Reference
     R = Reference()
@@ -163,7 +163,7 @@ def test_column_segments_equal_multiwave(M):
     seqs = [rng.integers(0, 20, L).astype(np.uint8) for L in range(41)]
     seqs += [rng.integers(0, 20, int(L)).astype(np.uint8) for L in rng.integers(41, 3000, 60)]
     seqs += ca.core_emit(prof.hmm, M + 7, 3)
-    nseg = -(-M // 1920)                                # the runtime's segment width rule
+    nseg = -(-M // 1792)                                # the runtime's segment width rule
     edge = 128 * -(-(-(-M // nseg)) // 128)
     hot = [_best_residues(vp, max(0, edge - w // 2), min(M, edge + w // 2), rng) for w in (6, 10, 20, 40)]
     hot += [_best_residues(vp, 0, M, rng), _best_residues(vp, M - 200, M, rng)]

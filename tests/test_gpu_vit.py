@@ -107,8 +107,8 @@ def _skewed_targets(seed, n, n_long):
 def test_skewed_batch_vs_reference_every_vote_mode(vote):
     """More wave units than the launch has slots (several units per slot,
     dealt greedily by rows), targets of very different lengths, an empty one:
-    one sampled profile per band shape (16, 32 and 64 lanes) and one
-    multi-wave profile, in one multi-profile call, against the compiled
+    one sampled profile per band shape (16, 32 and 64 lanes) and one in
+    column segments, in one multi-profile call, against the compiled
     reference's p7_ViterbiFilter, with the rows voting on the lazy-F test
     (1), closing the delete paths on every row (0), or chosen by the
     profile's estimated rate (-1)."""
@@ -121,7 +121,7 @@ def test_skewed_batch_vs_reference_every_vote_mode(vote):
     did = ctx.add_database(db)
     refs, pids, lists = [], [], []
     rng = np.random.default_rng(5)
-    for k, M in enumerate([400, 900, 1700, 2300]):
+    for k, M in enumerate([400, 900, 1700, 2300, 4100]):
         rp = R.sample(700 + k, M, calibrate=False)
         f = rp.vit_fields()
         vd = {"rwv": f["rwv_pad"][:, :M], "twv": f["twv_pad"][:, :M], "e_loop": f["e_loop"], "e_move": f["e_move"],
@@ -190,23 +190,36 @@ def test_column_segments_equal_multiwave(M):
 
 def test_many_profiles_equal_separate_searches():
     """msvg_vit_search_many over profiles of every kernel shape (16- and
-    32-lane groups, one, two and four waves per target), one of them with an
-    empty list, equals one msvg_vit_search per profile, bit for bit."""
-    Ms = [60, 300, 700, 1500, 2300, 120]
+    32-lane groups, one wave per target, column segments), one of them with
+    an empty list, equals one msvg_vit_search per profile, bit for bit; the
+    targets include each long model's best-residue runs, so several
+    profiles hand pairs back to the multi-wave kernel in the same call."""
+    Ms = [60, 300, 700, 1500, 2300, 120, 3000, 5200]
     ctx = ca.MSVContext(0, tiles=None)
     rng = np.random.default_rng(17)
+    profs = [ca.sample_profile(60 + k, M) for k, M in enumerate(Ms)]
     seqs = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in rng.integers(1, 2500, 300)]
+    hot = []
+    for prof in profs[6:]:
+        vp = prof.viterbi()
+        for w in (40, 400):
+            k0 = int(rng.integers(0, prof.M - w))
+            hot.append(len(seqs))
+            seqs.append(_best_residues(vp, k0, k0 + w, rng))
     db = ca.Database.from_list(seqs)
     did = ctx.add_database(db)
     pids, lists = [], []
-    for k, M in enumerate(Ms):
-        prof = ca.sample_profile(60 + k, M)
+    for k, prof in enumerate(profs):
         pid = ctx.add_profile(prof)
         ctx.set_viterbi(pid, prof.viterbi())
         pids.append(pid)
-        lists.append(np.zeros(0, np.int64) if k == 5 else rng.choice(db.n, 120 + 10 * k, replace=False))
+        t = np.zeros(0, np.int64) if k == 5 else rng.choice(300, 120 + 10 * k, replace=False)
+        if k >= 6:
+            t = np.concatenate([t, hot])
+        lists.append(np.asarray(t, np.int64))
     print(f"profiles M={Ms} targets={db.n}", flush=True)
     many = ctx.vit_search_many(pids, did, lists)
+    assert ctx.stats()["n_slow"] >= 2
     for pid, t, (sc, st) in zip(pids, lists, many):
         sc1, st1 = ctx.vit_search(pid, did, t)
         assert _same(sc, st, sc1, st1)

@@ -132,6 +132,24 @@ struct JArgs {
   int32_t cap;
 };
 
+// msv_jrow_many: the J-aware pairs of many profiles in one launch.
+// Workgroup b works on profile b % np; its waves take that profile's pairs
+// at a fixed stride (no device counter).
+struct JProf {
+  ProfDev prof;
+  const uint4 *tab;            // the profile's fast table for 64-lane groups, q registers
+  const SlowItem *items;
+  const int32_t *count;        // pairs queued (by the J-free pass)
+  int32_t cap, q;
+};
+
+struct JManyArgs {
+  DbDev db;
+  const JProf *profs;
+  int32_t np;
+  int32_t wpp;                 // workgroups per profile (grid = np * wpp)
+};
+
 struct OneArgs {               // one target, one workgroup (the per-call drop-in)
   ProfDev prof;
   const uint4 *tab;            // 64-lane fast table over residue codes 0..R-1, null row R

@@ -483,6 +483,51 @@ __global__ void __launch_bounds__(256) msv_jrow(JArgs a) {
   }
 }
 
+// The J-aware pairs of many profiles in one launch (a batched search's J
+// stage): one launch per profile lasts as long as its longest pair's row
+// chain, so a profile's launch after another's on the same stream adds its
+// whole chain; here every profile's pairs run side by side.  Workgroup b
+// stages profile b % np's table; wave w of the profile's k-th workgroup
+// takes pairs k * waves + w, then every wpp * waves further -- msv_jrow's
+// static stride, with no device counter (a lane-0 atomic feeding a loop
+// around this body was compiled into a nested loop that re-read a reset
+// item index).  A workgroup whose profile has no pair for it leaves before
+// staging.
+template <int Q>
+__device__ __forceinline__ void jrow_stride(const JManyArgs &a, const JProf &jp, const uint4 *lds, int n, int first,
+                                            int stride, int lane) {
+  for (int it = first; it < n; it += stride) {
+    const int64_t s = jp.items[it].seq;
+    int xEmax;
+    const uint32_t w = jrow_task<Q>(lds + lane, a.db.res + a.db.off[s], a.db.len[s], a.db.tjb[s], jp.prof, lane, xEmax);
+    if (lane == 0) jp.prof.res[s] = (uint16_t) w;
+  }
+}
+
+__global__ void __launch_bounds__(512) msv_jrow_many(JManyArgs a) {
+  extern __shared__ uint4 lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int) (threadIdx.x >> 6));
+  const int nwaves = blockDim.x >> 6;
+  const JProf jp = a.profs[blockIdx.x % a.np];
+  const int n = min(*jp.count, jp.cap);
+  const int k = blockIdx.x / a.np;                       // the profile's k-th workgroup
+  if (k * nwaves >= n) return;                           // no pair for it: leave before staging
+  const int ntab = (a.db.R + 1) * ((jp.q + 3) / 4) * 64;
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = jp.tab[i];
+  __syncthreads();
+  const int first = k * nwaves + wave, stride = a.wpp * nwaves;
+  switch (jp.q) {
+#define MSVG_JQ(q) case q: jrow_stride<q>(a, jp, lds, n, first, stride, lane); break;
+    MSVG_JQ(1) MSVG_JQ(2) MSVG_JQ(3) MSVG_JQ(4) MSVG_JQ(5) MSVG_JQ(6) MSVG_JQ(7) MSVG_JQ(8)
+    MSVG_JQ(9) MSVG_JQ(10) MSVG_JQ(11) MSVG_JQ(12) MSVG_JQ(13) MSVG_JQ(14) MSVG_JQ(15) MSVG_JQ(16)
+    MSVG_JQ(17) MSVG_JQ(18) MSVG_JQ(19) MSVG_JQ(20) MSVG_JQ(21) MSVG_JQ(22) MSVG_JQ(23) MSVG_JQ(24)
+    MSVG_JQ(25) MSVG_JQ(26) MSVG_JQ(27) MSVG_JQ(28) MSVG_JQ(29) MSVG_JQ(30) MSVG_JQ(31) MSVG_JQ(32)
+#undef MSVG_JQ
+    default: break;                    // the host sends q in 1..32 only
+  }
+}
+
 // ---------------------------------------------------------------------------
 // One-target composite for the per-call drop-in (msvfilter.c:78-213 for one
 // dsq): the J-aware row loop above from the start of the target, then the
@@ -863,6 +908,18 @@ extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t l
   return dispatch_jrow(Q, *a, grid, lds, st, MSVG_QSEQ{});
 }
 
+// np * wpp workgroups of 512 threads; lds: the largest of the profiles' tables.
+extern "C" hipError_t msvk_launch_jrow_many(const JManyArgs *a, size_t lds, hipStream_t st) {
+  if (a->np < 1 || a->wpp < 1 || lds > 160 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(msv_jrow_many, dim3(a->np * a->wpp), dim3(512), lds, st, *a);
+  return hipGetLastError();
+}
+
+// Workgroups of msv_jrow_many resident per CU with this much LDS.
+extern "C" hipError_t msvk_jrow_many_occupancy(size_t lds, int *per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, reinterpret_cast<const void *>(&msv_jrow_many), 512, lds);
+}
+
 template <int Q>
 static hipError_t launch_one_t(const OneArgs &a, size_t lds, hipStream_t st) {
   hipLaunchKernelGGL((msv_one<Q>), dim3(1), dim3(1024), lds, st, a);
@@ -925,6 +982,8 @@ extern "C" hipError_t msvk_prepare() {
   if ((e = set_lds_all<8>(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_seg(MSVG_SEGQ{})) != hipSuccess) return e;
   if ((e = set_lds_jrow(MSVG_QSEQ{})) != hipSuccess) return e;
+  if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&msv_jrow_many),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)) != hipSuccess) return e;
   if ((e = set_lds_one(MSVG_QSEQ{})) != hipSuccess) return e;
   if ((e = set_lds_scan(MSVG_SCANQ{})) != hipSuccess) return e;
   if ((e = set_lds_all<16>(MSVG_QSEQ{})) != hipSuccess) return e;

@@ -57,6 +57,8 @@ extern "C" hipError_t msvk_launch_vit_seg(int Q, const VitSegArgs *a, int grid, 
 extern "C" int msvk_vit_seg_cap(int Q, int R, int ncu);
 extern "C" hipError_t msvk_launch_slow(const SlowArgs *a, int grid, hipStream_t st);
 extern "C" hipError_t msvk_launch_jrow(int Q, const JArgs *a, int grid, size_t lds, hipStream_t st);
+extern "C" hipError_t msvk_launch_jrow_many(const JManyArgs *a, size_t lds, hipStream_t st);
+extern "C" hipError_t msvk_jrow_many_occupancy(size_t lds, int *per_cu);
 extern "C" hipError_t msvk_launch_vit(int Q, int W, const VitArgs *a, int ncu, hipStream_t st);
 extern "C" int msvk_vit_slots(int Q, int W, const VitArgs *a, int ncu);
 extern "C" hipError_t msvk_launch_vit_band(int G, int Q, int vote, const VitArgs *a, int grid, hipStream_t st);
@@ -232,6 +234,7 @@ struct Tile { int lo, hi, lanes, waves; };
 struct Bufs {                             // device buffers one search writes
   DevMem d_res, d_prof, d_slow, d_counters, d_usc, d_status, d_pass, d_thr, d_usctab, d_seg, d_jslow;
   DevMem d_bat;                          // batched launches: BatProf arrays and first-profile lists
+  DevMem d_jprof;                        // batched searches' J stage: JProf per profile
   int64_t slow_cap = 0;
   int64_t jcap = 0;                      // per-profile J-aware queue capacity
   Arena *arena = nullptr;                // allocation-free search: tables, chunk lists and buffers from here
@@ -239,7 +242,7 @@ struct Bufs {                             // device buffers one search writes
   Bufs() = default;
   explicit Bufs(Arena *a) : arena(a) {
     for (DevMem *m : {&d_res, &d_prof, &d_slow, &d_counters, &d_usc, &d_status, &d_pass, &d_thr, &d_usctab,
-                      &d_seg, &d_jslow, &d_bat})
+                      &d_seg, &d_jslow, &d_bat, &d_jprof})
       m->arena = a;
   }
 };
@@ -1045,42 +1048,44 @@ int enqueue_search(msvg_ctx *c, Bufs &b, const int32_t *ids, int np, int dbid, d
     HIPCHK(c, hipStreamWaitEvent(st, c->join, 0));
   }
   if (batch) {
-    // J-aware launches of the batched profiles, once the batched launches
-    // have filled their queues: one per profile, round-robin over kJStreams
-    // streams forked from st (in line on st when no side stream is given).
-    // Each lasts as long as its longest pair; workgroups without a pair leave
-    // before staging.  (Forking each group's J launches right after its own
-    // batched launch, beside the later groups' launches, measured slower.)
-    std::vector<int> jl;
-    for (int i = 0; i < np; ++i)
-      if (Gb[i] && jtab[i]) jl.push_back(i);
-    const int nst = (side && !jl.empty()) ? std::min<int>(kJStreams, (int) jl.size()) : 0;
+    // The J-aware pairs of the batched profiles, once the batched launches
+    // have filled their queues: one msv_jrow_many launch, every profile's
+    // pairs side by side.  (One launch per profile lasts as long as its
+    // longest pair's row chain, so even on four streams the launches after
+    // the first on each stream add their whole chains.)
+    std::vector<JProf> hjp;
+    size_t jlds = 0;
+    for (int i = 0; i < np; ++i) {
+      if (!Gb[i] || !jtab[i]) continue;
+      JProf jp{};
+      jp.prof = pds[i];
+      jp.prof.res = b.d_res.as<uint16_t>() + (size_t) i * n;
+      jp.tab = jtab[i];
+      jp.items = b.d_jslow.as<SlowItem>() + (size_t) i * b.jcap;
+      jp.count = jcounts + i;
+      jp.cap = (int32_t) b.jcap;
+      jp.q = qfor(c->profiles[ids[i]]->M, 128);
+      jlds = std::max(jlds, fast_lds(d.R, jp.q));
+      hjp.push_back(jp);
+    }
     if (timed) HIPCHK(c, hipEventRecord(c->bev[8], st));
-    if (nst) {
-      HIPCHK(c, hipEventRecord(c->fork, st));
-      for (int k = 0; k < nst; ++k) HIPCHK(c, hipStreamWaitEvent(c->jst[k], c->fork, 0));
-    }
-    for (size_t u = 0; u < jl.size(); ++u) {
-      const int i = jl[u];
-      hipStream_t js = nst ? c->jst[u % nst] : st;
-      JArgs ja;
-      ja.prof = pds[i];
-      ja.prof.res = b.d_res.as<uint16_t>() + (size_t) i * n;
-      ja.db = dd;
-      ja.tab = jtab[i];
-      ja.items = b.d_jslow.as<SlowItem>() + (size_t) i * b.jcap;
-      ja.count = jcounts + i;
-      ja.cap = (int32_t) b.jcap;
-      const int Q64 = qfor(c->profiles[ids[i]]->M, 128);
-      const size_t jlds = fast_lds(d.R, Q64);
-      const int per_cu = (int) std::max<size_t>(1, std::min<size_t>(4, kLdsBytes / jlds));
-      HIPCHK(c, msvk_launch_jrow(Q64, &ja, c->ncu * per_cu, jlds, js));
-    }
-    if (nst)
-      for (int k = 0; k < nst; ++k) {
-        HIPCHK(c, hipEventRecord(c->jdone[k], c->jst[k]));
-        HIPCHK(c, hipStreamWaitEvent(st, c->jdone[k], 0));
+    if (!hjp.empty()) {
+      HIPCHK(c, b.d_jprof.ensure(hjp.size() * sizeof(JProf)));
+      if (!capturing) {
+        HIPCHK(c, hipMemcpyAsync(b.d_jprof.p, hjp.data(), hjp.size() * sizeof(JProf), hipMemcpyHostToDevice, st));
+        HIPCHK(c, hipStreamSynchronize(st));           // hjp ends with this block
       }
+      int per_cu = 0;
+      HIPCHK(c, msvk_jrow_many_occupancy(jlds, &per_cu));
+      JManyArgs ja;
+      ja.db = dd;
+      ja.profs = b.d_jprof.as<JProf>();
+      ja.np = (int32_t) hjp.size();
+      // about twice the workgroups the device holds, each profile the same
+      // number (its workgroups beyond its pairs leave at once)
+      ja.wpp = std::max(1, (2 * c->ncu * std::max(1, per_cu) + ja.np - 1) / ja.np);
+      HIPCHK(c, msvk_launch_jrow_many(&ja, jlds, st));
+    }
     if (timed) HIPCHK(c, hipEventRecord(c->bev[9], st));
   }
   if (timed) HIPCHK(c, hipEventRecord(c->ev[1], st));
@@ -1135,6 +1140,7 @@ size_t noalloc_bytes(msvg_ctx *c, const int32_t *ids, int np, int dbid) {
   add((size_t) np * sizeof(ProfDev));
   add((size_t) noalloc_slow_cap(np, n) * sizeof(SlowItem));
   add((size_t) (2 * np + 2) * sizeof(int32_t));
+  add((size_t) np * sizeof(JProf));
   add((size_t) np * std::max<int64_t>(1024, n / 16) * sizeof(SlowItem));
   add((size_t) np * d.Ls.size() * sizeof(uint16_t) + 16);
   add((size_t) np * 511 * sizeof(float));

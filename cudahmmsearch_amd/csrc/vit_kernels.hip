@@ -637,10 +637,9 @@ vit_band(VitArgs a) {
         const uint32_t mm = LT ? tl[(0 * Q + j) * G] : tMM[LT ? 0 : j];
         const uint32_t im = LT ? tl[(1 * Q + j) * G] : tIM[LT ? 0 : j];
         const uint32_t dm = LT ? tl[(2 * Q + j) * G] : tDM[LT ? 0 : j];
-        uint32_t sv = bm[j];                                         // :146-150
-        sv = maxs(sv, adds(pm, mm));
-        sv = maxs(sv, adds(pi, im));
-        sv = maxs(sv, adds(pd, dm));
+        // :146-150 as a tree (max is exact in any order): two independent
+        // maxima, so dependent packed ops need not follow each other
+        uint32_t sv = maxs(maxs(bm[j], adds(pm, mm)), maxs(adds(pi, im), adds(pd, dm)));
         sv = adds(sv, e[j]);
         xEv = maxs(xEv, sv);                                         // :151
         Ir[j] = maxs(adds(Mr[j], tMI[j]), adds(Ir[j], tII[j]));      // :171-172
@@ -900,10 +899,9 @@ vit_seg(VitSegArgs a) {
         const uint32_t mm = LT ? tl[(0 * Q + j) * G] : tMM[LT ? 0 : j];
         const uint32_t im = LT ? tl[(1 * Q + j) * G] : tIM[LT ? 0 : j];
         const uint32_t dm = LT ? tl[(2 * Q + j) * G] : tDM[LT ? 0 : j];
-        uint32_t sv = bm[j];                                         // :146-150
-        sv = maxs(sv, adds(pm, mm));
-        sv = maxs(sv, adds(pi, im));
-        sv = maxs(sv, adds(pd, dm));
+        // :146-150 as a tree (max is exact in any order): two independent
+        // maxima, so dependent packed ops need not follow each other
+        uint32_t sv = maxs(maxs(bm[j], adds(pm, mm)), maxs(adds(pi, im), adds(pd, dm)));
         sv = adds(sv, e[j]);
         xEv = maxs(xEv, sv);                                         // :151
         Ir[j] = maxs(adds(Mr[j], tMI[j]), adds(Ir[j], tII[j]));      // :171-172

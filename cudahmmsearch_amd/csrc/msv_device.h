@@ -77,6 +77,8 @@ struct FastArgs {
   int64_t scan_stride;         // no queues are written
   int32_t scan_nprof;          // mode 3: profiles; the grid is a multiple of it
   const uint8_t *scan_rowcode; // mode 3: LDS row r holds table row scan_rowcode[r] (r = 0..db.R)
+  const int32_t *scan_soff;    // mode 3: wave slot w = split * waves + wave scores the tasks
+  const int32_t *scan_stask;   //   scan_stask[scan_soff[w] .. scan_soff[w + 1])
 };
 
 // Batched J-free pass over many profiles in one persistent launch per lane

@@ -132,14 +132,16 @@ __global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
   if (a.mode == 3) {
     // Scan: this workgroup's profile against its targets (queries of an
     // hmmscan batch, sorted by length so a wave's groups finish together);
-    // task t is targets t*NG .. t*NG+NG-1, tasks round-robin over the waves
-    // of the profile's workgroups.
+    // task t is targets t*NG .. t*NG+NG-1.  The tasks of a wave slot (its
+    // split and wave) come from the host's schedule, which balances the
+    // slots' rows; every profile of the launch shares it.
     const ProfDev pd = a.scan_profs[sp];
     uint16_t *res = a.scan_out + sp;
     const int split = (int) (blockIdx.x / (unsigned) a.scan_nprof);
-    const int nsplit = (int) (gridDim.x / (unsigned) a.scan_nprof);
-    for (int t = split * nwaves + wave; t < a.ntasks; t += nsplit * nwaves)
-      fast_task<G, Q, TWG>(a, pd, res, a.scan_stride, lds, (int64_t) t * NG + g, lane);
+    const int slot = __builtin_amdgcn_readfirstlane(split * nwaves + wave);
+    const int i1 = a.scan_soff[slot + 1];
+    for (int i = a.scan_soff[slot]; i < i1; ++i)
+      fast_task<G, Q, TWG>(a, pd, res, a.scan_stride, lds, (int64_t) a.scan_stask[i] * NG + g, lane);
   } else if (a.mode == 0) {
     for (;;) {
       int c = 0;

@@ -36,7 +36,6 @@
 #include "msv_device.h"
 #include "msv_host.h"
 #include "../../include/msv_prep.h"
-#include "../../include/msv_prep.h"
 
 // Scratch memory of the allocation-free search (msvg_search_noalloc): one
 // array per GPU, held until the process ends.  Host code never uses this
@@ -2624,15 +2623,128 @@ int scan_prepare_grouped(msvg_ctx *c, ScanSet *set) {
   return MSVG_OK;
 }
 
-// Batches of at least this many queries take the grouped kernel.  Its
-// workgroups (one per profile) have kScanWaves waves when the class has
-// enough profiles to fill the device (tables G lanes wide let several
-// workgroups share a CU, and a wave's query groups, taken round-robin
-// longest first, even out its share of a skewed batch), else 16.
+// Batches of at least this many queries take the grouped kernel.
 constexpr int kScanGroupMin = 4;
-constexpr int kScanWaves = 8;
-constexpr int kScanManyProfiles = 128;
 constexpr int kScanCUs = 256;
+
+// Wave-slot schedule of one grouped launch.  Task t (queries t*NG ..
+// t*NG+NG-1 of the batch sorted longest first) costs its longest query's
+// padded rows plus 16 (its set-up and decision); every profile of a class
+// sees the same tasks, so one schedule serves all of its workgroups.
+// S slots are split * waves + wave over nsplit workgroups per profile.
+struct ScanPlan {
+  int waves = 1, nsplit = 1;
+  int64_t load = 0, sum = 0;         // rows of the busiest slot, of all tasks
+  std::vector<int32_t> off, task;    // slot w's tasks: task[off[w] .. off[w + 1])
+};
+
+// Longest processing time first over S slots: each task, in order (cost is
+// non-increasing), goes to the slot with the fewest rows.  Returns the
+// largest slot load; fills the slot lists when plan is given.
+int64_t scan_lpt(const std::vector<int64_t> &cost, int S, ScanPlan *plan) {
+  using Load = std::pair<int64_t, int>;
+  std::priority_queue<Load, std::vector<Load>, std::greater<Load>> h;
+  for (int s = 0; s < S; ++s) h.push({0, s});
+  std::vector<int32_t> slot_of(plan ? cost.size() : 0);
+  int64_t mx = 0;
+  for (size_t t = 0; t < cost.size(); ++t) {
+    Load l = h.top();
+    h.pop();
+    l.first += cost[t];
+    mx = std::max(mx, l.first);
+    if (plan) slot_of[t] = l.second;
+    h.push(l);
+  }
+  if (plan) {
+    plan->off.assign((size_t) S + 1, 0);
+    for (int32_t s : slot_of) ++plan->off[(size_t) s + 1];
+    for (int s = 0; s < S; ++s) plan->off[(size_t) s + 1] += plan->off[(size_t) s];
+    plan->task.assign(cost.size(), 0);
+    std::vector<int32_t> fill(plan->off.begin(), plan->off.end() - 1);
+    for (size_t t = 0; t < cost.size(); ++t) plan->task[(size_t) fill[(size_t) slot_of[t]]++] = (int32_t) t;
+  }
+  return mx;
+}
+
+// Slots per profile: the most whose schedule keeps at least 93% of the slot
+// rows busy (sum of costs / (S * largest load)); a class of few profiles,
+// whose waves alone would not cover the device, may go down to 85% for up
+// to about four waves per CU, so its long models do not hold the batch up.
+// At most 16 waves per workgroup; more slots are split over workgroups.
+ScanPlan scan_plan(const std::vector<int64_t> &cost, int nprof) {
+  const int nt = (int) cost.size();
+  int64_t sum = 0;
+  for (int64_t x : cost) sum += x;
+  const int smax = std::max(1, std::min(nt, 64));
+  int s_many = 1, s_few = 0;
+  for (int S = smax; S >= 1; --S) {                 // largest S first; stop at the first 93% schedule
+    const double eff = (double) sum / ((double) S * (double) std::max<int64_t>(1, scan_lpt(cost, S, nullptr)));
+    if (s_few == 0 && eff >= 0.85) s_few = S;
+    if (eff >= 0.93) { s_many = S; break; }
+  }
+  int S = s_many;
+  if ((int64_t) nprof * s_many < 4 * kScanCUs)
+    S = std::max(s_many, std::min(std::max(s_few, 1), (4 * kScanCUs + nprof - 1) / nprof));
+  ScanPlan p;
+  p.nsplit = (S + 15) / 16;
+  p.waves = (S + p.nsplit - 1) / p.nsplit;
+  p.load = scan_lpt(cost, p.waves * p.nsplit, &p);
+  p.sum = sum;
+  return p;
+}
+
+// Streams of a grouped scan's launches.  A stream runs its launches one
+// after the other.  A class with fewer workgroups than CUs is bound by its
+// busiest slot's row chain, not by the device's throughput, so such classes
+// get streams of their own (two of four), longest chain (register count x
+// rows) first, each to the stream whose chains add up to the least; the
+// other classes share the remaining streams the same way by their work
+// (profiles x registers x rows).  Returns each class's stream and fills
+// `order` with the launch order.
+std::vector<int> scan_streams(const std::vector<std::unique_ptr<ScanSet::GCls>> &gcls,
+                              const std::vector<ScanPlan> &plans, int ns, std::vector<size_t> &order) {
+  const size_t n = gcls.size();
+  std::vector<int> stream(n, 0);
+  std::vector<int64_t> key(n);
+  std::vector<size_t> small, big;
+  for (size_t ci = 0; ci < n; ++ci) {
+    const int64_t wgs = (int64_t) gcls[ci]->members.size() * plans[ci].nsplit;
+    if (wgs < kScanCUs) {
+      small.push_back(ci);
+      key[ci] = (int64_t) gcls[ci]->Q * plans[ci].load;
+    } else {
+      big.push_back(ci);
+      key[ci] = (int64_t) gcls[ci]->members.size() * gcls[ci]->Q * plans[ci].sum;
+    }
+  }
+  auto by_key = [&key](size_t x, size_t y) { return key[x] != key[y] ? key[x] > key[y] : x < y; };
+  std::sort(small.begin(), small.end(), by_key);
+  std::sort(big.begin(), big.end(), by_key);
+  auto deal = [&](const std::vector<size_t> &cls, int s0, int s1) {
+    std::vector<int64_t> acc((size_t) (s1 - s0), 0);
+    for (size_t ci : cls) {
+      const int k = (int) (std::min_element(acc.begin(), acc.end()) - acc.begin());
+      acc[(size_t) k] += key[ci];
+      stream[ci] = s0 + k;
+    }
+  };
+  if (small.empty() || big.empty() || ns < 4) {
+    std::vector<size_t> all(small);
+    all.insert(all.end(), big.begin(), big.end());
+    std::sort(all.begin(), all.end(), by_key);
+    deal(all, 0, ns);
+    order = all;
+  } else {
+    deal(small, 0, ns / 2);
+    deal(big, ns / 2, ns);
+    order.clear();                     // alternate, so each stream's first launch goes out at once
+    for (size_t i = 0; i < std::max(small.size(), big.size()); ++i) {
+      if (i < big.size()) order.push_back(big[i]);
+      if (i < small.size()) order.push_back(small[i]);
+    }
+  }
+  return stream;
+}
 
 // msvg_scan_many for a batch: the fast kernel in scan mode (one workgroup
 // per profile, 64/G queries per wave, longest queries first), then the
@@ -2659,7 +2771,24 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
   const size_t o_len = o_off + (size_t) nl * sizeof(int64_t);
   const size_t o_tjb = o_len + (size_t) nl * sizeof(int32_t);
   const size_t o_rc = o_tjb + (size_t) nl;
-  const size_t total = o_rc + rowcode.size();
+  // the wave-slot schedules, one per class, after the row list
+  std::vector<ScanPlan> plans;
+  std::vector<size_t> o_plan;
+  size_t o_sched = (o_rc + rowcode.size() + 15) & ~(size_t) 15;
+  {
+    std::map<int, std::vector<int64_t>> costs;                // per G
+    for (auto &gc : set->gcls) {
+      std::vector<int64_t> &cost = costs[gc->G];
+      if (cost.empty()) {
+        const int NG = 64 / gc->G;
+        for (int32_t k = 0; k < nl; k += NG) cost.push_back((((int64_t) qlen[order[(size_t) k]] + 15) & ~(int64_t) 15) + 16);
+      }
+      plans.push_back(scan_plan(cost, (int) gc->members.size()));
+      o_plan.push_back(o_sched);
+      o_sched += ((plans.back().off.size() + plans.back().task.size()) * sizeof(int32_t) + 15) & ~(size_t) 15;
+    }
+  }
+  const size_t total = o_sched;
   if (c->h_one_cap < total + 64) {
     if (c->h_one_res) { (void) hipHostFree(c->h_one_res); c->h_one_res = nullptr; c->h_one_cap = 0; }
     const size_t cap = std::max<size_t>(total + 64, 4096);
@@ -2671,6 +2800,11 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
   int32_t *h_len = reinterpret_cast<int32_t *>(h + o_len);
   uint8_t *h_tjb = h + o_tjb;
   std::memcpy(h + o_rc, rowcode.data(), rowcode.size());
+  for (size_t ci = 0; ci < plans.size(); ++ci) {
+    const ScanPlan &pl = plans[ci];
+    std::memcpy(h + o_plan[ci], pl.off.data(), pl.off.size() * sizeof(int32_t));
+    std::memcpy(h + o_plan[ci] + pl.off.size() * sizeof(int32_t), pl.task.data(), pl.task.size() * sizeof(int32_t));
+  }
   std::vector<int> tjb_k((size_t) nl);
   int64_t pos = 0;
   for (int32_t k = 0; k < nl; ++k) {
@@ -2718,28 +2852,26 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
     HIPCHK(c, hipEventRecord(c->fork, st));
     for (int k = 0; k < ns; ++k) HIPCHK(c, hipStreamWaitEvent(c->vstream[k], c->fork, 0));
   }
-  size_t base = 0;
-  for (size_t ci = 0; ci < set->gcls.size(); ++ci) {
+  std::vector<size_t> cbase(set->gcls.size(), 0), lorder;       // each class's first result row
+  for (size_t ci = 1; ci < set->gcls.size(); ++ci) cbase[ci] = cbase[ci - 1] + set->gcls[ci - 1]->members.size();
+  const std::vector<int> cstream = scan_streams(set->gcls, plans, ns, lorder);
+  for (size_t ci : lorder) {
     auto &gc = set->gcls[ci];
     FastArgs a{};
     a.db = qdb;
     a.mode = 3;
     a.ntasks = (nl + (64 / gc->G) - 1) / (64 / gc->G);
     a.scan_profs = gc->d_profs.as<ProfDev>();
-    a.scan_out = d_words + base;                            // words [sorted query][row]
+    a.scan_out = d_words + cbase[ci];                       // words [sorted query][row]
     a.scan_stride = (int64_t) nmem;
     a.scan_rowcode = d + o_rc;
-    // few profiles (long models): all query groups at once; many: fewer
-    // waves per workgroup so that several workgroups share each CU
+    const ScanPlan &pl = plans[ci];
+    a.scan_soff = reinterpret_cast<const int32_t *>(d + o_plan[ci]);
+    a.scan_stask = a.scan_soff + pl.off.size();
     const int nprof = (int) gc->members.size();
-    const int waves = std::min(nprof >= kScanManyProfiles ? kScanWaves : 16, a.ntasks);
-    // a class of few profiles splits its query groups over several
-    // workgroups per profile, up to about one workgroup per CU
-    const int nsplit = std::max(1, std::min((kScanCUs + nprof - 1) / nprof, (a.ntasks + waves - 1) / waves));
     a.scan_nprof = nprof;
-    hipStream_t cs = ns > 1 ? c->vstream[ci % ns] : st;
-    HIPCHK(c, msvk_launch_fast(gc->G, gc->Q, &a, nprof * nsplit, 64 * waves, fast_lds_g(R, gc->Q, gc->G), cs));
-    base += gc->members.size();
+    hipStream_t cs = ns > 1 ? c->vstream[cstream[ci]] : st;
+    HIPCHK(c, msvk_launch_fast(gc->G, gc->Q, &a, nprof * pl.nsplit, 64 * pl.waves, fast_lds_g(R, gc->Q, gc->G), cs));
   }
   if (ns > 1) {
     for (int k = 0; k < ns; ++k) {
@@ -2779,7 +2911,7 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
   // outputs at [query][profile position], so reads and writes run along rows.
   struct Row { int32_t i; int base; float scale, mu, lambda; };
   std::vector<Row> rows(nmem);
-  base = 0;
+  size_t base = 0;
   for (auto &gc : set->gcls) {
     for (size_t m = 0; m < gc->members.size(); ++m) {
       const int32_t i = gc->members[m];

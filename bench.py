@@ -45,7 +45,8 @@ cudahmmsearch_amd.workloads; Easel-compatible generators):
   scan  hmmscan direction: --scan-profiles sampled profiles (M log-normal
       median 250, sigma 0.6, clipped 30..2560, seeds 100000..) against
       --scan-queries query sequences (log-normal median 350, sigma 0.7,
-      seed 6), one msvg_scan call per query
+      seed 6), one msvg_scan call per query, then all of them as one
+      msvg_scan_many batch
   forms  the packed-form probe (msvg_probe_forms): the J-free recurrence in
       the kernels' f16 clamp form and in an unsigned 16-bit form on the same
       cells, value = the f16 form's speed-up
@@ -294,7 +295,7 @@ def scan_workload(ca, args):
         ms_k += ctx.stats()["ms_fast"]
     wall = time.perf_counter() - t0
     # the same queries as one msvg_scan_many batch: each profile's table is
-    # staged once per batch and its workgroup's four waves share the queries
+    # staged once per batch and its workgroup's waves share the queries
     ctx.scan_many(pids, qdb)
     walls, kms = [], []
     for _ in range(5):                                        # median of five calls

@@ -91,15 +91,39 @@ __device__ __forceinline__ void row_step(uint32_t (&r)[Q], const uint32_t *e, ui
   }
 }
 
+// Where a pair's result word goes.  A search: the profile's result array,
+// and a pair whose J state may fire or that needs the byte-exact pass is
+// queued.  A grouped scan: the word only, in the profile's column of the
+// [target][profile] word array (the host lists such pairs).
+struct BatOut {
+  uint16_t *res;
+  SlowItem *slow;
+  int32_t *slow_count;
+  SlowItem *jslow;
+  int32_t *jslow_count;
+  int32_t prof_idx, slow_cap, jslow_cap;
+  __device__ __forceinline__ void operator()(int64_t s, uint32_t w) const {
+    res[s] = (uint16_t) w;
+    if ((w >> 8) == RES_SLOW) defer(w, prof_idx, s, slow, slow_count, slow_cap, jslow, jslow_count, jslow_cap);
+  }
+};
+
+struct ScanOut {
+  uint16_t *col;
+  int64_t stride;
+  __device__ __forceinline__ void operator()(int64_t s, uint32_t w) const { col[s * stride] = (uint16_t) w; }
+};
+
 // G = 64: one target per wave (s is wave-uniform).
-template <int Q>
-__device__ __forceinline__ void task64(const BatArgs &a, const BatProf &bp, const uint4 *lds, int s, int lane) {
+template <int Q, class Out>
+__device__ __forceinline__ void task64(const DbDev &db, const ProfDev &pd, const uint4 *lds, int s, int lane,
+                                       const Out &out) {
   constexpr int QB = (Q + 3) / 4;
-  const int L = sload(a.db.len + s);
+  const int L = sload(db.len + s);
   const int Lpad = (L + 15) & ~15;
-  const uint8_t *p = a.db.res + sload(a.db.off + s);
+  const uint8_t *p = db.res + sload(db.off + s);
   const uint4 *tl = lds + lane;
-  const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
+  const uint32_t nb = (uint32_t) db.R * 0x01010101u;
   const uint4 null4 = make_uint4(nb, nb, nb, nb);
 
   uint32_t r[Q];
@@ -158,10 +182,7 @@ __device__ __forceinline__ void task64(const BatArgs &a, const BatProf &bp, cons
   const float m = wave_max_f32(fmaxf((float) ah.x, (float) ah.y));
   if (lane == 0) {
     const int Rp = (int) (m * 2048.0f + 0.5f);
-    const uint32_t w = decide(Rp, sload(a.db.tjb + s), bp.pd);
-    bp.pd.res[s] = (uint16_t) w;
-    if ((w >> 8) == RES_SLOW)
-      defer(w, bp.prof_idx, s, a.slow, a.slow_count, a.slow_cap, bp.jslow, bp.jslow_count, a.jslow_cap);
+    out(s, decide(Rp, sload(db.tjb + s), pd));
   }
 }
 
@@ -185,16 +206,16 @@ __device__ __forceinline__ uint32_t row_addr(const uint4 &w, uint32_t col) {
 }
 
 // G = 8, 16, 32: 64/G targets per wave, s is this lane group's target.
-template <int G, int Q>
-__device__ __forceinline__ void task_narrow(const BatArgs &a, const BatProf &bp, const uint4 *lds, int64_t s,
-                                            int lane) {
+template <int G, int Q, class Out>
+__device__ __forceinline__ void task_narrow(const DbDev &db, const ProfDev &pd, const uint4 *lds, int64_t s,
+                                            int lane, const Out &out) {
   constexpr int QB = (Q + 3) / 4;
   constexpr int TW = G < 16 ? 16 : G;
   constexpr uint32_t JB = 32u * TW * 16u;               // bytes per register block
-  const bool valid = s < a.db.n;
-  const int L = valid ? a.db.len[s] : 0;
+  const bool valid = s < db.n;
+  const int L = valid ? db.len[s] : 0;
   const int Lpad = (L + 15) & ~15;
-  const uint8_t *p = a.db.res + (valid ? a.db.off[s] : 0);
+  const uint8_t *p = db.res + (valid ? db.off[s] : 0);
 
   int Lmax = L;                                         // the wave's longest target
 #pragma unroll
@@ -202,7 +223,7 @@ __device__ __forceinline__ void task_narrow(const BatArgs &a, const BatProf &bp,
   Lmax = __builtin_amdgcn_readfirstlane(Lmax);
   const int Lfull = Lmax & ~15;
 
-  const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
+  const uint32_t nb = (uint32_t) db.R * 0x01010101u;
   const uint4 null4 = make_uint4(nb, nb, nb, nb);
   const uint32_t col = (uint32_t) (lane & 15) * 16u;
   const uint32_t half = (TW == 32) ? (uint32_t) ((lane >> 4) & 1) * 0x01010101u : 0u;
@@ -259,10 +280,7 @@ __device__ __forceinline__ void task_narrow(const BatArgs &a, const BatProf &bp,
   for (int o = G / 2; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
   if (valid && (lane % G) == 0) {
     const int Rp = (int) (m * 2048.0f + 0.5f);
-    const uint32_t w = decide(Rp, a.db.tjb[s], bp.pd);
-    bp.pd.res[s] = (uint16_t) w;
-    if ((w >> 8) == RES_SLOW)
-      defer(w, bp.prof_idx, s, a.slow, a.slow_count, a.slow_cap, bp.jslow, bp.jslow_count, a.jslow_cap);
+    out(s, decide(Rp, db.tjb[s], pd));
   }
 }
 
@@ -272,6 +290,7 @@ template <int G, int Q>
 __device__ __forceinline__ void drain(const BatArgs &a, const BatProf &bp, const uint4 *lds, int lane) {
   constexpr int NG = 64 / G;
   const int g = lane / G;
+  const BatOut out{bp.pd.res, a.slow, a.slow_count, bp.jslow, bp.jslow_count, bp.prof_idx, a.slow_cap, a.jslow_cap};
   for (;;) {
     int c = 0;
     if (lane == 0) c = atomicAdd(bp.counter, 1);
@@ -279,8 +298,8 @@ __device__ __forceinline__ void drain(const BatArgs &a, const BatProf &bp, const
     if (c >= a.nchunks) break;
     const int t0 = sload(a.chunk_start + c), t1 = sload(a.chunk_start + c + 1);
     for (int t = t0; t < t1; ++t) {
-      if constexpr (G == 64) task64<Q>(a, bp, lds, t, lane);
-      else task_narrow<G, Q>(a, bp, lds, (int64_t) t * NG + g, lane);
+      if constexpr (G == 64) task64<Q>(a.db, bp.pd, lds, t, lane, out);
+      else task_narrow<G, Q>(a.db, bp.pd, lds, (int64_t) t * NG + g, lane, out);
     }
   }
 }
@@ -344,6 +363,68 @@ __global__ void __launch_bounds__(1024) msv_batch(BatArgs a) {
     }
     __syncthreads();
     p = *slot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Grouped scan (the hmmscan direction, a batch of queries): workgroup b
+// scores profile b % np against the tasks of its wave slots, with the row
+// loops above.  Not persistent and no counters: the host's slot schedule
+// (the same for every profile of the launch) says which tasks each wave
+// takes.  QB is the launch's register-block count; pd.Q picks among the four
+// register counts it holds.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int G, int Q>
+__device__ __forceinline__ void scan_slot(const ScanGroupArgs &a, const ProfDev &pd, const uint4 *lds, int slot,
+                                          int lane, const ScanOut &out) {
+  if constexpr (Q <= bat_qmax<G>()) {
+    constexpr int NG = 64 / G;
+    const int i1 = sload(a.soff + slot + 1);
+    for (int i = sload(a.soff + slot); i < i1; ++i) {
+      const int t = sload(a.stask + i);
+      if constexpr (G == 64) task64<Q>(a.db, pd, lds, t, lane, out);
+      else task_narrow<G, Q>(a.db, pd, lds, (int64_t) t * NG + lane / G, lane, out);
+    }
+  }
+}
+
+}  // namespace
+
+template <int G, int QB>
+__global__ void __launch_bounds__(1024) msv_scan_group(ScanGroupArgs a) {
+  extern __shared__ uint4 lds[];
+  constexpr int TW = G < 16 ? 16 : G;
+  if (a.prio) __builtin_amdgcn_s_setprio(2);
+  const int lane = threadIdx.x & 63;
+  const int sp = (int) (blockIdx.x % (unsigned) a.np);
+  const int split = (int) (blockIdx.x / (unsigned) a.np);
+  const ProfDev pd = sload(a.profs + sp);
+  const int R1 = a.db.R + 1;
+  if constexpr (G == 64) {                 // [row][jb][64]: whole rows
+    constexpr int row = QB * 64;
+    for (int i = threadIdx.x; i < R1 * row; i += blockDim.x) {
+      const int r = i / row;
+      lds[i] = pd.ftab[(int) a.rowcode[r] * row + (i - r * row)];
+    }
+  } else {                                 // [jb][32 rows][TW]: R + 1 rows of each block
+    const int blk = R1 * TW;
+    for (int i = threadIdx.x; i < QB * blk; i += blockDim.x) {
+      const int jb = i / blk, rem = i - jb * blk;
+      const int r = rem / TW, col = rem - r * TW;
+      lds[(jb * 32 + r) * TW + col] = pd.ftab[(jb * 32 + (int) a.rowcode[r]) * TW + col];
+    }
+  }
+  __syncthreads();
+  const int slot = __builtin_amdgcn_readfirstlane(split * (int) (blockDim.x >> 6) + (int) (threadIdx.x >> 6));
+  const ScanOut out{a.out + sp, a.stride};
+  switch (pd.Q - 4 * (QB - 1)) {
+    case 1: scan_slot<G, 4 * QB - 3>(a, pd, lds, slot, lane, out); break;
+    case 2: scan_slot<G, 4 * QB - 2>(a, pd, lds, slot, lane, out); break;
+    case 3: scan_slot<G, 4 * QB - 1>(a, pd, lds, slot, lane, out); break;
+    case 4: scan_slot<G, 4 * QB>(a, pd, lds, slot, lane, out); break;
+    default: break;                        // not a register count of this launch: the host never sends one
   }
 }
 
@@ -418,4 +499,42 @@ extern "C" hipError_t msvk_build_bat_table(const uint8_t *rbv, int M, const uint
   if (grid > 65535) grid = 65535;
   hipLaunchKernelGGL(build_bat_table, dim3(grid), dim3(256), 0, st, rbv, M, rowcode, R, G, Q, TW, bias, out);
   return hipGetLastError();
+}
+
+// The grouped-scan kernels built: G = 8 up to 2 register blocks (M <= 128),
+// 16 up to 6, 32 up to 4 and 64 up to 8.
+#define MSVG_SCAN_GROUPS(X) \
+  X(8, 1) X(8, 2) X(16, 1) X(16, 2) X(16, 3) X(16, 4) X(16, 5) X(16, 6) X(32, 1) X(32, 2) X(32, 3) X(32, 4) \
+  X(64, 1) X(64, 2) X(64, 3) X(64, 4) X(64, 5) X(64, 6) X(64, 7) X(64, 8)
+
+extern "C" int msvk_scan_group_built(int G, int QB) {
+#define MSVG_SG_IS(g, qb) if (G == g && QB == qb) return 1;
+  MSVG_SCAN_GROUPS(MSVG_SG_IS)
+#undef MSVG_SG_IS
+  return 0;
+}
+
+extern "C" hipError_t msvk_prepare_scan_group() {
+  hipError_t e = hipSuccess;
+#define MSVG_SG_ATTR(g, qb)                                                                                  \
+  if (e == hipSuccess)                                                                                       \
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&msv_scan_group<g, qb>),                          \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  MSVG_SCAN_GROUPS(MSVG_SG_ATTR)
+#undef MSVG_SG_ATTR
+  return e;
+}
+
+extern "C" hipError_t msvk_launch_scan_group(int G, int QB, const ScanGroupArgs *a, int grid, int block, size_t lds,
+                                             hipStream_t st) {
+  if (grid < 1 || block < 64 || block > 1024 || block % 64 != 0 || lds > 160 * 1024 || a->np < 1 || grid % a->np != 0)
+    return hipErrorInvalidValue;
+#define MSVG_SG_LAUNCH(g, qb)                                                                                \
+  if (G == g && QB == qb) {                                                                                  \
+    hipLaunchKernelGGL((msv_scan_group<g, qb>), dim3(grid), dim3(block), lds, st, *a);                       \
+    return hipGetLastError();                                                                                \
+  }
+  MSVG_SCAN_GROUPS(MSVG_SG_LAUNCH)
+#undef MSVG_SG_LAUNCH
+  return hipErrorInvalidValue;
 }

@@ -63,8 +63,7 @@ struct FastArgs {
   int32_t *counter;            // dynamic mode: next chunk (zeroed before the launch)
   const int32_t *order;        // static mode: task t -> sorted target index
   int32_t ntasks;              // static mode: number of tasks
-  int32_t mode;                // 0 dynamic chunks, 1 static round-robin, 2 one chunk per workgroup,
-                               // 3 scan: workgroup b scores profile scan_profs[b] against every target
+  int32_t mode;                // 0 dynamic chunks, 1 static round-robin, 2 one chunk per workgroup
   int32_t prof_idx;
   SlowItem *slow;
   int32_t *slow_count;
@@ -72,13 +71,6 @@ struct FastArgs {
   SlowItem *jslow;             // this profile's J-aware queue (null: use the byte queue)
   int32_t *jslow_count;
   int32_t jslow_cap;
-  const ProfDev *scan_profs;   // mode 3: per-workgroup profile (its ftab and constants; res unused)
-  uint16_t *scan_out;          // mode 3: result word of (workgroup b, target s) at scan_out[s * scan_stride + b];
-  int64_t scan_stride;         // no queues are written
-  int32_t scan_nprof;          // mode 3: profiles; the grid is a multiple of it
-  const uint8_t *scan_rowcode; // mode 3: LDS row r holds table row scan_rowcode[r] (r = 0..db.R)
-  const int32_t *scan_soff;    // mode 3: wave slot w = split * waves + wave scores the tasks
-  const int32_t *scan_stask;   //   scan_stask[scan_soff[w] .. scan_soff[w + 1])
 };
 
 // Batched J-free pass over many profiles in one persistent launch per lane
@@ -107,6 +99,27 @@ struct BatProf {
   int32_t *jslow_count;
   int32_t prof_idx;            // index into the search's profile array
   int32_t tab_u4;              // table size in uint4 entries
+};
+
+// Grouped scan (msv_scan_group<G, QB>): the J-free pass of many profiles,
+// one per workgroup (b % np; the (b / np)-th of its splits), against a batch
+// of queries.  Every profile of a launch has QB register blocks; pd.Q is
+// its own register count and pd.ftab its table over the whole alphabet, in
+// the batched pass's layout for G (above), over codes 0..Kp-1 and the
+// floor rows after them.  The workgroup stages rows rowcode[0..db.R] of it
+// (the codes the queries hold, then a floor row as the null row R) as LDS
+// rows 0..R, each register block keeping its stride of 32 rows.
+struct ScanGroupArgs {
+  const ProfDev *profs;
+  int32_t np;
+  DbDev db;                    // the queries, longest first, residues as LDS rows (db.R = null row)
+  const uint8_t *rowcode;      // LDS row r holds table row rowcode[r] (r = 0..db.R)
+  const int32_t *soff;         // wave slot w = split * waves + wave scores the tasks
+  const int32_t *stask;        //   stask[soff[w] .. soff[w + 1]) (task t = queries t*64/G ..)
+  uint16_t *out;               // word of (profile b % np, query s) at out[s * stride + b % np]
+  int64_t stride;
+  int32_t prio;                // 1: the waves raise their issue priority (a launch whose few
+                               // workgroups' row chains bound the batch, beside busier launches)
 };
 
 struct BatArgs {

@@ -28,7 +28,7 @@
 // Stage 1: fast J-free pass.
 // ---------------------------------------------------------------------------
 
-template <int G, int Q, int TW = 64>
+template <int G, int Q>
 __device__ __forceinline__ void fast_task(const FastArgs &a, const ProfDev &pd, uint16_t *res, int64_t rstride,
                                           const uint4 *lds, int64_t s, int lane) {
   constexpr int QB = (Q + 3) / 4;
@@ -46,7 +46,7 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const ProfDev &pd, 
 
   const uint32_t nb = (uint32_t) a.db.R * 0x01010101u;
   const uint4 null4 = make_uint4(nb, nb, nb, nb);
-  const uint4 *tl = lds + lane % TW;
+  const uint4 *tl = lds + lane;
 
   uint32_t r[Q];
 #pragma unroll
@@ -59,7 +59,7 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const ProfDev &pd, 
   uint4 cnext = *reinterpret_cast<const uint4 *>(p);
   uint4 c = (0 < Lpad) ? cnext : null4;
   uint32_t e[QB * 4];
-  load_row<Q, TW>(tl, byte_of(c, 0), e);
+  load_row<Q>(tl, byte_of(c, 0), e);
   for (int i0 = 0; i0 < Lmax; i0 += 16) {
     const int i1 = i0 + 16;
     cnext = *reinterpret_cast<const uint4 *>(p + (i1 < Lpad ? i1 : 0));
@@ -67,7 +67,7 @@ __device__ __forceinline__ void fast_task(const FastArgs &a, const ProfDev &pd, 
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
       uint32_t en[QB * 4];
-      load_row<Q, TW>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
+      load_row<Q>(tl, (rr < 15) ? byte_of(c, rr + 1) : byte_of(cn, 0), en);
       const uint32_t sh = shift_slot<G>(r[Q - 1], lane);
 #pragma unroll
       for (int j = Q - 1; j >= 1; --j) r[j] = cell(r[j - 1], e[j]);
@@ -107,42 +107,12 @@ __global__ void __launch_bounds__(1024) msv_fast(FastArgs a) {
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
 
-  // Search tables are 64 lanes wide.  Scan tables (mode 3) are TWG =
-  // max(G, 16) lanes wide: the wave's lane groups share one copy, so more
-  // workgroups (one per profile) fit a CU, and each 16-lane phase of a
-  // 16-byte LDS read still covers 256 contiguous bytes.
-  constexpr int TWG = G < 16 ? 16 : G;
-  const int ntab = (a.db.R + 1) * QB * (a.mode == 3 ? TWG : 64);
-  // mode 3: workgroup b scores profile b % scan_nprof against its share of
-  // the targets (the (b / scan_nprof)-th of gridDim.x / scan_nprof splits)
-  const int sp = (a.mode == 3) ? (int) (blockIdx.x % (unsigned) a.scan_nprof) : 0;
-  const uint4 *ftab = (a.mode == 3) ? a.scan_profs[sp].ftab : a.prof.ftab;
-  if (a.mode == 3) {                   // the rows of the codes present, then the null row
-    const int row = QB * TWG;
-    for (int i = threadIdx.x; i < ntab; i += blockDim.x) {
-      const int r = i / row;
-      lds[i] = ftab[(int) a.scan_rowcode[r] * row + (i - r * row)];
-    }
-  } else {
-    for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = ftab[i];
-  }
+  const int ntab = (a.db.R + 1) * QB * 64;
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) lds[i] = a.prof.ftab[i];
   __syncthreads();
 
   const int g = lane / G;
-  if (a.mode == 3) {
-    // Scan: this workgroup's profile against its targets (queries of an
-    // hmmscan batch, sorted by length so a wave's groups finish together);
-    // task t is targets t*NG .. t*NG+NG-1.  The tasks of a wave slot (its
-    // split and wave) come from the host's schedule, which balances the
-    // slots' rows; every profile of the launch shares it.
-    const ProfDev pd = a.scan_profs[sp];
-    uint16_t *res = a.scan_out + sp;
-    const int split = (int) (blockIdx.x / (unsigned) a.scan_nprof);
-    const int slot = __builtin_amdgcn_readfirstlane(split * nwaves + wave);
-    const int i1 = a.scan_soff[slot + 1];
-    for (int i = a.scan_soff[slot]; i < i1; ++i)
-      fast_task<G, Q, TWG>(a, pd, res, a.scan_stride, lds, (int64_t) a.scan_stask[i] * NG + g, lane);
-  } else if (a.mode == 0) {
+  if (a.mode == 0) {
     for (;;) {
       int c = 0;
       if (lane == 0) c = atomicAdd(a.counter, 1);

@@ -46,6 +46,10 @@ static __device__ __align__(256) uint8_t scratch[kScratchBytes];
 
 extern "C" hipError_t msvk_prepare();
 extern "C" hipError_t msvk_prepare_batch();
+extern "C" hipError_t msvk_prepare_scan_group();
+extern "C" int msvk_scan_group_built(int G, int QB);
+extern "C" hipError_t msvk_launch_scan_group(int G, int QB, const ScanGroupArgs *a, int grid, int block, size_t lds,
+                                             hipStream_t st);
 extern "C" hipError_t msvk_launch_batch(int G, const BatArgs *a, int grid, int block, size_t lds, hipStream_t st);
 extern "C" hipError_t msvk_batch_occupancy(int G, int block, size_t lds, int *per_cu);
 extern "C" hipError_t msvk_build_bat_table(const uint8_t *rbv, int M, const uint8_t *rowcode, int R, int G, int Q,
@@ -194,10 +198,13 @@ struct ScanSet {                   // the profiles of msvg_scan calls, grouped b
   std::vector<std::unique_ptr<Cls>> cls;
   std::vector<int32_t> fallback;   // positions whose table does not fit LDS
   int64_t sumM = 0;
-  // Batched scans: the same profiles grouped by fast-kernel tile (G lanes per
-  // query, Q registers), built on the first batched call.
+  // Batched scans: the same profiles grouped by G lanes per query and QB
+  // register blocks (msv_scan_group<G, QB>), members by register count,
+  // most first; built on the first batched call.
   struct GCls {
-    int G = 0, Q = 0;
+    int G = 0, QB = 0;
+    int Qmax = 0;                  // registers of the first member
+    int64_t Qsum = 0;              // over the members
     std::vector<int32_t> members;
     DevMem d_profs;                // ProfDev per member
   };
@@ -360,10 +367,6 @@ void choose_tile_tp(const msvg_ctx *c, int M, int R, int &G, int &waves) {
 }
 
 size_t fast_lds(int R, int Q) { return (size_t) (R + 1) * ((Q + 3) / 4) * 1024; }
-// Lanes of the fast kernel's table in scan mode for G lanes per query
-// (max(G, 16): the lane groups of a wave share one copy), and its bytes.
-int table_width(int G) { return G < 16 ? 16 : G; }
-size_t fast_lds_g(int R, int Q, int G) { return fast_lds(R, Q) / 64 * (size_t) table_width(G); }
 
 // The tile for profile length M over database d.  choose_tile_tp picks for
 // throughput; a launch also lasts at least as long as its longest target's
@@ -1359,6 +1362,7 @@ int msvg_init(int device, msvg_ctx **ret) {
     if (hipEventCreate(&e) != hipSuccess) return MSVG_EDEVICE;
   if (msvk_prepare() != hipSuccess) return MSVG_EDEVICE;
   if (msvk_prepare_batch() != hipSuccess) return MSVG_EDEVICE;
+  if (msvk_prepare_scan_group() != hipSuccess) return MSVG_EDEVICE;
   if (hipGetSymbolAddress((void **) &c->scratch_dev, HIP_SYMBOL(scratch)) != hipSuccess || !c->scratch_dev)
     return MSVG_EDEVICE;
   *ret = c.release();
@@ -2548,10 +2552,14 @@ int scan_prepare(msvg_ctx *c, const int32_t *pids, int32_t np, ScanSet **ret) {
 // 1024, a whole wave beyond).
 int scan_lanes(int M) { return M <= 128 ? 8 : M <= 768 ? 16 : M <= 1024 ? 32 : 64; }
 
-// Registers per lane for the batched scan: exact up to 8, then rounded up to
-// even up to 16 and to a multiple of 4 beyond, so that a database's profiles
-// fall into fewer (larger) launches at a padding cost below 1/8.
-int scan_regs(int Q) { return Q <= 8 ? Q : Q <= 16 ? (Q + 1) & ~1 : std::min(32, (Q + 3) & ~3); }
+// LDS of a grouped-scan workgroup staging R + 1 rows of a table with QB
+// register blocks: G = 64 whole rows of the fast layout; G < 64 the narrow
+// layout's 32-row block stride, the last block only as far as row R.
+size_t scan_group_lds(int G, int R, int QB) {
+  if (G == 64) return (size_t) (R + 1) * QB * 1024;
+  const size_t tw = (size_t) (G < 16 ? 16 : G);
+  return ((size_t) (QB - 1) * 32 + (size_t) (R + 1)) * tw * 16;
+}
 
 // The table rows a batch of queries uses: the codes present (rowcode[r],
 // r < R, ascending) and the null row R, which takes table row Kp; rowof maps
@@ -2581,30 +2589,46 @@ int scan_prepare_grouped(msvg_ctx *c, ScanSet *set) {
   for (size_t i = 0; i < set->pids.size(); ++i) {
     if (!has_cls[i]) continue;
     Profile &p = *set->prof[i];
-    int G = scan_lanes(p.M), Q = scan_regs(qfor(p.M, 2 * G));
-    while (G < 64 && (Q > 32 || fast_lds_g(p.Kp, Q, G) > (size_t) kLdsBytes)) {
+    // the tuned lanes for M, wider where the registers or the table (over
+    // the whole alphabet) would not fit; the narrow layouts take at most 32
+    // rows (Kp + 1 <= 30 for every alphabet)
+    int G = scan_lanes(p.M), Q = qfor(p.M, 2 * G);
+    while (G < 64 && (Q > bat_qmax(G) || scan_group_lds(G, p.Kp, (Q + 3) / 4) > (size_t) kLdsBytes)) {
       G *= 2;
-      Q = scan_regs(qfor(p.M, 2 * G));
+      Q = qfor(p.M, 2 * G);
     }
+    const int QB = (Q + 3) / 4;
+    if (Q > bat_qmax(G) || scan_group_lds(G, p.Kp, QB) > (size_t) kLdsBytes || !msvk_scan_group_built(G, QB) ||
+        (G < 64 && p.Kp + 1 > 32))
+      return fail(c, MSVG_EINVAL, "scan profile without a grouped kernel");
     if (p.gG != G || p.gQ != Q) {
-      HIPCHK(c, p.d_gtab.ensure(fast_lds_g(p.Kp, Q, G)));
-      HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, G, Q,
-                                      table_width(G), p.bias, p.d_gtab.as<uint32_t>(), st));
+      if (G == 64) {
+        HIPCHK(c, p.d_gtab.ensure(fast_lds(p.Kp, Q)));
+        HIPCHK(c, msvk_build_fast_table(p.d_rbv.as<uint8_t>(), p.M, p.Kp, c->d_idrow.as<uint8_t>(), p.Kp, 64, Q, 64,
+                                        p.bias, p.d_gtab.as<uint32_t>(), st));
+      } else {
+        HIPCHK(c, p.d_gtab.ensure(bat_bytes(G, p.Kp, Q)));
+        HIPCHK(c, msvk_build_bat_table(p.d_rbv.as<uint8_t>(), p.M, c->d_idrow.as<uint8_t>(), p.Kp, G, Q, p.bias,
+                                       p.d_gtab.as<uint32_t>(), st));
+      }
       p.gG = G; p.gQ = Q;
     }
-    auto it = by_tile.find({G, Q});
+    auto it = by_tile.find({G, QB});
     if (it == by_tile.end()) {
       set->gcls.push_back(std::make_unique<ScanSet::GCls>());
-      set->gcls.back()->G = G; set->gcls.back()->Q = Q;
-      it = by_tile.emplace(std::make_pair(G, Q), set->gcls.back().get()).first;
+      set->gcls.back()->G = G; set->gcls.back()->QB = QB;
+      it = by_tile.emplace(std::make_pair(G, QB), set->gcls.back().get()).first;
     }
     it->second->members.push_back((int32_t) i);
   }
-  // largest tables first: their launches hold the longest row chains
-  std::sort(set->gcls.begin(), set->gcls.end(), [](const std::unique_ptr<ScanSet::GCls> &x,
-                                                   const std::unique_ptr<ScanSet::GCls> &y) {
-    return x->Q * x->G != y->Q * y->G ? x->Q * x->G > y->Q * y->G : x->G > y->G;
-  });
+  for (auto &gc : set->gcls) {         // most registers first: their workgroups go out first
+    std::stable_sort(gc->members.begin(), gc->members.end(), [set](int32_t x, int32_t y) {
+      return set->prof[(size_t) x]->gQ > set->prof[(size_t) y]->gQ;
+    });
+    gc->Qmax = set->prof[(size_t) gc->members[0]]->gQ;
+    gc->Qsum = 0;
+    for (int32_t i : gc->members) gc->Qsum += set->prof[(size_t) i]->gQ;
+  }
   for (auto &gc : set->gcls) {
     std::vector<ProfDev> h(gc->members.size());
     for (size_t k = 0; k < gc->members.size(); ++k) {
@@ -2612,7 +2636,7 @@ int scan_prepare_grouped(msvg_ctx *c, ScanSet *set) {
       ProfDev &d = h[k];
       std::memset(&d, 0, sizeof d);
       d.ftab = p.d_gtab.as<uint4>();
-      d.M = p.M; d.Q = gc->Q; d.QB = (gc->Q + 3) / 4; d.Qs = qfor(p.M, 128);
+      d.M = p.M; d.Q = p.gQ; d.QB = gc->QB; d.Qs = qfor(p.M, 128);
       d.tbm = p.tbm; d.tec = p.tec; d.base = p.base; d.bias = p.bias;
     }
     HIPCHK(c, gc->d_profs.ensure(h.size() * sizeof(ProfDev)));
@@ -2700,21 +2724,25 @@ ScanPlan scan_plan(const std::vector<int64_t> &cost, int nprof) {
 // rows) first, each to the stream whose chains add up to the least; the
 // other classes share the remaining streams the same way by their work
 // (profiles x registers x rows).  Returns each class's stream and fills
-// `order` with the launch order.
+// `order` with the launch order and `prio` with the classes whose waves take
+// a higher issue priority (the chain-bound ones, when both kinds are there:
+// among the busier launches' waves their chains would stretch several-fold).
 std::vector<int> scan_streams(const std::vector<std::unique_ptr<ScanSet::GCls>> &gcls,
-                              const std::vector<ScanPlan> &plans, int ns, std::vector<size_t> &order) {
+                              const std::vector<ScanPlan> &plans, int ns, std::vector<size_t> &order,
+                              std::vector<char> &prio) {
   const size_t n = gcls.size();
   std::vector<int> stream(n, 0);
   std::vector<int64_t> key(n);
+  prio.assign(n, 0);
   std::vector<size_t> small, big;
   for (size_t ci = 0; ci < n; ++ci) {
     const int64_t wgs = (int64_t) gcls[ci]->members.size() * plans[ci].nsplit;
     if (wgs < kScanCUs) {
       small.push_back(ci);
-      key[ci] = (int64_t) gcls[ci]->Q * plans[ci].load;
+      key[ci] = (int64_t) gcls[ci]->Qmax * plans[ci].load;
     } else {
       big.push_back(ci);
-      key[ci] = (int64_t) gcls[ci]->members.size() * gcls[ci]->Q * plans[ci].sum;
+      key[ci] = gcls[ci]->Qsum * plans[ci].sum;
     }
   }
   auto by_key = [&key](size_t x, size_t y) { return key[x] != key[y] ? key[x] > key[y] : x < y; };
@@ -2737,6 +2765,7 @@ std::vector<int> scan_streams(const std::vector<std::unique_ptr<ScanSet::GCls>> 
   } else {
     deal(small, 0, ns / 2);
     deal(big, ns / 2, ns);
+    for (size_t ci : small) prio[ci] = 1;
     order.clear();                     // alternate, so each stream's first launch goes out at once
     for (size_t i = 0; i < std::max(small.size(), big.size()); ++i) {
       if (i < big.size()) order.push_back(big[i]);
@@ -2854,24 +2883,25 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
   }
   std::vector<size_t> cbase(set->gcls.size(), 0), lorder;       // each class's first result row
   for (size_t ci = 1; ci < set->gcls.size(); ++ci) cbase[ci] = cbase[ci - 1] + set->gcls[ci - 1]->members.size();
-  const std::vector<int> cstream = scan_streams(set->gcls, plans, ns, lorder);
+  std::vector<char> cprio;
+  const std::vector<int> cstream = scan_streams(set->gcls, plans, ns, lorder, cprio);
   for (size_t ci : lorder) {
     auto &gc = set->gcls[ci];
-    FastArgs a{};
-    a.db = qdb;
-    a.mode = 3;
-    a.ntasks = (nl + (64 / gc->G) - 1) / (64 / gc->G);
-    a.scan_profs = gc->d_profs.as<ProfDev>();
-    a.scan_out = d_words + cbase[ci];                       // words [sorted query][row]
-    a.scan_stride = (int64_t) nmem;
-    a.scan_rowcode = d + o_rc;
     const ScanPlan &pl = plans[ci];
-    a.scan_soff = reinterpret_cast<const int32_t *>(d + o_plan[ci]);
-    a.scan_stask = a.scan_soff + pl.off.size();
     const int nprof = (int) gc->members.size();
-    a.scan_nprof = nprof;
+    ScanGroupArgs a{};
+    a.profs = gc->d_profs.as<ProfDev>();
+    a.np = nprof;
+    a.db = qdb;
+    a.rowcode = d + o_rc;
+    a.soff = reinterpret_cast<const int32_t *>(d + o_plan[ci]);
+    a.stask = a.soff + pl.off.size();
+    a.out = d_words + cbase[ci];                            // words [sorted query][row]
+    a.stride = (int64_t) nmem;
+    a.prio = cprio[ci];
     hipStream_t cs = ns > 1 ? c->vstream[cstream[ci]] : st;
-    HIPCHK(c, msvk_launch_fast(gc->G, gc->Q, &a, nprof * pl.nsplit, 64 * pl.waves, fast_lds_g(R, gc->Q, gc->G), cs));
+    HIPCHK(c, msvk_launch_scan_group(gc->G, gc->QB, &a, nprof * pl.nsplit, 64 * pl.waves,
+                                     scan_group_lds(gc->G, R, gc->QB), cs));
   }
   if (ns > 1) {
     for (int k = 0; k < ns; ++k) {

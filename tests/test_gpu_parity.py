@@ -494,6 +494,64 @@ def test_scan_dna_and_mixed_alphabets(ctx, golden):
     ctx.remove_profile(pa)
 
 
+def _scan_many_vs_search_and_oracle(ctx, profs, qs):
+    pids = [ctx.add_profile(p) for p in profs]
+    db = ca.Database.from_list(qs)
+    u, s, ps = ctx.scan_many(pids, db)
+    jrow = ctx.stats()["n_jrow"]
+    did = ctx.add_database(db)
+    ru, rs, rp = ctx.search(pids, did, mode=ca.MODE_PLAIN)
+    ctx.remove_database(did)
+    assert np.array_equal(u.view(np.uint32), ru.T.view(np.uint32))
+    assert np.array_equal(s, rs.T) and np.array_equal(ps, rp.T)
+    O = Oracle()
+    for j, p in enumerate(profs):
+        ost, ousc = O.msv_batch(p.as_dict(), db.res, db.off, db.lens)
+        assert np.array_equal(s[:, j].astype(np.int32), ost), (j, p.M)
+        assert np.array_equal(u[:, j].view(np.uint32), ousc.view(np.uint32)), (j, p.M)
+    for pid in pids:
+        ctx.remove_profile(pid)
+    return jrow
+
+
+def test_scan_many_every_group_kernel(ctx):
+    """The grouped scan kernel for every lane count and register-block count
+    it is built for, at odd and even register counts (the narrow row loop
+    pairs an odd count's leftover register across two rows) and at the
+    block boundaries: amino models from 1 to 2,560 positions (8 lanes up to
+    128, 16 up to 768, 32 up to 1,024, a whole wave beyond) and 18-code
+    models of 3,000 to 4,096 positions (a whole wave, 6 to 8 blocks, tables
+    that fit LDS only over the smaller alphabet), against batches of 4, 23
+    and 90 queries with planted homologs and an empty query; equal to the
+    search path and to the oracle."""
+    rng = np.random.default_rng(2024)
+    Ms = [1, 33, 64, 65, 100, 128, 129, 161, 200, 257, 300, 385, 450, 513, 600, 641, 700, 768,
+          769, 900, 1024, 1025, 1300, 1537, 1800, 2049, 2300, 2560]
+    profs = [ca.sample_profile(8100 + i, M) for i, M in enumerate(Ms)]
+    for p in profs:
+        p.mu, p.lam = -8.0, 0.69
+    jrow = []
+    for nq in (4, 23, 90):
+        hi = 1500 if nq < 50 else 400
+        qs = [rng.integers(0, 20, int(L)).astype(np.uint8) for L in rng.integers(1, hi, nq)]
+        qs[1] = np.zeros(0, np.uint8)
+        qs[2] = ca.core_emit(profs[9].hmm, 30 + nq, 1)[0]
+        qs[-1] = ca.core_emit(profs[24].hmm, 40 + nq, 1)[0]
+        print(f"scan_many every group kernel: amino, {nq} queries", flush=True)
+        jrow.append(_scan_many_vs_search_and_oracle(ctx, profs, qs))
+    assert max(jrow) > 0                        # the homologs' pairs took the J-aware stage
+    # 18-code models (the nucleotide alphabets' Kp) from sampled models' first 18 rows
+    dna = []
+    for i, M in enumerate((100, 500, 3000, 3500, 4096)):
+        d = ca.sample_profile(8200 + i, M).as_dict()
+        dna.append(ca.ByteProfile(d["rbv"][:18], d["tbm"], d["tec"], d["base"], d["bias"], d["scale"], -8.0, 0.69,
+                                  alphabet=ca.DNA))
+    qs = [rng.integers(0, 4, int(L)).astype(np.uint8) for L in rng.integers(1, 900, 11)]
+    qs += [rng.integers(0, 18, 300).astype(np.uint8), np.zeros(0, np.uint8)]
+    print("scan_many every group kernel: 18 codes", flush=True)
+    _scan_many_vs_search_and_oracle(ctx, dna, qs)
+
+
 def test_scan_guard_regime_fallback(ctx):
     """In the guard regime a query that keeps every cell at the floor needs
     the exact byte MSV: msvg_scan hands that profile to the search path

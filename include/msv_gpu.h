@@ -165,7 +165,7 @@ int msvg_scan(msvg_ctx *ctx, int32_t np, const int32_t *profile_ids, const uint8
 /* msvg_scan for a batch of nq queries (digital codes, no sentinels: query q
  * is res[qoff[q] .. qoff[q] + qlen[q])): the result of profile i against
  * query q at usc[q * np + i] (status, pass likewise).  Each workgroup stages
- * its profile's table once and its four waves share the queries, so a batch
+ * its profile's table once and its waves share the queries, so a batch
  * costs far less than nq single calls.  No counterpart in the reference. */
 int msvg_scan_many(msvg_ctx *ctx, int32_t np, const int32_t *profile_ids, int32_t nq, const uint8_t *res,
                    const int64_t *qoff, const int32_t *qlen, double F1, float *usc, uint8_t *status,

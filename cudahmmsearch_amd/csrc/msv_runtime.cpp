@@ -196,6 +196,14 @@ struct ScanSet {                   // the profiles of msvg_scan calls, grouped b
     std::vector<int32_t> slot;     // position in pids -> member index, -1 if not a member
   };
   std::vector<std::unique_ptr<Cls>> cls;
+  // Per class member, in class order: its position and the constants the
+  // host's decoding reads (one sequential array rather than each profile's
+  // record); rebuilt after msvg_profile_set_evparams changes mu or lambda.
+  struct Row { int32_t i; int base; float scale, mu, lambda; };
+  std::vector<Row> rows;
+  bool rows_ok = false;
+  std::vector<Row> grows;          // the same in the grouped classes' order (gcls)
+  bool grows_ok = false;
   std::vector<int32_t> fallback;   // positions whose table does not fit LDS
   int64_t sumM = 0;
   // Batched scans: the same profiles grouped by G lanes per query and QB
@@ -1727,6 +1735,7 @@ int msvg_profile_set_evparams(msvg_ctx *c, int32_t id, float mmu, float mlambda)
   if (!c || !c->profiles.count(id)) return fail(c, MSVG_EINVAL, "unknown profile id");
   c->profiles[id]->mu = mmu;
   c->profiles[id]->lambda = mlambda;
+  if (c->scanset) c->scanset->rows_ok = c->scanset->grows_ok = false;
   for (auto it = c->finals.begin(); it != c->finals.end();)
     it = (std::get<0>(it->first) == id) ? c->finals.erase(it) : std::next(it);
   c->last = nullptr;
@@ -2943,17 +2952,16 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
   // The words are decoded on host threads, each over a range of queries:
   // word (k, r) at words[k * nmem + r], row r = a class member, and its
   // outputs at [query][profile position], so reads and writes run along rows.
-  struct Row { int32_t i; int base; float scale, mu, lambda; };
-  std::vector<Row> rows(nmem);
-  size_t base = 0;
-  for (auto &gc : set->gcls) {
-    for (size_t m = 0; m < gc->members.size(); ++m) {
-      const int32_t i = gc->members[m];
-      const Profile &p = *set->prof[(size_t) i];
-      rows[base + m] = Row{i, p.base, p.scale, p.mu, p.lambda};
-    }
-    base += gc->members.size();
+  if (!set->grows_ok) {
+    set->grows.clear();
+    for (auto &gc : set->gcls)
+      for (int32_t i : gc->members) {
+        const Profile &p = *set->prof[(size_t) i];
+        set->grows.push_back(ScanSet::Row{i, p.base, p.scale, p.mu, p.lambda});
+      }
+    set->grows_ok = true;
   }
+  const std::vector<ScanSet::Row> &rows = set->grows;
   const int nth = (int) std::max<size_t>(1, std::min<size_t>({(size_t) 8, (size_t) nl,
                                                              nmem * (size_t) nl / 32768 + 1,
                                                              (size_t) std::max(1u, std::thread::hardware_concurrency())}));
@@ -2971,7 +2979,7 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
       uint8_t *sq = status + (size_t) q * np, *pq = pass + (size_t) q * np;
       for (size_t r = 0; r < nmem; ++r) {
         const uint32_t w = wk[r];
-        const Row &R = rows[r];
+        const ScanSet::Row &R = rows[r];
         const uint32_t code = w >> 8;
         if (code == RES_SLOW) {
           if (w & 1u) {
@@ -3221,6 +3229,15 @@ int msvg_scan_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t nq, con
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->stats.ms_fast = ms;
     c->stats.ms_total = ms;
+    if (!set->rows_ok) {
+      set->rows.clear();
+      for (auto &cl : set->cls)
+        for (int32_t i : cl->members) {
+          const Profile &p = *set->prof[(size_t) i];
+          set->rows.push_back(ScanSet::Row{i, p.base, p.scale, p.mu, p.lambda});
+        }
+      set->rows_ok = true;
+    }
     for (int32_t k = 0; k < nl; ++k) {
       const int32_t q = qlive[(size_t) k];
       const int tjb = msvh_tjb(qlen[q]);
@@ -3228,23 +3245,18 @@ int msvg_scan_many(msvg_ctx *c, int32_t np, const int32_t *pids, int32_t nq, con
       const uint32_t *wq = c->h_scanout + (size_t) k * np;
       float *uq = usc + (size_t) q * np;
       uint8_t *sq = status + (size_t) q * np, *pq = pass + (size_t) q * np;
-      base = 0;
-      for (auto &cl : set->cls) {
-        for (size_t m = 0; m < cl->members.size(); ++m) {
-          const int32_t i = cl->members[m];
-          const uint32_t w = wq[base + (int64_t) m];
-          const uint32_t code = w >> 8;
-          const Profile &p = *set->prof[(size_t) i];
-          if (code == RES_SLOW) { slow.emplace_back(i, q); continue; }
-          if (code == RES_ERANGE) {
-            uq[i] = std::numeric_limits<float>::infinity(); sq[i] = MSVG_ERANGE; pq[i] = 1;
-          } else {
-            uq[i] = usc_of((int) (w & 255u) - tjb, p.base, p.scale);
-            sq[i] = MSVG_OK;
-            pq[i] = pass_rule(uq[i], p.mu, p.lambda) ? 1 : 0;
-          }
+      for (size_t m = 0; m < set->rows.size(); ++m) {
+        const ScanSet::Row &R = set->rows[m];
+        const uint32_t w = wq[m];
+        const uint32_t code = w >> 8;
+        if (code == RES_SLOW) { slow.emplace_back(R.i, q); continue; }
+        if (code == RES_ERANGE) {
+          uq[R.i] = std::numeric_limits<float>::infinity(); sq[R.i] = MSVG_ERANGE; pq[R.i] = 1;
+        } else {
+          uq[R.i] = usc_of((int) (w & 255u) - tjb, R.base, R.scale);
+          sq[R.i] = MSVG_OK;
+          pq[R.i] = pass_rule(uq[R.i], R.mu, R.lambda) ? 1 : 0;
         }
-        base += (int64_t) cl->members.size();
       }
     }
   }

@@ -453,6 +453,22 @@ def test_scan_direction_equals_search(ctx):
             assert ps[j] == _oracle_pass(O, ost, ousc, one.lens, p.mu, p.lam)[0], (j, q.size)
     st = ctx.stats()
     assert st["cells"] == 0 or st["cells"] > 0
+    # new Gumbel parameters for every profile after the scan set was built:
+    # the next scan of the same profile list applies them
+    q = queries[0]
+    _, _, before = ctx.scan(pids, q)
+    for pid, p in zip(pids, profs):
+        p.mu = p.mu - 8.0
+        ctx.set_evparams(pid, p.mu, p.lam)
+    u, s, ps = ctx.scan(pids, q)
+    did = ctx.add_database(ca.Database.from_list([q]))
+    _, _, rp = ctx.search(pids, did, mode=ca.MODE_PLAIN)
+    ctx.remove_database(did)
+    assert np.array_equal(ps, rp[:, 0]) and not np.array_equal(ps, before)
+    one = ca.Database.from_list([q])
+    for j, p in enumerate(profs):
+        ost, ousc = O.msv_batch(p.as_dict(), one.res, one.off, one.lens)
+        assert ps[j] == _oracle_pass(O, ost, ousc, one.lens, p.mu, p.lam)[0], j
     with pytest.raises(ca.MSVError):
         ctx.scan(pids, np.array([1, 29], np.uint8))              # code outside the alphabet
     for pid in pids:
@@ -622,6 +638,17 @@ def test_scan_many_equals_search(ctx):
             u1, s1, p1 = ctx.scan(pids, qs[0])
             assert np.array_equal(u1.view(np.uint32), u[0].view(np.uint32)) and np.array_equal(p1, ps[0])
     assert max(jrow[2:]) > 0                    # the homologs' pairs took the J-aware stage
+    # new Gumbel parameters after the batch's scan set was built: the next
+    # batch applies them
+    _, _, before = ctx.scan_many(pids, db)
+    for pid, p in zip(pids, profs):
+        p.mu = p.mu - 8.0
+        ctx.set_evparams(pid, p.mu, p.lam)
+    u, s, ps = ctx.scan_many(pids, db)
+    did = ctx.add_database(db)
+    _, _, rp = ctx.search(pids, did, mode=ca.MODE_PLAIN)
+    ctx.remove_database(did)
+    assert np.array_equal(ps, rp.T) and not np.array_equal(ps, before)
     u, s, ps = ctx.scan_many(pids, [])
     assert u.shape == (0, len(pids))
     for pid in pids:

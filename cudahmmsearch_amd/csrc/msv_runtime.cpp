@@ -3060,6 +3060,19 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
   HIPCHK(c, hipMemcpyAsync(c->d_scanj.p, hb.data(), hb.size(), hipMemcpyHostToDevice, st));
   uint8_t *dj = c->d_scanj.as<uint8_t>();
   HIPCHK(c, hipEventRecord(c->ev[2], st));
+  // the classes' J-aware launches side by side on the scan's streams
+  const int nj_st = std::min<int>(kVitStreams, (int) parts.size());
+  if (nj_st > 1) {
+    if (!c->vstream[0]) {
+      for (int k = 0; k < kVitStreams; ++k) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->vstream[k], hipStreamNonBlocking));
+        HIPCHK(c, hipEventCreateWithFlags(&c->vdone[k], hipEventDisableTiming));
+      }
+    }
+    HIPCHK(c, hipEventRecord(c->fork, st));
+    for (int k = 0; k < nj_st; ++k) HIPCHK(c, hipStreamWaitEvent(c->vstream[k], c->fork, 0));
+  }
+  size_t jpart = 0;
   for (const Part &pt : parts) {
     ScanArgs a{};
     a.profs = reinterpret_cast<const ScanProf *>(dj + pt.o_prof);
@@ -3071,7 +3084,14 @@ int scan_grouped(msvg_ctx *c, ScanSet *set, int32_t np, const uint8_t *res, cons
     a.wq_off = reinterpret_cast<const int32_t *>(dj + pt.o_woff);
     a.wq_idx = reinterpret_cast<const int32_t *>(dj + pt.o_widx);
     const int Q = set->cls[pt.ci]->Q;
-    HIPCHK(c, msvk_launch_scan(Q, &a, (int) pt.nwg, fast_lds(R, Q), st));
+    hipStream_t js = nj_st > 1 ? c->vstream[jpart++ % (size_t) nj_st] : st;
+    HIPCHK(c, msvk_launch_scan(Q, &a, (int) pt.nwg, fast_lds(R, Q), js));
+  }
+  if (nj_st > 1) {
+    for (int k = 0; k < nj_st; ++k) {
+      HIPCHK(c, hipEventRecord(c->vdone[k], c->vstream[k]));
+      HIPCHK(c, hipStreamWaitEvent(st, c->vdone[k], 0));
+    }
   }
   HIPCHK(c, hipEventRecord(c->ev[3], st));
   std::vector<uint32_t> jw(nout);
